@@ -1,0 +1,26 @@
+"""coolpdlp_hip -- MI355X-native PDLP/PDHG engine with CoolPDLP.jl's API.
+
+Exports mirror src/CoolPDLP.jl:53-63: the matrix plug-ins, MILP and its counters,
+PrimalDualSolution, solve / PDHG / PDLP / Algorithm, is_feasible, objective_value.
+All computation runs in libcoolpdlp_hip.so (hand-written HIP kernels for gfx950).
+"""
+from ._lib import ArgumentError, DimensionMismatch, HIPError
+from .algorithms import (ITERATION_LIMIT, OPTIMAL, STILL_RUNNING, TIME_LIMIT, Algorithm,
+                         ConvergenceStats, Engine, KKTErrors, PDHG, PDLP, TerminationStatus,
+                         absolute, relative, solve)
+from .matrices import GPUSparseMatrixCSR, GPUSparseMatrixELL
+from .problems import (MILP, PrimalDualSolution, is_feasible, nbcons, nbcons_eq, nbcons_ineq,
+                       nbvar, nbvar_cont, nbvar_int, objective_value)
+
+__all__ = [
+    "GPUSparseMatrixCSR", "GPUSparseMatrixELL", "MILP", "nbvar", "nbvar_int", "nbvar_cont",
+    "nbcons", "nbcons_eq", "nbcons_ineq", "PrimalDualSolution", "solve", "PDHG", "PDLP",
+    "Algorithm", "is_feasible", "objective_value", "TerminationStatus", "OPTIMAL",
+    "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING", "KKTErrors", "ConvergenceStats",
+    "Engine", "relative", "absolute", "DimensionMismatch", "ArgumentError", "HIPError",
+]
+
+
+def version() -> str:
+    from ._lib import lib
+    return lib.cpdlp_version().decode()

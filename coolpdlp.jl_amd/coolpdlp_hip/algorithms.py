@@ -1,0 +1,268 @@
+"""Algorithm configs, results and solve(), mirroring src/algorithms/*.jl and the
+components' public structs (KKTErrors, ConvergenceStats, TerminationStatus).
+
+Every solve runs on the HIP engine through the C ABI; nothing here computes the
+method itself.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, check, lib
+from .problems import MILP, PrimalDualSolution
+
+
+class TerminationStatus(enum.IntEnum):
+    """src/components/termination.jl:13 (same order)"""
+    OPTIMAL = 0
+    TIME_LIMIT = 1
+    ITERATION_LIMIT = 2
+    STILL_RUNNING = 3
+
+
+OPTIMAL = TerminationStatus.OPTIMAL
+TIME_LIMIT = TerminationStatus.TIME_LIMIT
+ITERATION_LIMIT = TerminationStatus.ITERATION_LIMIT
+STILL_RUNNING = TerminationStatus.STILL_RUNNING
+
+
+@dataclass
+class KKTErrors:
+    """src/components/errors.jl:8-21"""
+    primal: float = math.nan
+    primal_scale: float = math.nan
+    dual: float = math.nan
+    dual_scale: float = math.nan
+    gap: float = math.nan
+    gap_scale: float = math.nan
+
+    @classmethod
+    def from_c(cls, e):
+        return cls(e.primal, e.primal_scale, e.dual, e.dual_scale, e.gap, e.gap_scale)
+
+    def relative(self) -> float:
+        """errors.jl:55-58"""
+        return max(self.primal / self.primal_scale, self.dual / self.dual_scale,
+                   self.gap / self.gap_scale)
+
+    def absolute(self, omega: float) -> float:
+        """errors.jl:60-63"""
+        return math.sqrt(omega ** 2 * self.primal ** 2 + omega ** -2 * self.dual ** 2
+                         + self.gap ** 2)
+
+    def isapprox(self, other: "KKTErrors", rtol=1e-8, atol=0.0) -> bool:
+        """errors.jl:33-42"""
+        return all(math.isclose(getattr(self, f), getattr(other, f), rel_tol=rtol, abs_tol=atol)
+                   for f in ("primal", "dual", "gap", "primal_scale", "dual_scale", "gap_scale"))
+
+    def __str__(self):
+        return (f"KKT relative errors: primal {self.primal / self.primal_scale:.3e}, "
+                f"dual {self.dual / self.dual_scale:.3e}, gap {self.gap / self.gap_scale:.3e}")
+
+
+relative = KKTErrors.relative
+absolute = KKTErrors.absolute
+
+
+@dataclass
+class ConvergenceStats:
+    """src/components/termination.jl:44-56, plus the engine's counters."""
+    err: KKTErrors
+    time_elapsed: float
+    kkt_passes: int
+    termination_status: TerminationStatus
+    error_history: list = field(default_factory=list)
+    eta: float = math.nan
+    omega: float = math.nan
+    powm_iters: int = 0
+    restarts: int = 0
+    fp32_passes: int = 0
+    setup_seconds: float = 0.0
+    loop_seconds: float = 0.0
+    step_kernel_ms: float = 0.0
+    step_kernel_launches: int = 0
+    format_A: int = 0
+    format_At: int = 0
+
+
+_FLOAT_TYPES = {"float64": _lib.PREC_FP64, "Float64": _lib.PREC_FP64, np.float64: _lib.PREC_FP64,
+                "float32": _lib.PREC_FP32, "Float32": _lib.PREC_FP32, np.float32: _lib.PREC_FP32,
+                "mixed": _lib.PREC_MIXED}
+
+
+class Algorithm:
+    """Algorithm{A}(T, Ti, M; backend, kwargs...), src/algorithms/common.jl:53-116.
+
+    Positional: float type T ("float64" | "float32" | "mixed"), index type Ti
+    (accepted; the engine indexes with int32), matrix type M (GPUSparseMatrixCSR,
+    GPUSparseMatrixELL or "auto").  Keyword arguments are the reference's, with the same
+    defaults, plus engine options (device, powm_tol, powm_maxiter, powm_seed,
+    polish_reltol, use_graph, long_row)."""
+
+    name = "PDLP"
+    _algo = _lib.ALGO_PDLP
+
+    def __init__(self, float_type="float64", int_type=None, matrix_type=None, *, backend="hip",
+                 **kwargs):
+        if backend not in ("hip", "rocm", "gpu"):
+            raise ArgumentError(f"unsupported backend {backend!r}: this engine runs on HIP")
+        self.params = _lib.default_params(self._algo)
+        if float_type not in _FLOAT_TYPES:
+            raise ArgumentError(f"unsupported float type {float_type!r}")
+        self.params.precision = _FLOAT_TYPES[float_type]
+        self.float_type = float_type
+        self.int_type = int_type
+        from .matrices import format_of
+        self.params.format = format_of(matrix_type)
+        self.matrix_type = matrix_type
+        names = {f for f, _ in _lib.CParams._fields_}
+        for k, v in kwargs.items():
+            if k not in names or k in ("algorithm", "precision", "format"):
+                raise ArgumentError(f"unknown keyword argument {k!r}")
+            setattr(self.params, k, v)
+
+    def __repr__(self):
+        p = self.params
+        return (f"{self.name} algorithm: float_type={self.float_type}, check_every={p.check_every}, "
+                f"termination_reltol={p.termination_reltol}, max_kkt_passes={p.max_kkt_passes}, "
+                f"time_limit={p.time_limit}, ruiz_iter={p.ruiz_iter}, "
+                f"chambolle_pock_alpha={p.chambolle_pock_alpha}")
+
+
+class PDLP(Algorithm):
+    """PDLP(args...; kwargs...), src/algorithms/pdlp.jl:6-11"""
+    name = "PDLP"
+    _algo = _lib.ALGO_PDLP
+
+
+class PDHG(Algorithm):
+    """PDHG(args...; kwargs...), src/algorithms/pdhg.jl:6-16 (ruiz_iter=0, no restarts)"""
+    name = "PDHG"
+    _algo = _lib.ALGO_PDHG
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Engine:
+    """One LP resident on the GPU (a cpdlp_handle): build once, solve, query."""
+
+    def __init__(self, milp: MILP, algo: Algorithm):
+        self.milp = milp
+        self.algo = algo
+        A = milp.A
+        self._colptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        self._rowval = np.ascontiguousarray(A.indices, dtype=np.int64)
+        self._nzval = np.ascontiguousarray(A.data, dtype=np.float64)
+        self._vecs = [np.ascontiguousarray(v, dtype=np.float64)
+                      for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc)]
+        lp = _lib.CLP()
+        lp.m, lp.n = A.shape
+        lp.index_base, lp.index_bits = 0, 64
+        lp.colptr, lp.rowval, lp.nzval = (_ptr(self._colptr), _ptr(self._rowval),
+                                          _ptr(self._nzval))
+        lp.c, lp.lv, lp.uv, lp.lc, lp.uc = [_ptr(v) for v in self._vecs]
+        h = C.c_void_p()
+        check(lib.cpdlp_create(C.byref(lp), C.byref(algo.params), C.byref(h)))
+        self.h = h
+        self.m, self.n = A.shape
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.cpdlp_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def prepare(self):
+        check(lib.cpdlp_prepare(self.h))
+
+    def solve(self, sol: PrimalDualSolution | None = None):
+        x0 = y0 = None
+        if sol is not None:
+            x0 = np.ascontiguousarray(sol.x, dtype=np.float64)
+            y0 = np.ascontiguousarray(sol.y, dtype=np.float64)
+            if len(x0) != self.n or len(y0) != self.m:
+                raise _lib.DimensionMismatch("starting point has the wrong size")
+        check(lib.cpdlp_solve(self.h, _ptr(x0), _ptr(y0)))
+        return self.solution(), self.stats()
+
+    def solution(self) -> PrimalDualSolution:
+        x = np.zeros(self.n)
+        y = np.zeros(self.m)
+        check(lib.cpdlp_get_solution(self.h, _ptr(x), _ptr(y)))
+        return PrimalDualSolution(x, y)
+
+    def stats(self) -> ConvergenceStats:
+        s = _lib.CStats()
+        check(lib.cpdlp_get_stats(self.h, C.byref(s)))
+        cnt = C.c_int64()
+        check(lib.cpdlp_get_history(self.h, None, None, 0, C.byref(cnt)))
+        k = cnt.value
+        passes = np.zeros(max(k, 1), np.int64)
+        errs = (_lib.CKKT * max(k, 1))()
+        check(lib.cpdlp_get_history(self.h, _ptr(passes), errs, k, C.byref(cnt)))
+        hist = [(int(passes[i]), KKTErrors.from_c(errs[i])) for i in range(k)]
+        return ConvergenceStats(
+            err=KKTErrors.from_c(s.err), time_elapsed=s.time_elapsed, kkt_passes=s.kkt_passes,
+            termination_status=TerminationStatus(s.termination_status), error_history=hist,
+            eta=s.eta, omega=s.omega, powm_iters=s.powm_iters, restarts=s.restarts,
+            fp32_passes=s.fp32_passes, setup_seconds=s.setup_seconds,
+            loop_seconds=s.loop_seconds, step_kernel_ms=s.step_kernel_ms,
+            step_kernel_launches=s.step_kernel_launches, format_A=s.format_A,
+            format_At=s.format_At)
+
+    def kkt_errors(self, sol: PrimalDualSolution) -> KKTErrors:
+        """kkt_errors! on the preconditioned problem for an unscaled point, errors.jl:65-109"""
+        x = np.ascontiguousarray(sol.x, dtype=np.float64)
+        y = np.ascontiguousarray(sol.y, dtype=np.float64)
+        e = _lib.CKKT()
+        check(lib.cpdlp_kkt_errors(self.h, _ptr(x), _ptr(y), C.byref(e)))
+        return KKTErrors.from_c(e)
+
+    def scaling(self):
+        """(D1, D2, scaled A values, scaled At values) in CSC(A) layout"""
+        nnz = len(self._nzval)
+        D1, D2 = np.zeros(max(self.m, 1)), np.zeros(max(self.n, 1))
+        a, at = np.zeros(max(nnz, 1)), np.zeros(max(nnz, 1))
+        check(lib.cpdlp_get_scaling(self.h, _ptr(D1), _ptr(D2), _ptr(a), _ptr(at)))
+        return D1[: self.m], D2[: self.n], a[:nnz], at[:nnz]
+
+    def csr(self):
+        nnz = len(self._nzval)
+        rp = np.zeros(self.m + 1, np.int64)
+        col = np.zeros(max(nnz, 1), np.int32)
+        check(lib.cpdlp_get_csr(self.h, _ptr(rp), _ptr(col)))
+        return rp, col[:nnz]
+
+    def step_size(self):
+        eta, omega, it = C.c_double(), C.c_double(), C.c_int64()
+        check(lib.cpdlp_get_step_size(self.h, C.byref(eta), C.byref(omega), C.byref(it)))
+        return eta.value, omega.value, it.value
+
+
+def solve(milp: MILP, *args):
+    """solve(milp, algo) / solve(milp, sol, algo), src/algorithms/common.jl:186-209.
+
+    Returns (PrimalDualSolution, ConvergenceStats); the solution is unscaled."""
+    if len(args) == 1:
+        sol, algo = None, args[0]
+    elif len(args) == 2:
+        sol, algo = args
+    else:
+        raise TypeError("solve(milp, algo) or solve(milp, sol, algo)")
+    with Engine(milp, algo) as eng:
+        return eng.solve(sol)

@@ -1,0 +1,197 @@
+// layout.hpp -- host-side construction of the device matrix layouts.
+//
+// Input: the reference's SparseMatrixCSC (colptr n+1, rowval nnz, 0-based here, rows
+// sorted within columns).  Output, all int32:
+//   * CSR(A) in the original row order: the stable counting-sort transpose,
+//     i.e. sparse(transpose(A)) as GPUSparseMatrixCSR(A) builds it
+//     (src/utils/mat_csr.jl:58-61), with perm[k_csr] = k_csc;
+//   * CSR(A') = the CSC(A) arrays themselves (src/utils/mat_csr.jl:58-61 applied to
+//     MILP.At, src/problems/milp.jl:60);
+//   * the internal layouts used by the kernels: rows of A (resp. A') longer than
+//     `long_row` are moved to the front by a stable partition, and column indices are
+//     relabelled through the other side's partition.  Entries inside each row keep
+//     their original order, so the sequential row sums are unchanged.
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "spmv.hpp"
+
+namespace cpdlp {
+
+struct HostSide {
+    int32_t rows = 0, cols = 0;
+    int64_t nnz = 0;
+    int32_t n_long = 0;
+    std::vector<int32_t> rowptr, col, vmap, tile_ptr;
+};
+
+struct HostLayout {
+    int64_t m = 0, n = 0, nnz = 0;
+    std::vector<int32_t> colptr, rowval;           // CSC(A) = CSR(A')
+    std::vector<int32_t> csr_rowptr, csr_col, perm; // CSR(A), original order
+    std::vector<int32_t> row_of_int, int_of_row, col_of_int, int_of_col;
+    HostSide A, At;
+};
+
+// Normalise the caller's CSC (0/1-based, 32/64-bit) into 0-based int32 arrays and
+// validate it (sizes, monotone colptr, row bounds, strictly increasing rows).
+inline void read_csc(int64_t m, int64_t n, int32_t base, int32_t bits, const void* colptr,
+                     const void* rowval, std::vector<int32_t>& cp, std::vector<int32_t>& rv) {
+    if (m < 0 || n < 0) throw Error(CPDLP_ERR_DIMENSION, "negative matrix size");
+    if (base != 0 && base != 1) throw Error(CPDLP_ERR_ARGUMENT, "index_base must be 0 or 1");
+    if (bits != 32 && bits != 64) throw Error(CPDLP_ERR_ARGUMENT, "index_bits must be 32 or 64");
+    if (m > INT32_MAX - 1 || n > INT32_MAX - 1)
+        throw Error(CPDLP_ERR_ARGUMENT, "matrix dimension exceeds int32 indexing");
+    if (n > 0 && !colptr) throw Error(CPDLP_ERR_ARGUMENT, "colptr is NULL");
+    cp.resize(n + 1);
+    auto cp_at = [&](int64_t j) -> int64_t {
+        return bits == 32 ? (int64_t)((const int32_t*)colptr)[j] : ((const int64_t*)colptr)[j];
+    };
+    const int64_t c0 = n >= 0 && colptr ? cp_at(0) - base : 0;
+    if (c0 != 0) throw Error(CPDLP_ERR_ARGUMENT, "colptr must start at index_base");
+    const int64_t nnz = colptr ? cp_at(n) - base : 0;
+    if (nnz < 0 || nnz > INT32_MAX) throw Error(CPDLP_ERR_ARGUMENT, "nnz out of int32 range");
+    if (nnz > 0 && !rowval) throw Error(CPDLP_ERR_ARGUMENT, "rowval is NULL");
+    for (int64_t j = 0; j <= n; ++j) {
+        const int64_t v = colptr ? cp_at(j) - base : 0;
+        if (v < 0 || v > nnz || (j > 0 && v < cp[j - 1]))
+            throw Error(CPDLP_ERR_ARGUMENT, "colptr is not monotone");
+        cp[j] = (int32_t)v;
+    }
+    rv.resize(nnz);
+    for (int64_t k = 0; k < nnz; ++k) {
+        const int64_t v = (bits == 32 ? (int64_t)((const int32_t*)rowval)[k]
+                                      : ((const int64_t*)rowval)[k]) - base;
+        if (v < 0 || v >= m) throw Error(CPDLP_ERR_DIMENSION, "row index out of range");
+        rv[k] = (int32_t)v;
+    }
+    for (int64_t j = 0; j < n; ++j)
+        for (int32_t k = cp[j] + 1; k < cp[j + 1]; ++k)
+            if (rv[k] <= rv[k - 1])
+                throw Error(CPDLP_ERR_ARGUMENT, "rows must be strictly increasing within a column");
+}
+
+// stable counting-sort transpose: CSR(A) with perm[k_csr] = k_csc
+inline void csc_to_csr(int64_t m, int64_t n, const std::vector<int32_t>& cp,
+                       const std::vector<int32_t>& rv, std::vector<int32_t>& rp,
+                       std::vector<int32_t>& col, std::vector<int32_t>& perm) {
+    const int64_t nnz = cp[n];
+    rp.assign(m + 1, 0);
+    for (int64_t k = 0; k < nnz; ++k) rp[rv[k] + 1]++;
+    for (int64_t i = 0; i < m; ++i) rp[i + 1] += rp[i];
+    std::vector<int32_t> next(rp.begin(), rp.end() - 1);
+    col.resize(nnz);
+    perm.resize(nnz);
+    for (int64_t j = 0; j < n; ++j)
+        for (int32_t k = cp[j]; k < cp[j + 1]; ++k) {
+            const int32_t d = next[rv[k]]++;
+            col[d] = (int32_t)j;
+            perm[d] = k;
+        }
+}
+
+// stable partition: long rows (length > long_row) first
+inline void partition_rows(const std::vector<int32_t>& ptr, int32_t long_row,
+                           std::vector<int32_t>& of_int, std::vector<int32_t>& int_of,
+                           int32_t& n_long) {
+    const int64_t rows = (int64_t)ptr.size() - 1;
+    of_int.clear();
+    of_int.reserve(rows);
+    for (int64_t i = 0; i < rows; ++i)
+        if (ptr[i + 1] - ptr[i] > long_row) of_int.push_back((int32_t)i);
+    n_long = (int32_t)of_int.size();
+    for (int64_t i = 0; i < rows; ++i)
+        if (ptr[i + 1] - ptr[i] <= long_row) of_int.push_back((int32_t)i);
+    int_of.assign(rows, 0);
+    for (int64_t r = 0; r < rows; ++r) int_of[of_int[r]] = (int32_t)r;
+}
+
+// greedy short tiles over internal rows [n_long, rows): <= BLOCK rows, <= TILE_NNZ nnz
+inline void build_tiles(HostSide& S) {
+    S.tile_ptr.clear();
+    int32_t r = S.n_long;
+    S.tile_ptr.push_back(r);
+    while (r < S.rows) {
+        int32_t rows_in = 0;
+        int64_t nz = 0;
+        while (r < S.rows && rows_in < BLOCK) {
+            const int64_t L = S.rowptr[r + 1] - S.rowptr[r];
+            if (rows_in > 0 && nz + L > TILE_NNZ) break;
+            nz += L;
+            ++rows_in;
+            ++r;
+        }
+        S.tile_ptr.push_back(r);
+    }
+}
+
+// Internal side: row r of the internal matrix is source row src_row[r]; its entries
+// are src_ptr/src_col/src_k of that row in order, columns mapped by col_map.
+inline void build_side(HostSide& S, int32_t rows, int32_t cols, const std::vector<int32_t>& src_ptr,
+                       const std::vector<int32_t>& src_col, const std::vector<int32_t>* src_k,
+                       const std::vector<int32_t>& row_of_int, const std::vector<int32_t>* col_map,
+                       int32_t n_long) {
+    S.rows = rows;
+    S.cols = cols;
+    S.n_long = n_long;
+    S.nnz = src_ptr[rows];
+    S.rowptr.assign(rows + 1, 0);
+    for (int32_t r = 0; r < rows; ++r) {
+        const int32_t i = row_of_int[r];
+        S.rowptr[r + 1] = S.rowptr[r] + (src_ptr[i + 1] - src_ptr[i]);
+    }
+    S.col.resize(S.nnz);
+    S.vmap.resize(S.nnz);
+    for (int32_t r = 0; r < rows; ++r) {
+        const int32_t i = row_of_int[r];
+        int32_t d = S.rowptr[r];
+        for (int32_t k = src_ptr[i]; k < src_ptr[i + 1]; ++k, ++d) {
+            S.col[d] = col_map ? (*col_map)[src_col[k]] : src_col[k];
+            S.vmap[d] = src_k ? (*src_k)[k] : k;
+        }
+    }
+    build_tiles(S);
+}
+
+inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
+                               std::vector<int32_t>&& rv, int32_t long_row) {
+    HostLayout L;
+    L.m = m;
+    L.n = n;
+    L.nnz = cp[n];
+    L.colptr = std::move(cp);
+    L.rowval = std::move(rv);
+    csc_to_csr(m, n, L.colptr, L.rowval, L.csr_rowptr, L.csr_col, L.perm);
+    int32_t nlA = 0, nlAt = 0;
+    partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA);
+    partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt);
+    build_side(L.A, (int32_t)m, (int32_t)n, L.csr_rowptr, L.csr_col, &L.perm, L.row_of_int,
+               &L.int_of_col, nlA);
+    build_side(L.At, (int32_t)n, (int32_t)m, L.colptr, L.rowval, nullptr, L.col_of_int,
+               &L.int_of_row, nlAt);
+    return L;
+}
+
+// ELL (column-major rows x width, -1 padding) of an internal side, or width -1 if the
+// padding would exceed `max_fill` times nnz.
+inline int32_t build_ell(const HostSide& S, double max_fill, std::vector<int32_t>& ecol,
+                         std::vector<int32_t>& evmap) {
+    int32_t w = 0;
+    for (int32_t r = 0; r < S.rows; ++r) w = std::max(w, S.rowptr[r + 1] - S.rowptr[r]);
+    if (S.rows == 0 || (double)w * S.rows > max_fill * std::max<int64_t>(S.nnz, 1)) return -1;
+    ecol.assign((size_t)w * S.rows, -1);
+    evmap.assign((size_t)w * S.rows, -1);
+    for (int32_t r = 0; r < S.rows; ++r)
+        for (int32_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k) {
+            const size_t s = (size_t)(k - S.rowptr[r]) * S.rows + r;
+            ecol[s] = S.col[k];
+            evmap[s] = S.vmap[k];
+        }
+    return w;
+}
+
+}  // namespace cpdlp

@@ -1,0 +1,1232 @@
+// solver.hip -- libcoolpdlp_hip.so: handle, setup (scaling, step size), the PDLP/PDHG
+// loop and the C ABI declared in include/coolpdlp_hip.h.
+//
+// Data layout in HBM (internal order, see layout.hpp): problem vectors c, lv, uv, D2
+// over the n internal columns and lc, uc, D1 over the m internal rows, FP64 (plus
+// FP32 copies for the FP32/mixed loop); CSR of A and of A' with int32 indices.  The
+// iterate set of the reference's PDLPState (pdlp.jl:20-43) -- sol, sol_last,
+// sol_avg, sol_avg_last, sol_restart -- lives on the device for the whole solve; only
+// the check scalars (about 30 doubles) come back to the host once per check.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <mutex>
+
+#include "kernels.hpp"
+#include "layout.hpp"
+#include "solver.hpp"
+
+namespace cpdlp {
+
+static thread_local std::string g_last_error;
+void set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
+
+static double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ================================================================ setup kernels
+
+// norm over CSC column j (rows ascending): p = inf -> max|v|; p = 1 -> sequential
+// sum of |v|; p = 2 -> sqrt of sequential sum of squares; else (sum |v|^p)^(1/p).
+// d = iszero(norm) ? 1 : inv(sqrt(norm))   (preconditioning.jl:102-111)
+__device__ __forceinline__ double pn_acc(double acc, double v, double p) {
+    const double a = fabs(v);
+    if (isinf(p)) return a > acc ? a : acc;
+    if (p == 1.0) return acc + a;
+    if (p == 2.0) return acc + a * a;
+    return acc + pow(a, p);
+}
+__device__ __forceinline__ double pn_fin(double acc, double p) {
+    if (isinf(p) || p == 1.0) return acc;
+    if (p == 2.0) return sqrt(acc);
+    return pow(acc, 1.0 / p);
+}
+
+__global__ void k_col_norm(int32_t n, const int32_t* colptr, const double* vals, double p,
+                           double* d) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double acc = 0.0;
+    for (int k = colptr[j]; k < colptr[j + 1]; ++k) acc = pn_acc(acc, vals[k], p);
+    const double nv = pn_fin(acc, p);
+    d[j] = nv == 0.0 ? 1.0 : 1.0 / sqrt(nv);
+}
+
+// row norms of A from the A' values: row i's entries in CSR(A) order through perm
+__global__ void k_row_norm(int32_t m, const int32_t* rowptr, const int32_t* perm,
+                           const double* vals, double p, double* d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double acc = 0.0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) acc = pn_acc(acc, vals[perm[k]], p);
+    const double nv = pn_fin(acc, p);
+    d[i] = nv == 0.0 ? 1.0 : 1.0 / sqrt(nv);
+}
+
+// precondition(cons, prec), preconditioning.jl:37-43:
+// A <- (d1_i * a_ij) * d2_j ; A' <- (d2_j * a'_ji) * d1_i   (CSC(A) layout)
+__global__ void k_apply_prec(int32_t n, const int32_t* colptr, const int32_t* rowval,
+                             const double* a_in, const double* at_in, const double* d1,
+                             const double* d2, double* a_out, double* at_out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double dj = d2[j];
+    for (int k = colptr[j]; k < colptr[j + 1]; ++k) {
+        const double di = d1[rowval[k]];
+        const double a = a_in[k], at = at_in[k];
+        a_out[k] = (di * a) * dj;
+        at_out[k] = (dj * at) * di;
+    }
+}
+
+__global__ void k_mul_into(int32_t n, double* D, const double* d) {   // D .*= d
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) D[i] = d[i] * D[i];
+}
+__global__ void k_fill(int64_t n, double* v, double a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = a;
+}
+template <class T>
+__global__ void k_fill_t(int64_t n, T* v, T a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = a;
+}
+
+// values of an internal side: out[k] = src[vmap[k]] (vmap < 0: padding -> 0)
+template <class T>
+__global__ void k_gather_vals(int64_t nnz, const int32_t* vmap, const double* src, T* out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int32_t s = vmap[k];
+    out[k] = s >= 0 ? (T)src[s] : T(0);
+}
+
+// internal scaled column vectors: j = col_of_int[s]; c~ = D2*c, lv~ = lv/D2, uv~ = uv/D2
+// (precondition(milp, prec), preconditioning.jl:61-91)
+__global__ void k_scale_cols(int32_t n, const int32_t* col_of_int, const double* D2,
+                             const double* c, const double* lv, const double* uv, double* cs,
+                             double* lvs, double* uvs, double* D2s, double* invD2s) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int j = col_of_int[s];
+    const double d = D2[j];
+    cs[s] = d * c[j];
+    lvs[s] = lv[j] / d;
+    uvs[s] = uv[j] / d;
+    D2s[s] = d;
+    invD2s[s] = 1.0 / d;
+}
+__global__ void k_scale_rows(int32_t m, const int32_t* row_of_int, const double* D1,
+                             const double* lc, const double* uc, double* lcs, double* ucs,
+                             double* D1s, double* invD1s) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    const int i = row_of_int[r];
+    const double d = D1[i];
+    lcs[r] = d * lc[i];
+    ucs[r] = d * uc[i];
+    D1s[r] = d;
+    invD1s[r] = 1.0 / d;
+}
+
+// original -> internal and scaled: out[s] = v[of_int[s]] / D[of_int[s]]
+template <class T>
+__global__ void k_to_internal(int32_t n, const int32_t* of_int, const double* v, const double* D,
+                              T* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int j = of_int[s];
+    out[s] = (T)(v[j] / D[j]);
+}
+// internal -> original and unscaled: out[j] = D[j] * v[int_of[j]]  (unprecondition)
+template <class T>
+__global__ void k_from_internal(int32_t n, const int32_t* int_of, const double* D, const T* v,
+                                double* out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    out[j] = D[j] * (double)v[int_of[j]];
+}
+template <class S, class T>
+__global__ void k_convert(int64_t n, const S* in, T* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (T)in[i];
+}
+
+// constant scales of kkt_errors!: sum ((1/D1) combine(lc, uc))^2 and sum ((1/D2) c)^2
+__global__ void k_scale_sums(int32_t m, int32_t n, const double* lc, const double* uc,
+                             const double* invD1, const double* c, const double* invD2,
+                             double* partials) {
+    double q[4] = {0, 0, 0, 0};
+    for (int r = blockIdx.x * BLOCK + threadIdx.x; r < m; r += gridDim.x * BLOCK) {
+        const double v = invD1[r] * combine(lc[r], uc[r]);
+        q[0] += v * v;
+        const double w = combine(lc[r], uc[r]);
+        q[2] += w * w;
+    }
+    for (int s = blockIdx.x * BLOCK + threadIdx.x; s < n; s += gridDim.x * BLOCK) {
+        const double v = invD2[s] * c[s];
+        q[1] += v * v;
+        q[3] += c[s] * c[s];
+    }
+    block_sum<4>(q, partials + (size_t)blockIdx.x * 4);
+}
+
+// deterministic final reduction of partials[nb][nq] -> out[nq] (one block)
+__global__ __launch_bounds__(BLOCK) void k_reduce_partials(int32_t nb, int32_t nq,
+                                                           const double* partials, double* out,
+                                                           const PowState* st) {
+    if (st && st->done) return;
+    __shared__ double red[BLOCK / WAVE];
+    for (int q = 0; q < nq; ++q) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += BLOCK) s += partials[(size_t)b * nq + q];
+        s = wave_sum(s);
+        if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < BLOCK / WAVE; ++w) t += red[w];
+            out[q] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- power method kernels
+// start vector: sum of 12 counter-hashed uniforms - 6 (exact adds; same bits as the
+// oracle's orc_powm_start on any IEEE device)
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+template <class V>
+__global__ void k_powm_start(int32_t n, const int32_t* col_of_int, uint64_t seed, V* x) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint64_t i = (uint64_t)col_of_int[s];
+    double acc = 0.0;
+    for (int r = 0; r < 12; ++r) {
+        const uint64_t h = splitmix64(seed * 0x100000001B3ull ^ splitmix64(i * 16u + (uint64_t)r));
+        acc += (double)(h >> 11) * 0x1.0p-53;
+    }
+    x[s] = (V)(acc - 6.0);
+}
+template <class V>
+__global__ void k_sumsq(int32_t n, const V* x, double* partials) {
+    double q[1] = {0.0};
+    for (int s = blockIdx.x * BLOCK + threadIdx.x; s < n; s += gridDim.x * BLOCK)
+        q[0] += (double)(x[s] * x[s]);
+    block_sum<1>(q, partials + blockIdx.x);
+}
+template <class V>
+__global__ void k_scale_by_inv_sqrt(int32_t n, V* x, const double* sumsq) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const V inv = (V)(1.0 / sqrt(sumsq[0]));
+    x[s] = x[s] * inv;
+}
+// theta and |Ax| from the sums (one thread)
+__global__ void k_pow_fin1(PowState* st, const double* sums) {
+    if (st->done) return;
+    st->theta = sums[0];
+    st->nrm = sqrt(sums[1]);
+}
+// residual sum of (Ax - theta x)^2 and x <- Ax * inv(|Ax|)
+template <class V>
+__global__ void k_pow_update(int32_t n, V* x, const V* Ax, const PowState* st, double* partials) {
+    if (st->done) return;
+    double q[1] = {0.0};
+    const V th = (V)st->theta;
+    const V inv = (V)(1.0 / st->nrm);
+    for (int s = blockIdx.x * BLOCK + threadIdx.x; s < n; s += gridDim.x * BLOCK) {
+        const V a = Ax[s];
+        const V r = a + (-th) * x[s];
+        q[0] += (double)(r * r);
+        x[s] = a * inv;
+    }
+    block_sum<1>(q, partials + blockIdx.x);
+}
+__global__ void k_pow_fin2(PowState* st, const double* sums) {
+    if (st->done) return;
+    st->residual = sqrt(sums[0]);
+    st->iter += 1;
+    st->done = (st->iter > st->maxiter) || (st->residual <= st->tol * fabs(st->theta));
+}
+
+// ---------------------------------------------------------------- restart!
+// pdlp.jl:189-192: sol <- candidate; avg <- 0; restart <- sol
+template <class V>
+__global__ void k_restart(int64_t len, int32_t from_avg, V* sol, V* avg, V* rst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    V s = sol[i];
+    if (from_avg) s = avg[i];
+    sol[i] = s;
+    avg[i] = V(0);
+    rst[i] = s;
+}
+
+// ================================================================ device side of a matrix
+
+template <class V>
+MatView<V> DevSide::view() const {
+    MatView<V> v;
+    v.rows = rows;
+    v.rowptr = rowptr.p;
+    v.col = col.p;
+    v.val = vals<V>();
+    v.n_long = n_long;
+    v.n_long_blocks = ceil_div(n_long, LONG_ROWS_PER_BLOCK);
+    v.tile_ptr = tile_ptr.p;
+    v.n_tiles = n_tiles;
+    v.format = format;
+    v.ell_width = ell_w;
+    v.ell_col = ell_col.p;
+    v.ell_val = ell_vals<V>();
+    return v;
+}
+template <>
+const double* DevSide::vals<double>() const { return val64.p; }
+template <>
+const float* DevSide::vals<float>() const { return val32.p; }
+template <>
+const double* DevSide::ell_vals<double>() const { return ell_val64.p; }
+template <>
+const float* DevSide::ell_vals<float>() const { return ell_val32.p; }
+
+void DevSide::upload_structure(const HostSide& S, hipStream_t s) {
+    rows = S.rows;
+    cols = S.cols;
+    nnz = S.nnz;
+    n_long = S.n_long;
+    n_tiles = (int32_t)S.tile_ptr.size() - 1;
+    rowptr.upload(S.rowptr, s);
+    col.alloc(std::max<int64_t>(nnz, 1));
+    col.upload(S.col.data(), S.col.size(), s);
+    vmap.alloc(std::max<int64_t>(nnz, 1));
+    vmap.upload(S.vmap.data(), S.vmap.size(), s);
+    tile_ptr.upload(S.tile_ptr, s);
+}
+
+void DevSide::fill_values(const double* src_csc, bool want32, hipStream_t s) {
+    if (val64.n < (size_t)std::max<int64_t>(nnz, 1)) val64.alloc(std::max<int64_t>(nnz, 1));
+    if (nnz)
+        hipLaunchKernelGGL(k_gather_vals<double>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
+                           nnz, vmap.p, src_csc, val64.p);
+    if (want32) {
+        if (val32.n < (size_t)std::max<int64_t>(nnz, 1)) val32.alloc(std::max<int64_t>(nnz, 1));
+        if (nnz)
+            hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
+                               nnz, vmap.p, src_csc, val32.p);
+    }
+    if (ell_w > 0) {
+        const int64_t len = (int64_t)ell_w * rows;
+        if (ell_val64.n < (size_t)len) ell_val64.alloc(len);
+        hipLaunchKernelGGL(k_gather_vals<double>, dim3(ceil_div(len, BLOCK)), dim3(BLOCK), 0, s,
+                           len, ell_vmap.p, src_csc, ell_val64.p);
+        if (want32) {
+            if (ell_val32.n < (size_t)len) ell_val32.alloc(len);
+            hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(len, BLOCK)), dim3(BLOCK), 0, s,
+                               len, ell_vmap.p, src_csc, ell_val32.p);
+        }
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void DevSide::build_ell(const HostSide& S, hipStream_t s) {
+    std::vector<int32_t> ecol, evmap;
+    const int32_t w = cpdlp::build_ell(S, 1.5, ecol, evmap);
+    if (w <= 0) {
+        ell_w = 0;
+        return;
+    }
+    ell_w = w;
+    ell_col.upload(ecol, s);
+    ell_vmap.upload(evmap, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template MatView<double> DevSide::view<double>() const;
+template MatView<float> DevSide::view<float>() const;
+
+// ================================================================ iterate state
+
+template <class V>
+void IterState<V>::alloc(int64_t n, int64_t m) {
+    const int64_t nn = std::max<int64_t>(n, 1), mm = std::max<int64_t>(m, 1);
+    for (int b = 0; b < 2; ++b) {
+        x[b].alloc(nn);
+        y[b].alloc(mm);
+    }
+    xbar.alloc(nn);
+    xavg.alloc(nn);
+    yavg.alloc(mm);
+    xavgl.alloc(nn);
+    yavgl.alloc(mm);
+    xrst.alloc(nn);
+    yrst.alloc(mm);
+    tmp_n.alloc(nn);
+    tmp_m.alloc(mm);
+    cur = 0;
+}
+template struct IterState<double>;
+template struct IterState<float>;
+
+// ================================================================ handle
+
+static int64_t wrap_cube(int64_t n) {
+    const uint64_t u = (uint64_t)n;
+    return (int64_t)(u * u * u);
+}
+
+Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
+    if (P.check_every <= 0) throw Error(CPDLP_ERR_ARGUMENT, "check_every must be positive");
+    if (P.precision < 0 || P.precision > 2) throw Error(CPDLP_ERR_ARGUMENT, "unknown precision");
+    if (P.algorithm != CPDLP_ALGO_PDLP && P.algorithm != CPDLP_ALGO_PDHG)
+        throw Error(CPDLP_ERR_ARGUMENT, "unknown algorithm");
+    if (P.long_row <= 0) P.long_row = 64;
+    if (P.long_row > TILE_NNZ) P.long_row = TILE_NNZ;
+    m = lp.m;
+    n = lp.n;
+    for (const void* v : {(const void*)lp.c, (const void*)lp.lv, (const void*)lp.uv})
+        if (n > 0 && !v) throw Error(CPDLP_ERR_ARGUMENT, "NULL variable vector");
+    for (const void* v : {(const void*)lp.lc, (const void*)lp.uc})
+        if (m > 0 && !v) throw Error(CPDLP_ERR_ARGUMENT, "NULL constraint vector");
+    std::vector<int32_t> cp, rv;
+    read_csc(m, n, lp.index_base, lp.index_bits, lp.colptr, lp.rowval, cp, rv);
+    nnz = cp[n];
+    if (nnz > 0 && !lp.nzval) throw Error(CPDLP_ERR_ARGUMENT, "nzval is NULL");
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (P.device < 0 || P.device >= ndev) throw Error(CPDLP_ERR_ARGUMENT, "invalid device ordinal");
+    device = P.device;
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreate(&ev0));
+    HIP_CHECK(hipEventCreate(&ev1));
+    L = build_layout(m, n, std::move(cp), std::move(rv), P.long_row);
+    const hipStream_t s = stream;
+    auto up64 = [&](DevBuf<double>& d, const double* h, int64_t len) {
+        d.alloc(std::max<int64_t>(len, 1));
+        d.upload(h, len, s);
+    };
+    up64(d_nzval, lp.nzval, nnz);
+    up64(d_c, lp.c, n);
+    up64(d_lv, lp.lv, n);
+    up64(d_uv, lp.uv, n);
+    up64(d_lc, lp.lc, m);
+    up64(d_uc, lp.uc, m);
+    d_colptr.upload(L.colptr, s);
+    d_rowval.alloc(std::max<int64_t>(nnz, 1));
+    d_rowval.upload(L.rowval.data(), nnz, s);
+    d_csr_rowptr.upload(L.csr_rowptr, s);
+    d_perm.alloc(std::max<int64_t>(nnz, 1));
+    d_perm.upload(L.perm.data(), nnz, s);
+    auto upi = [&](DevBuf<int32_t>& d, const std::vector<int32_t>& h) {
+        d.alloc(std::max<size_t>(h.size(), 1));
+        d.upload(h.data(), h.size(), s);
+    };
+    upi(d_row_of_int, L.row_of_int);
+    upi(d_col_of_int, L.col_of_int);
+    upi(d_int_of_row, L.int_of_row);
+    upi(d_int_of_col, L.int_of_col);
+    A.upload_structure(L.A, s);
+    At.upload_structure(L.At, s);
+    if (P.format == CPDLP_FMT_ELL || P.format == CPDLP_FMT_AUTO) {
+        A.build_ell(L.A, s);
+        At.build_ell(L.At, s);
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    // free the big host-side copies that are no longer needed
+    std::vector<int32_t>().swap(L.A.col);
+    std::vector<int32_t>().swap(L.A.vmap);
+    std::vector<int32_t>().swap(L.At.col);
+    std::vector<int32_t>().swap(L.At.vmap);
+    std::vector<int32_t>().swap(L.perm);
+}
+
+Handle::~Handle() {
+    (void)hipSetDevice(device);
+    for (auto& g : graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Handle::grid_n() const { return std::max(1, ceil_div(n, BLOCK)); }
+int Handle::grid_m() const { return std::max(1, ceil_div(m, BLOCK)); }
+
+void Handle::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+
+// reduce nb blocks of nq partials on the device and copy the nq sums to the host
+void Handle::reduce_to_host(int nb, int nq, const double* partials, double* host_out) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, stream, nb, nq, partials,
+                       d_sums.p, (const PowState*)nullptr);
+    d_sums.download(host_out, nq, stream);
+    sync();
+}
+
+void Handle::ensure_partials(size_t count) {
+    if (d_partials.n >= count && d_partials2.n >= count) return;
+    sync();   // no kernel may still use the old buffers
+    if (d_partials.n < count) d_partials.alloc(count);
+    if (d_partials2.n < count) d_partials2.alloc(count);
+}
+
+// ---------------------------------------------------------------- preprocess (device)
+// pdlp_preconditioner (preconditioning.jl:146-155) then precondition(milp, prec)
+void Handle::scale() {
+    const hipStream_t s = stream;
+    const int32_t mm = (int32_t)m, nn = (int32_t)n;
+    const int64_t nz1 = std::max<int64_t>(nnz, 1);
+    d_D1o.alloc(std::max<int64_t>(m, 1));
+    d_D2o.alloc(std::max<int64_t>(n, 1));
+    d_a_csc.alloc(nz1);
+    d_at_csc.alloc(nz1);
+    DevBuf<double> d1(std::max<int64_t>(m, 1)), d2(std::max<int64_t>(n, 1));
+    const int gm = grid_m(), gn = grid_n();
+    hipLaunchKernelGGL(k_fill, dim3(gm), dim3(BLOCK), 0, s, (int64_t)m, d_D1o.p, 1.0);
+    hipLaunchKernelGGL(k_fill, dim3(gn), dim3(BLOCK), 0, s, (int64_t)n, d_D2o.p, 1.0);
+    if (nnz) {
+        HIP_CHECK(hipMemcpyAsync(d_a_csc.p, d_nzval.p, nnz * 8, hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(d_at_csc.p, d_nzval.p, nnz * 8, hipMemcpyDeviceToDevice, s));
+    }
+    for (int it = 0; it < P.ruiz_iter; ++it) {   // ruiz_preconditioner, :117-125
+        hipLaunchKernelGGL(k_col_norm, dim3(gn), dim3(BLOCK), 0, s, nn, d_colptr.p, d_a_csc.p,
+                           (double)INFINITY, d2.p);
+        hipLaunchKernelGGL(k_row_norm, dim3(gm), dim3(BLOCK), 0, s, mm, d_csr_rowptr.p, d_perm.p,
+                           d_at_csc.p, (double)INFINITY, d1.p);
+        hipLaunchKernelGGL(k_apply_prec, dim3(gn), dim3(BLOCK), 0, s, nn, d_colptr.p, d_rowval.p,
+                           d_a_csc.p, d_at_csc.p, d1.p, d2.p, d_a_csc.p, d_at_csc.p);
+        hipLaunchKernelGGL(k_mul_into, dim3(gm), dim3(BLOCK), 0, s, mm, d_D1o.p, d1.p);
+        hipLaunchKernelGGL(k_mul_into, dim3(gn), dim3(BLOCK), 0, s, nn, d_D2o.p, d2.p);
+    }
+    // cons_r = precondition(cons, prec_r) from the original values; CP(alpha)
+    const double alpha = P.chambolle_pock_alpha;
+    hipLaunchKernelGGL(k_apply_prec, dim3(gn), dim3(BLOCK), 0, s, nn, d_colptr.p, d_rowval.p,
+                       d_nzval.p, d_nzval.p, d_D1o.p, d_D2o.p, d_a_csc.p, d_at_csc.p);
+    hipLaunchKernelGGL(k_col_norm, dim3(gn), dim3(BLOCK), 0, s, nn, d_colptr.p, d_a_csc.p, alpha,
+                       d2.p);
+    hipLaunchKernelGGL(k_row_norm, dim3(gm), dim3(BLOCK), 0, s, mm, d_csr_rowptr.p, d_perm.p,
+                       d_at_csc.p, 2.0 - alpha, d1.p);
+    hipLaunchKernelGGL(k_mul_into, dim3(gm), dim3(BLOCK), 0, s, mm, d_D1o.p, d1.p);
+    hipLaunchKernelGGL(k_mul_into, dim3(gn), dim3(BLOCK), 0, s, nn, d_D2o.p, d2.p);
+    // A_p = D1*A*D2, At_p = D2*At*D1 from the originals
+    hipLaunchKernelGGL(k_apply_prec, dim3(gn), dim3(BLOCK), 0, s, nn, d_colptr.p, d_rowval.p,
+                       d_nzval.p, d_nzval.p, d_D1o.p, d_D2o.p, d_a_csc.p, d_at_csc.p);
+    HIP_CHECK(hipGetLastError());
+    const bool want32 = P.precision != CPDLP_PREC_FP64;
+    A.fill_values(d_a_csc.p, want32, s);
+    At.fill_values(d_at_csc.p, want32, s);
+    // scaled vectors in internal order
+    const int64_t n1 = std::max<int64_t>(n, 1), m1 = std::max<int64_t>(m, 1);
+    c_s.alloc(n1); lv_s.alloc(n1); uv_s.alloc(n1); D2_s.alloc(n1); invD2_s.alloc(n1);
+    lc_s.alloc(m1); uc_s.alloc(m1); D1_s.alloc(m1); invD1_s.alloc(m1);
+    hipLaunchKernelGGL(k_scale_cols, dim3(gn), dim3(BLOCK), 0, s, nn, d_col_of_int.p, d_D2o.p,
+                       d_c.p, d_lv.p, d_uv.p, c_s.p, lv_s.p, uv_s.p, D2_s.p, invD2_s.p);
+    hipLaunchKernelGGL(k_scale_rows, dim3(gm), dim3(BLOCK), 0, s, mm, d_row_of_int.p, d_D1o.p,
+                       d_lc.p, d_uc.p, lc_s.p, uc_s.p, D1_s.p, invD1_s.p);
+    if (want32) {
+        c_32.alloc(n1); lv_32.alloc(n1); uv_32.alloc(n1); lc_32.alloc(m1); uc_32.alloc(m1);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gn), dim3(BLOCK), 0, s, (int64_t)n, c_s.p, c_32.p);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gn), dim3(BLOCK), 0, s, (int64_t)n, lv_s.p, lv_32.p);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gn), dim3(BLOCK), 0, s, (int64_t)n, uv_s.p, uv_32.p);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gm), dim3(BLOCK), 0, s, (int64_t)m, lc_s.p, lc_32.p);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gm), dim3(BLOCK), 0, s, (int64_t)m, uc_s.p, uc_32.p);
+    }
+    HIP_CHECK(hipGetLastError());
+    // constant scales of the KKT errors + primal_weight_init norms
+    const int nb = 64;
+    ensure_partials((size_t)nb * 4);
+    hipLaunchKernelGGL(k_scale_sums, dim3(nb), dim3(BLOCK), 0, s, mm, nn, lc_s.p, uc_s.p,
+                       invD1_s.p, c_s.p, invD2_s.p, d_partials.p);
+    double sums[4];
+    reduce_to_host(nb, 4, d_partials.p, sums);
+    primal_scale = 1.0 + std::sqrt(sums[0]);
+    dual_scale = 1.0 + std::sqrt(sums[1]);
+    comb_norm = std::sqrt(sums[2]);
+    c_norm = std::sqrt(sums[3]);
+}
+
+// ---------------------------------------------------------------- step size (device)
+// fixed_stepsize / spectral_norm via the powm! rule (step_size.jl:22-27, linalg.jl:86-96)
+template <class V>
+void Handle::power_method(double tol, int64_t maxiter) {
+    const hipStream_t s = stream;
+    const int32_t nn = (int32_t)n;
+    DevBuf<V> x(std::max<int64_t>(n, 1)), Ax(std::max<int64_t>(n, 1)), t(std::max<int64_t>(m, 1));
+    DevBuf<PowState> st(1);
+    PowState h0{};
+    h0.theta = 0.0;
+    h0.residual = 1.7976931348623157e308;
+    h0.tol = tol;
+    h0.iter = 0;
+    h0.maxiter = maxiter;
+    h0.done = (0 > maxiter) || (h0.residual <= tol * 0.0);
+    st.upload(&h0, 1, s);
+    const int gn = grid_n();
+    const int nbr = std::min(gn, 1024);
+    const int gA = A.view<V>().grid(), gAt = At.view<V>().grid();
+    ensure_partials((size_t)std::max({gA, gAt, nbr}) * 2 + 16);
+    hipLaunchKernelGGL(k_powm_start<V>, dim3(gn), dim3(BLOCK), 0, s, nn, d_col_of_int.p,
+                       P.powm_seed, x.p);
+    hipLaunchKernelGGL(k_sumsq<V>, dim3(nbr), dim3(BLOCK), 0, s, nn, (const V*)x.p, d_partials.p);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, nbr, 1, d_partials.p,
+                       d_sums.p, (const PowState*)nullptr);
+    hipLaunchKernelGGL(k_scale_by_inv_sqrt<V>, dim3(gn), dim3(BLOCK), 0, s, nn, x.p,
+                       (const double*)d_sums.p);
+    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
+    PowState hs{};
+    for (;;) {
+        for (int c = 0; c < 32; ++c) {
+            EpiStore<V> e1;
+            e1.out = t.p;
+            e1.st = st.p;
+            launch_spmv<V, V, 1>(vA, XPack<V, 1>{{x.p}}, e1, s);
+            EpiPowAt<V> e2;
+            e2.partials = d_partials.p;
+            e2.Ax = Ax.p;
+            e2.x = x.p;
+            e2.st = st.p;
+            launch_spmv<V, V, 1>(vAt, XPack<V, 1>{{t.p}}, e2, s);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gAt, 2, d_partials.p,
+                               d_sums.p, (const PowState*)st.p);
+            hipLaunchKernelGGL(k_pow_fin1, dim3(1), dim3(1), 0, s, st.p, (const double*)d_sums.p);
+            hipLaunchKernelGGL(k_pow_update<V>, dim3(nbr), dim3(BLOCK), 0, s, nn, x.p,
+                               (const V*)Ax.p, (const PowState*)st.p, d_partials2.p);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, nbr, 1,
+                               d_partials2.p, d_sums.p + 8, (const PowState*)st.p);
+            hipLaunchKernelGGL(k_pow_fin2, dim3(1), dim3(1), 0, s, st.p, (const double*)d_sums.p + 8);
+        }
+        HIP_CHECK(hipGetLastError());
+        st.download(&hs, 1, s);
+        sync();
+        if (hs.done) break;
+    }
+    powm_iters = hs.iter;
+    eta = P.invnorm_scaling * (1.0 / std::sqrt(hs.theta));
+}
+
+// Format autotuner: for each side with a feasible ELL (padding <= 1.5x nnz), time the
+// plain product in both formats on the actual matrix and keep the faster.
+void Handle::choose_formats() {
+    for (DevSide* S : {&A, &At}) {
+        S->format = CPDLP_FMT_CSR;
+        if (P.format == CPDLP_FMT_CSR || S->ell_w <= 0) continue;
+        if (P.format == CPDLP_FMT_ELL) {
+            S->format = CPDLP_FMT_ELL;
+            continue;
+        }
+        DevBuf<double> xin(std::max<int64_t>(S->cols, 1)), out(std::max<int64_t>(S->rows, 1));
+        hipLaunchKernelGGL(k_fill, dim3(std::max(1, ceil_div(S->cols, BLOCK))), dim3(BLOCK), 0,
+                           stream, (int64_t)S->cols, xin.p, 1.0);
+        float best[2] = {0.f, 0.f};
+        for (int f = 0; f < 2; ++f) {
+            S->format = f == 0 ? CPDLP_FMT_CSR : CPDLP_FMT_ELL;
+            const MatView<double> v = S->view<double>();
+            EpiStore<double> e;
+            e.out = out.p;
+            e.st = nullptr;
+            for (int rep = 0; rep < 3; ++rep)   // warm-up
+                launch_spmv<double, double, 1>(v, XPack<double, 1>{{xin.p}}, e, stream);
+            HIP_CHECK(hipEventRecord(ev0, stream));
+            for (int rep = 0; rep < 10; ++rep)
+                launch_spmv<double, double, 1>(v, XPack<double, 1>{{xin.p}}, e, stream);
+            HIP_CHECK(hipEventRecord(ev1, stream));
+            HIP_CHECK(hipEventSynchronize(ev1));
+            HIP_CHECK(hipEventElapsedTime(&best[f], ev0, ev1));
+        }
+        S->format = best[1] < best[0] ? CPDLP_FMT_ELL : CPDLP_FMT_CSR;
+    }
+}
+
+void Handle::prepare() {
+    HIP_CHECK(hipSetDevice(device));
+    const double t0 = now_s();
+    if (d_sums.n == 0) d_sums.alloc(64);
+    scale();
+    choose_formats();
+    // power method in T: Float32 semantics only for the all-FP32 precision
+    const bool t32 = P.precision == CPDLP_PREC_FP32;
+    const double eps = t32 ? (double)FLT_EPSILON : DBL_EPSILON;
+    const double tol = std::isnan(P.powm_tol) ? eps * (double)wrap_cube(n) : P.powm_tol;
+    const int64_t maxiter = P.powm_maxiter < 0 ? n : P.powm_maxiter;
+    if (t32)
+        power_method<float>(tol, maxiter);
+    else
+        power_method<double>(tol, maxiter);
+    // primal_weight_init (step_size.jl:29-40); PDHG: omega = 1 (pdhg.jl:48)
+    omega0 = 1.0;
+    if (P.algorithm == CPDLP_ALGO_PDLP && c_norm > P.zero_tol && comb_norm > P.zero_tol)
+        omega0 = c_norm / comb_norm;
+    prepared = true;
+    setup_seconds = now_s() - t0;
+}
+
+// ---------------------------------------------------------------- KKT evaluation
+template <class V, int K>
+void Handle::kkt_launch(const V* const* X, const V* const* Y, const V* xr, const V* yr, bool dist,
+                        double* host_sums) {
+    const hipStream_t s = stream;
+    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
+    const int gA = vA.grid(), gAt = vAt.grid();
+    constexpr int NQR = EpiKktRows<V, K>::NQ, NQC = EpiKktCols<V, K>::NQ;
+    ensure_partials((size_t)std::max(gA * NQR, gAt * NQC) + 16);
+    EpiKktRows<V, K> er;
+    er.partials = d_partials.p;
+    for (int k = 0; k < K; ++k) er.Y[k] = Y[k];
+    er.y_rst = yr ? yr : Y[0];
+    er.lc = lc_s.p;
+    er.uc = uc_s.p;
+    er.invD1 = invD1_s.p;
+    er.dist = dist;
+    XPack<V, K> xp;
+    for (int k = 0; k < K; ++k) xp.p[k] = X[k];
+    launch_spmv<V, double, K>(vA, xp, er, s);
+    EpiKktCols<V, K> ec;
+    ec.partials = d_partials2.p;
+    for (int k = 0; k < K; ++k) ec.X[k] = X[k];
+    ec.x_rst = xr ? xr : X[0];
+    ec.c = c_s.p;
+    ec.lv = lv_s.p;
+    ec.uv = uv_s.p;
+    ec.invD2 = invD2_s.p;
+    ec.dist = dist;
+    XPack<V, K> yp;
+    for (int k = 0; k < K; ++k) yp.p[k] = Y[k];
+    launch_spmv<V, double, K>(vAt, yp, ec, s);
+    if (gA > 0)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gA, NQR, d_partials.p,
+                           d_sums.p, (const PowState*)nullptr);
+    else
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQR, d_sums.p, 0.0);
+    if (gAt > 0)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gAt, NQC, d_partials2.p,
+                           d_sums.p + NQR, (const PowState*)nullptr);
+    else
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQC, d_sums.p + NQR, 0.0);
+    HIP_CHECK(hipGetLastError());
+    d_sums.download(host_sums, NQR + NQC, s);
+    sync();
+}
+
+// errors.jl:77-98 from the raw sums of candidate k
+cpdlp_kkt Handle::kkt_from_sums(const double* sums, int K, int k) const {
+    const double* r = sums;
+    const double* c = sums + 2 * K + 2;
+    cpdlp_kkt e;
+    e.primal = std::sqrt(r[2 * k]);
+    e.primal_scale = primal_scale;
+    e.dual = std::sqrt(c[3 * k]);
+    e.dual_scale = dual_scale;
+    const double pc = r[2 * k + 1], pv = c[3 * k + 1], cx = c[3 * k + 2];
+    e.gap = std::fabs(cx + pc + pv);
+    e.gap_scale = 1.0 + std::fabs(pc + pv) + std::fabs(cx);
+    return e;
+}
+
+static double relative(const cpdlp_kkt& e) {   // errors.jl:55-58
+    return std::max(std::max(e.primal / e.primal_scale, e.dual / e.dual_scale), e.gap / e.gap_scale);
+}
+static double absolute(const cpdlp_kkt& e, double w) {   // errors.jl:60-63
+    return std::sqrt(w * w * e.primal * e.primal + (1.0 / (w * w)) * e.dual * e.dual + e.gap * e.gap);
+}
+
+// ---------------------------------------------------------------- the loop
+template <class V>
+void Handle::launch_steps(IterState<V>& S, int nsteps, bool pdlp) {
+    const hipStream_t s = stream;
+    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
+    const bool f64 = std::is_same<V, double>::value;
+    const V* c = f64 ? (const V*)c_s.p : (const V*)c_32.p;
+    const V* lv = f64 ? (const V*)lv_s.p : (const V*)lv_32.p;
+    const V* uv = f64 ? (const V*)uv_s.p : (const V*)uv_32.p;
+    const V* lc = f64 ? (const V*)lc_s.p : (const V*)lc_32.p;
+    const V* uc = f64 ? (const V*)uc_s.p : (const V*)uc_32.p;
+    for (int k = 0; k < nsteps; ++k) {
+        const int p = S.cur, q = 1 - S.cur;   // read sol_last = buffers p, write sol = q
+        EpiPrimal<V> ep;
+        ep.x = S.x[p].p;
+        ep.c = c;
+        ep.lv = lv;
+        ep.uv = uv;
+        ep.xn = S.x[q].p;
+        ep.xbar = S.xbar.p;
+        ep.xavg = S.xavg.p;
+        ep.xavgl = S.xavgl.p;
+        ep.coef = d_coef.p;
+        ep.step = k;
+        ep.pdlp = pdlp;
+        launch_spmv<V, V, 1>(vAt, XPack<V, 1>{{S.y[p].p}}, ep, s);
+        EpiDual<V> ed;
+        ed.y = S.y[p].p;
+        ed.lc = lc;
+        ed.uc = uc;
+        ed.yn = S.y[q].p;
+        ed.yavg = S.yavg.p;
+        ed.yavgl = S.yavgl.p;
+        ed.coef = d_coef.p;
+        ed.step = k;
+        ed.pdlp = pdlp;
+        launch_spmv<V, V, 1>(vA, XPack<V, 1>{{S.xbar.p}}, ed, s);
+        S.cur = q;
+    }
+}
+
+template <class V>
+void Handle::run_chunk(IterState<V>& S, bool pdlp) {
+    const int ce = P.check_every;
+    const bool use_graph = P.use_graph != 0;
+    if (!use_graph) {
+        launch_steps(S, ce, pdlp);
+        return;
+    }
+    const int parity = S.cur;
+    const int gid = (std::is_same<V, double>::value ? 0 : 2) + parity;
+    GraphCache& g = graphs[gid];
+    if (!g.exec) {   // capture once per (precision, start parity); kernels only
+        hipGraph_t graph;
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        launch_steps(S, ce, pdlp);
+        HIP_CHECK(hipStreamEndCapture(stream, &graph));
+        HIP_CHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphDestroy(graph));
+        g.end_parity = S.cur;
+        S.cur = parity;
+    }
+    HIP_CHECK(hipGraphLaunch(g.exec, stream));
+    S.cur = g.end_parity;
+}
+
+// one PDLP/PDHG solve phase in precision V (pdlp.jl:69-94, pdhg.jl:252-270).
+// Returns when the phase terminates; phase_reltol replaces termination_reltol (mixed
+// precision's FP32 phase), final = false marks a non-final phase (no duplicate entry).
+template <class V>
+void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase) {
+    const bool pdlp = P.algorithm == CPDLP_ALGO_PDLP;
+    const int ce = P.check_every;
+    std::vector<StepCoef> coefs(ce);
+    if (d_coef.n < (size_t)ce) {
+        sync();
+        d_coef.alloc(ce);
+    }
+    double sums[64];
+    auto term_status = [&](const cpdlp_kkt& e) -> int32_t {   // termination.jl:89-101
+        if (relative(e) <= phase_reltol) return CPDLP_OPTIMAL;
+        if (stats.time_elapsed >= P.time_limit) return CPDLP_TIME_LIMIT;
+        if (stats.kkt_passes >= P.max_kkt_passes) return CPDLP_ITERATION_LIMIT;
+        return CPDLP_STILL_RUNNING;
+    };
+    auto push_history = [&]() {
+        if (P.record_error_history) history.emplace_back(stats.kkt_passes, stats.err);
+    };
+    for (;;) {
+        // step coefficients of this chunk: tau, sigma fixed; eta_sum advances per step
+        const double tau = eta / omega, sigma = eta * omega, isig = 1.0 / sigma;
+        for (int k = 0; k < ce; ++k) {
+            StepCoef& cf = coefs[k];
+            cf.tau = tau;
+            cf.sigma = sigma;
+            cf.isig = isig;
+            cf.wa = eta / (eta + eta_sum);
+            cf.wb = eta_sum / (eta + eta_sum);
+            cf.last = (k == ce - 1);
+            cf.pad = 0;
+            if (pdlp) eta_sum += eta;
+        }
+        d_coef.upload(coefs.data(), ce, stream);
+        HIP_CHECK(hipEventRecord(ev0, stream));
+        run_chunk(S, pdlp);
+        HIP_CHECK(hipEventRecord(ev1, stream));
+        HIP_CHECK(hipGetLastError());
+        stats.kkt_passes += ce;
+        if (pdlp) {
+            inner += ce;
+            total += ce;
+        }
+        step_launches += 2 * ce;
+        // check: KKT of sol (and, for PDLP, avg, sol_last, avg_last, restart)
+        const int c = S.cur, l = 1 - S.cur;
+        int K = 1;
+        if (pdlp) {
+            const V* X[5] = {S.x[c].p, S.xavg.p, S.x[l].p, S.xavgl.p, S.xrst.p};
+            const V* Y[5] = {S.y[c].p, S.yavg.p, S.y[l].p, S.yavgl.p, S.yrst.p};
+            kkt_launch<V, 5>(X, Y, S.xrst.p, S.yrst.p, true, sums);
+            K = 5;
+        } else {
+            const V* X[1] = {S.x[c].p};
+            const V* Y[1] = {S.y[c].p};
+            kkt_launch<V, 1>(X, Y, nullptr, nullptr, false, sums);
+        }
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        step_ms += ms;
+        // termination_check!, common.jl:218-231
+        stats.time_elapsed = now_s() - t_start;
+        stats.err = kkt_from_sums(sums, K, 0);
+        push_history();
+        stats.termination_status = term_status(stats.err);
+        if (stats.termination_status != CPDLP_STILL_RUNNING) {
+            if (final_phase && pdlp) {   // second check after the inner loop, pdlp.jl:88-90
+                stats.time_elapsed = now_s() - t_start;
+                push_history();
+                stats.termination_status = term_status(stats.err);
+            }
+            return;
+        }
+        if (!pdlp) continue;
+        // restart_check!, pdlp.jl:138-170 and should_restart, restart.jl:40-57
+        const cpdlp_kkt e_s = stats.err, e_a = kkt_from_sums(sums, K, 1);
+        const cpdlp_kkt e_l = kkt_from_sums(sums, K, 2), e_al = kkt_from_sums(sums, K, 3);
+        const cpdlp_kkt e_r = kkt_from_sums(sums, K, 4);
+        const bool from_avg = !(absolute(e_s, omega) < absolute(e_a, omega));
+        const cpdlp_kkt cand = from_avg ? e_a : e_s;
+        const cpdlp_kkt cand_last = absolute(e_l, omega) < absolute(e_al, omega) ? e_l : e_al;
+        const double ac = absolute(cand, omega), ar = absolute(e_r, omega);
+        const double acl = absolute(cand_last, omega);
+        const bool sufficient = ac <= P.sufficient_decay * ar;
+        const bool necessary = ac <= P.necessary_decay * ar;
+        const bool no_local = ac > acl;
+        const bool long_inner = (double)inner >= P.artificial_decay * (double)total;
+        if (sufficient || (necessary && no_local) || long_inner) {
+            // restart!, pdlp.jl:172-196; primal_weight_update!, step_size.jl:58-75
+            eta_sum = 0.0;
+            const double* rs = sums;
+            const double* cs = sums + 2 * K + 2;
+            const int di = from_avg ? 1 : 0;
+            const double dx = std::sqrt(cs[3 * K + di]), dy = std::sqrt(rs[2 * K + di]);
+            const double th = P.primal_weight_damping;
+            if (dx > P.zero_tol && dy > P.zero_tol)
+                omega = std::exp(th * std::log(dy / dx) + (1.0 - th) * std::log(omega));
+            hipLaunchKernelGGL(k_restart<V>, dim3(grid_n()), dim3(BLOCK), 0, stream, (int64_t)n,
+                               (int32_t)from_avg, S.x[c].p, S.xavg.p, S.xrst.p);
+            hipLaunchKernelGGL(k_restart<V>, dim3(grid_m()), dim3(BLOCK), 0, stream, (int64_t)m,
+                               (int32_t)from_avg, S.y[c].p, S.yavg.p, S.yrst.p);
+            HIP_CHECK(hipGetLastError());
+            inner = 0;
+            restarts += 1;
+        }
+    }
+}
+
+template <class V>
+void Handle::init_state(IterState<V>& S, const double* x0_dev, const double* y0_dev) {
+    const hipStream_t s = stream;
+    S.alloc(n, m);
+    const int gn = grid_n(), gm = grid_m();
+    const int32_t nn = (int32_t)n, mm = (int32_t)m;
+    // sol = precondition(sol_init): x/D2, y/D1 (internal order)
+    hipLaunchKernelGGL(k_to_internal<V>, dim3(gn), dim3(BLOCK), 0, s, nn, d_col_of_int.p, x0_dev,
+                       d_D2o.p, S.x[0].p);
+    hipLaunchKernelGGL(k_to_internal<V>, dim3(gm), dim3(BLOCK), 0, s, mm, d_row_of_int.p, y0_dev,
+                       d_D1o.p, S.y[0].p);
+    // sol_last = 0, avg = sol, avg_last = 0, restart = sol  (pdlp.jl:51-54)
+    hipLaunchKernelGGL(k_fill_t<V>, dim3(gn), dim3(BLOCK), 0, s, (int64_t)n, S.x[1].p, V(0));
+    hipLaunchKernelGGL(k_fill_t<V>, dim3(gm), dim3(BLOCK), 0, s, (int64_t)m, S.y[1].p, V(0));
+    hipLaunchKernelGGL(k_fill_t<V>, dim3(gn), dim3(BLOCK), 0, s, (int64_t)n, S.xavgl.p, V(0));
+    hipLaunchKernelGGL(k_fill_t<V>, dim3(gm), dim3(BLOCK), 0, s, (int64_t)m, S.yavgl.p, V(0));
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(S.xavg.p, S.x[0].p, n * sizeof(V), hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(S.xrst.p, S.x[0].p, n * sizeof(V), hipMemcpyDeviceToDevice, s));
+    }
+    if (m) {
+        HIP_CHECK(hipMemcpyAsync(S.yavg.p, S.y[0].p, m * sizeof(V), hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(S.yrst.p, S.y[0].p, m * sizeof(V), hipMemcpyDeviceToDevice, s));
+    }
+    S.cur = 0;
+    HIP_CHECK(hipGetLastError());
+}
+
+// FP32 phase -> FP64 polish: the FP64 state restarts from the FP32 phase's sol
+static void upcast(IterState<float>& a, IterState<double>& b, int64_t n, int64_t m, hipStream_t s) {
+    const int gn = std::max(1, ceil_div(n, BLOCK)), gm = std::max(1, ceil_div(m, BLOCK));
+    const int c = a.cur;
+    for (DevBuf<double>* t : {&b.x[0], &b.xavg, &b.xrst})
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gn), dim3(BLOCK), 0, s, n, (const float*)a.x[c].p, t->p);
+    for (DevBuf<double>* t : {&b.y[0], &b.yavg, &b.yrst})
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gm), dim3(BLOCK), 0, s, m, (const float*)a.y[c].p, t->p);
+    hipLaunchKernelGGL(k_fill_t<double>, dim3(gn), dim3(BLOCK), 0, s, n, b.x[1].p, 0.0);
+    hipLaunchKernelGGL(k_fill_t<double>, dim3(gm), dim3(BLOCK), 0, s, m, b.y[1].p, 0.0);
+    hipLaunchKernelGGL(k_fill_t<double>, dim3(gn), dim3(BLOCK), 0, s, n, b.xavgl.p, 0.0);
+    hipLaunchKernelGGL(k_fill_t<double>, dim3(gm), dim3(BLOCK), 0, s, m, b.yavgl.p, 0.0);
+    b.cur = 0;
+}
+
+void Handle::solve(const double* x0, const double* y0) {
+    HIP_CHECK(hipSetDevice(device));
+    t_start = now_s();
+    stats = cpdlp_stats{};
+    history.clear();
+    stats.err = cpdlp_kkt{NAN, NAN, NAN, NAN, NAN, NAN};
+    stats.termination_status = CPDLP_STILL_RUNNING;
+    if (!prepared) prepare();
+    // start point (unscaled, original order) on the device
+    DevBuf<double> x0d(std::max<int64_t>(n, 1)), y0d(std::max<int64_t>(m, 1));
+    if (x0) x0d.upload(x0, n, stream);
+    else hipLaunchKernelGGL(k_fill, dim3(grid_n()), dim3(BLOCK), 0, stream, (int64_t)n, x0d.p, 0.0);
+    if (y0) y0d.upload(y0, m, stream);
+    else hipLaunchKernelGGL(k_fill, dim3(grid_m()), dim3(BLOCK), 0, stream, (int64_t)m, y0d.p, 0.0);
+    omega = omega0;
+    eta_sum = 0.0;
+    inner = total = 0;
+    restarts = 0;
+    step_ms = 0.0;
+    step_launches = 0;
+    fp32_passes = 0;
+    for (auto& g : graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+    }
+    const double t_loop = now_s();
+    const bool f32 = P.precision != CPDLP_PREC_FP64;
+    // early exit for no constraints and c == 0 (common.jl:194-198)
+    bool c_zero = true;
+    {
+        std::vector<double> hc(n);
+        if (n) {
+            d_c.download(hc.data(), n, stream);
+            sync();
+        }
+        for (double v : hc) if (v != 0.0) { c_zero = false; break; }
+    }
+    if (m == 0 && c_zero) {
+        init_state<double>(st64, x0d.p, y0d.p);
+        std::vector<double> lvh(n), uvh(n);
+        if (n) {
+            lv_s.download(lvh.data(), n, stream);
+            uv_s.download(uvh.data(), n, stream);
+            sync();
+            std::vector<double> xs(n);
+            for (int64_t j = 0; j < n; ++j) xs[j] = proj_box(0.0, lvh[j], uvh[j]);
+            st64.x[0].upload(xs.data(), n, stream);
+        }
+        sync();
+        active64 = true;
+        stats.termination_status = CPDLP_OPTIMAL;
+    } else if (!f32) {
+        init_state<double>(st64, x0d.p, y0d.p);
+        active64 = true;
+        loop<double>(st64, P.termination_reltol, true);
+    } else {
+        init_state<float>(st32, x0d.p, y0d.p);
+        active64 = false;
+        const bool polish = P.precision == CPDLP_PREC_MIXED;
+        const double r32 = polish ? std::max(P.polish_reltol, P.termination_reltol) : P.termination_reltol;
+        loop<float>(st32, r32, !polish);
+        fp32_passes = stats.kkt_passes;
+        if (polish && stats.termination_status == CPDLP_OPTIMAL && r32 > P.termination_reltol) {
+            st64.alloc(n, m);
+            upcast(st32, st64, n, m, stream);
+            eta_sum = 0.0;
+            inner = 0;
+            active64 = true;
+            stats.termination_status = CPDLP_STILL_RUNNING;
+            loop<double>(st64, P.termination_reltol, true);
+        }
+    }
+    sync();
+    loop_seconds = now_s() - t_loop;
+    stats.eta = eta;
+    stats.omega = omega;
+    stats.powm_iters = powm_iters;
+    stats.restarts = restarts;
+    stats.n_history = (int64_t)history.size();
+    stats.fp32_passes = fp32_passes;
+    stats.setup_seconds = setup_seconds;
+    stats.loop_seconds = loop_seconds;
+    stats.step_kernel_ms = step_ms;
+    stats.step_kernel_launches = step_launches;
+    stats.format_A = A.format;
+    stats.format_At = At.format;
+    solved = true;
+}
+
+void Handle::get_solution(double* x, double* y) {
+    HIP_CHECK(hipSetDevice(device));
+    if (!solved) throw Error(CPDLP_ERR_STATE, "no solution: call cpdlp_solve first");
+    DevBuf<double> xo(std::max<int64_t>(n, 1)), yo(std::max<int64_t>(m, 1));
+    const int32_t nn = (int32_t)n, mm = (int32_t)m;
+    if (active64) {
+        hipLaunchKernelGGL(k_from_internal<double>, dim3(grid_n()), dim3(BLOCK), 0, stream, nn,
+                           d_int_of_col.p, d_D2o.p, (const double*)st64.x[st64.cur].p, xo.p);
+        hipLaunchKernelGGL(k_from_internal<double>, dim3(grid_m()), dim3(BLOCK), 0, stream, mm,
+                           d_int_of_row.p, d_D1o.p, (const double*)st64.y[st64.cur].p, yo.p);
+    } else {
+        hipLaunchKernelGGL(k_from_internal<float>, dim3(grid_n()), dim3(BLOCK), 0, stream, nn,
+                           d_int_of_col.p, d_D2o.p, (const float*)st32.x[st32.cur].p, xo.p);
+        hipLaunchKernelGGL(k_from_internal<float>, dim3(grid_m()), dim3(BLOCK), 0, stream, mm,
+                           d_int_of_row.p, d_D1o.p, (const float*)st32.y[st32.cur].p, yo.p);
+    }
+    HIP_CHECK(hipGetLastError());
+    if (x) xo.download(x, n, stream);
+    if (y) yo.download(y, m, stream);
+    sync();
+}
+
+cpdlp_kkt Handle::kkt_host(const double* x, const double* y) {
+    HIP_CHECK(hipSetDevice(device));
+    if (!prepared) prepare();
+    DevBuf<double> xo(std::max<int64_t>(n, 1)), yo(std::max<int64_t>(m, 1));
+    DevBuf<double> xi(std::max<int64_t>(n, 1)), yi(std::max<int64_t>(m, 1));
+    xo.upload(x, n, stream);
+    yo.upload(y, m, stream);
+    hipLaunchKernelGGL(k_to_internal<double>, dim3(grid_n()), dim3(BLOCK), 0, stream, (int32_t)n,
+                       d_col_of_int.p, xo.p, d_D2o.p, xi.p);
+    hipLaunchKernelGGL(k_to_internal<double>, dim3(grid_m()), dim3(BLOCK), 0, stream, (int32_t)m,
+                       d_row_of_int.p, yo.p, d_D1o.p, yi.p);
+    const double* X[1] = {xi.p};
+    const double* Y[1] = {yi.p};
+    double sums[16];
+    kkt_launch<double, 1>(X, Y, nullptr, nullptr, false, sums);
+    return kkt_from_sums(sums, 1, 0);
+}
+
+}  // namespace cpdlp
+
+// ================================================================ C ABI
+using namespace cpdlp;
+
+struct cpdlp_handle {
+    std::unique_ptr<Handle> h;
+};
+
+extern "C" {
+
+int cpdlp_default_params(cpdlp_params* p, int32_t algorithm) {
+    return guarded([&] {
+        if (!p) throw Error(CPDLP_ERR_ARGUMENT, "NULL params");
+        if (algorithm != CPDLP_ALGO_PDLP && algorithm != CPDLP_ALGO_PDHG)
+            throw Error(CPDLP_ERR_ARGUMENT, "unknown algorithm");
+        std::memset(p, 0, sizeof(*p));
+        const bool pdlp = algorithm == CPDLP_ALGO_PDLP;
+        p->algorithm = algorithm;
+        p->ruiz_iter = pdlp ? 10 : 0;
+        p->chambolle_pock_alpha = 1.0;
+        p->invnorm_scaling = 0.9;
+        p->primal_weight_damping = pdlp ? 0.5 : NAN;
+        p->zero_tol = 1e-8;
+        p->sufficient_decay = pdlp ? 0.2 : NAN;
+        p->necessary_decay = pdlp ? 0.8 : NAN;
+        p->artificial_decay = pdlp ? 0.36 : NAN;
+        p->show_progress = 0;
+        p->check_every = 100;
+        p->record_error_history = 1;
+        p->precision = CPDLP_PREC_FP64;
+        p->termination_reltol = 1e-4;
+        p->max_kkt_passes = 100000;
+        p->time_limit = 100.0;
+        p->format = CPDLP_FMT_CSR;
+        p->device = 0;
+        p->powm_tol = NAN;
+        p->powm_maxiter = -1;
+        p->powm_seed = 0;
+        p->polish_reltol = 1e-5;
+        p->use_graph = 1;
+        p->long_row = 64;
+    });
+}
+
+int cpdlp_create(const cpdlp_lp* lp, const cpdlp_params* params, cpdlp_handle** out) {
+    return guarded([&] {
+        if (!lp || !out) throw Error(CPDLP_ERR_ARGUMENT, "NULL argument");
+        cpdlp_params p;
+        if (params) p = *params;
+        else cpdlp_default_params(&p, CPDLP_ALGO_PDLP);
+        auto* h = new cpdlp_handle;
+        try {
+            h->h = std::make_unique<Handle>(*lp, p);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+#define HANDLE_OR_THROW(hp)                                                      \
+    if (!(hp) || !(hp)->h) throw Error(CPDLP_ERR_ARGUMENT, "NULL handle");
+
+int cpdlp_solve(cpdlp_handle* h, const double* x0, const double* y0) {
+    return guarded([&] { HANDLE_OR_THROW(h); h->h->solve(x0, y0); });
+}
+int cpdlp_prepare(cpdlp_handle* h) {
+    return guarded([&] { HANDLE_OR_THROW(h); HIP_CHECK(hipSetDevice(h->h->device)); h->h->prepare(); });
+}
+int cpdlp_get_solution(cpdlp_handle* h, double* x, double* y) {
+    return guarded([&] { HANDLE_OR_THROW(h); h->h->get_solution(x, y); });
+}
+int cpdlp_get_stats(cpdlp_handle* h, cpdlp_stats* out) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        if (!out) throw Error(CPDLP_ERR_ARGUMENT, "NULL stats");
+        *out = h->h->stats;
+    });
+}
+int cpdlp_get_history(cpdlp_handle* h, int64_t* passes, cpdlp_kkt* errs, int64_t cap, int64_t* count) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        const auto& H = h->h->history;
+        const int64_t k = std::min<int64_t>(cap, (int64_t)H.size());
+        for (int64_t i = 0; i < k; ++i) {
+            if (passes) passes[i] = H[i].first;
+            if (errs) errs[i] = H[i].second;
+        }
+        if (count) *count = (int64_t)H.size();
+    });
+}
+int cpdlp_kkt_errors(cpdlp_handle* h, const double* x, const double* y, cpdlp_kkt* out) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        if ((!x && h->h->n) || (!y && h->h->m) || !out) throw Error(CPDLP_ERR_ARGUMENT, "NULL argument");
+        *out = h->h->kkt_host(x, y);
+    });
+}
+int cpdlp_get_scaling(cpdlp_handle* h, double* D1, double* D2, double* a_csc, double* at_csc) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        Handle& H = *h->h;
+        HIP_CHECK(hipSetDevice(H.device));
+        if (!H.prepared) H.prepare();
+        if (D1) H.d_D1o.download(D1, H.m, H.stream);
+        if (D2) H.d_D2o.download(D2, H.n, H.stream);
+        if (a_csc) H.d_a_csc.download(a_csc, H.nnz, H.stream);
+        if (at_csc) H.d_at_csc.download(at_csc, H.nnz, H.stream);
+        H.sync();
+    });
+}
+int cpdlp_get_csr(cpdlp_handle* h, int64_t* rowptr, int32_t* colval) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        const Handle& H = *h->h;
+        if (rowptr)
+            for (int64_t i = 0; i <= H.m; ++i) rowptr[i] = H.L.csr_rowptr[i];
+        if (colval) std::memcpy(colval, H.L.csr_col.data(), H.nnz * sizeof(int32_t));
+    });
+}
+int cpdlp_get_step_size(cpdlp_handle* h, double* eta, double* omega, int64_t* powm_iters) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        Handle& H = *h->h;
+        if (!H.prepared) {
+            HIP_CHECK(hipSetDevice(H.device));
+            H.prepare();
+        }
+        if (eta) *eta = H.eta;
+        if (omega) *omega = H.omega0;
+        if (powm_iters) *powm_iters = H.powm_iters;
+    });
+}
+void cpdlp_destroy(cpdlp_handle* h) { delete h; }
+
+const char* cpdlp_last_error(void) { return g_last_error.c_str(); }
+
+const char* cpdlp_version(void) { return "coolpdlp_hip 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// solver.hpp -- host-side objects of libcoolpdlp_hip.so.
+#pragma once
+#include <cfloat>
+#include <utility>
+#include <vector>
+
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace cpdlp {
+
+// one matrix (A or A') on the device, internal layout; FP64 values always, FP32 copy
+// for the FP32 / mixed loop; optional column-major ELL of the same rows
+struct DevSide {
+    int32_t rows = 0, cols = 0, n_long = 0, n_tiles = 0;
+    int64_t nnz = 0;
+    int32_t format = CPDLP_FMT_CSR;
+    int32_t ell_w = 0;
+    DevBuf<int32_t> rowptr, col, vmap, tile_ptr;
+    DevBuf<double> val64;
+    DevBuf<float> val32;
+    DevBuf<int32_t> ell_col, ell_vmap;
+    DevBuf<double> ell_val64;
+    DevBuf<float> ell_val32;
+
+    template <class V>
+    const V* vals() const;
+    template <class V>
+    const V* ell_vals() const;
+    template <class V>
+    MatView<V> view() const;
+    void upload_structure(const HostSide& S, hipStream_t s);
+    void build_ell(const HostSide& S, hipStream_t s);
+    void fill_values(const double* src_csc, bool want32, hipStream_t s);
+};
+
+// PDLPState's solutions (pdlp.jl:20-43): x[cur], y[cur] = sol; x[1-cur], y[1-cur] =
+// sol_last (the two swap every step, pdlp.jl:101); xbar = scratch.x of the step.
+template <class V>
+struct IterState {
+    DevBuf<V> x[2], y[2];
+    DevBuf<V> xbar, xavg, yavg, xavgl, yavgl, xrst, yrst, tmp_n, tmp_m;
+    int cur = 0;
+    void alloc(int64_t n, int64_t m);
+};
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    int end_parity = 0;
+};
+
+struct Handle {
+    cpdlp_params P;
+    int64_t m = 0, n = 0, nnz = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    HostLayout L;
+    // inputs, original order
+    DevBuf<double> d_nzval, d_c, d_lv, d_uv, d_lc, d_uc;
+    DevBuf<int32_t> d_colptr, d_rowval, d_csr_rowptr, d_perm;
+    DevBuf<int32_t> d_row_of_int, d_col_of_int, d_int_of_row, d_int_of_col;
+    // scaling results, original order / CSC(A) layout
+    DevBuf<double> d_D1o, d_D2o, d_a_csc, d_at_csc;
+    // scaled problem, internal order
+    DevBuf<double> c_s, lv_s, uv_s, lc_s, uc_s, D1_s, D2_s, invD1_s, invD2_s;
+    DevBuf<float> c_32, lv_32, uv_32, lc_32, uc_32;
+    DevSide A, At;
+    // scratch
+    DevBuf<double> d_partials, d_partials2, d_sums;
+    DevBuf<StepCoef> d_coef;
+    GraphCache graphs[4];
+    // solver scalars
+    bool prepared = false, solved = false, active64 = true;
+    double eta = 0, omega0 = 1, omega = 1, eta_sum = 0;
+    double primal_scale = 1, dual_scale = 1, c_norm = 0, comb_norm = 0;
+    int64_t powm_iters = 0, inner = 0, total = 0, restarts = 0, step_launches = 0, fp32_passes = 0;
+    double setup_seconds = 0, loop_seconds = 0, step_ms = 0, t_start = 0;
+    IterState<double> st64;
+    IterState<float> st32;
+    cpdlp_stats stats{};
+    std::vector<std::pair<int64_t, cpdlp_kkt>> history;
+
+    Handle(const cpdlp_lp& lp, const cpdlp_params& prm);
+    ~Handle();
+    int grid_n() const;
+    int grid_m() const;
+    void sync();
+    void ensure_partials(size_t count);
+    void reduce_to_host(int nb, int nq, const double* partials, double* host_out);
+    void scale();
+    void choose_formats();
+    template <class V>
+    void power_method(double tol, int64_t maxiter);
+    void prepare();
+    template <class V, int K>
+    void kkt_launch(const V* const* X, const V* const* Y, const V* xr, const V* yr, bool dist,
+                    double* host_sums);
+    cpdlp_kkt kkt_from_sums(const double* sums, int K, int k) const;
+    template <class V>
+    void launch_steps(IterState<V>& S, int nsteps, bool pdlp);
+    template <class V>
+    void run_chunk(IterState<V>& S, bool pdlp);
+    template <class V>
+    void loop(IterState<V>& S, double phase_reltol, bool final_phase);
+    template <class V>
+    void init_state(IterState<V>& S, const double* x0_dev, const double* y0_dev);
+    void solve(const double* x0, const double* y0);
+    void get_solution(double* x, double* y);
+    cpdlp_kkt kkt_host(const double* x, const double* y);
+};
+
+}  // namespace cpdlp

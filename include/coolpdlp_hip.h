@@ -1,0 +1,211 @@
+/*
+ * coolpdlp_hip.h -- C ABI of libcoolpdlp_hip.so, the MI355X (gfx950) PDLP/PDHG engine.
+ *
+ * Plain C types only (pointers, sizes, PODs); no C++ types or exceptions cross it.
+ * Every function returns an int status (CPDLP_OK = 0); the message of the last
+ * error on the calling thread is cpdlp_last_error().  Solver outcomes (OPTIMAL,
+ * TIME_LIMIT, ...) are not errors: they are reported in cpdlp_stats.
+ *
+ * The reference (CoolPDLP.jl) is pure Julia and has no FFI of its own.  Each entry
+ * point below names the reference function or plug-in method it replaces; the
+ * Julia-side ccall binding a maintainer would add is in INTEGRATION.md and
+ * coolpdlp.jl_amd/julia/CoolPDLPHIP.jl.  Paths are relative to the reference repo.
+ *
+ * Threading: one handle owns one HIP stream and its device memory; calls on
+ * different handles may run on different threads; a single handle is not
+ * thread-safe (like the reference's single-task solve!).
+ */
+#ifndef COOLPDLP_HIP_H
+#define COOLPDLP_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define CPDLP_OK 0
+#define CPDLP_ERR_DIMENSION 1   /* DimensionMismatch, src/problems/milp.jl:71-76 */
+#define CPDLP_ERR_ARGUMENT 2    /* ArgumentError, milp.jl:83-92, utils/device.jl:6-12 */
+#define CPDLP_ERR_MEMORY 3      /* device allocation failed */
+#define CPDLP_ERR_HIP 4         /* HIP runtime error */
+#define CPDLP_ERR_STATE 5       /* call out of order (e.g. solution before solve) */
+
+/* ---- enums ------------------------------------------------------------------- */
+/* TerminationStatus, src/components/termination.jl:13 (same order) */
+#define CPDLP_OPTIMAL 0
+#define CPDLP_TIME_LIMIT 1
+#define CPDLP_ITERATION_LIMIT 2
+#define CPDLP_STILL_RUNNING 3
+
+#define CPDLP_ALGO_PDLP 0       /* Algorithm{:PDLP}, src/algorithms/pdlp.jl:6-11 */
+#define CPDLP_ALGO_PDHG 1       /* Algorithm{:PDHG}, src/algorithms/pdhg.jl:6-16 */
+
+#define CPDLP_PREC_FP64 0       /* T = Float64 everywhere */
+#define CPDLP_PREC_FP32 1       /* T = Float32 (the reference's float_type = Float32) */
+#define CPDLP_PREC_MIXED 2      /* FP32 matrix/iterates/SpMV; FP64 checks; FP64 polish */
+
+#define CPDLP_FMT_AUTO 0        /* autotuned per matrix */
+#define CPDLP_FMT_CSR 1         /* GPUSparseMatrixCSR, src/utils/mat_csr.jl */
+#define CPDLP_FMT_ELL 2         /* GPUSparseMatrixELL, src/utils/mat_ell.jl */
+
+/* ---- problem input --------------------------------------------------------------
+ * MILP(; c, lv, uv, A, lc, uc), src/problems/milp.jl:55-115, with A given as the
+ * reference's SparseMatrixCSC (colptr n+1, rowval nnz, nzval nnz; rows sorted within
+ * each column).  index_base 1 accepts Julia's arrays unchanged.  +-inf bounds allowed.
+ * Host pointers; read only during cpdlp_create. */
+typedef struct cpdlp_lp {
+    int64_t m, n;
+    int32_t index_base;      /* 0 or 1 */
+    int32_t index_bits;      /* 32 or 64: width of colptr and rowval */
+    const void* colptr;
+    const void* rowval;
+    const double* nzval;
+    const double* c;
+    const double* lv;
+    const double* uv;
+    const double* lc;
+    const double* uc;
+} cpdlp_lp;
+
+/* ---- parameters -------------------------------------------------------------------
+ * Algorithm{A}(T, Ti, M; kwargs...), src/algorithms/common.jl:53-116: same names,
+ * same defaults (cpdlp_default_params), plus engine options. */
+typedef struct cpdlp_params {
+    int32_t algorithm;              /* CPDLP_ALGO_* */
+    int32_t ruiz_iter;              /* 10 (PDHG: 0) */
+    double chambolle_pock_alpha;    /* 1.0 */
+    double invnorm_scaling;         /* 0.9 */
+    double primal_weight_damping;   /* 0.5 (PDHG: NaN) */
+    double zero_tol;                /* 1e-8 */
+    double sufficient_decay;        /* 0.2 (PDHG: NaN) */
+    double necessary_decay;         /* 0.8 (PDHG: NaN) */
+    double artificial_decay;        /* 0.36 (PDHG: NaN) */
+    int32_t show_progress;          /* 0 (accepted, ignored: no progress bar) */
+    int32_t check_every;            /* 100 */
+    int32_t record_error_history;   /* 1 */
+    int32_t precision;              /* CPDLP_PREC_* (T) */
+    double termination_reltol;      /* 1e-4 */
+    int64_t max_kkt_passes;         /* 100000 */
+    double time_limit;              /* 100.0 s */
+    /* engine options (no reference counterpart) */
+    int32_t format;                 /* CPDLP_FMT_* (M) */
+    int32_t device;                 /* HIP device ordinal */
+    double powm_tol;                /* NaN: eps(T)*n^3 rule of powm!, linalg.jl:94 */
+    int64_t powm_maxiter;           /* -1: n */
+    uint64_t powm_seed;             /* power-method start-vector seed */
+    double polish_reltol;           /* MIXED: FP32 phase stops at max(this, reltol) */
+    int32_t use_graph;              /* capture the check_every-step chunk in a hipGraph */
+    int32_t long_row;               /* rows longer than this use a wavefront per row (64) */
+} cpdlp_params;
+
+/* KKTErrors, src/components/errors.jl:8-21 */
+typedef struct cpdlp_kkt {
+    double primal, primal_scale, dual, dual_scale, gap, gap_scale;
+} cpdlp_kkt;
+
+/* ConvergenceStats, src/components/termination.jl:44-56, plus engine counters */
+typedef struct cpdlp_stats {
+    cpdlp_kkt err;
+    double time_elapsed;            /* seconds since solve start (incl. preprocess) */
+    int64_t kkt_passes;
+    int32_t termination_status;     /* CPDLP_OPTIMAL ... */
+    int32_t format_A, format_At;    /* chosen formats */
+    int32_t pad;
+    double eta, omega;
+    int64_t powm_iters;
+    int64_t restarts;
+    int64_t n_history;
+    int64_t fp32_passes;            /* MIXED: passes done in FP32 before polishing */
+    double setup_seconds;           /* scaling + step size on the device */
+    double loop_seconds;            /* PDHG loop incl. checks */
+    double step_kernel_ms;          /* device time of all step kernels (HIP events) */
+    int64_t step_kernel_launches;
+} cpdlp_stats;
+
+typedef struct cpdlp_handle cpdlp_handle;
+typedef struct cpdlp_matrix cpdlp_matrix;
+
+/* defaults of Algorithm{:PDLP}(...) / PDHG(...): common.jl:53-78, pdhg.jl:6-16 */
+int cpdlp_default_params(cpdlp_params* p, int32_t algorithm);
+
+/* preprocess + initialize inputs: copies the LP to HBM, builds CSR(A) and CSR(At)
+ * (perform_conversion, src/components/conversion.jl:37-55; set_matrix_type,
+ * src/problems/modify.jl:80-105).  Validates like MILP(...), milp.jl:71-92. */
+int cpdlp_create(const cpdlp_lp* lp, const cpdlp_params* params, cpdlp_handle** out);
+
+/* solve(milp, sol, algo) / solve(milp, algo), src/algorithms/common.jl:186-209:
+ * scaling, step size, primal weight, the PDLP/PDHG loop to termination.
+ * x0 (n) / y0 (m) are the unscaled start (NULL = zero, common.jl:207). */
+int cpdlp_solve(cpdlp_handle* h, const double* x0, const double* y0);
+
+/* get_solution + Array(...), common.jl:233-235 and src/moi.jl:215-216:
+ * unscaled x (n) and y (m) into host buffers. */
+int cpdlp_get_solution(cpdlp_handle* h, double* x, double* y);
+
+/* state.stats, src/components/termination.jl:44-76 */
+int cpdlp_get_stats(cpdlp_handle* h, cpdlp_stats* out);
+
+/* stats.error_history, termination.jl:56: up to cap entries; *count = total */
+int cpdlp_get_history(cpdlp_handle* h, int64_t* kkt_passes, cpdlp_kkt* errs, int64_t cap,
+                      int64_t* count);
+
+/* kkt_errors!(scratch, sol, milp), src/components/errors.jl:65-109, on the
+ * preconditioned problem for an UNSCALED host point (x, y); errors are in the
+ * original space as in the reference.  Requires cpdlp_solve or cpdlp_prepare. */
+int cpdlp_kkt_errors(cpdlp_handle* h, const double* x, const double* y, cpdlp_kkt* out);
+
+/* preprocess(...) + initialize's step sizes without running the loop:
+ * pdlp_preconditioner / precondition (preconditioning.jl:146-155, :61-91),
+ * fixed_stepsize (step_size.jl:22-27), primal_weight_init (:29-40). */
+int cpdlp_prepare(cpdlp_handle* h);
+
+/* Scaling results in the CSC(A) layout: D1 (m), D2 (n), scaled A values (nnz) and
+ * scaled At values (nnz, transposing CSC(A) entry k).  Any pointer may be NULL. */
+int cpdlp_get_scaling(cpdlp_handle* h, double* D1, double* D2, double* a_csc, double* at_csc);
+
+/* CSR(A) structure built by cpdlp_create (0-based): rowptr (m+1), colval (nnz).
+ * GPUSparseMatrixCSR(A), src/utils/mat_csr.jl:58-61. */
+int cpdlp_get_csr(cpdlp_handle* h, int64_t* rowptr, int32_t* colval);
+
+/* eta (after cpdlp_prepare or cpdlp_solve) and power-method iterations. */
+int cpdlp_get_step_size(cpdlp_handle* h, double* eta, double* omega, int64_t* powm_iters);
+
+void cpdlp_destroy(cpdlp_handle* h);
+
+/* thread-local message of the last failing call on this thread ("" if none) */
+const char* cpdlp_last_error(void);
+
+/* library version string and the device architecture it was built for */
+const char* cpdlp_version(void);
+
+/* ---- matrix plug-in: GPUSparseMatrixCSR / GPUSparseMatrixELL + mul! ----------------
+ * GPUSparseMatrixCSR(A::SparseMatrixCSC), src/utils/mat_csr.jl:58-61;
+ * GPUSparseMatrixELL(A), src/utils/mat_ell.jl:54-70; values FP64.  transpose != 0
+ * builds the matrix of A' instead (the reference's MILP.At, src/problems/milp.jl:60). */
+int cpdlp_matrix_create(int64_t m, int64_t n, int32_t index_base, int32_t index_bits,
+                        const void* colptr, const void* rowval, const double* nzval,
+                        int32_t format, int32_t transpose, int32_t device, cpdlp_matrix** out);
+
+/* LinearAlgebra.mul!(c, A, b, alpha, beta), mat_csr.jl:81-92 / mat_ell.jl:87-98:
+ * c = alpha*A*b + beta*c on DEVICE pointers, asynchronously on `stream`
+ * (a hipStream_t; NULL = the matrix's own stream).  beta == 0 does not read c. */
+int cpdlp_matrix_mul(cpdlp_matrix* A, double alpha, const double* b_dev, double beta,
+                     double* c_dev, void* stream);
+
+/* Same on host buffers (copies in and out; synchronous). */
+int cpdlp_matrix_mul_host(cpdlp_matrix* A, double alpha, const double* b, double beta, double* c);
+
+/* size(A), nnz(A), the format in use and, for ELL, the width d. */
+int cpdlp_matrix_info(cpdlp_matrix* A, int64_t* m, int64_t* n, int64_t* nnz, int32_t* format,
+                      int64_t* ell_width);
+
+/* ELL arrays (column-major m x d, padding column -1, value 0), mat_ell.jl:54-70. */
+int cpdlp_matrix_get_ell(cpdlp_matrix* A, int32_t* colval, double* nzval);
+
+void cpdlp_matrix_destroy(cpdlp_matrix* A);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,228 @@
+"""GPU parity of the plain (one LP) path, through the C ABI, against the oracle.
+
+Bars: index maps and scaling bit-exact; SpMV bit-exact on rows that the reference's
+thread-per-row order covers (rows <= 64 nonzeros), 1e-13 relative otherwise; solve
+results: same termination status, objective within 1e-6 relative of the oracle's and
+of the closed form c'x0, kkt_passes within two checks.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+from coolpdlp_hip import generators as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cp():
+    import coolpdlp_hip
+    return coolpdlp_hip
+
+
+def _lp(A, c, lv, uv, lc, uc):
+    A = sp.csc_matrix(np.asarray(A, float))
+    g = G.GeneratedLP(A.shape[0], A.shape[1], A.indptr.astype(np.int64), A.indices.astype(np.int32),
+                      A.data.astype(float), *[np.asarray(v, float) for v in (c, lv, uv, lc, uc)])
+    return g
+
+
+# ------------------------------------------------------------------ matrices
+def _rand_mats(seed=0):
+    rng = np.random.default_rng(seed)
+    mats = []
+    for m in (10, 20, 30):
+        for n in (10, 20, 30):
+            for p in (0.01, 0.1, 0.2, 0.3):   # test/utils/matrices.jl:10-15
+                mats.append(sp.random(m, n, density=p, format="csc", random_state=rng))
+    # long rows (wavefront path) mixed with short ones, and an empty row / column
+    A = sp.random(400, 3000, density=0.002, format="lil", random_state=rng)
+    for i in (0, 7, 399):
+        A[i, rng.choice(3000, 700, replace=False)] = rng.standard_normal(700)
+    mats.append(sp.csc_matrix(A))
+    return mats
+
+
+@pytest.mark.parametrize("fmt", ["csr", "ell", "auto"])
+def test_mul_matches_dense(cp, fmt):
+    rng = np.random.default_rng(1)
+    for A in _rand_mats():
+        cls = cp.GPUSparseMatrixELL if fmt == "ell" else cp.GPUSparseMatrixCSR
+        kw = {} if fmt != "auto" else {"format": 0}
+        try:
+            M = cls(A, **kw)
+        except cp.ArgumentError:
+            assert fmt == "ell"   # ELL refused: padding above 1.5x nnz
+            continue
+        assert M.shape == A.shape and M.nnz == A.nnz
+        b = rng.random(A.shape[1])
+        c = rng.random(A.shape[0])
+        alpha, beta = 0.37, 0.81
+        out = M.mul(c.copy(), b, alpha, beta)
+        ref = alpha * (A @ b) + beta * c
+        np.testing.assert_allclose(out, ref, rtol=1e-13, atol=1e-14)
+        # beta == 0 must not read c (NaN there)
+        out0 = M.mul(np.full(A.shape[0], np.nan), b, 1.0, 0.0)
+        assert np.all(np.isfinite(out0))
+        # bit-exact vs the oracle's thread-per-row sum on short rows
+        rp, col, val = O.csc_to_csr(A.shape[0], A.shape[1], A.indptr, A.indices, A.data)
+        seq = O.spmv_csr(rp, col, val, 1.0, b, 0.0, np.zeros(A.shape[0]))
+        short = np.diff(rp) <= 64
+        assert np.array_equal(out0[short], seq[short])
+        np.testing.assert_allclose(out0, seq, rtol=1e-13, atol=1e-14)
+
+
+def test_mul_transpose_and_torch(cp):
+    import torch
+    rng = np.random.default_rng(2)
+    A = sp.random(500, 800, density=0.01, format="csc", random_state=rng)
+    Mt = cp.GPUSparseMatrixCSR(A, transpose=True)
+    assert Mt.shape == (800, 500)
+    y = torch.tensor(rng.standard_normal(500), device="cuda:0")
+    out = torch.empty(800, dtype=torch.float64, device="cuda:0")
+    Mt.mul(out, y)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(out.cpu().numpy(), A.T @ y.cpu().numpy(), rtol=1e-13, atol=1e-14)
+    with pytest.raises(cp.ArgumentError):
+        Mt.mul(np.zeros(800), y)   # mixed backends, utils/device.jl:6-12
+
+
+def test_ell_layout_matches_oracle(cp):
+    g = G.column_lp(60, 90, 3, seed=4)
+    A = g.csc()
+    M = cp.GPUSparseMatrixELL(A)
+    col, val = M.ell()
+    rp, c, v = O.csc_to_csr(g.m, g.n, g.colptr, g.rowval, g.nzval)
+    d, ec, ev = O.csr_to_ell(rp, c, v)
+    assert M.ell_width == d
+    assert np.array_equal(col, ec) and np.array_equal(val, ev)
+
+
+# ------------------------------------------------------------------ preprocessing
+def test_csr_and_scaling_bit_exact(cp):
+    lp, _, _ = G.random_milp_and_sol(120, 260, 0.05, seed=7)
+    milp = cp.MILP.from_generated(lp)
+    with cp.Engine(milp, cp.PDLP()) as eng:
+        rp, col = eng.csr()
+        orp, ocol, _ = O.csc_to_csr(lp.m, lp.n, lp.colptr, lp.rowval, lp.nzval)
+        assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+        D1, D2, a, at = eng.scaling()
+    oD1, oD2, oa, oat = O.scaling(lp.m, lp.n, lp.colptr, lp.rowval, lp.nzval, 10, 1.0)
+    assert np.array_equal(D1, oD1) and np.array_equal(D2, oD2)
+    assert np.array_equal(a, oa) and np.array_equal(at, oat)
+
+
+def test_step_size_matches_oracle(cp):
+    g = G.column_lp(200, 500, 5, seed=3)
+    milp = cp.MILP.from_generated(g)
+    with cp.Engine(milp, cp.PDLP()) as eng:
+        eta, omega, iters = eng.step_size()
+    D1, D2, a, at = O.scaling(g.m, g.n, g.colptr, g.rowval, g.nzval, 10, 1.0)
+    A_csr = O.csc_to_csr(g.m, g.n, g.colptr, g.rowval, a)
+    At_csr = (g.colptr, g.rowval, at)
+    s, it = O.spectral_norm(g.m, g.n, A_csr, At_csr)
+    assert abs(iters - it) <= 2
+    assert abs(eta - 0.9 / s) <= 1e-9 * eta
+
+
+# ------------------------------------------------------------------ solves
+def test_moi_two_variable_lp(cp):
+    # test/moi.jl:44-52: min 12x + 20y, 6x + 8y >= 100, 7x + 12y >= 120, x >= 0, 0 <= y <= 3
+    g = _lp([[6, 8], [7, 12]], [12, 20], [0, 0], [np.inf, 3], [100, 120], [np.inf, np.inf])
+    milp = cp.MILP.from_generated(g)
+    sol, st = cp.solve(milp, cp.PDLP())
+    assert st.termination_status == cp.OPTIMAL
+    assert abs(cp.objective_value(sol.x, milp) - 205.0) <= 1e-2
+    assert cp.is_feasible(sol.x, milp, cons_tol=1e-2)
+
+
+def test_termination_not_skipped(cp):
+    # test/components/termination.jl:1-10
+    g = _lp([[1, 1]], [1, 1], [0, 0], [np.inf, np.inf], [1], [np.inf])
+    sol, st = cp.solve(cp.MILP.from_generated(g), cp.PDLP())
+    assert st.termination_status == cp.OPTIMAL
+    assert abs(sol.x.sum() - 1.0) < 1e-3
+
+
+@pytest.mark.parametrize("algo", ["PDLP", "PDHG"])
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_solve_matches_oracle(cp, algo, use_graph):
+    g = G.column_lp(400, 1000, 6, seed=21)
+    milp = cp.MILP.from_generated(g)
+    A = cp.PDLP if algo == "PDLP" else cp.PDHG
+    reltol = 1e-6 if algo == "PDLP" else 1e-4
+    sol, st = cp.solve(milp, A(termination_reltol=reltol, use_graph=use_graph))
+    ref = O.solve(g, O.default_params(algo, termination_reltol=reltol))
+    assert st.termination_status.name == ref.status == "OPTIMAL"
+    assert abs(st.kkt_passes - ref.stats.kkt_passes) <= 200
+    assert st.err.relative() <= reltol
+    obj = cp.objective_value(sol.x, milp)
+    obj_ref = float(ref.x @ g.c)
+    assert abs(obj - obj_ref) <= 10 * reltol * (1 + abs(obj_ref))
+    assert abs(obj - g.objective) <= 10 * reltol * (1 + abs(g.objective))
+    assert abs(st.eta - ref.stats.eta) <= 1e-9 * ref.stats.eta
+    # the history is recorded every check, with the duplicate final entry of PDLP
+    assert len(st.error_history) == ref.stats.n_history
+
+
+def test_trajectory_matches_oracle_exactly_early(cp):
+    """Short rows + FP64: the first checks' KKT errors agree to reduction-order
+    rounding (sums differ only in order)."""
+    g = G.column_lp(300, 700, 6, seed=5)
+    milp = cp.MILP.from_generated(g)
+    _, st = cp.solve(milp, cp.PDLP(max_kkt_passes=500, termination_reltol=1e-12))
+    ref = O.solve(g, O.default_params(max_kkt_passes=500, termination_reltol=1e-12))
+    assert st.termination_status == cp.ITERATION_LIMIT and ref.status == "ITERATION_LIMIT"
+    assert st.kkt_passes == ref.stats.kkt_passes == 500
+    for (p1, e1), (p2, e2) in zip(st.error_history, ref.history):
+        assert p1 == p2
+        for f in ("primal", "dual", "gap"):
+            assert abs(getattr(e1, f) - e2[f]) <= 1e-9 * (1 + abs(e2[f])), (p1, f)
+
+
+def test_kkt_errors_match_oracle(cp):
+    lp, x, y = G.random_milp_and_sol(80, 150, 0.1, seed=9)
+    milp = cp.MILP.from_generated(lp)
+    with cp.Engine(milp, cp.PDLP()) as eng:
+        e = eng.kkt_errors(cp.PrimalDualSolution(x, y))
+    D1, D2, a, at = O.scaling(lp.m, lp.n, lp.colptr, lp.rowval, lp.nzval, 10, 1.0)
+    A_csr = O.csc_to_csr(lp.m, lp.n, lp.colptr, lp.rowval, a)
+    e_ref = O.kkt_errors(lp.m, lp.n, A_csr, (lp.colptr, lp.rowval, at), D2 * lp.c, lp.lv / D2,
+                         lp.uv / D2, D1 * lp.lc, D1 * lp.uc, D1, D2, x / D2, y / D1)
+    for f in ("primal", "primal_scale", "dual", "dual_scale", "gap", "gap_scale"):
+        assert abs(getattr(e, f) - getattr(e_ref, f)) <= 1e-12 * (1 + abs(getattr(e_ref, f))), f
+
+
+def test_limits(cp):
+    g = G.column_lp(200, 400, 5, seed=8)
+    milp = cp.MILP.from_generated(g)
+    _, st = cp.solve(milp, cp.PDLP(max_kkt_passes=1, termination_reltol=1e-12))
+    assert st.termination_status == cp.ITERATION_LIMIT and st.kkt_passes == 100
+    _, st = cp.solve(milp, cp.PDLP(time_limit=0.0, termination_reltol=1e-12))
+    assert st.termination_status == cp.TIME_LIMIT and st.kkt_passes == 100
+
+
+def test_warm_start_and_empty_constraints(cp):
+    g = G.column_lp(200, 400, 5, seed=8)
+    milp = cp.MILP.from_generated(g)
+    sol, st = cp.solve(milp, cp.PrimalDualSolution(g.x0, g.y0), cp.PDLP())
+    assert st.termination_status == cp.OPTIMAL and st.kkt_passes == 100
+    # no constraints and c == 0: early exit, x = proj_box(0, lv, uv) (common.jl:194-198)
+    e = cp.MILP(c=np.zeros(3), lv=np.array([1.0, -2.0, -np.inf]), uv=np.array([2.0, -1.0, np.inf]),
+                A=sp.csc_matrix((0, 3)), lc=np.zeros(0), uc=np.zeros(0))
+    sol, st = cp.solve(e, cp.PDLP())
+    assert st.termination_status == cp.OPTIMAL
+    assert np.array_equal(sol.x, [1.0, -1.0, 0.0])
+
+
+def test_float32_and_mixed(cp):
+    g = G.column_lp(300, 700, 6, seed=13)
+    milp = cp.MILP.from_generated(g)
+    sol, st = cp.solve(milp, cp.PDLP("mixed", termination_reltol=1e-7))
+    assert st.termination_status == cp.OPTIMAL
+    assert st.fp32_passes > 0 and st.kkt_passes > st.fp32_passes
+    assert abs(cp.objective_value(sol.x, milp) - g.objective) <= 1e-5 * (1 + abs(g.objective))
+    sol, st = cp.solve(milp, cp.PDLP("float32", termination_reltol=1e-4))
+    assert st.termination_status == cp.OPTIMAL
+    assert abs(cp.objective_value(sol.x, milp) - g.objective) <= 1e-2 * (1 + abs(g.objective))
