@@ -107,6 +107,16 @@ if not os.path.exists(LIB):
         f"libcoolpdlp_hip.so not found at {LIB}: the HIP engine is required (there is no "
         "CPU fallback); build it with __graft_entry__.build()")
 
+# One HIP runtime per process: PyTorch ships its own libamdhip64 (same SONAME as
+# ROCm's).  Loading torch first makes this library bind to that copy, so device
+# pointers and streams from torch tensors are valid here; loading ours first would
+# put two runtimes in the process and torch's would then fail to initialise.
+if os.environ.get("COOLPDLP_NO_TORCH") != "1":
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
 lib = C.CDLL(LIB)
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)

@@ -1006,8 +1006,8 @@ void Handle::solve(const double* x0, const double* y0) {
             std::vector<double> xs(n);
             for (int64_t j = 0; j < n; ++j) xs[j] = proj_box(0.0, lvh[j], uvh[j]);
             st64.x[0].upload(xs.data(), n, stream);
+            sync();
         }
-        sync();
         active64 = true;
         stats.termination_status = CPDLP_OPTIMAL;
     } else if (!f32) {

@@ -276,11 +276,14 @@ void orc_scaling(int64_t m, int64_t n, const int64_t* colptr, const int32_t* row
         for (int64_t i = 0; i < m; ++i) D1[i] = d1[i] * D1[i];   /* prec_next * prec */
         for (int64_t j = 0; j < n; ++j) D2[j] = D2[j] * d2[j];
     }
-    /* cons_r = precondition(cons, prec_r) from the ORIGINAL matrices */
-    apply_prec(n, colptr, rowval, nzval, nzval, D1, D2, a, at);
-    diag_norm_prec(m, n, colptr, rowptr, perm, a, at, 2.0 - cp_alpha, cp_alpha, d1, d2);
-    for (int64_t i = 0; i < m; ++i) D1[i] = D1[i] * d1[i];       /* prec_r * prec_cp */
-    for (int64_t j = 0; j < n; ++j) D2[j] = d2[j] * D2[j];
+    /* cons_r = precondition(cons, prec_r) from the ORIGINAL matrices; cp_alpha = NaN
+     * skips Chambolle-Pock (test hook: Ruiz alone, as ruiz_preconditioner) */
+    if (!isnan(cp_alpha)) {
+        apply_prec(n, colptr, rowval, nzval, nzval, D1, D2, a, at);
+        diag_norm_prec(m, n, colptr, rowptr, perm, a, at, 2.0 - cp_alpha, cp_alpha, d1, d2);
+        for (int64_t i = 0; i < m; ++i) D1[i] = D1[i] * d1[i];   /* prec_r * prec_cp */
+        for (int64_t j = 0; j < n; ++j) D2[j] = d2[j] * D2[j];
+    }
     /* precondition(milp, prec): A_p = D1*A*D2, At_p = D2*At*D1 from the originals */
     apply_prec(n, colptr, rowval, nzval, nzval, D1, D2, a_csc_out, at_csc_out);
     free(rowptr); free(colval); free(perm); free(a); free(at); free(d1); free(d2);
