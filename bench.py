@@ -1,0 +1,174 @@
+"""Benchmark: solve time to relative KKT 1e-6 on one MI355X.
+
+Default workload: the single large LP -- m = 1,000,000 rows (half =, half <=),
+n = 2,000,000 columns, ~20M nonzeros with power-law row lengths (Zipf 1.5, capped at
+5,000, rescaled to 20M), N(0,1) values, 0 <= x <= 10 on 30% of the columns; plain
+FP64 PDLP with Ruiz + Chambolle-Pock scaling and adaptive restarts, reltol 1e-6.
+One step = one full solve (scaling, step size, the PDLP loop to termination) with
+the LP already resident in HBM.  `value` is LPs solved per second (1 / solve time).
+
+Also reported: the roofline of the PDHG iteration (the two fused step kernels, timed
+live with HIP events on the solver's stream), per-kernel times, and the CPU oracle
+(FP64, OpenMP) on a bounded sample of the same LP on this host.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|plain]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "coolpdlp.jl_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def iteration_bytes(m, n, nnz, V=8, I=4):
+    """Algorithmic HBM bytes of one PDHG iteration (both step kernels):
+    primal kernel: A' (rowptr + col/val), gather y once, read x,c,lv,uv, write x+,xbar,
+                   read+write x_avg   -> 4(n+1) + nnz(I+V) + mV + 8nV
+    dual kernel:   A (rowptr + col/val), gather xbar once, read y,lc,uc, write y+,
+                   read+write y_avg   -> 4(m+1) + nnz(I+V) + nV + 6mV"""
+    primal = I * (n + 1) + nnz * (I + V) + m * V + 8 * n * V
+    dual = I * (m + 1) + nnz * (I + V) + n * V + 6 * m * V
+    return primal, dual
+
+
+def make_workload(name):
+    from coolpdlp_hip import generators as G
+    if name == "large":
+        g = G.power_law_lp()
+        desc = dict(workload="single_large_lp_power_law", m=g.m, n=g.n, nnz=g.nnz,
+                    row_length="zipf1.5_cap5000_rescaled", box_frac=0.3, reltol=1e-6,
+                    algorithm="PDLP", scaling="ruiz10+chambolle_pock", precision="fp64")
+        return g, 1e-6, desc
+    if name == "plain":
+        g = G.small_dense_lp()
+        desc = dict(workload="plain_lp_m2000_n5000", m=g.m, n=g.n, nnz=g.nnz, reltol=1e-6,
+                    algorithm="PDLP", precision="fp64")
+        return g, 1e-6, desc
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(g, gpu_passes, reltol, budget_passes):
+    """Oracle FP64 PDLP on this host: a bounded sample of `budget_passes` passes (plus
+    their checks), projected to the GPU solve's pass count."""
+    import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", cores))
+    threads = max(1, min(cores, omp))
+    O.set_threads(threads)
+    p = O.default_params(termination_reltol=reltol, max_kkt_passes=budget_passes, time_limit=1e9)
+    t = time.perf_counter()
+    r = O.solve(g, p, hist_cap=16)
+    wall = time.perf_counter() - t
+    loop = max(wall - r.stats.setup_seconds, 1e-9)
+    per_pass = loop / max(r.stats.kkt_passes, 1)
+    projected = r.stats.setup_seconds + per_pass * gpu_passes
+    return {
+        "value": 1.0 / projected, "unit": "LPs/s", "cores": threads, "kind": "port",
+        "sample": (f"oracle FP64 PDLP, {r.stats.kkt_passes} passes incl. their checks "
+                   f"({wall:.1f}s wall, setup {r.stats.setup_seconds:.1f}s, "
+                   f"{1e3 * per_pass:.1f} ms/pass) projected to the GPU solve's "
+                   f"{gpu_passes} passes: {projected:.0f}s per LP"),
+        "status_at_sample": r.status,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--workload", default="large")
+    ap.add_argument("--cpu-passes", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--time-limit", type=float, default=600.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
+              "multi-GPU path; see DESIGN.md for how the solve would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    import coolpdlp_hip as cp
+    import torch
+
+    t = time.perf_counter()
+    g, reltol, desc = make_workload(args.workload)
+    log(f"[bench] generated {desc['workload']} m={g.m} n={g.n} nnz={g.nnz} in "
+        f"{time.perf_counter() - t:.1f}s")
+    milp = cp.MILP.from_generated(g)
+    algo = cp.PDLP(termination_reltol=reltol, time_limit=args.time_limit,
+                   max_kkt_passes=10**7, device=0)
+    t = time.perf_counter()
+    eng = cp.Engine(milp, algo)
+    log(f"[bench] uploaded to HBM in {time.perf_counter() - t:.1f}s")
+
+    for w in range(args.warmup):
+        _, st = eng.solve()
+        log(f"[bench] warmup {w}: {st.termination_status.name} passes={st.kkt_passes} "
+            f"t={st.time_elapsed:.2f}s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for k in range(args.steps):
+        _, st = eng.solve()
+        stats.append(st)
+        log(f"[bench] step {k}: {st.termination_status.name} passes={st.kkt_passes} "
+            f"t={st.time_elapsed:.2f}s rel={st.err.relative():.3e}")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = stats[-1]
+    sol = eng.solution()
+    obj = cp.objective_value(sol.x, milp)
+
+    # roofline of the PDHG iteration, live (events around every chunk of steps)
+    bp, bd = iteration_bytes(g.m, g.n, g.nnz)
+    iters = st.step_kernel_launches // 2
+    t_iter_ms = st.step_kernel_ms / max(iters, 1)
+    achieved = (bp + bd) / (t_iter_ms * 1e-3) / 1e9
+    ms_p, ms_d = eng.profile_steps(20)
+    kernels = {
+        "primal_step": {"ms": ms_p, "bytes": bp, "GBs": bp / (ms_p * 1e-3) / 1e9},
+        "dual_step": {"ms": ms_d, "bytes": bd, "GBs": bd / (ms_d * 1e-3) / 1e9},
+    }
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "pdhg_iteration (primal_step + dual_step)",
+                "bytes_per_iteration": bp + bd, "ms_per_iteration": t_iter_ms,
+                "kernels": kernels}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(g, st.kkt_passes, reltol, args.cpu_passes)
+        log(f"[bench] cpu baseline: {cpu['sample']}")
+
+    value = args.steps / elapsed
+    out = {
+        "metric": "solve time to rel-KKT 1e-6; batched LPs/s + p50/p99 vs plain PDLP, 1 GPU",
+        "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator with a known optimum)",
+        "config": desc,
+        "solve_seconds": elapsed / args.steps,
+        "termination_status": st.termination_status.name, "kkt_passes": st.kkt_passes,
+        "relative_kkt": st.err.relative(), "restarts": st.restarts,
+        "objective": obj, "objective_closed_form": g.objective,
+        "objective_rel_err": abs(obj - g.objective) / (1 + abs(g.objective)),
+        "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

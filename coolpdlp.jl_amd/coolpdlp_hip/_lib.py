@@ -86,6 +86,7 @@ _SIGS = {
     "cpdlp_get_csr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cpdlp_get_step_size": (C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_double),
                                       _P(C.c_int64)]),
+    "cpdlp_profile_steps": (C.c_int, [C.c_void_p, C.c_int32, _P(C.c_double), _P(C.c_double)]),
     "cpdlp_destroy": (None, [C.c_void_p]),
     "cpdlp_last_error": (C.c_char_p, []),
     "cpdlp_version": (C.c_char_p, []),

@@ -248,6 +248,12 @@ class Engine:
         check(lib.cpdlp_get_csr(self.h, _ptr(rp), _ptr(col)))
         return rp, col[:nnz]
 
+    def profile_steps(self, reps=20):
+        """(ms per primal-step launch, ms per dual-step launch), HIP-event timed."""
+        a, b = C.c_double(), C.c_double()
+        check(lib.cpdlp_profile_steps(self.h, int(reps), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def step_size(self):
         eta, omega, it = C.c_double(), C.c_double(), C.c_int64()
         check(lib.cpdlp_get_step_size(self.h, C.byref(eta), C.byref(omega), C.byref(it)))

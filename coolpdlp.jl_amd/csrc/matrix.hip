@@ -91,7 +91,9 @@ int cpdlp_matrix_create(int64_t m, int64_t n, int32_t index_base, int32_t index_
         HostSide H;
         build_side(H, rows, cols, rp, col, transpose ? nullptr : &perm, M->row_of_int, nullptr, n_long);
         M->S.upload_structure(H, M->stream);
-        if (format != CPDLP_FMT_CSR) M->S.build_ell(H, M->stream);
+        // an explicit ELL is built whatever its padding (as GPUSparseMatrixELL does);
+        // AUTO only considers it within 1.5x nnz
+        if (format != CPDLP_FMT_CSR) M->S.build_ell(H, M->stream, format == CPDLP_FMT_ELL ? 1e30 : 1.5);
         DevBuf<double> src(std::max<int64_t>(nnz, 1));
         src.upload(nzval, nnz, M->stream);
         M->S.val64.alloc(std::max<int64_t>(nnz, 1));
@@ -123,8 +125,6 @@ int cpdlp_matrix_create(int64_t m, int64_t n, int32_t index_base, int32_t index_
         // format choice
         M->S.format = CPDLP_FMT_CSR;
         if (format == CPDLP_FMT_ELL && M->S.ell_w > 0) M->S.format = CPDLP_FMT_ELL;
-        if (format == CPDLP_FMT_ELL && M->S.ell_w <= 0 && rows > 0)
-            throw Error(CPDLP_ERR_ARGUMENT, "ELL padding exceeds 1.5x nnz for this matrix");
         if (format == CPDLP_FMT_AUTO && M->S.ell_w > 0) {
             DevBuf<double> b(std::max<int64_t>(cols, 1)), c(std::max<int64_t>(rows, 1));
             hipLaunchKernelGGL(k_fill_m, dim3(std::max(1, ceil_div(cols, BLOCK))), dim3(BLOCK), 0,

@@ -339,9 +339,9 @@ void DevSide::fill_values(const double* src_csc, bool want32, hipStream_t s) {
     HIP_CHECK(hipGetLastError());
 }
 
-void DevSide::build_ell(const HostSide& S, hipStream_t s) {
+void DevSide::build_ell(const HostSide& S, hipStream_t s, double max_fill) {
     std::vector<int32_t> ecol, evmap;
-    const int32_t w = cpdlp::build_ell(S, 1.5, ecol, evmap);
+    const int32_t w = cpdlp::build_ell(S, max_fill, ecol, evmap);
     if (w <= 0) {
         ell_w = 0;
         return;
@@ -439,8 +439,11 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
     A.upload_structure(L.A, s);
     At.upload_structure(L.At, s);
     if (P.format == CPDLP_FMT_ELL || P.format == CPDLP_FMT_AUTO) {
-        A.build_ell(L.A, s);
-        At.build_ell(L.At, s);
+        // AUTO considers ELL only when padding stays within 1.5x nnz; an explicit ELL
+        // request accepts up to 8x (beyond that the side stays CSR, see stats.format_*)
+        const double fill = P.format == CPDLP_FMT_ELL ? 8.0 : 1.5;
+        A.build_ell(L.A, s, fill);
+        At.build_ell(L.At, s, fill);
     }
     HIP_CHECK(hipStreamSynchronize(s));
     // free the big host-side copies that are no longer needed
@@ -966,7 +969,7 @@ void Handle::solve(const double* x0, const double* y0) {
     history.clear();
     stats.err = cpdlp_kkt{NAN, NAN, NAN, NAN, NAN, NAN};
     stats.termination_status = CPDLP_STILL_RUNNING;
-    if (!prepared) prepare();
+    prepare();   // solve = preprocess + initialize + solve!, every call (common.jl:186-201)
     // start point (unscaled, original order) on the device
     DevBuf<double> x0d(std::max<int64_t>(n, 1)), y0d(std::max<int64_t>(m, 1));
     if (x0) x0d.upload(x0, n, stream);
@@ -1068,6 +1071,74 @@ void Handle::get_solution(double* x, double* y) {
     if (x) xo.download(x, n, stream);
     if (y) yo.download(y, m, stream);
     sync();
+}
+
+// Device time per launch of the two step kernels, each timed alone with HIP events
+// on the handle's stream, on a scratch copy of the current iterates (the solve's
+// state is not touched).  Uses the coefficients of the last chunk.
+template <class V>
+void Handle::profile_steps_t(IterState<V>& S, int reps, double* ms_primal, double* ms_dual) {
+    IterState<V> T;
+    T.alloc(n, m);
+    const int c = S.cur;
+    if (n) {
+        HIP_CHECK(hipMemcpyAsync(T.x[0].p, S.x[c].p, n * sizeof(V), hipMemcpyDeviceToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(T.xavg.p, S.xavg.p, n * sizeof(V), hipMemcpyDeviceToDevice, stream));
+    }
+    if (m) {
+        HIP_CHECK(hipMemcpyAsync(T.y[0].p, S.y[c].p, m * sizeof(V), hipMemcpyDeviceToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(T.yavg.p, S.yavg.p, m * sizeof(V), hipMemcpyDeviceToDevice, stream));
+    }
+    if (d_coef.n == 0) throw Error(CPDLP_ERR_STATE, "no step coefficients: call cpdlp_solve first");
+    const bool pdlp = P.algorithm == CPDLP_ALGO_PDLP;
+    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
+    const bool f64 = std::is_same<V, double>::value;
+    EpiPrimal<V> ep;
+    ep.x = T.x[0].p;
+    ep.c = f64 ? (const V*)c_s.p : (const V*)c_32.p;
+    ep.lv = f64 ? (const V*)lv_s.p : (const V*)lv_32.p;
+    ep.uv = f64 ? (const V*)uv_s.p : (const V*)uv_32.p;
+    ep.xn = T.x[1].p;
+    ep.xbar = T.xbar.p;
+    ep.xavg = T.xavg.p;
+    ep.xavgl = T.xavgl.p;
+    ep.coef = d_coef.p;
+    ep.step = 0;
+    ep.pdlp = pdlp;
+    EpiDual<V> ed;
+    ed.y = T.y[0].p;
+    ed.lc = f64 ? (const V*)lc_s.p : (const V*)lc_32.p;
+    ed.uc = f64 ? (const V*)uc_s.p : (const V*)uc_32.p;
+    ed.yn = T.y[1].p;
+    ed.yavg = T.yavg.p;
+    ed.yavgl = T.yavgl.p;
+    ed.coef = d_coef.p;
+    ed.step = 0;
+    ed.pdlp = pdlp;
+    launch_spmv<V, V, 1>(vAt, XPack<V, 1>{{T.y[0].p}}, ep, stream);   // warm-up
+    launch_spmv<V, V, 1>(vA, XPack<V, 1>{{T.xbar.p}}, ed, stream);
+    float ms = 0.f;
+    HIP_CHECK(hipEventRecord(ev0, stream));
+    for (int r = 0; r < reps; ++r) launch_spmv<V, V, 1>(vAt, XPack<V, 1>{{T.y[0].p}}, ep, stream);
+    HIP_CHECK(hipEventRecord(ev1, stream));
+    HIP_CHECK(hipEventSynchronize(ev1));
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    *ms_primal = ms / reps;
+    HIP_CHECK(hipEventRecord(ev0, stream));
+    for (int r = 0; r < reps; ++r) launch_spmv<V, V, 1>(vA, XPack<V, 1>{{T.xbar.p}}, ed, stream);
+    HIP_CHECK(hipEventRecord(ev1, stream));
+    HIP_CHECK(hipEventSynchronize(ev1));
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    *ms_dual = ms / reps;
+    HIP_CHECK(hipGetLastError());
+}
+
+void Handle::profile_steps(int reps, double* ms_primal, double* ms_dual) {
+    HIP_CHECK(hipSetDevice(device));
+    if (!solved) throw Error(CPDLP_ERR_STATE, "call cpdlp_solve first");
+    if (reps <= 0) throw Error(CPDLP_ERR_ARGUMENT, "reps must be positive");
+    if (active64) profile_steps_t<double>(st64, reps, ms_primal, ms_dual);
+    else profile_steps_t<float>(st32, reps, ms_primal, ms_dual);
 }
 
 cpdlp_kkt Handle::kkt_host(const double* x, const double* y) {
@@ -1223,6 +1294,14 @@ int cpdlp_get_step_size(cpdlp_handle* h, double* eta, double* omega, int64_t* po
         if (powm_iters) *powm_iters = H.powm_iters;
     });
 }
+int cpdlp_profile_steps(cpdlp_handle* h, int32_t reps, double* ms_primal, double* ms_dual) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        if (!ms_primal || !ms_dual) throw Error(CPDLP_ERR_ARGUMENT, "NULL output");
+        h->h->profile_steps(reps, ms_primal, ms_dual);
+    });
+}
+
 void cpdlp_destroy(cpdlp_handle* h) { delete h; }
 
 const char* cpdlp_last_error(void) { return g_last_error.c_str(); }

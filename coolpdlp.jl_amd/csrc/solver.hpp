@@ -30,7 +30,7 @@ struct DevSide {
     template <class V>
     MatView<V> view() const;
     void upload_structure(const HostSide& S, hipStream_t s);
-    void build_ell(const HostSide& S, hipStream_t s);
+    void build_ell(const HostSide& S, hipStream_t s, double max_fill);
     void fill_values(const double* src_csc, bool want32, hipStream_t s);
 };
 
@@ -108,6 +108,9 @@ struct Handle {
     void solve(const double* x0, const double* y0);
     void get_solution(double* x, double* y);
     cpdlp_kkt kkt_host(const double* x, const double* y);
+    template <class V>
+    void profile_steps_t(IterState<V>& S, int reps, double* ms_primal, double* ms_dual);
+    void profile_steps(int reps, double* ms_primal, double* ms_dual);
 };
 
 }  // namespace cpdlp

@@ -171,6 +171,12 @@ int cpdlp_get_csr(cpdlp_handle* h, int64_t* rowptr, int32_t* colval);
 /* eta (after cpdlp_prepare or cpdlp_solve) and power-method iterations. */
 int cpdlp_get_step_size(cpdlp_handle* h, double* eta, double* omega, int64_t* powm_iters);
 
+/* Measurement hook (no reference counterpart): average device milliseconds per launch
+ * of the primal half-step kernel (A'y + projection + extrapolation + averaging) and of
+ * the dual half-step kernel (A xbar + projection + averaging), each timed alone with
+ * HIP events over `reps` launches on a scratch copy of the iterates.  After a solve. */
+int cpdlp_profile_steps(cpdlp_handle* h, int32_t reps, double* ms_primal, double* ms_dual);
+
 void cpdlp_destroy(cpdlp_handle* h);
 
 /* thread-local message of the last failing call on this thread ("" if none) */

@@ -532,6 +532,7 @@ int orc_solve(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval
     }
     double eta_sum = 0.0;
     int64_t inner = 0, total = 0;
+    st->setup_seconds = now_s() - t_start;
 
     /* early exit (common.jl:194-198) */
     int all_zero_c = 1;

@@ -58,6 +58,7 @@ typedef struct {
     int64_t powm_iters;
     int64_t restarts;
     int64_t n_history;              /* total entries produced (may exceed capacity) */
+    double setup_seconds;           /* scaling + step size (before the first step) */
 } orc_stats;
 
 void orc_default_params(orc_params* p, int32_t algorithm);

@@ -65,7 +65,7 @@ class Stats(C.Structure):
     _fields_ = [("err", KKT), ("time_elapsed", C.c_double), ("kkt_passes", C.c_int64),
                 ("termination_status", C.c_int32), ("pad", C.c_int32),
                 ("eta", C.c_double), ("omega", C.c_double), ("powm_iters", C.c_int64),
-                ("restarts", C.c_int64), ("n_history", C.c_int64)]
+                ("restarts", C.c_int64), ("n_history", C.c_int64), ("setup_seconds", C.c_double)]
 
 
 _lib = None

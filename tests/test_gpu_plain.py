@@ -50,11 +50,7 @@ def test_mul_matches_dense(cp, fmt):
     for A in _rand_mats():
         cls = cp.GPUSparseMatrixELL if fmt == "ell" else cp.GPUSparseMatrixCSR
         kw = {} if fmt != "auto" else {"format": 0}
-        try:
-            M = cls(A, **kw)
-        except cp.ArgumentError:
-            assert fmt == "ell"   # ELL refused: padding above 1.5x nnz
-            continue
+        M = cls(A, **kw)
         assert M.shape == A.shape and M.nnz == A.nnz
         b = rng.random(A.shape[1])
         c = rng.random(A.shape[0])
