@@ -91,6 +91,12 @@ def main():
     ap.add_argument("--cpu-passes", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-limit", type=float, default=600.0)
+    ap.add_argument("--powm-tol", type=float, default=1e-4)
+    # profiling runs only: cap the solve at this many passes (same kernels, fewer of
+    # them); the default solves to the tolerance
+    ap.add_argument("--max-passes", type=int, default=10**7)
+    # PMC calibration: also run a plain 512 MiB device-to-device copy kernel 5 times
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
@@ -105,8 +111,12 @@ def main():
     log(f"[bench] generated {desc['workload']} m={g.m} n={g.n} nnz={g.nnz} in "
         f"{time.perf_counter() - t:.1f}s")
     milp = cp.MILP.from_generated(g)
+    # powm_tol: the reference's power-method rule (tol = eps*n^3) stops after one
+    # Rayleigh quotient of a random vector at n = 2e6, which underestimates ||A|| and
+    # makes eta*||A|| > 1; the benchmark converges the power method to 1e-4 instead
     algo = cp.PDLP(termination_reltol=reltol, time_limit=args.time_limit,
-                   max_kkt_passes=10**7, device=0)
+                   max_kkt_passes=args.max_passes, device=0, powm_tol=args.powm_tol,
+                   show_progress=1)
     t = time.perf_counter()
     eng = cp.Engine(milp, algo)
     log(f"[bench] uploaded to HBM in {time.perf_counter() - t:.1f}s")
@@ -145,6 +155,14 @@ def main():
                 "bytes_per_iteration": bp + bd, "ms_per_iteration": t_iter_ms,
                 "kernels": kernels}
 
+    if args.calibrate_copy:
+        a = torch.empty(64 * 2**20, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 5 x 512 MiB read + 512 MiB written")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
@@ -164,7 +182,10 @@ def main():
         "relative_kkt": st.err.relative(), "restarts": st.restarts,
         "objective": obj, "objective_closed_form": g.objective,
         "objective_rel_err": abs(obj - g.objective) / (1 + abs(g.objective)),
+        "formats": {"A": ["auto", "csr", "ell", "xslice"][st.format_A],
+                    "At": ["auto", "csr", "ell", "xslice"][st.format_At]},
         "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
+        "powm_tol": args.powm_tol,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

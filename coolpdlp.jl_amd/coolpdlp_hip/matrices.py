@@ -110,12 +110,16 @@ class GPUSparseMatrixELL(_GPUSparseMatrix):
 
 
 def format_of(matrix_type) -> int:
-    if matrix_type is None or matrix_type in ("csr", "CSR"):
-        return _lib.FMT_CSR
-    if matrix_type in ("auto", "AUTO"):
+    """Matrix type of Algorithm(T, Ti, M): None / "auto" lets the engine's autotuner pick
+    per matrix (CSR, ELL or XCD column slices); the others force one format."""
+    if matrix_type is None or matrix_type in ("auto", "AUTO"):
         return _lib.FMT_AUTO
+    if matrix_type in ("csr", "CSR"):
+        return _lib.FMT_CSR
     if matrix_type in ("ell", "ELL"):
         return _lib.FMT_ELL
+    if matrix_type in ("xslice", "XSLICE"):
+        return _lib.FMT_XSLICE
     if isinstance(matrix_type, type) and issubclass(matrix_type, _GPUSparseMatrix):
         return matrix_type._format
     if matrix_type in ("SparseMatrixCSC",):

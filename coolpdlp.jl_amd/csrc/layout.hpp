@@ -194,4 +194,71 @@ inline int32_t build_ell(const HostSide& S, double max_fill, std::vector<int32_t
     return w;
 }
 
+// XCD column slices of an internal side: ns sub-CSRs over all rows, slice s holding
+// the entries whose (internal) column lies in [s*W, (s+1)*W), W = ceil(cols/ns), in
+// the row's original entry order.  rowptr: ns*(rows+1) absolute offsets.  Rows
+// [0, n_long) use the wavefront path in every slice; the rest is cut into short tiles
+// per slice (<= BLOCK rows, <= TILE_NNZ nnz).
+struct HostSlices {
+    int32_t ns = 0, max_tiles = 0;
+    std::vector<int32_t> rowptr, col, vmap, tile_ptr, tile_off;
+};
+
+inline void build_slices(const HostSide& S, int32_t ns, HostSlices& H) {
+    const int32_t rows = S.rows;
+    const int64_t W = std::max<int64_t>(1, (S.cols + ns - 1) / ns);
+    H.ns = ns;
+    H.rowptr.assign((size_t)ns * (rows + 1), 0);
+    std::vector<int32_t> cnt((size_t)ns * rows, 0);
+    for (int32_t r = 0; r < rows; ++r)
+        for (int32_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k)
+            cnt[(size_t)(S.col[k] / W) * rows + r]++;
+    int64_t off = 0;
+    for (int32_t sl = 0; sl < ns; ++sl) {
+        int32_t* rp = H.rowptr.data() + (size_t)sl * (rows + 1);
+        for (int32_t r = 0; r < rows; ++r) {
+            rp[r] = (int32_t)off;
+            off += cnt[(size_t)sl * rows + r];
+        }
+        rp[rows] = (int32_t)off;
+    }
+    H.col.resize(S.nnz);
+    H.vmap.resize(S.nnz);
+    std::vector<int32_t> next(H.rowptr);
+    for (int32_t r = 0; r < rows; ++r)
+        for (int32_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k) {
+            const int32_t sl = (int32_t)(S.col[k] / W);
+            const int32_t d = next[(size_t)sl * (rows + 1) + r]++;
+            H.col[d] = S.col[k];
+            H.vmap[d] = S.vmap[k];
+        }
+    // per slice: tile row boundaries (nt + 1 entries, the last one = rows);
+    // tile_off[s] = index of slice s's first boundary, tile_off[ns] = total size, so
+    // slice s has tile_off[s+1] - tile_off[s] - 1 tiles
+    H.tile_ptr.clear();
+    H.tile_off.assign(ns + 1, 0);
+    H.max_tiles = 0;
+    for (int32_t sl = 0; sl < ns; ++sl) {
+        const int32_t* rp = H.rowptr.data() + (size_t)sl * (rows + 1);
+        H.tile_off[sl] = (int32_t)H.tile_ptr.size();
+        int32_t r = S.n_long, nt = 0;
+        H.tile_ptr.push_back(r);
+        while (r < rows) {
+            int32_t rows_in = 0;
+            int64_t nz = 0;
+            while (r < rows && rows_in < BLOCK) {
+                const int64_t L = rp[r + 1] - rp[r];
+                if (rows_in > 0 && nz + L > TILE_NNZ) break;
+                nz += L;
+                ++rows_in;
+                ++r;
+            }
+            H.tile_ptr.push_back(r);
+            ++nt;
+        }
+        H.max_tiles = std::max(H.max_tiles, nt);
+    }
+    H.tile_off[ns] = (int32_t)H.tile_ptr.size();
+}
+
 }  // namespace cpdlp

@@ -275,7 +275,7 @@ __global__ void k_restart(int64_t len, int32_t from_avg, V* sol, V* avg, V* rst)
 // ================================================================ device side of a matrix
 
 template <class V>
-MatView<V> DevSide::view() const {
+MatView<V> DevSide::view(bool allow_slices) const {
     MatView<V> v;
     v.rows = rows;
     v.rowptr = rowptr.p;
@@ -289,6 +289,15 @@ MatView<V> DevSide::view() const {
     v.ell_width = ell_w;
     v.ell_col = ell_col.p;
     v.ell_val = ell_vals<V>();
+    v.ns = sl_ns;
+    v.sl_rowptr = sl_rowptr.p;
+    v.sl_col = sl_col.p;
+    v.sl_val = sl_vals<V>();
+    v.sl_tile_ptr = sl_tile_ptr.p;
+    v.sl_tile_off = sl_tile_off.p;
+    v.sl_blocks = sl_blocks;
+    v.sl_part = sl_part.p;
+    if (format == CPDLP_FMT_XSLICE && !allow_slices) v.format = CPDLP_FMT_CSR;
     return v;
 }
 template <>
@@ -299,6 +308,28 @@ template <>
 const double* DevSide::ell_vals<double>() const { return ell_val64.p; }
 template <>
 const float* DevSide::ell_vals<float>() const { return ell_val32.p; }
+template <>
+const double* DevSide::sl_vals<double>() const { return sl_val64.p; }
+template <>
+const float* DevSide::sl_vals<float>() const { return sl_val32.p; }
+
+// NS = 8 column slices, one per XCD of the MI355X
+void DevSide::build_slices(const HostSide& S, hipStream_t s) {
+    constexpr int32_t NS = 8;
+    HostSlices H;
+    cpdlp::build_slices(S, NS, H);
+    sl_ns = NS;
+    sl_blocks = ceil_div(S.n_long, LONG_ROWS_PER_BLOCK) + H.max_tiles;
+    sl_rowptr.upload(H.rowptr, s);
+    sl_col.alloc(std::max<size_t>(H.col.size(), 1));
+    sl_col.upload(H.col.data(), H.col.size(), s);
+    sl_vmap.alloc(std::max<size_t>(H.vmap.size(), 1));
+    sl_vmap.upload(H.vmap.data(), H.vmap.size(), s);
+    sl_tile_ptr.upload(H.tile_ptr, s);
+    sl_tile_off.upload(H.tile_off, s);
+    sl_part.alloc(std::max<int64_t>((int64_t)NS * S.rows, 1));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
 
 void DevSide::upload_structure(const HostSide& S, hipStream_t s) {
     rows = S.rows;
@@ -324,6 +355,16 @@ void DevSide::fill_values(const double* src_csc, bool want32, hipStream_t s) {
         if (nnz)
             hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
                                nnz, vmap.p, src_csc, val32.p);
+    }
+    if (sl_ns > 0 && nnz > 0) {
+        if (sl_val64.n < (size_t)nnz) sl_val64.alloc(nnz);
+        hipLaunchKernelGGL(k_gather_vals<double>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
+                           nnz, sl_vmap.p, src_csc, sl_val64.p);
+        if (want32) {
+            if (sl_val32.n < (size_t)nnz) sl_val32.alloc(nnz);
+            hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
+                               nnz, sl_vmap.p, src_csc, sl_val32.p);
+        }
     }
     if (ell_w > 0) {
         const int64_t len = (int64_t)ell_w * rows;
@@ -352,8 +393,8 @@ void DevSide::build_ell(const HostSide& S, hipStream_t s, double max_fill) {
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
-template MatView<double> DevSide::view<double>() const;
-template MatView<float> DevSide::view<float>() const;
+template MatView<double> DevSide::view<double>(bool) const;
+template MatView<float> DevSide::view<float>(bool) const;
 
 // ================================================================ iterate state
 
@@ -444,6 +485,10 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         const double fill = P.format == CPDLP_FMT_ELL ? 8.0 : 1.5;
         A.build_ell(L.A, s, fill);
         At.build_ell(L.At, s, fill);
+    }
+    if (P.format == CPDLP_FMT_XSLICE || P.format == CPDLP_FMT_AUTO) {
+        A.build_slices(L.A, s);
+        At.build_slices(L.At, s);
     }
     HIP_CHECK(hipStreamSynchronize(s));
     // free the big host-side copies that are no longer needed
@@ -576,7 +621,7 @@ void Handle::power_method(double tol, int64_t maxiter) {
     st.upload(&h0, 1, s);
     const int gn = grid_n();
     const int nbr = std::min(gn, 1024);
-    const int gA = A.view<V>().grid(), gAt = At.view<V>().grid();
+    const int gA = A.view<V>(false).grid(), gAt = At.view<V>(false).grid();
     ensure_partials((size_t)std::max({gA, gAt, nbr}) * 2 + 16);
     hipLaunchKernelGGL(k_powm_start<V>, dim3(gn), dim3(BLOCK), 0, s, nn, d_col_of_int.p,
                        P.powm_seed, x.p);
@@ -585,7 +630,7 @@ void Handle::power_method(double tol, int64_t maxiter) {
                        d_sums.p, (const PowState*)nullptr);
     hipLaunchKernelGGL(k_scale_by_inv_sqrt<V>, dim3(gn), dim3(BLOCK), 0, s, nn, x.p,
                        (const double*)d_sums.p);
-    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
+    const MatView<V> vA = A.view<V>(false), vAt = At.view<V>(false);
     PowState hs{};
     for (;;) {
         for (int c = 0; c < 32; ++c) {
@@ -622,31 +667,47 @@ void Handle::power_method(double tol, int64_t maxiter) {
 void Handle::choose_formats() {
     for (DevSide* S : {&A, &At}) {
         S->format = CPDLP_FMT_CSR;
-        if (P.format == CPDLP_FMT_CSR || S->ell_w <= 0) continue;
+        if (P.format == CPDLP_FMT_CSR) continue;
         if (P.format == CPDLP_FMT_ELL) {
-            S->format = CPDLP_FMT_ELL;
+            if (S->ell_w > 0) S->format = CPDLP_FMT_ELL;
             continue;
         }
+        if (P.format == CPDLP_FMT_XSLICE) {
+            if (S->sl_ns > 0) S->format = CPDLP_FMT_XSLICE;
+            continue;
+        }
+        // AUTO: time the plain product (K = 1, store epilogue) of every available
+        // format; another format replaces CSR only when it is >= 10% faster
+        std::vector<int32_t> cand{CPDLP_FMT_CSR};
+        if (S->ell_w > 0) cand.push_back(CPDLP_FMT_ELL);
+        if (S->sl_ns > 0) cand.push_back(CPDLP_FMT_XSLICE);
+        if (cand.size() == 1) continue;
         DevBuf<double> xin(std::max<int64_t>(S->cols, 1)), out(std::max<int64_t>(S->rows, 1));
         hipLaunchKernelGGL(k_fill, dim3(std::max(1, ceil_div(S->cols, BLOCK))), dim3(BLOCK), 0,
                            stream, (int64_t)S->cols, xin.p, 1.0);
-        float best[2] = {0.f, 0.f};
-        for (int f = 0; f < 2; ++f) {
-            S->format = f == 0 ? CPDLP_FMT_CSR : CPDLP_FMT_ELL;
+        float best_ms = 0.f;
+        int32_t best = CPDLP_FMT_CSR;
+        for (int32_t f : cand) {
+            S->format = f;
             const MatView<double> v = S->view<double>();
             EpiStore<double> e;
             e.out = out.p;
             e.st = nullptr;
             for (int rep = 0; rep < 3; ++rep)   // warm-up
                 launch_spmv<double, double, 1>(v, XPack<double, 1>{{xin.p}}, e, stream);
+            float ms = 0.f;
             HIP_CHECK(hipEventRecord(ev0, stream));
             for (int rep = 0; rep < 10; ++rep)
                 launch_spmv<double, double, 1>(v, XPack<double, 1>{{xin.p}}, e, stream);
             HIP_CHECK(hipEventRecord(ev1, stream));
             HIP_CHECK(hipEventSynchronize(ev1));
-            HIP_CHECK(hipEventElapsedTime(&best[f], ev0, ev1));
+            HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            if (f == CPDLP_FMT_CSR || ms < 0.9f * best_ms) {
+                best_ms = ms;
+                best = f;
+            }
         }
-        S->format = best[1] < best[0] ? CPDLP_FMT_ELL : CPDLP_FMT_CSR;
+        S->format = best;
     }
 }
 
@@ -678,7 +739,7 @@ template <class V, int K>
 void Handle::kkt_launch(const V* const* X, const V* const* Y, const V* xr, const V* yr, bool dist,
                         double* host_sums) {
     const hipStream_t s = stream;
-    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
+    const MatView<V> vA = A.view<V>(false), vAt = At.view<V>(false);
     const int gA = vA.grid(), gAt = vAt.grid();
     constexpr int NQR = EpiKktRows<V, K>::NQ, NQC = EpiKktCols<V, K>::NQ;
     ensure_partials((size_t)std::max(gA * NQR, gAt * NQC) + 16);
@@ -876,7 +937,25 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase) {
         stats.err = kkt_from_sums(sums, K, 0);
         push_history();
         stats.termination_status = term_status(stats.err);
+        if (P.show_progress) progress(false);
+        if (stats.termination_status == CPDLP_STILL_RUNNING && !final_phase) {
+            // mixed precision, FP32 phase: keep the best iterate; hand over to FP64 when
+            // the relative KKT error has not improved for STALL_CHECKS checks or has
+            // grown 100x past its best (FP32 rounding floor reached)
+            const double rel = relative(stats.err);
+            if (rel < best_rel) {
+                best_rel = rel;
+                best_omega = omega;
+                stall = 0;
+                if (n) HIP_CHECK(hipMemcpyAsync(S.tmp_n.p, S.x[c].p, n * sizeof(V), hipMemcpyDeviceToDevice, stream));
+                if (m) HIP_CHECK(hipMemcpyAsync(S.tmp_m.p, S.y[c].p, m * sizeof(V), hipMemcpyDeviceToDevice, stream));
+            } else if (++stall >= STALL_CHECKS || rel > 100.0 * best_rel) {
+                phase_switch = true;
+                return;
+            }
+        }
         if (stats.termination_status != CPDLP_STILL_RUNNING) {
+            if (P.show_progress) progress(true);
             if (final_phase && pdlp) {   // second check after the inner loop, pdlp.jl:88-90
                 stats.time_elapsed = now_s() - t_start;
                 push_history();
@@ -919,6 +998,20 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase) {
     }
 }
 
+// show_progress: one line on stderr at most every 2 s and at termination (the
+// reference's ProgressMeter line, src/algorithms/common.jl:134-145)
+void Handle::progress(bool final_line) {
+    const double t = now_s();
+    if (!final_line && t - last_progress < 2.0) return;
+    last_progress = t;
+    const cpdlp_kkt& e = stats.err;
+    std::fprintf(stderr, "%s iterations: %lld  primal %.3e  dual %.3e  gap %.3e  (%.1f s)%s\n",
+                 P.algorithm == CPDLP_ALGO_PDLP ? "PDLP" : "PDHG", (long long)stats.kkt_passes,
+                 e.primal / e.primal_scale, e.dual / e.dual_scale, e.gap / e.gap_scale,
+                 stats.time_elapsed, final_line ? "  done" : "");
+    std::fflush(stderr);
+}
+
 template <class V>
 void Handle::init_state(IterState<V>& S, const double* x0_dev, const double* y0_dev) {
     const hipStream_t s = stream;
@@ -947,14 +1040,14 @@ void Handle::init_state(IterState<V>& S, const double* x0_dev, const double* y0_
     HIP_CHECK(hipGetLastError());
 }
 
-// FP32 phase -> FP64 polish: the FP64 state restarts from the FP32 phase's sol
+// FP32 phase -> FP64 polish: the FP64 state restarts (sol = avg = restart) from the
+// FP32 phase's best iterate (kept in tmp_n / tmp_m)
 static void upcast(IterState<float>& a, IterState<double>& b, int64_t n, int64_t m, hipStream_t s) {
     const int gn = std::max(1, ceil_div(n, BLOCK)), gm = std::max(1, ceil_div(m, BLOCK));
-    const int c = a.cur;
     for (DevBuf<double>* t : {&b.x[0], &b.xavg, &b.xrst})
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(gn), dim3(BLOCK), 0, s, n, (const float*)a.x[c].p, t->p);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gn), dim3(BLOCK), 0, s, n, (const float*)a.tmp_n.p, t->p);
     for (DevBuf<double>* t : {&b.y[0], &b.yavg, &b.yrst})
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(gm), dim3(BLOCK), 0, s, m, (const float*)a.y[c].p, t->p);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gm), dim3(BLOCK), 0, s, m, (const float*)a.tmp_m.p, t->p);
     hipLaunchKernelGGL(k_fill_t<double>, dim3(gn), dim3(BLOCK), 0, s, n, b.x[1].p, 0.0);
     hipLaunchKernelGGL(k_fill_t<double>, dim3(gm), dim3(BLOCK), 0, s, m, b.y[1].p, 0.0);
     hipLaunchKernelGGL(k_fill_t<double>, dim3(gn), dim3(BLOCK), 0, s, n, b.xavgl.p, 0.0);
@@ -965,6 +1058,7 @@ static void upcast(IterState<float>& a, IterState<double>& b, int64_t n, int64_t
 void Handle::solve(const double* x0, const double* y0) {
     HIP_CHECK(hipSetDevice(device));
     t_start = now_s();
+    last_progress = t_start;
     stats = cpdlp_stats{};
     history.clear();
     stats.err = cpdlp_kkt{NAN, NAN, NAN, NAN, NAN, NAN};
@@ -1022,13 +1116,26 @@ void Handle::solve(const double* x0, const double* y0) {
         active64 = false;
         const bool polish = P.precision == CPDLP_PREC_MIXED;
         const double r32 = polish ? std::max(P.polish_reltol, P.termination_reltol) : P.termination_reltol;
+        best_rel = INFINITY;
+        best_omega = omega;
+        stall = 0;
+        phase_switch = false;
         loop<float>(st32, r32, !polish);
         fp32_passes = stats.kkt_passes;
-        if (polish && stats.termination_status == CPDLP_OPTIMAL && r32 > P.termination_reltol) {
+        const bool reached = stats.termination_status == CPDLP_OPTIMAL && r32 > P.termination_reltol;
+        if (polish && (reached || phase_switch)) {
+            if (reached) {   // the terminating check's sol is the hand-over point
+                const int c = st32.cur;
+                if (n) HIP_CHECK(hipMemcpyAsync(st32.tmp_n.p, st32.x[c].p, n * 4, hipMemcpyDeviceToDevice, stream));
+                if (m) HIP_CHECK(hipMemcpyAsync(st32.tmp_m.p, st32.y[c].p, m * 4, hipMemcpyDeviceToDevice, stream));
+            } else {
+                omega = best_omega;
+            }
             st64.alloc(n, m);
             upcast(st32, st64, n, m, stream);
             eta_sum = 0.0;
             inner = 0;
+            total = 0;
             active64 = true;
             stats.termination_status = CPDLP_STILL_RUNNING;
             loop<double>(st64, P.termination_reltol, true);
@@ -1193,7 +1300,7 @@ int cpdlp_default_params(cpdlp_params* p, int32_t algorithm) {
         p->termination_reltol = 1e-4;
         p->max_kkt_passes = 100000;
         p->time_limit = 100.0;
-        p->format = CPDLP_FMT_CSR;
+        p->format = CPDLP_FMT_AUTO;
         p->device = 0;
         p->powm_tol = NAN;
         p->powm_maxiter = -1;

@@ -22,14 +22,24 @@ struct DevSide {
     DevBuf<int32_t> ell_col, ell_vmap;
     DevBuf<double> ell_val64;
     DevBuf<float> ell_val32;
+    // XCD column slices (step kernels only)
+    int32_t sl_ns = 0, sl_blocks = 0;
+    DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_ptr, sl_tile_off;
+    DevBuf<double> sl_val64, sl_part;
+    DevBuf<float> sl_val32;
 
     template <class V>
     const V* vals() const;
     template <class V>
     const V* ell_vals() const;
     template <class V>
-    MatView<V> view() const;
+    const V* sl_vals() const;
+    // allow_slices = false: the KKT / power-method products (K > 1 or block-partial
+    // reductions) use the unsliced CSR instead of the XCD slices
+    template <class V>
+    MatView<V> view(bool allow_slices = true) const;
     void upload_structure(const HostSide& S, hipStream_t s);
+    void build_slices(const HostSide& S, hipStream_t s);
     void build_ell(const HostSide& S, hipStream_t s, double max_fill);
     void fill_values(const double* src_csc, bool want32, hipStream_t s);
 };
@@ -76,6 +86,11 @@ struct Handle {
     double primal_scale = 1, dual_scale = 1, c_norm = 0, comb_norm = 0;
     int64_t powm_iters = 0, inner = 0, total = 0, restarts = 0, step_launches = 0, fp32_passes = 0;
     double setup_seconds = 0, loop_seconds = 0, step_ms = 0, t_start = 0;
+    // mixed precision hand-over (FP32 phase -> FP64 polish)
+    static constexpr int STALL_CHECKS = 10;
+    double best_rel = INFINITY, best_omega = 1.0, last_progress = 0.0;
+    int stall = 0;
+    bool phase_switch = false;
     IterState<double> st64;
     IterState<float> st32;
     cpdlp_stats stats{};
@@ -106,6 +121,7 @@ struct Handle {
     template <class V>
     void init_state(IterState<V>& S, const double* x0_dev, const double* y0_dev);
     void solve(const double* x0, const double* y0);
+    void progress(bool final_line);
     void get_solution(double* x, double* y);
     cpdlp_kkt kkt_host(const double* x, const double* y);
     template <class V>

@@ -42,9 +42,22 @@ struct MatView {
     int32_t ell_width = 0;
     const int32_t* ell_col = nullptr;
     const V* ell_val = nullptr;
+    // XCD column slices (format 3): NS sub-CSRs over all rows, one per column slice;
+    // slice s owns the blocks with blockIdx % NS == s (workgroups are dispatched to
+    // the XCDs round-robin, so each XCD's L2 holds one slice of the gathered vector)
+    int32_t ns = 0;
+    const int32_t* sl_rowptr = nullptr;   // ns * (rows + 1), absolute offsets
+    const int32_t* sl_col = nullptr;
+    const V* sl_val = nullptr;
+    const int32_t* sl_tile_ptr = nullptr; // row boundaries of slice s's tiles, from sl_tile_off[s]
+    const int32_t* sl_tile_off = nullptr; // ns + 1; slice s has off[s+1] - off[s] - 1 tiles
+    int32_t sl_blocks = 0;                // blocks per slice (long blocks + max tiles)
+    void* sl_part = nullptr;              // K * ns * rows partial sums (Acc)
 
     int grid() const {
-        return format == CPDLP_FMT_ELL ? ceil_div(rows, BLOCK) : n_long_blocks + n_tiles;
+        if (format == CPDLP_FMT_ELL) return ceil_div(rows, BLOCK);
+        if (format == CPDLP_FMT_XSLICE) return ns * sl_blocks;
+        return n_long_blocks + n_tiles;
     }
 };
 
@@ -53,6 +66,85 @@ struct XPack {
     const V* p[K];
 };
 
+// Long row r (one wavefront): lane l takes k = l, l+64, ...; U entries are loaded
+// (clamped, so every load is unconditional and all are in flight together) before any
+// is accumulated; accumulation stays in ascending k per lane; then a butterfly.
+template <class V, class Acc, int K>
+__device__ __forceinline__ void long_row_sums(const int32_t* rowptr, const int32_t* col,
+                                              const V* val, int r, const XPack<V, K>& X,
+                                              Acc (&acc)[K]) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
+    const int k0 = rowptr[r], k1 = rowptr[r + 1];
+    constexpr int U = 4;
+    for (int kb = k0 + lane; kb < k1; kb += U * WAVE) {
+        int c[U];
+        Acc v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = min(kb + u * WAVE, k1 - 1);
+            c[u] = col[k];
+            v[u] = (Acc)val[k];
+        }
+#pragma unroll
+        for (int kv = 0; kv < K; ++kv) {
+            Acc xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[kv][c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (kb + u * WAVE < k1) acc[kv] += v[u] * xv[u];
+        }
+    }
+#pragma unroll
+    for (int kv = 0; kv < K; ++kv) acc[kv] = wave_sum(acc[kv]);
+}
+
+// Short tile rows [r0, r1): thread tid owns row r0 + tid.  Every load unconditional
+// (index clamped to the tile), so all PER_T (col, val) pairs and then all PER_T
+// gathers are in flight together; products go to LDS and each thread sums its row
+// sequentially from zero in ascending k.  Slots past the tile's nnz are written but
+// never summed.  All threads of the block must call this (it synchronises).
+template <class V, class Acc, int K>
+__device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* col,
+                                          const V* val, int r0, int r1, const XPack<V, K>& X,
+                                          Acc* prod, Acc (&acc)[K]) {
+    const int tid = threadIdx.x;
+    const int base = rowptr[r0];
+    const int cnt = rowptr[r1] - base;
+    const int r = r0 + tid;
+    int lo = 0, hi = 0;
+    if (r < r1) {
+        lo = rowptr[r] - base;
+        hi = rowptr[r + 1] - base;
+    }
+#pragma unroll
+    for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
+    if (cnt <= 0) return;   // uniform per block
+    int cc[PER_T];
+    V vv[PER_T];
+#pragma unroll
+    for (int j = 0; j < PER_T; ++j) {
+        const int e = min(tid + j * BLOCK, cnt - 1);
+        cc[j] = col[base + e];
+        vv[j] = val[base + e];
+    }
+#pragma unroll
+    for (int kv = 0; kv < K; ++kv) {
+        Acc xv[PER_T];
+#pragma unroll
+        for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
+#pragma unroll
+        for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
+        __syncthreads();
+        Acc sum = Acc(0);
+        for (int e = lo; e < hi; ++e) sum += prod[e];
+        acc[kv] = sum;
+        __syncthreads();
+    }
+}
+
 template <class V, class Acc, int K, class Epi>
 __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
@@ -60,64 +152,71 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X,
 #pragma unroll
     for (int i = 0; i < (Epi::NQ > 0 ? Epi::NQ : 1); ++i) q[i] = 0.0;
     if (epi.skip()) return;
-    const int tid = threadIdx.x;
+    Acc acc[K];
     if ((int)blockIdx.x < A.n_long_blocks) {
-        const int lane = tid & (WAVE - 1);
-        const int r = blockIdx.x * LONG_ROWS_PER_BLOCK + tid / WAVE;
+        const int r = blockIdx.x * LONG_ROWS_PER_BLOCK + threadIdx.x / WAVE;
         if (r < A.n_long) {
-            Acc acc[K];
-#pragma unroll
-            for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
-            const int k1 = A.rowptr[r + 1];
-            for (int k = A.rowptr[r] + lane; k < k1; k += WAVE) {
-                const int c = A.col[k];
-                const Acc v = (Acc)A.val[k];
-#pragma unroll
-                for (int kv = 0; kv < K; ++kv) acc[kv] += v * (Acc)X.p[kv][c];
-            }
-#pragma unroll
-            for (int kv = 0; kv < K; ++kv) acc[kv] = wave_sum(acc[kv]);
-            if (lane == 0) epi.row(r, acc, q);
+            long_row_sums<V, Acc, K>(A.rowptr, A.col, A.val, r, X, acc);
+            if ((threadIdx.x & (WAVE - 1)) == 0) epi.row(r, acc, q);
         }
     } else {
         const int t = blockIdx.x - A.n_long_blocks;
         const int r0 = A.tile_ptr[t], r1 = A.tile_ptr[t + 1];
-        const int base = A.rowptr[r0];
-        const int cnt = A.rowptr[r1] - base;
-        const int r = r0 + tid;
-        const bool has = r < r1;
-        int lo = 0, hi = 0;
-        if (has) {
-            lo = A.rowptr[r] - base;
-            hi = A.rowptr[r + 1] - base;
-        }
-        int cc[PER_T];
-        V vv[PER_T];
+        tile_sums<V, Acc, K>(A.rowptr, A.col, A.val, r0, r1, X, prod, acc);
+        if (r0 + (int)threadIdx.x < r1) epi.row(r0 + threadIdx.x, acc, q);
+    }
+    if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
+}
+
+// XCD-sliced product, pass 1: block b works on slice s = b % ns and writes the
+// slice's partial row sums part[(kv * ns + s) * rows + r].
+template <class V, class Acc, int K>
+__global__ __launch_bounds__(BLOCK) void k_spmv_slices(MatView<V> A, XPack<V, K> X) {
+    __shared__ Acc prod[TILE_NNZ];
+    const int s = blockIdx.x % A.ns;
+    const int j = blockIdx.x / A.ns;
+    const int32_t* rp = A.sl_rowptr + (size_t)s * (A.rows + 1);
+    Acc* part = (Acc*)A.sl_part;
+    Acc acc[K];
+    if (j < A.n_long_blocks) {
+        const int r = j * LONG_ROWS_PER_BLOCK + threadIdx.x / WAVE;
+        if (r < A.n_long) {
+            long_row_sums<V, Acc, K>(rp, A.sl_col, A.sl_val, r, X, acc);
+            if ((threadIdx.x & (WAVE - 1)) == 0)
 #pragma unroll
-        for (int j = 0; j < PER_T; ++j) {
-            const int e = tid + j * BLOCK;
-            cc[j] = 0;
-            vv[j] = V(0);
-            if (e < cnt) {
-                cc[j] = A.col[base + e];
-                vv[j] = A.val[base + e];
-            }
+                for (int kv = 0; kv < K; ++kv) part[((size_t)kv * A.ns + s) * A.rows + r] = acc[kv];
         }
+    } else {
+        const int t = A.sl_tile_off[s] + (j - A.n_long_blocks);
+        if (t + 1 >= A.sl_tile_off[s + 1]) return;   // uniform per block: slice has fewer tiles
+        const int r0 = A.sl_tile_ptr[t], r1 = A.sl_tile_ptr[t + 1];
+        tile_sums<V, Acc, K>(rp, A.sl_col, A.sl_val, r0, r1, X, prod, acc);
+        const int r = r0 + threadIdx.x;
+        if (r < r1)
+#pragma unroll
+            for (int kv = 0; kv < K; ++kv) part[((size_t)kv * A.ns + s) * A.rows + r] = acc[kv];
+    }
+}
+
+// XCD-sliced product, pass 2: row sum = sum of the slice partials in slice order,
+// then the epilogue (thread per row, coalesced).
+template <class V, class Acc, int K, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_slices_combine(MatView<V> A, Epi epi) {
+    double q[Epi::NQ > 0 ? Epi::NQ : 1];
+#pragma unroll
+    for (int i = 0; i < (Epi::NQ > 0 ? Epi::NQ : 1); ++i) q[i] = 0.0;
+    if (epi.skip()) return;
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    if (r < A.rows) {
+        const Acc* part = (const Acc*)A.sl_part;
         Acc acc[K];
 #pragma unroll
         for (int kv = 0; kv < K; ++kv) {
-#pragma unroll
-            for (int j = 0; j < PER_T; ++j) {
-                const int e = tid + j * BLOCK;
-                if (e < cnt) prod[e] = (Acc)vv[j] * (Acc)X.p[kv][cc[j]];
-            }
-            __syncthreads();
-            Acc s = Acc(0);
-            for (int e = lo; e < hi; ++e) s += prod[e];
-            acc[kv] = s;
-            __syncthreads();
+            Acc sum = part[((size_t)kv * A.ns) * A.rows + r];
+            for (int s = 1; s < A.ns; ++s) sum += part[((size_t)kv * A.ns + s) * A.rows + r];
+            acc[kv] = sum;
         }
-        if (has) epi.row(r, acc, q);
+        epi.row(r, acc, q);
     }
     if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
 }
@@ -137,12 +236,26 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(MatView<V> A, XPack<V, K> X,
 #pragma unroll
         for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
         const size_t stride = (size_t)A.rows;
-        for (int s = 0; s < A.ell_width; ++s) {
-            const int c = A.ell_col[s * stride + r];
-            if (c < 0) break;   // left-packed: padding only after the row's entries
-            const Acc v = (Acc)A.ell_val[s * stride + r];
+        constexpr int U = 4;
+        for (int s0 = 0; s0 < A.ell_width; s0 += U) {
+            int c[U];
+            Acc v[U];
 #pragma unroll
-            for (int kv = 0; kv < K; ++kv) acc[kv] += v * (Acc)X.p[kv][c];
+            for (int u = 0; u < U; ++u) {
+                const int sl = min(s0 + u, A.ell_width - 1);
+                c[u] = A.ell_col[sl * stride + r];
+                v[u] = (Acc)A.ell_val[sl * stride + r];
+            }
+#pragma unroll
+            for (int kv = 0; kv < K; ++kv) {
+                Acc xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[kv][c[u] < 0 ? 0 : c[u]];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (s0 + u < A.ell_width && c[u] >= 0) acc[kv] += v[u] * xv[u];
+            }
+            if (c[U - 1] < 0) break;   // left-packed: the rest of the row is padding
         }
         epi.row(r, acc, q);
     }
@@ -153,7 +266,11 @@ template <class V, class Acc, int K, class Epi>
 inline void launch_spmv(const MatView<V>& A, const XPack<V, K>& X, const Epi& epi, hipStream_t s) {
     const int g = A.grid();
     if (g <= 0) return;
-    if (A.format == CPDLP_FMT_ELL)
+    if (A.format == CPDLP_FMT_XSLICE) {
+        hipLaunchKernelGGL((k_spmv_slices<V, Acc, K>), dim3(g), dim3(BLOCK), 0, s, A, X);
+        hipLaunchKernelGGL((k_slices_combine<V, Acc, K, Epi>), dim3(ceil_div(A.rows, BLOCK)),
+                           dim3(BLOCK), 0, s, A, epi);
+    } else if (A.format == CPDLP_FMT_ELL)
         hipLaunchKernelGGL((k_spmv_ell<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
     else
         hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);

@@ -48,6 +48,7 @@ extern "C" {
 #define CPDLP_FMT_AUTO 0        /* autotuned per matrix */
 #define CPDLP_FMT_CSR 1         /* GPUSparseMatrixCSR, src/utils/mat_csr.jl */
 #define CPDLP_FMT_ELL 2         /* GPUSparseMatrixELL, src/utils/mat_ell.jl */
+#define CPDLP_FMT_XSLICE 3      /* CSR cut into 8 column slices, one per XCD's L2 */
 
 /* ---- problem input --------------------------------------------------------------
  * MILP(; c, lv, uv, A, lc, uc), src/problems/milp.jl:55-115, with A given as the
@@ -81,7 +82,7 @@ typedef struct cpdlp_params {
     double sufficient_decay;        /* 0.2 (PDHG: NaN) */
     double necessary_decay;         /* 0.8 (PDHG: NaN) */
     double artificial_decay;        /* 0.36 (PDHG: NaN) */
-    int32_t show_progress;          /* 0 (accepted, ignored: no progress bar) */
+    int32_t show_progress;          /* 0; 1: a progress line on stderr every 2 s */
     int32_t check_every;            /* 100 */
     int32_t record_error_history;   /* 1 */
     int32_t precision;              /* CPDLP_PREC_* (T) */

@@ -141,14 +141,17 @@ def test_termination_not_skipped(cp):
     assert abs(sol.x.sum() - 1.0) < 1e-3
 
 
-@pytest.mark.parametrize("algo", ["PDLP", "PDHG"])
-@pytest.mark.parametrize("use_graph", [0, 1])
-def test_solve_matches_oracle(cp, algo, use_graph):
+@pytest.mark.parametrize("algo,use_graph,fmt", [
+    ("PDLP", 0, "csr"), ("PDLP", 1, "csr"), ("PDLP", 1, "ell"), ("PDLP", 1, "xslice"),
+    ("PDLP", 1, "auto"), ("PDHG", 0, "csr"), ("PDHG", 1, "auto")])
+def test_solve_matches_oracle(cp, algo, use_graph, fmt):
     g = G.column_lp(400, 1000, 6, seed=21)
     milp = cp.MILP.from_generated(g)
     A = cp.PDLP if algo == "PDLP" else cp.PDHG
     reltol = 1e-6 if algo == "PDLP" else 1e-4
-    sol, st = cp.solve(milp, A(termination_reltol=reltol, use_graph=use_graph))
+    sol, st = cp.solve(milp, A("float64", None, fmt, termination_reltol=reltol, use_graph=use_graph))
+    if fmt in ("ell", "xslice"):
+        assert (st.format_A, st.format_At) == ((2, 2) if fmt == "ell" else (3, 3))
     ref = O.solve(g, O.default_params(algo, termination_reltol=reltol))
     assert st.termination_status.name == ref.status == "OPTIMAL"
     assert abs(st.kkt_passes - ref.stats.kkt_passes) <= 200
@@ -167,7 +170,7 @@ def test_trajectory_matches_oracle_exactly_early(cp):
     rounding (sums differ only in order)."""
     g = G.column_lp(300, 700, 6, seed=5)
     milp = cp.MILP.from_generated(g)
-    _, st = cp.solve(milp, cp.PDLP(max_kkt_passes=500, termination_reltol=1e-12))
+    _, st = cp.solve(milp, cp.PDLP("float64", None, "csr", max_kkt_passes=500, termination_reltol=1e-12))
     ref = O.solve(g, O.default_params(max_kkt_passes=500, termination_reltol=1e-12))
     assert st.termination_status == cp.ITERATION_LIMIT and ref.status == "ITERATION_LIMIT"
     assert st.kkt_passes == ref.stats.kkt_passes == 500
@@ -219,6 +222,23 @@ def test_float32_and_mixed(cp):
     assert st.termination_status == cp.OPTIMAL
     assert st.fp32_passes > 0 and st.kkt_passes > st.fp32_passes
     assert abs(cp.objective_value(sol.x, milp) - g.objective) <= 1e-5 * (1 + abs(g.objective))
-    sol, st = cp.solve(milp, cp.PDLP("float32", termination_reltol=1e-4))
+    # the eps(Float32)*n^3 power-method rule stops after one Rayleigh quotient here
+    # (eta*||A|| > 1 diverges, as it would in the reference): converge it instead
+    sol, st = cp.solve(milp, cp.PDLP("float32", termination_reltol=1e-4, powm_tol=1e-6))
     assert st.termination_status == cp.OPTIMAL
     assert abs(cp.objective_value(sol.x, milp) - g.objective) <= 1e-2 * (1 + abs(g.objective))
+
+
+@pytest.mark.parametrize("fmt", ["csr", "xslice", "auto"])
+def test_power_law_lp_formats(cp, fmt):
+    """Long rows (wavefront path) + XCD slices on a small power-law LP: same status and
+    objective as the oracle; the slice format's row sums differ only in rounding."""
+    g = G.power_law_lp(3000, 6000, 60_000, seed=2)
+    milp = cp.MILP.from_generated(g)
+    kw = dict(termination_reltol=1e-5, powm_tol=1e-6, max_kkt_passes=400_000)
+    sol, st = cp.solve(milp, cp.PDLP("float64", None, fmt, **kw))
+    ref = O.solve(g, O.default_params(termination_reltol=1e-5, powm_tol=1e-6, max_kkt_passes=400_000))
+    assert st.termination_status == cp.OPTIMAL and ref.status == "OPTIMAL"
+    obj = cp.objective_value(sol.x, milp)
+    assert abs(obj - g.objective) <= 1e-4 * (1 + abs(g.objective))
+    assert abs(obj - float(ref.x @ g.c)) <= 1e-4 * (1 + abs(g.objective))
