@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--max-passes", type=int, default=10**7)
     # PMC calibration: also run a plain 512 MiB device-to-device copy kernel 5 times
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--format", default="auto", choices=["auto", "csr", "ell", "xslice"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
@@ -114,7 +115,7 @@ def main():
     # powm_tol: the reference's power-method rule (tol = eps*n^3) stops after one
     # Rayleigh quotient of a random vector at n = 2e6, which underestimates ||A|| and
     # makes eta*||A|| > 1; the benchmark converges the power method to 1e-4 instead
-    algo = cp.PDLP(termination_reltol=reltol, time_limit=args.time_limit,
+    algo = cp.PDLP("float64", None, args.format, termination_reltol=reltol, time_limit=args.time_limit,
                    max_kkt_passes=args.max_passes, device=0, powm_tol=args.powm_tol,
                    show_progress=1)
     t = time.perf_counter()

@@ -29,19 +29,29 @@ struct EpiPrimal {
     V *xn, *xbar, *xavg, *xavgl;
     const StepCoef* coef;
     int32_t step, pdlp;
+    struct Pre {
+        V x, c, lv, uv, avg;
+    };
     __device__ bool skip() const { return false; }
+    __device__ __forceinline__ Pre load(int j) const {
+        Pre p;
+        p.x = x[j];
+        p.c = c[j];
+        p.lv = lv[j];
+        p.uv = uv[j];
+        p.avg = pdlp ? xavg[j] : V(0);
+        return p;
+    }
     template <class Acc>
-    __device__ __forceinline__ void row(int j, const Acc* acc, double*) const {
+    __device__ __forceinline__ void row(int j, const Acc* acc, double*, const Pre& p) const {
         const StepCoef& cf = coef[step];
         const V tau = (V)cf.tau;
-        const V xj = x[j];
-        const V xp = proj_box<V>(xj - tau * (c[j] - (V)acc[0]), lv[j], uv[j]);
+        const V xp = proj_box<V>(p.x - tau * (p.c - (V)acc[0]), p.lv, p.uv);
         xn[j] = xp;
-        xbar[j] = V(2) * xp - xj;
+        xbar[j] = V(2) * xp - p.x;
         if (pdlp) {
-            const V a = xavg[j];
-            if (cf.last) xavgl[j] = a;
-            xavg[j] = (V)cf.wa * xp + (V)cf.wb * a;
+            if (cf.last) xavgl[j] = p.avg;
+            xavg[j] = (V)cf.wa * xp + (V)cf.wb * p.avg;
         }
     }
 };
@@ -56,18 +66,28 @@ struct EpiDual {
     V *yn, *yavg, *yavgl;
     const StepCoef* coef;
     int32_t step, pdlp;
+    struct Pre {
+        V y, lc, uc, avg;
+    };
     __device__ bool skip() const { return false; }
+    __device__ __forceinline__ Pre load(int i) const {
+        Pre p;
+        p.y = y[i];
+        p.lc = lc[i];
+        p.uc = uc[i];
+        p.avg = pdlp ? yavg[i] : V(0);
+        return p;
+    }
     template <class Acc>
-    __device__ __forceinline__ void row(int i, const Acc* acc, double*) const {
+    __device__ __forceinline__ void row(int i, const Acc* acc, double*, const Pre& p) const {
         const StepCoef& cf = coef[step];
         const V sigma = (V)cf.sigma, isig = (V)cf.isig;
-        const V yi = y[i], ax = (V)acc[0];
-        const V yp = (yi - sigma * ax) - sigma * proj_box<V>(isig * yi - ax, -uc[i], -lc[i]);
+        const V ax = (V)acc[0];
+        const V yp = (p.y - sigma * ax) - sigma * proj_box<V>(isig * p.y - ax, -p.uc, -p.lc);
         yn[i] = yp;
         if (pdlp) {
-            const V a = yavg[i];
-            if (cf.last) yavgl[i] = a;
-            yavg[i] = (V)cf.wa * yp + (V)cf.wb * a;
+            if (cf.last) yavgl[i] = p.avg;
+            yavg[i] = (V)cf.wa * yp + (V)cf.wb * p.avg;
         }
     }
 };
@@ -84,21 +104,33 @@ struct EpiKktRows {
     const V* y_rst;
     const double *lc, *uc, *invD1;
     int32_t dist;
+    struct Pre {
+        double l, u, id, yr;
+        double y[K];
+    };
     __device__ bool skip() const { return false; }
+    __device__ __forceinline__ Pre load(int i) const {
+        Pre p;
+        p.l = lc[i];
+        p.u = uc[i];
+        p.id = invD1[i];
+#pragma unroll
+        for (int k = 0; k < K; ++k) p.y[k] = (double)Y[k][i];
+        p.yr = dist ? (double)y_rst[i] : 0.0;
+        return p;
+    }
     template <class Acc>
-    __device__ __forceinline__ void row(int i, const Acc* acc, double* q) const {
-        const double l = lc[i], u = uc[i], id = invD1[i];
+    __device__ __forceinline__ void row(int i, const Acc* acc, double* q, const Pre& p) const {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const double ax = (double)acc[k];
-            const double pd = id * (ax - proj_box(ax, l, u));
+            const double pd = p.id * (ax - proj_box(ax, p.l, p.u));
             q[2 * k] += pd * pd;
-            const double yk = (double)Y[k][i];
-            q[2 * k + 1] += safeprod_left(u, pos_part(-yk)) - safeprod_left(l, neg_part(-yk));
+            const double yk = p.y[k];
+            q[2 * k + 1] += safeprod_left(p.u, pos_part(-yk)) - safeprod_left(p.l, neg_part(-yk));
         }
         if (dist) {
-            const double yr = (double)y_rst[i];
-            const double d0 = (double)Y[0][i] - yr, d1 = (double)Y[K > 1 ? 1 : 0][i] - yr;
+            const double d0 = p.y[0] - p.yr, d1 = p.y[K > 1 ? 1 : 0] - p.yr;
             q[2 * K] += d0 * d0;
             q[2 * K + 1] += d1 * d1;
         }
@@ -116,22 +148,35 @@ struct EpiKktCols {
     const V* x_rst;
     const double *c, *lv, *uv, *invD2;
     int32_t dist;
+    struct Pre {
+        double c, l, u, id, xr;
+        double x[K];
+    };
     __device__ bool skip() const { return false; }
+    __device__ __forceinline__ Pre load(int j) const {
+        Pre p;
+        p.c = c[j];
+        p.l = lv[j];
+        p.u = uv[j];
+        p.id = invD2[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) p.x[k] = (double)X[k][j];
+        p.xr = dist ? (double)x_rst[j] : 0.0;
+        return p;
+    }
     template <class Acc>
-    __device__ __forceinline__ void row(int j, const Acc* acc, double* q) const {
-        const double cj = c[j], l = lv[j], u = uv[j], id = invD2[j];
+    __device__ __forceinline__ void row(int j, const Acc* acc, double* q, const Pre& p) const {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double g = cj - (double)acc[k];
-            const double r = proj_multiplier(g, l, u);
-            const double dd = id * (g - r);
+            const double g = p.c - (double)acc[k];
+            const double r = proj_multiplier(g, p.l, p.u);
+            const double dd = p.id * (g - r);
             q[3 * k] += dd * dd;
-            q[3 * k + 1] += safeprod_left(u, pos_part(-r)) - safeprod_left(l, neg_part(-r));
-            q[3 * k + 2] += cj * (double)X[k][j];
+            q[3 * k + 1] += safeprod_left(p.u, pos_part(-r)) - safeprod_left(p.l, neg_part(-r));
+            q[3 * k + 2] += p.c * p.x[k];
         }
         if (dist) {
-            const double xr = (double)x_rst[j];
-            const double d0 = (double)X[0][j] - xr, d1 = (double)X[K > 1 ? 1 : 0][j] - xr;
+            const double d0 = p.x[0] - p.xr, d1 = p.x[K > 1 ? 1 : 0] - p.xr;
             q[3 * K] += d0 * d0;
             q[3 * K + 1] += d1 * d1;
         }
@@ -145,15 +190,21 @@ struct PowState {
     int32_t done, pad;
 };
 
+struct NoPre {};
+
 template <class V>
 struct EpiStore {   // t = A x (skipped once the power method has converged)
     static constexpr int NQ = 0;
     double* partials = nullptr;
     V* out;
     const PowState* st;
+    using Pre = NoPre;
     __device__ bool skip() const { return st && st->done; }
+    __device__ __forceinline__ Pre load(int) const { return {}; }
     template <class Acc>
-    __device__ __forceinline__ void row(int i, const Acc* acc, double*) const { out[i] = (V)acc[0]; }
+    __device__ __forceinline__ void row(int i, const Acc* acc, double*, const Pre&) const {
+        out[i] = (V)acc[0];
+    }
 };
 
 template <class V>
@@ -163,12 +214,16 @@ struct EpiPowAt {   // Ax = A' t; sums of x.Ax and Ax.Ax
     V* Ax;
     const V* x;
     const PowState* st;
+    struct Pre {
+        V x;
+    };
     __device__ bool skip() const { return st->done; }
+    __device__ __forceinline__ Pre load(int j) const { return Pre{x[j]}; }
     template <class Acc>
-    __device__ __forceinline__ void row(int j, const Acc* acc, double* q) const {
+    __device__ __forceinline__ void row(int j, const Acc* acc, double* q, const Pre& p) const {
         const V a = (V)acc[0];
         Ax[j] = a;
-        q[0] += (double)(x[j] * a);
+        q[0] += (double)(p.x * a);
         q[1] += (double)(a * a);
     }
 };
@@ -180,9 +235,11 @@ struct EpiAxpby {
     double* out;
     const int32_t* out_index;
     double alpha, beta;
+    using Pre = NoPre;
     __device__ bool skip() const { return false; }
+    __device__ __forceinline__ Pre load(int) const { return {}; }
     template <class Acc>
-    __device__ __forceinline__ void row(int r, const Acc* acc, double*) const {
+    __device__ __forceinline__ void row(int r, const Acc* acc, double*, const Pre&) const {
         const int i = out_index ? out_index[r] : r;
         const double s = (double)acc[0];
         out[i] = beta == 0.0 ? alpha * s : alpha * s + beta * out[i];

@@ -26,7 +26,8 @@ struct HostSide {
     int32_t rows = 0, cols = 0;
     int64_t nnz = 0;
     int32_t n_long = 0;
-    std::vector<int32_t> rowptr, col, vmap, tile_ptr;
+    std::vector<int32_t> rowptr, col, vmap;
+    std::vector<TileMeta> tiles;
 };
 
 struct HostLayout {
@@ -94,7 +95,9 @@ inline void csc_to_csr(int64_t m, int64_t n, const std::vector<int32_t>& cp,
         }
 }
 
-// stable partition: long rows (length > long_row) first
+// partition: long rows (length > long_row) first, longest first (ties by index), so a
+// block of four wavefronts gets rows of similar length and the longest start first;
+// the short rows follow in their original order
 inline void partition_rows(const std::vector<int32_t>& ptr, int32_t long_row,
                            std::vector<int32_t>& of_int, std::vector<int32_t>& int_of,
                            int32_t& n_long) {
@@ -103,6 +106,9 @@ inline void partition_rows(const std::vector<int32_t>& ptr, int32_t long_row,
     of_int.reserve(rows);
     for (int64_t i = 0; i < rows; ++i)
         if (ptr[i + 1] - ptr[i] > long_row) of_int.push_back((int32_t)i);
+    std::stable_sort(of_int.begin(), of_int.end(), [&](int32_t a, int32_t b) {
+        return ptr[a + 1] - ptr[a] > ptr[b + 1] - ptr[b];
+    });
     n_long = (int32_t)of_int.size();
     for (int64_t i = 0; i < rows; ++i)
         if (ptr[i + 1] - ptr[i] <= long_row) of_int.push_back((int32_t)i);
@@ -110,22 +116,22 @@ inline void partition_rows(const std::vector<int32_t>& ptr, int32_t long_row,
     for (int64_t r = 0; r < rows; ++r) int_of[of_int[r]] = (int32_t)r;
 }
 
-// greedy short tiles over internal rows [n_long, rows): <= BLOCK rows, <= TILE_NNZ nnz
-inline void build_tiles(HostSide& S) {
-    S.tile_ptr.clear();
-    int32_t r = S.n_long;
-    S.tile_ptr.push_back(r);
-    while (r < S.rows) {
+// greedy short tiles over rows [first, rows): <= BLOCK rows, <= TILE_NNZ nnz
+inline void build_tiles(const int32_t* rowptr, int32_t first, int32_t rows,
+                        std::vector<TileMeta>& tiles) {
+    int32_t r = first;
+    while (r < rows) {
+        const int32_t r0 = r;
         int32_t rows_in = 0;
         int64_t nz = 0;
-        while (r < S.rows && rows_in < BLOCK) {
-            const int64_t L = S.rowptr[r + 1] - S.rowptr[r];
+        while (r < rows && rows_in < BLOCK) {
+            const int64_t L = rowptr[r + 1] - rowptr[r];
             if (rows_in > 0 && nz + L > TILE_NNZ) break;
             nz += L;
             ++rows_in;
             ++r;
         }
-        S.tile_ptr.push_back(r);
+        tiles.push_back(TileMeta{r0, r, rowptr[r0], rowptr[r] - rowptr[r0]});
     }
 }
 
@@ -154,7 +160,8 @@ inline void build_side(HostSide& S, int32_t rows, int32_t cols, const std::vecto
             S.vmap[d] = src_k ? (*src_k)[k] : k;
         }
     }
-    build_tiles(S);
+    S.tiles.clear();
+    build_tiles(S.rowptr.data(), S.n_long, S.rows, S.tiles);
 }
 
 inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
@@ -201,7 +208,8 @@ inline int32_t build_ell(const HostSide& S, double max_fill, std::vector<int32_t
 // per slice (<= BLOCK rows, <= TILE_NNZ nnz).
 struct HostSlices {
     int32_t ns = 0, max_tiles = 0;
-    std::vector<int32_t> rowptr, col, vmap, tile_ptr, tile_off;
+    std::vector<int32_t> rowptr, col, vmap, tile_off;
+    std::vector<TileMeta> tiles;
 };
 
 inline void build_slices(const HostSide& S, int32_t ns, HostSlices& H) {
@@ -232,33 +240,16 @@ inline void build_slices(const HostSide& S, int32_t ns, HostSlices& H) {
             H.col[d] = S.col[k];
             H.vmap[d] = S.vmap[k];
         }
-    // per slice: tile row boundaries (nt + 1 entries, the last one = rows);
-    // tile_off[s] = index of slice s's first boundary, tile_off[ns] = total size, so
-    // slice s has tile_off[s+1] - tile_off[s] - 1 tiles
-    H.tile_ptr.clear();
+    // per slice: its tiles from tile_off[s] to tile_off[s+1]
+    H.tiles.clear();
     H.tile_off.assign(ns + 1, 0);
     H.max_tiles = 0;
     for (int32_t sl = 0; sl < ns; ++sl) {
-        const int32_t* rp = H.rowptr.data() + (size_t)sl * (rows + 1);
-        H.tile_off[sl] = (int32_t)H.tile_ptr.size();
-        int32_t r = S.n_long, nt = 0;
-        H.tile_ptr.push_back(r);
-        while (r < rows) {
-            int32_t rows_in = 0;
-            int64_t nz = 0;
-            while (r < rows && rows_in < BLOCK) {
-                const int64_t L = rp[r + 1] - rp[r];
-                if (rows_in > 0 && nz + L > TILE_NNZ) break;
-                nz += L;
-                ++rows_in;
-                ++r;
-            }
-            H.tile_ptr.push_back(r);
-            ++nt;
-        }
-        H.max_tiles = std::max(H.max_tiles, nt);
+        H.tile_off[sl] = (int32_t)H.tiles.size();
+        build_tiles(H.rowptr.data() + (size_t)sl * (rows + 1), S.n_long, rows, H.tiles);
+        H.max_tiles = std::max(H.max_tiles, (int32_t)H.tiles.size() - H.tile_off[sl]);
     }
-    H.tile_off[ns] = (int32_t)H.tile_ptr.size();
+    H.tile_off[ns] = (int32_t)H.tiles.size();
 }
 
 }  // namespace cpdlp

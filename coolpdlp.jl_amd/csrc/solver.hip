@@ -283,7 +283,7 @@ MatView<V> DevSide::view(bool allow_slices) const {
     v.val = vals<V>();
     v.n_long = n_long;
     v.n_long_blocks = ceil_div(n_long, LONG_ROWS_PER_BLOCK);
-    v.tile_ptr = tile_ptr.p;
+    v.tiles = tiles.p;
     v.n_tiles = n_tiles;
     v.format = format;
     v.ell_width = ell_w;
@@ -293,7 +293,7 @@ MatView<V> DevSide::view(bool allow_slices) const {
     v.sl_rowptr = sl_rowptr.p;
     v.sl_col = sl_col.p;
     v.sl_val = sl_vals<V>();
-    v.sl_tile_ptr = sl_tile_ptr.p;
+    v.sl_tiles = sl_tiles.p;
     v.sl_tile_off = sl_tile_off.p;
     v.sl_blocks = sl_blocks;
     v.sl_part = sl_part.p;
@@ -325,7 +325,7 @@ void DevSide::build_slices(const HostSide& S, hipStream_t s) {
     sl_col.upload(H.col.data(), H.col.size(), s);
     sl_vmap.alloc(std::max<size_t>(H.vmap.size(), 1));
     sl_vmap.upload(H.vmap.data(), H.vmap.size(), s);
-    sl_tile_ptr.upload(H.tile_ptr, s);
+    sl_tiles.upload(H.tiles, s);
     sl_tile_off.upload(H.tile_off, s);
     sl_part.alloc(std::max<int64_t>((int64_t)NS * S.rows, 1));
     HIP_CHECK(hipStreamSynchronize(s));
@@ -336,13 +336,13 @@ void DevSide::upload_structure(const HostSide& S, hipStream_t s) {
     cols = S.cols;
     nnz = S.nnz;
     n_long = S.n_long;
-    n_tiles = (int32_t)S.tile_ptr.size() - 1;
+    n_tiles = (int32_t)S.tiles.size();
     rowptr.upload(S.rowptr, s);
     col.alloc(std::max<int64_t>(nnz, 1));
     col.upload(S.col.data(), S.col.size(), s);
     vmap.alloc(std::max<int64_t>(nnz, 1));
     vmap.upload(S.vmap.data(), S.vmap.size(), s);
-    tile_ptr.upload(S.tile_ptr, s);
+    tiles.upload(S.tiles, s);
 }
 
 void DevSide::fill_values(const double* src_csc, bool want32, hipStream_t s) {

@@ -16,7 +16,8 @@ struct DevSide {
     int64_t nnz = 0;
     int32_t format = CPDLP_FMT_CSR;
     int32_t ell_w = 0;
-    DevBuf<int32_t> rowptr, col, vmap, tile_ptr;
+    DevBuf<int32_t> rowptr, col, vmap;
+    DevBuf<TileMeta> tiles;
     DevBuf<double> val64;
     DevBuf<float> val32;
     DevBuf<int32_t> ell_col, ell_vmap;
@@ -24,7 +25,8 @@ struct DevSide {
     DevBuf<float> ell_val32;
     // XCD column slices (step kernels only)
     int32_t sl_ns = 0, sl_blocks = 0;
-    DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_ptr, sl_tile_off;
+    DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_off;
+    DevBuf<TileMeta> sl_tiles;
     DevBuf<double> sl_val64, sl_part;
     DevBuf<float> sl_val32;
 

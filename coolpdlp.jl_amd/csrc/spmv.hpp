@@ -16,8 +16,10 @@
 //    order, as the reference's thread-per-row kernel (src/utils/mat_csr.jl:63-79),
 //    so these rows are bit-identical to it.
 //
-// Epilogues (struct with NQ and row()) consume the K row sums; when NQ > 0 each block
-// reduces NQ doubles deterministically into partials[blockIdx.x * NQ + q].
+// Epilogues consume the K row sums: load(r) returns the row's epilogue operands and is
+// issued before the gathers (so its latency overlaps them), row(r, acc, q, pre)
+// finishes the row; when NQ > 0 each block reduces NQ doubles deterministically into
+// partials[blockIdx.x * NQ + q].
 #pragma once
 #include "common.hpp"
 
@@ -27,6 +29,12 @@ constexpr int TILE_NNZ = 2048;
 constexpr int PER_T = TILE_NNZ / BLOCK;   // nonzeros held per thread in a short tile
 constexpr int LONG_ROWS_PER_BLOCK = BLOCK / WAVE;
 
+// one short tile: rows [r0, r1), nonzeros [base, base + cnt); read with one 16-byte
+// scalar load at block start
+struct alignas(16) TileMeta {
+    int32_t r0, r1, base, cnt;
+};
+
 template <class V>
 struct MatView {
     int32_t rows = 0;
@@ -35,7 +43,7 @@ struct MatView {
     const V* val = nullptr;
     int32_t n_long = 0;           // internal rows [0, n_long) are long
     int32_t n_long_blocks = 0;    // ceil(n_long / 4)
-    const int32_t* tile_ptr = nullptr;   // short tiles: rows [tile_ptr[t], tile_ptr[t+1])
+    const TileMeta* tiles = nullptr;     // short tiles
     int32_t n_tiles = 0;
     // column-major ELL (format 2): rows x width, col -1 = padding
     int32_t format = CPDLP_FMT_CSR;
@@ -49,8 +57,8 @@ struct MatView {
     const int32_t* sl_rowptr = nullptr;   // ns * (rows + 1), absolute offsets
     const int32_t* sl_col = nullptr;
     const V* sl_val = nullptr;
-    const int32_t* sl_tile_ptr = nullptr; // row boundaries of slice s's tiles, from sl_tile_off[s]
-    const int32_t* sl_tile_off = nullptr; // ns + 1; slice s has off[s+1] - off[s] - 1 tiles
+    const TileMeta* sl_tiles = nullptr;   // tiles of slice s: sl_tiles[sl_tile_off[s] ...]
+    const int32_t* sl_tile_off = nullptr; // ns + 1; slice s has off[s+1] - off[s] tiles
     int32_t sl_blocks = 0;                // blocks per slice (long blocks + max tiles)
     void* sl_part = nullptr;              // K * ns * rows partial sums (Acc)
 
@@ -108,14 +116,13 @@ __device__ __forceinline__ void long_row_sums(const int32_t* rowptr, const int32
 // never summed.  All threads of the block must call this (it synchronises).
 template <class V, class Acc, int K>
 __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* col,
-                                          const V* val, int r0, int r1, const XPack<V, K>& X,
+                                          const V* val, const TileMeta& tm, const XPack<V, K>& X,
                                           Acc* prod, Acc (&acc)[K]) {
     const int tid = threadIdx.x;
-    const int base = rowptr[r0];
-    const int cnt = rowptr[r1] - base;
-    const int r = r0 + tid;
+    const int base = tm.base, cnt = tm.cnt;
+    const int r = tm.r0 + tid;
     int lo = 0, hi = 0;
-    if (r < r1) {
+    if (r < tm.r1) {
         lo = rowptr[r] - base;
         hi = rowptr[r + 1] - base;
     }
@@ -156,14 +163,18 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X,
     if ((int)blockIdx.x < A.n_long_blocks) {
         const int r = blockIdx.x * LONG_ROWS_PER_BLOCK + threadIdx.x / WAVE;
         if (r < A.n_long) {
+            typename Epi::Pre pre{};
+            if ((threadIdx.x & (WAVE - 1)) == 0) pre = epi.load(r);
             long_row_sums<V, Acc, K>(A.rowptr, A.col, A.val, r, X, acc);
-            if ((threadIdx.x & (WAVE - 1)) == 0) epi.row(r, acc, q);
+            if ((threadIdx.x & (WAVE - 1)) == 0) epi.row(r, acc, q, pre);
         }
     } else {
-        const int t = blockIdx.x - A.n_long_blocks;
-        const int r0 = A.tile_ptr[t], r1 = A.tile_ptr[t + 1];
-        tile_sums<V, Acc, K>(A.rowptr, A.col, A.val, r0, r1, X, prod, acc);
-        if (r0 + (int)threadIdx.x < r1) epi.row(r0 + threadIdx.x, acc, q);
+        const TileMeta tm = A.tiles[blockIdx.x - A.n_long_blocks];
+        const int r = tm.r0 + (int)threadIdx.x;
+        typename Epi::Pre pre{};
+        if (r < tm.r1) pre = epi.load(r);   // issued before the gathers
+        tile_sums<V, Acc, K>(A.rowptr, A.col, A.val, tm, X, prod, acc);
+        if (r < tm.r1) epi.row(r, acc, q, pre);
     }
     if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
 }
@@ -188,11 +199,11 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_slices(MatView<V> A, XPack<V, K>
         }
     } else {
         const int t = A.sl_tile_off[s] + (j - A.n_long_blocks);
-        if (t + 1 >= A.sl_tile_off[s + 1]) return;   // uniform per block: slice has fewer tiles
-        const int r0 = A.sl_tile_ptr[t], r1 = A.sl_tile_ptr[t + 1];
-        tile_sums<V, Acc, K>(rp, A.sl_col, A.sl_val, r0, r1, X, prod, acc);
-        const int r = r0 + threadIdx.x;
-        if (r < r1)
+        if (t >= A.sl_tile_off[s + 1]) return;   // uniform per block: slice has fewer tiles
+        const TileMeta tm = A.sl_tiles[t];
+        tile_sums<V, Acc, K>(rp, A.sl_col, A.sl_val, tm, X, prod, acc);
+        const int r = tm.r0 + threadIdx.x;
+        if (r < tm.r1)
 #pragma unroll
             for (int kv = 0; kv < K; ++kv) part[((size_t)kv * A.ns + s) * A.rows + r] = acc[kv];
     }
@@ -208,6 +219,7 @@ __global__ __launch_bounds__(BLOCK) void k_slices_combine(MatView<V> A, Epi epi)
     if (epi.skip()) return;
     const int r = blockIdx.x * BLOCK + threadIdx.x;
     if (r < A.rows) {
+        const typename Epi::Pre pre = epi.load(r);
         const Acc* part = (const Acc*)A.sl_part;
         Acc acc[K];
 #pragma unroll
@@ -216,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void k_slices_combine(MatView<V> A, Epi epi)
             for (int s = 1; s < A.ns; ++s) sum += part[((size_t)kv * A.ns + s) * A.rows + r];
             acc[kv] = sum;
         }
-        epi.row(r, acc, q);
+        epi.row(r, acc, q, pre);
     }
     if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
 }
@@ -232,6 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(MatView<V> A, XPack<V, K> X,
     if (epi.skip()) return;
     const int r = blockIdx.x * BLOCK + threadIdx.x;
     if (r < A.rows) {
+        const typename Epi::Pre pre = epi.load(r);
         Acc acc[K];
 #pragma unroll
         for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
@@ -257,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(MatView<V> A, XPack<V, K> X,
             }
             if (c[U - 1] < 0) break;   // left-packed: the rest of the row is padding
         }
-        epi.row(r, acc, q);
+        epi.row(r, acc, q, pre);
     }
     if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
 }
