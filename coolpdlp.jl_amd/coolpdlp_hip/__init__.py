@@ -8,6 +8,7 @@ from ._lib import ArgumentError, DimensionMismatch, HIPError
 from .algorithms import (ITERATION_LIMIT, OPTIMAL, STILL_RUNNING, TIME_LIMIT, Algorithm,
                          ConvergenceStats, Engine, KKTErrors, PDHG, PDLP, TerminationStatus,
                          absolute, relative, solve)
+from .batch import BatchSolver
 from .matrices import GPUSparseMatrixCSR, GPUSparseMatrixELL
 from .problems import (MILP, PrimalDualSolution, is_feasible, nbcons, nbcons_eq, nbcons_ineq,
                        nbvar, nbvar_cont, nbvar_int, objective_value)
@@ -17,7 +18,7 @@ __all__ = [
     "nbcons", "nbcons_eq", "nbcons_ineq", "PrimalDualSolution", "solve", "PDHG", "PDLP",
     "Algorithm", "is_feasible", "objective_value", "TerminationStatus", "OPTIMAL",
     "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING", "KKTErrors", "ConvergenceStats",
-    "Engine", "relative", "absolute", "DimensionMismatch", "ArgumentError", "HIPError",
+    "Engine", "BatchSolver", "relative", "absolute", "DimensionMismatch", "ArgumentError", "HIPError",
 ]
 
 

@@ -90,6 +90,15 @@ _SIGS = {
     "cpdlp_destroy": (None, [C.c_void_p]),
     "cpdlp_last_error": (C.c_char_p, []),
     "cpdlp_version": (C.c_char_p, []),
+    "cpdlp_batch_create": (C.c_int, [_P(CParams), C.c_int64, _P(C.c_void_p)]),
+    "cpdlp_batch_add": (C.c_int, [C.c_void_p, _P(CLP), _P(C.c_int64)]),
+    "cpdlp_batch_footprint": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64]),
+    "cpdlp_batch_run": (C.c_int, [C.c_void_p, C.c_int32, _P(C.c_int64)]),
+    "cpdlp_batch_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _P(C.c_int64)]),
+    "cpdlp_batch_get": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, _P(CStats)]),
+    "cpdlp_batch_release": (C.c_int, [C.c_void_p, C.c_int64]),
+    "cpdlp_batch_memory": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int64)]),
+    "cpdlp_batch_destroy": (None, [C.c_void_p]),
     "cpdlp_matrix_create": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
                                       _P(C.c_void_p)]),

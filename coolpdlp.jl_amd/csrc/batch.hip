@@ -1,0 +1,1135 @@
+// batch.hip -- many independent LPs solved together (batched and served solves).
+//
+// One 256-thread workgroup owns one instance for the whole launch and runs the
+// reference's PDLP loop on it (src/algorithms/pdlp.jl:69-196): check_every PDHG steps,
+// the 5-candidate KKT check (src/components/errors.jl:65-109) in FP64, termination
+// (src/components/termination.jl:89-101), the restart rule (src/components/restart.jl:
+// 40-57) and the primal-weight update (src/components/step_size.jl:58-75) -- all on the
+// device.  Per-instance preprocessing (Ruiz + Chambolle-Pock scaling,
+// src/components/preconditioning.jl:117-155; the power method, src/utils/linalg.jl:86-96)
+// also runs on the device, one workgroup per instance.
+//
+// Each step gathers y (primal half-step) and xbar (dual half-step) from LDS: the
+// workgroup keeps both vectors of its instance in LDS, so the random gathers of the
+// sparse products never leave the CU.  Matrix values and indices stream from HBM with
+// L lanes per row (L chosen per instance from its mean row length).
+//
+// Precision MIXED: FP32 matrix, iterates and products (FP32 accumulation); KKT sums,
+// restart decisions, omega and eta in FP64; when the FP32 phase reaches polish_reltol
+// (or stops improving) the instance continues in FP64 from its best FP32 iterate.
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <type_traits>
+#include <deque>
+#include <map>
+#include <memory>
+
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace cpdlp {
+
+// ---------------------------------------------------------------- device structures
+struct BInst {
+    int32_t m, n, nnz, La, Lat, pad0;
+    int32_t *a_rp, *a_col, *at_rp, *at_col, *perm;
+    double *nz, *acsc, *atcsc, *a64, *at64;
+    float *a32, *at32;
+    double *c, *lv, *uv, *lc, *uc;
+    double *cs, *lvs, *uvs, *lcs, *ucs, *D1, *D2, *iD1, *iD2;
+    float *c32, *lv32, *uv32, *lc32, *uc32;
+    float *x32[2], *y32[2], *xa32, *ya32, *xal32, *yal32, *xr32, *yr32, *xb32, *yb32;
+    double *x64[2], *y64[2], *xa64, *ya64, *xal64, *yal64, *xr64, *yr64, *sn, *sm;
+};
+
+struct BState {
+    double eta, omega, eta_sum, best_rel, best_omega, primal_scale, dual_scale;
+    double err[6];
+    int64_t kkt_passes, inner, total, restarts, powm_iters, fp32_passes;
+    int32_t status, phase, cur, stall, setup, timeout;
+};
+
+struct BParams {
+    int32_t check_every, ruiz_iter, pdlp, precision;
+    double cp_alpha, invnorm, damping, zero_tol, suff, nec, art, reltol, polish_reltol;
+    double powm_tol;   // NaN: eps(Float64) * n^3 rule
+    int64_t max_kkt_passes, powm_maxiter;
+    uint64_t seed;
+};
+
+constexpr int BB = 256;   // threads per instance workgroup
+constexpr int STALL = 10;
+
+// block-wide sum of NQ doubles, result broadcast through sh[NQ] (deterministic order)
+template <int NQ>
+__device__ __forceinline__ void bsum(double (&q)[NQ], double* sh) {
+    block_sum<NQ>(q, sh);
+    __syncthreads();
+}
+
+// rows [0, rows) of a CSR with L lanes per row (L a power of two <= 64): lane l of a
+// group takes entries l, l+L, ... then a butterfly over the group; f(r, acc) runs on
+// lane 0 of the group.  Every thread runs the same number of outer iterations, so all
+// lanes of a group take part in the shuffles.
+template <class Acc, int K, class T, class GX, class F>
+__device__ __forceinline__ void vcsr(int rows, const int32_t* rp, const int32_t* col, const T* val,
+                                     int L, GX gx, F f) {
+    const int groups = BB / L, g = threadIdx.x / L, lane = threadIdx.x & (L - 1);
+    for (int base = 0; base < rows; base += groups) {
+        const int r = base + g;
+        Acc acc[K];
+#pragma unroll
+        for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
+        if (r < rows) {
+            const int k1 = rp[r + 1];
+            for (int k = rp[r] + lane; k < k1; k += L) {
+                const Acc v = (Acc)val[k];
+                const int c = col[k];
+#pragma unroll
+                for (int kv = 0; kv < K; ++kv) acc[kv] += v * gx(kv, c);
+            }
+        }
+#pragma unroll
+        for (int kv = 0; kv < K; ++kv)
+            for (int o = L >> 1; o > 0; o >>= 1) acc[kv] += __shfl_xor(acc[kv], o, L);
+        if (r < rows && lane == 0) f(r, acc);
+    }
+}
+
+__device__ __forceinline__ uint64_t b_splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double pnorm_acc(double acc, double v, double p) {
+    const double a = fabs(v);
+    if (isinf(p)) return a > acc ? a : acc;
+    if (p == 1.0) return acc + a;
+    if (p == 2.0) return acc + a * a;
+    return acc + pow(a, p);
+}
+__device__ __forceinline__ double pnorm_fin(double acc, double p) {
+    if (isinf(p) || p == 1.0) return acc;
+    if (p == 2.0) return sqrt(acc);
+    return pow(acc, 1.0 / p);
+}
+
+// ---------------------------------------------------------------- setup kernel
+// preprocess + initialize for each listed instance (one workgroup each)
+__global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const BInst* insts,
+                                                    BState* states, BParams P) {
+    __shared__ double sh[8];
+    const int slot = list[blockIdx.x];
+    const BInst I = insts[slot];
+    BState* S = states + slot;
+    const int m = I.m, n = I.n, nnz = I.nnz, tid = threadIdx.x;
+    double* d1 = I.sm;
+    double* d2 = I.sn;
+    // identity + working copies (CSC order: A values and A' values)
+    for (int i = tid; i < m; i += BB) I.D1[i] = 1.0;
+    for (int j = tid; j < n; j += BB) I.D2[j] = 1.0;
+    for (int k = tid; k < nnz; k += BB) {
+        I.acsc[k] = I.nz[k];
+        I.atcsc[k] = I.nz[k];
+    }
+    __syncthreads();
+    auto norms = [&](double p_row, double p_col) {
+        for (int j = tid; j < n; j += BB) {   // columns of A = rows of CSR(A')
+            double acc = 0.0;
+            for (int k = I.at_rp[j]; k < I.at_rp[j + 1]; ++k) acc = pnorm_acc(acc, I.acsc[k], p_col);
+            const double v = pnorm_fin(acc, p_col);
+            d2[j] = v == 0.0 ? 1.0 : 1.0 / sqrt(v);
+        }
+        for (int i = tid; i < m; i += BB) {   // rows of A, from the A' values
+            double acc = 0.0;
+            for (int k = I.a_rp[i]; k < I.a_rp[i + 1]; ++k) acc = pnorm_acc(acc, I.atcsc[I.perm[k]], p_row);
+            const double v = pnorm_fin(acc, p_row);
+            d1[i] = v == 0.0 ? 1.0 : 1.0 / sqrt(v);
+        }
+        __syncthreads();
+    };
+    // (d1_i * a) * d2_j into A, (d2_j * a') * d1_i into A'  (preconditioning.jl:37-43)
+    auto apply = [&](const double* ain, const double* atin, const double* D1, const double* D2) {
+        for (int j = tid; j < n; j += BB) {
+            const double dj = D2[j];
+            for (int k = I.at_rp[j]; k < I.at_rp[j + 1]; ++k) {
+                const double di = D1[I.at_col[k]];
+                const double a = ain[k], at = atin[k];
+                I.acsc[k] = (di * a) * dj;
+                I.atcsc[k] = (dj * at) * di;
+            }
+        }
+        __syncthreads();
+    };
+    for (int it = 0; it < P.ruiz_iter; ++it) {   // ruiz_preconditioner, :117-125
+        norms(INFINITY, INFINITY);
+        apply(I.acsc, I.atcsc, d1, d2);
+        for (int i = tid; i < m; i += BB) I.D1[i] = d1[i] * I.D1[i];
+        for (int j = tid; j < n; j += BB) I.D2[j] = I.D2[j] * d2[j];
+        __syncthreads();
+    }
+    apply(I.nz, I.nz, I.D1, I.D2);   // cons_r from the originals
+    norms(2.0 - P.cp_alpha, P.cp_alpha);
+    for (int i = tid; i < m; i += BB) I.D1[i] = I.D1[i] * d1[i];
+    for (int j = tid; j < n; j += BB) I.D2[j] = d2[j] * I.D2[j];
+    __syncthreads();
+    apply(I.nz, I.nz, I.D1, I.D2);   // A_p = D1*A*D2, At_p = D2*At*D1
+    for (int k = tid; k < nnz; k += BB) {
+        const double a = I.acsc[I.perm[k]];   // CSR(A) order
+        I.a64[k] = a;
+        I.a32[k] = (float)a;
+        I.at64[k] = I.atcsc[k];
+        I.at32[k] = (float)I.atcsc[k];
+    }
+    for (int j = tid; j < n; j += BB) {
+        const double d = I.D2[j];
+        I.cs[j] = d * I.c[j];
+        I.lvs[j] = I.lv[j] / d;
+        I.uvs[j] = I.uv[j] / d;
+        I.iD2[j] = 1.0 / d;
+        I.c32[j] = (float)I.cs[j];
+        I.lv32[j] = (float)I.lvs[j];
+        I.uv32[j] = (float)I.uvs[j];
+    }
+    for (int i = tid; i < m; i += BB) {
+        const double d = I.D1[i];
+        I.lcs[i] = d * I.lc[i];
+        I.ucs[i] = d * I.uc[i];
+        I.iD1[i] = 1.0 / d;
+        I.lc32[i] = (float)I.lcs[i];
+        I.uc32[i] = (float)I.ucs[i];
+    }
+    __syncthreads();
+    // constant KKT scales and the primal_weight_init norms
+    {
+        double q[4] = {0, 0, 0, 0};
+        for (int i = tid; i < m; i += BB) {
+            const double cb = combine(I.lcs[i], I.ucs[i]);
+            const double v = I.iD1[i] * cb;
+            q[0] += v * v;
+            q[2] += cb * cb;
+        }
+        for (int j = tid; j < n; j += BB) {
+            const double v = I.iD2[j] * I.cs[j];
+            q[1] += v * v;
+            q[3] += I.cs[j] * I.cs[j];
+        }
+        bsum<4>(q, sh);
+    }
+    const double primal_scale = 1.0 + sqrt(sh[0]), dual_scale = 1.0 + sqrt(sh[1]);
+    const double comb_norm = sqrt(sh[2]), c_norm = sqrt(sh[3]);
+    __syncthreads();
+    // power method on A'A (FP64): x = I.x64[0], Ax = I.x64[1], t = I.y64[0]
+    double* x = I.x64[0];
+    double* Ax = I.x64[1];
+    double* t = I.y64[0];
+    for (int j = tid; j < n; j += BB) {
+        double acc = 0.0;
+        for (int r = 0; r < 12; ++r) {
+            const uint64_t h = b_splitmix64(P.seed * 0x100000001B3ull ^ b_splitmix64((uint64_t)j * 16u + (uint64_t)r));
+            acc += (double)(h >> 11) * 0x1.0p-53;
+        }
+        x[j] = acc - 6.0;
+    }
+    __syncthreads();
+    {
+        double q[1] = {0.0};
+        for (int j = tid; j < n; j += BB) q[0] += x[j] * x[j];
+        bsum<1>(q, sh);
+    }
+    {
+        const double inv = 1.0 / sqrt(sh[0]);
+        __syncthreads();
+        for (int j = tid; j < n; j += BB) x[j] = x[j] * inv;
+    }
+    __syncthreads();
+    const uint64_t u = (uint64_t)n;
+    const double tol = isnan(P.powm_tol) ? DBL_EPSILON * (double)(int64_t)(u * u * u) : P.powm_tol;
+    const int64_t maxiter = P.powm_maxiter < 0 ? n : P.powm_maxiter;
+    double theta = 0.0, residual = 1.7976931348623157e308;
+    int64_t it = 0;
+    while (!(it > maxiter || residual <= tol * fabs(theta))) {
+        vcsr<double, 1>(m, I.a_rp, I.a_col, I.a64, I.La, [&](int, int c) { return x[c]; },
+                        [&](int r, const double* acc) { t[r] = acc[0]; });
+        __syncthreads();
+        vcsr<double, 1>(n, I.at_rp, I.at_col, I.at64, I.Lat, [&](int, int c) { return t[c]; },
+                        [&](int r, const double* acc) { Ax[r] = acc[0]; });
+        __syncthreads();
+        double q[2] = {0.0, 0.0};
+        for (int j = tid; j < n; j += BB) {
+            q[0] += x[j] * Ax[j];
+            q[1] += Ax[j] * Ax[j];
+        }
+        bsum<2>(q, sh);
+        theta = sh[0];
+        const double inv = 1.0 / sqrt(sh[1]);
+        __syncthreads();
+        double q2[1] = {0.0};
+        for (int j = tid; j < n; j += BB) {
+            const double r = Ax[j] + (-theta) * x[j];
+            q2[0] += r * r;
+            x[j] = Ax[j] * inv;
+        }
+        bsum<1>(q2, sh);
+        residual = sqrt(sh[0]);
+        __syncthreads();
+        ++it;
+    }
+    const double eta = P.invnorm * (1.0 / sqrt(theta));
+    double omega = 1.0;
+    if (P.pdlp && c_norm > P.zero_tol && comb_norm > P.zero_tol) omega = c_norm / comb_norm;
+    // initial iterates (zero start): sol = 0, sol_last = 0, avg = sol, avg_last = 0, restart = sol
+    for (int j = tid; j < n; j += BB) {
+        I.x32[0][j] = I.x32[1][j] = I.xa32[j] = I.xal32[j] = I.xr32[j] = I.xb32[j] = 0.f;
+        I.x64[0][j] = I.x64[1][j] = I.xa64[j] = I.xal64[j] = I.xr64[j] = 0.0;
+    }
+    for (int i = tid; i < m; i += BB) {
+        I.y32[0][i] = I.y32[1][i] = I.ya32[i] = I.yal32[i] = I.yr32[i] = I.yb32[i] = 0.f;
+        I.y64[0][i] = I.y64[1][i] = I.ya64[i] = I.yal64[i] = I.yr64[i] = 0.0;
+    }
+    if (tid == 0) {
+        S->eta = eta;
+        S->omega = omega;
+        S->eta_sum = 0.0;
+        S->best_rel = INFINITY;
+        S->best_omega = omega;
+        S->primal_scale = primal_scale;
+        S->dual_scale = dual_scale;
+        for (int f = 0; f < 6; ++f) S->err[f] = NAN;
+        S->kkt_passes = S->inner = S->total = S->restarts = S->fp32_passes = 0;
+        S->powm_iters = it;
+        S->status = CPDLP_STILL_RUNNING;
+        S->phase = P.precision == CPDLP_PREC_FP64 ? 1 : 0;
+        S->cur = 0;
+        S->stall = 0;
+        S->setup = 1;
+    }
+}
+
+// ---------------------------------------------------------------- iteration kernel
+template <class V>
+struct Iter {   // the precision-V view of an instance's iterates and problem data
+    V *x[2], *y[2], *xa, *ya, *xal, *yal, *xr, *yr;
+    const V *a, *at, *c, *lv, *uv, *lc, *uc;
+};
+__device__ __forceinline__ Iter<float> iter_of(const BInst& I, float) {
+    return Iter<float>{{I.x32[0], I.x32[1]}, {I.y32[0], I.y32[1]}, I.xa32, I.ya32, I.xal32, I.yal32,
+                       I.xr32, I.yr32, I.a32, I.at32, I.c32, I.lv32, I.uv32, I.lc32, I.uc32};
+}
+__device__ __forceinline__ Iter<double> iter_of(const BInst& I, double) {
+    return Iter<double>{{I.x64[0], I.x64[1]}, {I.y64[0], I.y64[1]}, I.xa64, I.ya64, I.xal64,
+                        I.yal64, I.xr64, I.yr64, I.a64, I.at64, I.cs, I.lvs, I.uvs, I.lcs, I.ucs};
+}
+
+__device__ __forceinline__ double rel_of(const double* e) {
+    return fmax(fmax(e[0] / e[1], e[2] / e[3]), e[4] / e[5]);
+}
+__device__ __forceinline__ double abs_of(const double* e, double w) {
+    return sqrt(w * w * e[0] * e[0] + (1.0 / (w * w)) * e[2] * e[2] + e[4] * e[4]);
+}
+
+// Run up to `checks` checks on each listed instance.  LDSG: gather y / xbar from LDS
+// (dynamic shared memory of (n + m) * sizeof(V) bytes); else from global memory.
+template <class V, bool LDSG>
+__global__ __launch_bounds__(BB) void k_batch_run(const int32_t* list, const BInst* insts,
+                                                  BState* states, const int32_t* timeout,
+                                                  BParams P, int32_t checks) {
+    extern __shared__ __align__(16) unsigned char dyn[];
+    __shared__ double sh[32];
+    const int slot = list[blockIdx.x];
+    const BInst I = insts[slot];
+    BState* S = states + slot;
+    const int m = I.m, n = I.n, tid = threadIdx.x;
+    const int want_phase = std::is_same<V, double>::value ? 1 : 0;
+    if (S->setup != 1 || S->status != CPDLP_STILL_RUNNING || S->phase != want_phase) return;
+    const Iter<V> it = iter_of(I, V(0));
+    V* xbs = LDSG ? (V*)dyn : nullptr;          // xbar (n)
+    V* ys = LDSG ? (V*)dyn + n : nullptr;       // y (m)
+    V* xbar_g = (V*)I.sn;   // global xbar when the vectors do not fit in LDS (n doubles)
+    // per-instance scalars (identical in every thread)
+    const double eta = S->eta;
+    double omega = S->omega, eta_sum = S->eta_sum;
+    int cur = S->cur;
+    int64_t passes = S->kkt_passes, inner = S->inner, total = S->total;
+    const double reltol = want_phase == 0 && P.precision == CPDLP_PREC_MIXED
+                              ? fmax(P.polish_reltol, P.reltol) : P.reltol;
+    const bool polish_next = want_phase == 0 && P.precision == CPDLP_PREC_MIXED;
+    if (LDSG) {   // y of the current solution into LDS
+        for (int i = tid; i < m; i += BB) ys[i] = it.y[cur][i];
+    }
+    __syncthreads();
+    int32_t status = CPDLP_STILL_RUNNING;
+    for (int chk = 0; chk < checks && status == CPDLP_STILL_RUNNING; ++chk) {
+        const double tau = eta / omega, sigma = eta * omega, isig = 1.0 / sigma;
+        const V tv = (V)tau, sv = (V)sigma, iv = (V)isig;
+        for (int s = 0; s < P.check_every; ++s) {
+            const int p = cur, q = 1 - cur;
+            const double wa = eta / (eta + eta_sum), wb = eta_sum / (eta + eta_sum);
+            const V wav = (V)wa, wbv = (V)wb;
+            const bool last = s == P.check_every - 1;
+            const V* yp = it.y[p];
+            // primal half-step over the rows of A'
+            vcsr<V, 1>(n, I.at_rp, I.at_col, it.at, I.Lat,
+                       [&](int, int c) { return LDSG ? ys[c] : yp[c]; },
+                       [&](int j, const V* acc) {
+                           const V xj = it.x[p][j];
+                           const V xp = proj_box<V>(xj - tv * (it.c[j] - acc[0]), it.lv[j], it.uv[j]);
+                           it.x[q][j] = xp;
+                           const V xb = V(2) * xp - xj;
+                           if (LDSG) xbs[j] = xb;
+                           else xbar_g[j] = xb;
+                           if (P.pdlp) {
+                               const V a = it.xa[j];
+                               if (last) it.xal[j] = a;
+                               it.xa[j] = wav * xp + wbv * a;
+                           }
+                       });
+            __syncthreads();
+            // dual half-step over the rows of A
+            vcsr<V, 1>(m, I.a_rp, I.a_col, it.a, I.La,
+                       [&](int, int c) { return LDSG ? xbs[c] : xbar_g[c]; },
+                       [&](int i, const V* acc) {
+                           const V yi = LDSG ? ys[i] : yp[i];
+                           const V ax = acc[0];
+                           const V yn = (yi - sv * ax) - sv * proj_box<V>(iv * yi - ax, -it.uc[i], -it.lc[i]);
+                           it.y[q][i] = yn;
+                           if (LDSG) ys[i] = yn;
+                           if (P.pdlp) {
+                               const V a = it.ya[i];
+                               if (last) it.yal[i] = a;
+                               it.ya[i] = wav * yn + wbv * a;
+                           }
+                       });
+            __syncthreads();
+            cur = q;
+            if (P.pdlp) {
+                eta_sum += eta;
+                ++inner;
+                ++total;
+            }
+            ++passes;
+        }
+        // ---- check: KKT errors (FP64) of sol, avg, sol_last, avg_last, restart
+        const int c = cur, l = 1 - cur;
+        constexpr int K = 5;
+        const V* X[K] = {it.x[c], it.xa, it.x[l], it.xal, it.xr};
+        const V* Y[K] = {it.y[c], it.ya, it.y[l], it.yal, it.yr};
+        const int KK = P.pdlp ? K : 1;
+        double qa[2 * K + 2], qb[3 * K + 2];
+        for (int t2 = 0; t2 < 2 * K + 2; ++t2) qa[t2] = 0.0;
+        for (int t2 = 0; t2 < 3 * K + 2; ++t2) qb[t2] = 0.0;
+        vcsr<double, K>(m, I.a_rp, I.a_col, it.a, I.La,
+                        [&](int kv, int col) { return kv < KK ? (double)X[kv][col] : 0.0; },
+                        [&](int i, const double* acc) {
+                            const double lo = I.lcs[i], up = I.ucs[i], id = I.iD1[i];
+                            for (int k = 0; k < KK; ++k) {
+                                const double pd = id * (acc[k] - proj_box(acc[k], lo, up));
+                                qa[2 * k] += pd * pd;
+                                const double yk = (double)Y[k][i];
+                                qa[2 * k + 1] += safeprod_left(up, pos_part(-yk)) - safeprod_left(lo, neg_part(-yk));
+                            }
+                            if (P.pdlp) {
+                                const double yr = (double)it.yr[i];
+                                const double d0 = (double)Y[0][i] - yr, d1 = (double)Y[1][i] - yr;
+                                qa[2 * K] += d0 * d0;
+                                qa[2 * K + 1] += d1 * d1;
+                            }
+                        });
+        vcsr<double, K>(n, I.at_rp, I.at_col, it.at, I.Lat,
+                        [&](int kv, int col) { return kv < KK ? (double)Y[kv][col] : 0.0; },
+                        [&](int j, const double* acc) {
+                            const double cj = I.cs[j], lo = I.lvs[j], up = I.uvs[j], id = I.iD2[j];
+                            for (int k = 0; k < KK; ++k) {
+                                const double g = cj - acc[k];
+                                const double r = proj_multiplier(g, lo, up);
+                                const double dd = id * (g - r);
+                                qb[3 * k] += dd * dd;
+                                qb[3 * k + 1] += safeprod_left(up, pos_part(-r)) - safeprod_left(lo, neg_part(-r));
+                                qb[3 * k + 2] += cj * (double)X[k][j];
+                            }
+                            if (P.pdlp) {
+                                const double xr = (double)it.xr[j];
+                                const double d0 = (double)X[0][j] - xr, d1 = (double)X[1][j] - xr;
+                                qb[3 * K] += d0 * d0;
+                                qb[3 * K + 1] += d1 * d1;
+                            }
+                        });
+        bsum<2 * K + 2>(qa, sh);
+        double ra[2 * K + 2];
+        for (int t2 = 0; t2 < 2 * K + 2; ++t2) ra[t2] = sh[t2];
+        __syncthreads();
+        bsum<3 * K + 2>(qb, sh);
+        double rb[3 * K + 2];
+        for (int t2 = 0; t2 < 3 * K + 2; ++t2) rb[t2] = sh[t2];
+        __syncthreads();
+        double e[K][6];
+        for (int k = 0; k < KK; ++k) {   // errors.jl:77-98
+            e[k][0] = sqrt(ra[2 * k]);
+            e[k][1] = S->primal_scale;
+            e[k][2] = sqrt(rb[3 * k]);
+            e[k][3] = S->dual_scale;
+            const double pc = ra[2 * k + 1], pv = rb[3 * k + 1], cx = rb[3 * k + 2];
+            e[k][4] = fabs(cx + pc + pv);
+            e[k][5] = 1.0 + fabs(pc + pv) + fabs(cx);
+        }
+        // termination (termination.jl:89-101); time limit comes from the host flag
+        const double rel = rel_of(e[0]);
+        if (rel <= reltol) status = CPDLP_OPTIMAL;
+        else if (timeout[slot]) status = CPDLP_TIME_LIMIT;
+        else if (passes >= P.max_kkt_passes) status = CPDLP_ITERATION_LIMIT;
+        if (tid == 0) {
+            for (int f = 0; f < 6; ++f) S->err[f] = e[0][f];
+        }
+        int restart = 0, from_avg = 0, switch_phase = 0, improve = 0;
+        if (status == CPDLP_STILL_RUNNING && polish_next) {
+            // FP32 phase: keep the best iterate; hand over on stall / blow-up
+            if (rel < S->best_rel) improve = 1;
+            else if (S->stall + 1 >= STALL || rel > 100.0 * S->best_rel) switch_phase = 1;
+        }
+        if (status == CPDLP_OPTIMAL && polish_next && P.reltol < reltol) {
+            improve = 1;          // the terminating sol is the hand-over point
+            switch_phase = 1;
+            status = CPDLP_STILL_RUNNING;
+        }
+        if (status == CPDLP_STILL_RUNNING && !switch_phase && P.pdlp) {
+            // restart_check! + should_restart (pdlp.jl:138-170, restart.jl:40-57)
+            from_avg = !(abs_of(e[0], omega) < abs_of(e[1], omega));
+            const double* cand = from_avg ? e[1] : e[0];
+            const double* cl = abs_of(e[2], omega) < abs_of(e[3], omega) ? e[2] : e[3];
+            const double ac = abs_of(cand, omega), ar = abs_of(e[4], omega), acl = abs_of(cl, omega);
+            const bool sufficient = ac <= P.suff * ar;
+            const bool necessary = ac <= P.nec * ar;
+            const bool no_local = ac > acl;
+            const bool long_inner = (double)inner >= P.art * (double)total;
+            if (sufficient || (necessary && no_local) || long_inner) {
+                restart = 1;
+                const double dx = sqrt(rb[3 * K + from_avg]), dy = sqrt(ra[2 * K + from_avg]);
+                if (dx > P.zero_tol && dy > P.zero_tol)
+                    omega = exp(P.damping * log(dy / dx) + (1.0 - P.damping) * log(omega));
+                eta_sum = 0.0;
+                inner = 0;
+            }
+        }
+        __syncthreads();
+        if (improve) {
+            for (int j = tid; j < n; j += BB) I.xb32[j] = (float)X[0][j];
+            for (int i = tid; i < m; i += BB) I.yb32[i] = (float)Y[0][i];
+            if (tid == 0) {
+                S->best_rel = rel;
+                S->best_omega = omega;
+                S->stall = 0;
+            }
+        } else if (polish_next && status == CPDLP_STILL_RUNNING && !switch_phase && tid == 0) {
+            S->stall += 1;
+        }
+        if (restart) {   // restart!: sol <- candidate; avg <- 0; restart <- sol
+            for (int j = tid; j < n; j += BB) {
+                V s0 = it.x[c][j];
+                if (from_avg) s0 = it.xa[j];
+                it.x[c][j] = s0;
+                it.xa[j] = V(0);
+                it.xr[j] = s0;
+            }
+            for (int i = tid; i < m; i += BB) {
+                V s0 = it.y[c][i];
+                if (from_avg) s0 = it.ya[i];
+                it.y[c][i] = s0;
+                it.ya[i] = V(0);
+                it.yr[i] = s0;
+                if (LDSG) ys[i] = s0;
+            }
+            if (tid == 0) S->restarts += 1;
+        }
+        __syncthreads();
+        if (switch_phase) {
+            // FP64 polish from the best FP32 iterate: sol = avg = restart = best
+            for (int j = tid; j < n; j += BB) {
+                const double v = (double)I.xb32[j];
+                I.x64[0][j] = I.xa64[j] = I.xr64[j] = v;
+                I.x64[1][j] = I.xal64[j] = 0.0;
+            }
+            for (int i = tid; i < m; i += BB) {
+                const double v = (double)I.yb32[i];
+                I.y64[0][i] = I.ya64[i] = I.yr64[i] = v;
+                I.y64[1][i] = I.yal64[i] = 0.0;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                S->phase = 1;
+                S->fp32_passes = passes;
+                S->omega = S->best_omega;
+                S->eta_sum = 0.0;
+                S->inner = 0;
+                S->total = 0;
+                S->cur = 0;
+                S->kkt_passes = passes;
+                S->status = CPDLP_STILL_RUNNING;
+            }
+            return;
+        }
+    }
+    if (tid == 0) {
+        S->omega = omega;
+        S->eta_sum = eta_sum;
+        S->cur = cur;
+        S->kkt_passes = passes;
+        S->inner = inner;
+        S->total = total;
+        S->status = status;
+    }
+}
+
+// unscaled solution of an instance into a staging buffer: x = D2 .* x~, y = D1 .* y~
+__global__ void k_batch_unscale(const BInst* insts, const BState* states, int32_t slot, double* x,
+                                double* y) {
+    const BInst I = insts[slot];
+    const BState* S = states + slot;
+    const int c = S->cur;
+    const bool f64 = S->phase == 1;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < I.n; j += gridDim.x * blockDim.x)
+        x[j] = I.D2[j] * (f64 ? I.x64[c][j] : (double)I.x32[c][j]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < I.m; i += gridDim.x * blockDim.x)
+        y[i] = I.D1[i] * (f64 ? I.y64[c][i] : (double)I.y32[c][i]);
+}
+
+// ================================================================ host side
+
+static double bnow() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// bytes of one instance in the arena (same order as carve())
+static size_t footprint(int64_t m, int64_t n, int64_t nnz) {
+    size_t b = 0;
+    auto add = [&](int64_t count, size_t sz) { b += align256((size_t)std::max<int64_t>(count, 1) * sz); };
+    add(m + 1, 4); add(nnz, 4); add(n + 1, 4); add(nnz, 4); add(nnz, 4);       // ints
+    for (int k = 0; k < 5; ++k) add(nnz, 8);                                    // nz, acsc, atcsc, a64, at64
+    add(nnz, 4); add(nnz, 4);                                                   // a32, at32
+    for (int k = 0; k < 3; ++k) add(n, 8);                                      // c, lv, uv
+    for (int k = 0; k < 2; ++k) add(m, 8);                                      // lc, uc
+    for (int k = 0; k < 3; ++k) add(n, 8);                                      // cs, lvs, uvs
+    for (int k = 0; k < 2; ++k) add(m, 8);                                      // lcs, ucs
+    add(m, 8); add(n, 8); add(m, 8); add(n, 8);                                 // D1, D2, iD1, iD2
+    for (int k = 0; k < 3; ++k) add(n, 4);                                      // c32, lv32, uv32
+    for (int k = 0; k < 2; ++k) add(m, 4);                                      // lc32, uc32
+    for (int k = 0; k < 6; ++k) add(n, 4);                                      // x32 x2, xa, xal, xr, xb
+    for (int k = 0; k < 6; ++k) add(m, 4);                                      // y32 x2, ya, yal, yr, yb
+    for (int k = 0; k < 6; ++k) add(n, 8);                                      // x64 x2, xa, xal, xr, sn
+    for (int k = 0; k < 6; ++k) add(m, 8);                                      // y64 x2, ya, yal, yr, sm
+    return b;
+}
+
+static BInst carve(char* base, int64_t m, int64_t n, int64_t nnz) {
+    BInst I{};
+    size_t off = 0;
+    auto take = [&](int64_t count, size_t sz) {
+        char* p = base + off;
+        off += align256((size_t)std::max<int64_t>(count, 1) * sz);
+        return (void*)p;
+    };
+    I.m = (int32_t)m;
+    I.n = (int32_t)n;
+    I.nnz = (int32_t)nnz;
+    I.a_rp = (int32_t*)take(m + 1, 4);
+    I.a_col = (int32_t*)take(nnz, 4);
+    I.at_rp = (int32_t*)take(n + 1, 4);
+    I.at_col = (int32_t*)take(nnz, 4);
+    I.perm = (int32_t*)take(nnz, 4);
+    I.nz = (double*)take(nnz, 8);
+    I.acsc = (double*)take(nnz, 8);
+    I.atcsc = (double*)take(nnz, 8);
+    I.a64 = (double*)take(nnz, 8);
+    I.at64 = (double*)take(nnz, 8);
+    I.a32 = (float*)take(nnz, 4);
+    I.at32 = (float*)take(nnz, 4);
+    I.c = (double*)take(n, 8);
+    I.lv = (double*)take(n, 8);
+    I.uv = (double*)take(n, 8);
+    I.lc = (double*)take(m, 8);
+    I.uc = (double*)take(m, 8);
+    I.cs = (double*)take(n, 8);
+    I.lvs = (double*)take(n, 8);
+    I.uvs = (double*)take(n, 8);
+    I.lcs = (double*)take(m, 8);
+    I.ucs = (double*)take(m, 8);
+    I.D1 = (double*)take(m, 8);
+    I.D2 = (double*)take(n, 8);
+    I.iD1 = (double*)take(m, 8);
+    I.iD2 = (double*)take(n, 8);
+    I.c32 = (float*)take(n, 4);
+    I.lv32 = (float*)take(n, 4);
+    I.uv32 = (float*)take(n, 4);
+    I.lc32 = (float*)take(m, 4);
+    I.uc32 = (float*)take(m, 4);
+    I.x32[0] = (float*)take(n, 4);
+    I.x32[1] = (float*)take(n, 4);
+    I.xa32 = (float*)take(n, 4);
+    I.xal32 = (float*)take(n, 4);
+    I.xr32 = (float*)take(n, 4);
+    I.xb32 = (float*)take(n, 4);
+    I.y32[0] = (float*)take(m, 4);
+    I.y32[1] = (float*)take(m, 4);
+    I.ya32 = (float*)take(m, 4);
+    I.yal32 = (float*)take(m, 4);
+    I.yr32 = (float*)take(m, 4);
+    I.yb32 = (float*)take(m, 4);
+    I.x64[0] = (double*)take(n, 8);
+    I.x64[1] = (double*)take(n, 8);
+    I.xa64 = (double*)take(n, 8);
+    I.xal64 = (double*)take(n, 8);
+    I.xr64 = (double*)take(n, 8);
+    I.sn = (double*)take(n, 8);
+    I.y64[0] = (double*)take(m, 8);
+    I.y64[1] = (double*)take(m, 8);
+    I.ya64 = (double*)take(m, 8);
+    I.yal64 = (double*)take(m, 8);
+    I.yr64 = (double*)take(m, 8);
+    I.sm = (double*)take(m, 8);
+    return I;
+}
+
+// lanes per row: a power of two near half the mean row length, in [2, 64]
+static int lanes_for(int64_t nnz, int64_t rows) {
+    const double mean = rows > 0 ? (double)nnz / rows : 1.0;
+    int L = 2;
+    while (L < 64 && L * 2 <= mean) L *= 2;
+    return L;
+}
+
+// first-fit arena allocator with coalescing free list (offsets in bytes)
+struct Arena {
+    size_t size = 0;
+    std::map<size_t, size_t> free_;   // offset -> length
+    size_t used = 0;
+    void init(size_t s) {
+        size = s;
+        free_.clear();
+        free_[0] = s;
+        used = 0;
+    }
+    bool alloc(size_t len, size_t& off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= len) {
+                off = it->first;
+                const size_t rest = it->second - len;
+                free_.erase(it);
+                if (rest) free_[off + len] = rest;
+                used += len;
+                return true;
+            }
+        return false;
+    }
+    void release(size_t off, size_t len) {
+        auto it = free_.emplace(off, len).first;
+        used -= len;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct BHost {
+    int64_t id;
+    int32_t slot, m, n, nnz;
+    size_t off, bytes;
+    bool finished = false, polled = false;
+    double t_start = 0.0, t_done = 0.0;
+    int32_t status = CPDLP_STILL_RUNNING;
+};
+
+struct Batch {
+    cpdlp_params P;
+    int device = 0;
+    hipStream_t streams[4] = {};
+    DevBuf<char> arena_mem;
+    Arena arena;
+    int32_t max_slots = 0;
+    DevBuf<BInst> d_inst;
+    DevBuf<BState> d_state;
+    DevBuf<int32_t> d_timeout, d_list;
+    std::vector<BInst> h_inst;
+    std::vector<BState> h_state;
+    std::vector<int32_t> h_timeout, free_slots;
+    std::map<int64_t, BHost> inst;   // by id
+    std::deque<int64_t> done;
+    std::vector<int32_t> pending_setup;
+    int64_t next_id = 0;
+    int32_t hi_slot = 0;   // slots [0, hi_slot) may be in use
+    size_t lds_max = 65536;  // dynamic LDS a workgroup may take (device limit minus static use)
+
+    Batch(const cpdlp_params& p, int64_t bytes) : P(p) {
+        int ndev = 0;
+        HIP_CHECK(hipGetDeviceCount(&ndev));
+        if (P.device < 0 || P.device >= ndev) throw Error(CPDLP_ERR_ARGUMENT, "invalid device ordinal");
+        if (P.check_every <= 0) throw Error(CPDLP_ERR_ARGUMENT, "check_every must be positive");
+        device = P.device;
+        HIP_CHECK(hipSetDevice(device));
+        for (auto& s : streams) HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        if (bytes <= 0) throw Error(CPDLP_ERR_ARGUMENT, "arena size must be positive");
+        arena_mem.alloc((size_t)bytes);
+        arena.init((size_t)bytes);
+        int lds = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+        lds_max = lds > 4096 ? (size_t)lds - 4096 : 0;
+        max_slots = 1 << 16;
+        d_inst.alloc(max_slots);
+        d_state.alloc(max_slots);
+        d_timeout.alloc(max_slots);
+        d_list.alloc(max_slots);
+        h_inst.resize(max_slots);
+        h_state.resize(max_slots);
+        h_timeout.assign(max_slots, 0);
+        for (int32_t s = max_slots - 1; s >= 0; --s) free_slots.push_back(s);
+    }
+    ~Batch() {
+        (void)hipSetDevice(device);
+        for (auto& s : streams)
+            if (s) (void)hipStreamDestroy(s);
+    }
+
+    BParams params() const {
+        BParams b{};
+        b.check_every = P.check_every;
+        b.ruiz_iter = P.ruiz_iter;
+        b.pdlp = P.algorithm == CPDLP_ALGO_PDLP;
+        b.precision = P.precision == CPDLP_PREC_FP64 ? CPDLP_PREC_FP64 : CPDLP_PREC_MIXED;
+        b.cp_alpha = P.chambolle_pock_alpha;
+        b.invnorm = P.invnorm_scaling;
+        b.damping = P.primal_weight_damping;
+        b.zero_tol = P.zero_tol;
+        b.suff = P.sufficient_decay;
+        b.nec = P.necessary_decay;
+        b.art = P.artificial_decay;
+        b.reltol = P.termination_reltol;
+        b.polish_reltol = P.polish_reltol;
+        b.powm_tol = P.powm_tol;
+        b.max_kkt_passes = P.max_kkt_passes;
+        b.powm_maxiter = P.powm_maxiter;
+        b.seed = P.powm_seed;
+        return b;
+    }
+
+    int64_t add(const cpdlp_lp& lp) {
+        HIP_CHECK(hipSetDevice(device));
+        std::vector<int32_t> cp, rv, rp, col, perm;
+        read_csc(lp.m, lp.n, lp.index_base, lp.index_bits, lp.colptr, lp.rowval, cp, rv);
+        const int64_t m = lp.m, n = lp.n, nnz = cp[n];
+        if (m + n > 65536) throw Error(CPDLP_ERR_ARGUMENT, "batched instances need m + n <= 65536");
+        if (nnz > 0 && !lp.nzval) throw Error(CPDLP_ERR_ARGUMENT, "nzval is NULL");
+        for (const void* v : {(const void*)lp.c, (const void*)lp.lv, (const void*)lp.uv})
+            if (n > 0 && !v) throw Error(CPDLP_ERR_ARGUMENT, "NULL variable vector");
+        for (const void* v : {(const void*)lp.lc, (const void*)lp.uc})
+            if (m > 0 && !v) throw Error(CPDLP_ERR_ARGUMENT, "NULL constraint vector");
+        if (free_slots.empty()) throw Error(CPDLP_ERR_MEMORY, "no free batch slot");
+        const size_t bytes = footprint(m, n, nnz);
+        size_t off = 0;
+        if (!arena.alloc(bytes, off)) throw Error(CPDLP_ERR_MEMORY, "batch arena full");
+        csc_to_csr(m, n, cp, rv, rp, col, perm);
+        BInst I = carve(arena_mem.p + off, m, n, nnz);
+        I.La = lanes_for(nnz, m);
+        I.Lat = lanes_for(nnz, n);
+        const hipStream_t s = streams[0];
+        auto up = [&](void* dst, const void* src, size_t b) {
+            if (b) HIP_CHECK(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, s));
+        };
+        up(I.a_rp, rp.data(), (m + 1) * 4);
+        up(I.a_col, col.data(), nnz * 4);
+        up(I.at_rp, cp.data(), (n + 1) * 4);
+        up(I.at_col, rv.data(), nnz * 4);
+        up(I.perm, perm.data(), nnz * 4);
+        up(I.nz, lp.nzval, nnz * 8);
+        up(I.c, lp.c, n * 8);
+        up(I.lv, lp.lv, n * 8);
+        up(I.uv, lp.uv, n * 8);
+        up(I.lc, lp.lc, m * 8);
+        up(I.uc, lp.uc, m * 8);
+        const int32_t slot = free_slots.back();
+        free_slots.pop_back();
+        hi_slot = std::max(hi_slot, slot + 1);
+        h_inst[slot] = I;
+        BState st{};
+        st.status = CPDLP_STILL_RUNNING;
+        st.setup = 0;
+        h_state[slot] = st;
+        h_timeout[slot] = 0;
+        HIP_CHECK(hipMemcpyAsync(d_inst.p + slot, &h_inst[slot], sizeof(BInst), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(d_state.p + slot, &h_state[slot], sizeof(BState), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(d_timeout.p + slot, &h_timeout[slot], 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));   // host vectors above go out of scope
+        BHost h;
+        h.id = next_id++;
+        h.slot = slot;
+        h.m = (int32_t)m;
+        h.n = (int32_t)n;
+        h.nnz = (int32_t)nnz;
+        h.off = off;
+        h.bytes = bytes;
+        h.t_start = bnow();
+        inst[h.id] = h;
+        pending_setup.push_back(slot);
+        return h.id;
+    }
+
+    void sync_all() {
+        for (auto& s : streams) HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    int64_t run(int32_t checks) {
+        HIP_CHECK(hipSetDevice(device));
+        if (checks <= 0) throw Error(CPDLP_ERR_ARGUMENT, "max_checks must be positive");
+        const BParams bp = params();
+        std::vector<int32_t> lists;
+        // 1. setup of newly admitted instances
+        if (!pending_setup.empty()) {
+            lists = pending_setup;
+            HIP_CHECK(hipMemcpyAsync(d_list.p, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, streams[0]));
+            hipLaunchKernelGGL(k_batch_setup, dim3((int)lists.size()), dim3(BB), 0, streams[0], d_list.p,
+                               d_inst.p, d_state.p, bp);
+            HIP_CHECK(hipGetLastError());
+            d_state.download(h_state.data(), hi_slot, streams[0]);
+            HIP_CHECK(hipStreamSynchronize(streams[0]));
+            pending_setup.clear();
+        }
+        // 2. time limits (host clock per instance)
+        const double t = bnow();
+        bool any_timeout = false;
+        for (auto& kv : inst) {
+            BHost& h = kv.second;
+            if (h.finished) continue;
+            if (t - h.t_start >= P.time_limit && !h_timeout[h.slot]) {
+                h_timeout[h.slot] = 1;
+                any_timeout = true;
+            }
+        }
+        if (any_timeout) {
+            d_timeout.upload(h_timeout.data(), hi_slot, streams[0]);
+            HIP_CHECK(hipStreamSynchronize(streams[0]));
+        }
+        // 3. one launch per (phase, LDS class)
+        std::vector<int32_t> cls[2][4];
+        for (auto& kv : inst) {
+            const BHost& h = kv.second;
+            if (h.finished) continue;
+            const BState& st = h_state[h.slot];
+            const int ph = st.phase;
+            const size_t lds = (size_t)(h.n + h.m) * (ph ? 8 : 4);
+            const int c = lds <= std::min<size_t>(16384, lds_max) ? 0
+                        : lds <= std::min<size_t>(40960, lds_max) ? 1
+                        : lds <= std::min<size_t>(98304, lds_max) ? 2 : 3;
+            cls[ph][c].push_back(h.slot);
+        }
+        lists.clear();
+        std::vector<std::pair<int, int>> spans;   // (offset, count) into lists per launch
+        for (int ph = 0; ph < 2; ++ph)
+            for (int c = 0; c < 4; ++c) {
+                spans.emplace_back((int)lists.size(), (int)cls[ph][c].size());
+                lists.insert(lists.end(), cls[ph][c].begin(), cls[ph][c].end());
+            }
+        int64_t active = (int64_t)lists.size();
+        if (active > 0) {
+            HIP_CHECK(hipMemcpyAsync(d_list.p, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, streams[0]));
+            hipEvent_t ready;
+            HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+            HIP_CHECK(hipEventRecord(ready, streams[0]));
+            const size_t lds_bytes[4] = {std::min<size_t>(16384, lds_max), std::min<size_t>(40960, lds_max),
+                                         std::min<size_t>(98304, lds_max), 0};
+            int si = 0;
+            for (int ph = 0; ph < 2; ++ph)
+                for (int c = 0; c < 4; ++c) {
+                    const auto sp = spans[ph * 4 + c];
+                    if (sp.second == 0) continue;
+                    const hipStream_t s = streams[si++ % 4];
+                    HIP_CHECK(hipStreamWaitEvent(s, ready, 0));
+                    const int32_t* L = d_list.p + sp.first;
+                    if (ph == 0) {
+                        if (c < 3)
+                            hipLaunchKernelGGL((k_batch_run<float, true>), dim3(sp.second), dim3(BB), lds_bytes[c], s,
+                                               L, d_inst.p, d_state.p, d_timeout.p, bp, checks);
+                        else
+                            hipLaunchKernelGGL((k_batch_run<float, false>), dim3(sp.second), dim3(BB), 0, s, L,
+                                               d_inst.p, d_state.p, d_timeout.p, bp, checks);
+                    } else {
+                        if (c < 3)
+                            hipLaunchKernelGGL((k_batch_run<double, true>), dim3(sp.second), dim3(BB), lds_bytes[c], s,
+                                               L, d_inst.p, d_state.p, d_timeout.p, bp, checks);
+                        else
+                            hipLaunchKernelGGL((k_batch_run<double, false>), dim3(sp.second), dim3(BB), 0, s, L,
+                                               d_inst.p, d_state.p, d_timeout.p, bp, checks);
+                    }
+                    HIP_CHECK(hipGetLastError());
+                }
+            sync_all();
+            (void)hipEventDestroy(ready);
+        }
+        // 4. states back; harvest finished instances
+        if (hi_slot > 0) {
+            d_state.download(h_state.data(), hi_slot, streams[0]);
+            HIP_CHECK(hipStreamSynchronize(streams[0]));
+        }
+        int64_t running = 0;
+        const double t2 = bnow();
+        for (auto& kv : inst) {
+            BHost& h = kv.second;
+            if (h.finished) continue;
+            const BState& st = h_state[h.slot];
+            if (st.status != CPDLP_STILL_RUNNING) {
+                h.finished = true;
+                h.status = st.status;
+                h.t_done = t2;
+                done.push_back(h.id);
+            } else {
+                ++running;
+            }
+        }
+        return running;
+    }
+
+    BHost& get_host(int64_t id) {
+        auto it = inst.find(id);
+        if (it == inst.end()) throw Error(CPDLP_ERR_ARGUMENT, "unknown batch instance id");
+        return it->second;
+    }
+
+    void get(int64_t id, double* x, double* y, cpdlp_stats* st) {
+        HIP_CHECK(hipSetDevice(device));
+        BHost& h = get_host(id);
+        if (x || y) {
+            if (!h_state[h.slot].setup) throw Error(CPDLP_ERR_STATE, "instance not set up yet: run the batch first");
+            DevBuf<double> xo(std::max<int32_t>(h.n, 1)), yo(std::max<int32_t>(h.m, 1));
+            hipLaunchKernelGGL(k_batch_unscale, dim3(64), dim3(256), 0, streams[0], d_inst.p, d_state.p, h.slot,
+                               xo.p, yo.p);
+            HIP_CHECK(hipGetLastError());
+            if (x) xo.download(x, h.n, streams[0]);
+            if (y) yo.download(y, h.m, streams[0]);
+            HIP_CHECK(hipStreamSynchronize(streams[0]));
+        }
+        if (st) {
+            const BState& s = h_state[h.slot];
+            *st = cpdlp_stats{};
+            st->err = cpdlp_kkt{s.err[0], s.err[1], s.err[2], s.err[3], s.err[4], s.err[5]};
+            st->time_elapsed = (h.finished ? h.t_done : bnow()) - h.t_start;
+            st->kkt_passes = s.kkt_passes;
+            st->termination_status = s.status;
+            st->format_A = st->format_At = CPDLP_FMT_CSR;
+            st->eta = s.eta;
+            st->omega = s.omega;
+            st->powm_iters = s.powm_iters;
+            st->restarts = s.restarts;
+            st->fp32_passes = s.phase == 1 ? s.fp32_passes : s.kkt_passes;
+        }
+    }
+
+    void release(int64_t id) {
+        BHost& h = get_host(id);
+        if (!h.finished) throw Error(CPDLP_ERR_STATE, "instance still running");
+        arena.release(h.off, h.bytes);
+        free_slots.push_back(h.slot);
+        inst.erase(id);
+    }
+};
+
+}  // namespace cpdlp
+
+using namespace cpdlp;
+
+struct cpdlp_batch {
+    std::unique_ptr<Batch> b;
+};
+
+extern "C" {
+
+int cpdlp_batch_create(const cpdlp_params* params, int64_t hbm_bytes, cpdlp_batch** out) {
+    return guarded([&] {
+        if (!out) throw Error(CPDLP_ERR_ARGUMENT, "NULL output");
+        cpdlp_params p;
+        if (params) p = *params;
+        else cpdlp_default_params(&p, CPDLP_ALGO_PDLP);
+        auto* b = new cpdlp_batch;
+        try {
+            b->b = std::make_unique<Batch>(p, hbm_bytes);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+#define BATCH_OR_THROW(bp) \
+    if (!(bp) || !(bp)->b) throw Error(CPDLP_ERR_ARGUMENT, "NULL batch");
+
+int cpdlp_batch_add(cpdlp_batch* b, const cpdlp_lp* lp, int64_t* id) {
+    return guarded([&] {
+        BATCH_OR_THROW(b);
+        if (!lp || !id) throw Error(CPDLP_ERR_ARGUMENT, "NULL argument");
+        *id = b->b->add(*lp);
+    });
+}
+
+int64_t cpdlp_batch_footprint(int64_t m, int64_t n, int64_t nnz) {
+    return (int64_t)footprint(m, n, nnz);
+}
+
+int cpdlp_batch_run(cpdlp_batch* b, int32_t max_checks, int64_t* running) {
+    return guarded([&] {
+        BATCH_OR_THROW(b);
+        const int64_t r = b->b->run(max_checks);
+        if (running) *running = r;
+    });
+}
+
+int cpdlp_batch_poll(cpdlp_batch* b, int64_t* ids, int64_t cap, int64_t* count) {
+    return guarded([&] {
+        BATCH_OR_THROW(b);
+        int64_t k = 0;
+        auto& d = b->b->done;
+        while (!d.empty() && k < cap) {
+            if (ids) ids[k] = d.front();
+            d.pop_front();
+            ++k;
+        }
+        if (count) *count = k;
+    });
+}
+
+int cpdlp_batch_get(cpdlp_batch* b, int64_t id, double* x, double* y, cpdlp_stats* st) {
+    return guarded([&] {
+        BATCH_OR_THROW(b);
+        b->b->get(id, x, y, st);
+    });
+}
+
+int cpdlp_batch_release(cpdlp_batch* b, int64_t id) {
+    return guarded([&] {
+        BATCH_OR_THROW(b);
+        b->b->release(id);
+    });
+}
+
+int cpdlp_batch_memory(cpdlp_batch* b, int64_t* used, int64_t* size) {
+    return guarded([&] {
+        BATCH_OR_THROW(b);
+        if (used) *used = (int64_t)b->b->arena.used;
+        if (size) *size = (int64_t)b->b->arena.size;
+    });
+}
+
+void cpdlp_batch_destroy(cpdlp_batch* b) { delete b; }
+
+}  // extern "C"

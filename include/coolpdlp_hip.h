@@ -186,6 +186,47 @@ const char* cpdlp_last_error(void);
 /* library version string and the device architecture it was built for */
 const char* cpdlp_version(void);
 
+/* ---- batched / served solves (no reference counterpart: the reference solves one
+ * LP per call to solve, src/algorithms/common.jl:186-209) ----------------------------
+ * A batch holds many independent LPs ("instances") in HBM and runs the same PDLP
+ * algorithm on all of them, one workgroup per instance, with per-instance scaling,
+ * step size, restarts and termination (all on the device).  Instances can be added
+ * while others are running (continuous batching): cpdlp_batch_run advances every
+ * admitted instance by a bounded number of checks and returns; finished instances
+ * are reported by cpdlp_batch_poll and their HBM is returned by cpdlp_batch_release.
+ * Precision follows params->precision (MIXED: FP32 bulk, FP64 checks + FP64 polish).
+ * Each instance must satisfy m + n <= 65536 (its gathered vectors live in LDS). */
+typedef struct cpdlp_batch cpdlp_batch;
+
+/* hbm_bytes: arena size for instance data (bytes of HBM reserved up front). */
+int cpdlp_batch_create(const cpdlp_params* params, int64_t hbm_bytes, cpdlp_batch** out);
+
+/* Admit one LP (host arrays, copied); *id identifies it.  Returns CPDLP_ERR_MEMORY
+ * (and admits nothing) when the arena lacks room: the caller retries after a release. */
+int cpdlp_batch_add(cpdlp_batch* b, const cpdlp_lp* lp, int64_t* id);
+
+/* HBM footprint an LP of this size would take in the arena. */
+int64_t cpdlp_batch_footprint(int64_t m, int64_t n, int64_t nnz);
+
+/* Advance every admitted, unfinished instance by up to max_checks checks (each check
+ * = check_every PDHG steps + the KKT/restart check); blocks until done.  Returns the
+ * number of still-running instances in *running. */
+int cpdlp_batch_run(cpdlp_batch* b, int32_t max_checks, int64_t* running);
+
+/* Ids of instances finished since the last poll (up to cap); *count = number written. */
+int cpdlp_batch_poll(cpdlp_batch* b, int64_t* ids, int64_t cap, int64_t* count);
+
+/* Unscaled solution and stats of an instance (finished or not). */
+int cpdlp_batch_get(cpdlp_batch* b, int64_t id, double* x, double* y, cpdlp_stats* st);
+
+/* Return an instance's HBM to the arena (after cpdlp_batch_get). */
+int cpdlp_batch_release(cpdlp_batch* b, int64_t id);
+
+/* Bytes of the arena in use and its size. */
+int cpdlp_batch_memory(cpdlp_batch* b, int64_t* used, int64_t* size);
+
+void cpdlp_batch_destroy(cpdlp_batch* b);
+
 /* ---- matrix plug-in: GPUSparseMatrixCSR / GPUSparseMatrixELL + mul! ----------------
  * GPUSparseMatrixCSR(A::SparseMatrixCSC), src/utils/mat_csr.jl:58-61;
  * GPUSparseMatrixELL(A), src/utils/mat_ell.jl:54-70; values FP64.  transpose != 0

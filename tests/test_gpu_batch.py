@@ -1,0 +1,113 @@
+"""GPU tests of the batched engine (many LPs, one workgroup each), through the C ABI.
+
+Bars: every instance reaches OPTIMAL at its tolerance with the objective within
+20 x reltol (relative) of the closed form c'x0 and of the oracle's FP64 solve; an
+instance's result is bit-identical whether it is solved alone or in a mix, and
+whatever the round length; the FP64 batch agrees with the plain engine's FP64 solve.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from coolpdlp_hip import generators as G
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cp():
+    import coolpdlp_hip
+    return coolpdlp_hip
+
+
+def _mix(count, seed):
+    return [G.column_lp(m, n, (6, 12), s, name=f"b{i}")
+            for i, (m, n, s) in enumerate(G.batch_specs(count, 50, 600, seed))]
+
+
+@pytest.mark.parametrize("precision,reltol", [("mixed", 1e-6), ("float64", 1e-6)])
+def test_batch_matches_closed_form_and_oracle(cp, precision, reltol):
+    lps = _mix(24, seed=7)
+    with cp.BatchSolver(cp.PDLP(precision, termination_reltol=reltol), hbm_bytes=256 << 20) as B:
+        res = B.solve_all(lps, checks_per_round=7)
+    for g, (sol, st) in zip(lps, res):
+        assert st.termination_status == cp.OPTIMAL, (g.name, st)
+        assert st.err.relative() <= reltol
+        obj = float(sol.x @ g.c)
+        assert abs(obj - g.objective) <= 20 * reltol * (1 + abs(g.objective)), (g.name, obj, g.objective)
+        if precision == "mixed":
+            assert st.fp32_passes > 0
+    for g, (sol, st) in list(zip(lps, res))[:4]:
+        ref = O.solve(g, O.default_params(termination_reltol=reltol))
+        assert ref.status == "OPTIMAL"
+        assert abs(float(sol.x @ g.c) - float(ref.x @ g.c)) <= 20 * reltol * (1 + abs(g.objective))
+
+
+def test_batch_fp64_agrees_with_plain_engine(cp):
+    g = G.column_lp(200, 450, 6, seed=31)
+    milp = cp.MILP.from_generated(g)
+    sol_p, st_p = cp.solve(milp, cp.PDLP(termination_reltol=1e-7))
+    with cp.BatchSolver(cp.PDLP("float64", termination_reltol=1e-7), hbm_bytes=64 << 20) as B:
+        (sol_b, st_b), = B.solve_all([g])
+    assert st_p.termination_status == st_b.termination_status == cp.OPTIMAL
+    assert abs(st_p.eta - st_b.eta) <= 1e-9 * st_p.eta
+    assert abs(st_b.kkt_passes - st_p.kkt_passes) <= 300
+    assert abs(float(sol_b.x @ g.c) - float(sol_p.x @ g.c)) <= 1e-5 * (1 + abs(g.objective))
+
+
+def test_alone_equals_in_mix_and_round_length_free(cp):
+    lps = _mix(12, seed=9)
+    target = lps[5]
+    algo = cp.PDLP("mixed", termination_reltol=1e-6)
+    with cp.BatchSolver(algo, hbm_bytes=128 << 20) as B:
+        (alone, st_alone), = B.solve_all([target], checks_per_round=3)
+    with cp.BatchSolver(algo, hbm_bytes=128 << 20) as B:
+        mix = B.solve_all(lps, checks_per_round=11)
+    sol_mix, st_mix = mix[5]
+    assert st_alone.kkt_passes == st_mix.kkt_passes
+    assert np.array_equal(alone.x, sol_mix.x) and np.array_equal(alone.y, sol_mix.y)
+
+
+def test_continuous_admission_and_release(cp):
+    """Instances admitted while others run finish with the same results; the arena
+    refuses what does not fit and accepts it after a release."""
+    lps = _mix(6, seed=11)
+    algo = cp.PDLP("mixed", termination_reltol=1e-6)
+    with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
+        ref = B.solve_all(lps, checks_per_round=5)
+    with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
+        ids = {B.add(lps[0]): 0, B.add(lps[1]): 1}
+        B.run(2)
+        ids[B.add(lps[2])] = 2
+        B.run(1)
+        for k in (3, 4, 5):
+            ids[B.add(lps[k])] = k
+        got = {}
+        while len(got) < 6:
+            B.run(4)
+            for i in B.poll():
+                got[ids[i]] = B.get(i)
+                B.release(i)
+        for k in range(6):
+            assert np.array_equal(got[k][0].x, ref[k][0].x)
+            assert got[k][1].kkt_passes == ref[k][1].kkt_passes
+    g = lps[0]
+    fp = cp.batch.footprint(g.m, g.n, g.nnz)
+    with cp.BatchSolver(algo, hbm_bytes=fp + 1024) as B:
+        i0 = B.add(g)
+        with pytest.raises(MemoryError):
+            B.add(lps[1])
+        while B.run(10) > 0:
+            pass
+        B.get(i0)
+        B.release(i0)
+        B.add(lps[1])   # fits again
+
+
+def test_batch_limits(cp):
+    lps = _mix(3, seed=13)
+    with cp.BatchSolver(cp.PDLP("mixed", termination_reltol=1e-12, max_kkt_passes=300),
+                        hbm_bytes=64 << 20) as B:
+        res = B.solve_all(lps)
+    for sol, st in res:
+        assert st.termination_status == cp.ITERATION_LIMIT and st.kkt_passes == 300
