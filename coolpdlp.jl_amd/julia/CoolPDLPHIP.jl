@@ -1,0 +1,222 @@
+"""
+    CoolPDLPHIP
+
+Julia binding of `libcoolpdlp_hip.so` (the MI355X PDLP/PDHG engine) for CoolPDLP.jl.
+
+It plugs into the two extension points CoolPDLP already has, so `solve`,
+`MOI.optimize!` and every option name stay as they are:
+
+1. the matrix plug-in (`src/utils/mat_csr.jl:8-92`): `HIPSparseMatrixCSR(A)` builds the
+   matrix in HBM and implements the 5-argument `mul!` on device vectors;
+2. algorithm dispatch (`src/algorithms/common.jl:176,216`): `solve(milp, sol, algo)` with
+   `backend = HIPEngine()` runs the whole PDLP/PDHG loop (scaling, step size, restarts,
+   termination) on the GPU through `cpdlp_solve` and returns the reference's
+   `(PrimalDualSolution, ConvergenceStats)`.
+
+This file is not loaded by anything in this repository (there is no Julia in the
+image); it is the binding a maintainer would add, written against
+`include/coolpdlp_hip.h`.
+"""
+module CoolPDLPHIP
+
+using CoolPDLP
+using CoolPDLP: Algorithm, ConvergenceStats, KKTErrors, MILP, PrimalDualSolution,
+    OPTIMAL, TIME_LIMIT, ITERATION_LIMIT, STILL_RUNNING
+using LinearAlgebra
+using SparseArrays
+
+const libcpdlp = get(ENV, "COOLPDLP_HIP_LIB", "libcoolpdlp_hip.so")
+
+# ---- C structs (include/coolpdlp_hip.h) ----------------------------------------
+struct CLP
+    m::Int64
+    n::Int64
+    index_base::Int32
+    index_bits::Int32
+    colptr::Ptr{Cvoid}
+    rowval::Ptr{Cvoid}
+    nzval::Ptr{Float64}
+    c::Ptr{Float64}
+    lv::Ptr{Float64}
+    uv::Ptr{Float64}
+    lc::Ptr{Float64}
+    uc::Ptr{Float64}
+end
+
+mutable struct CParams
+    algorithm::Int32
+    ruiz_iter::Int32
+    chambolle_pock_alpha::Float64
+    invnorm_scaling::Float64
+    primal_weight_damping::Float64
+    zero_tol::Float64
+    sufficient_decay::Float64
+    necessary_decay::Float64
+    artificial_decay::Float64
+    show_progress::Int32
+    check_every::Int32
+    record_error_history::Int32
+    precision::Int32
+    termination_reltol::Float64
+    max_kkt_passes::Int64
+    time_limit::Float64
+    format::Int32
+    device::Int32
+    powm_tol::Float64
+    powm_maxiter::Int64
+    powm_seed::UInt64
+    polish_reltol::Float64
+    use_graph::Int32
+    long_row::Int32
+    CParams() = new()
+end
+
+struct CKKT
+    primal::Float64
+    primal_scale::Float64
+    dual::Float64
+    dual_scale::Float64
+    gap::Float64
+    gap_scale::Float64
+end
+
+struct CStats
+    err::CKKT
+    time_elapsed::Float64
+    kkt_passes::Int64
+    termination_status::Int32
+    format_A::Int32
+    format_At::Int32
+    pad::Int32
+    eta::Float64
+    omega::Float64
+    powm_iters::Int64
+    restarts::Int64
+    n_history::Int64
+    fp32_passes::Int64
+    setup_seconds::Float64
+    loop_seconds::Float64
+    step_kernel_ms::Float64
+    step_kernel_launches::Int64
+end
+
+function check(code::Cint)
+    code == 0 && return nothing
+    msg = unsafe_string(ccall((:cpdlp_last_error, libcpdlp), Cstring, ()))
+    code == 1 && throw(DimensionMismatch(msg))
+    code == 2 && throw(ArgumentError(msg))
+    code == 3 && throw(OutOfMemoryError())
+    error("libcoolpdlp_hip error $code: $msg")
+end
+
+# ---- 1. matrix plug-in ----------------------------------------------------------
+"""
+    HIPSparseMatrixCSR(A::SparseMatrixCSC{Float64})
+
+`GPUSparseMatrixCSR` equivalent stored in HBM by libcoolpdlp_hip
+(`cpdlp_matrix_create`, replaces `GPUSparseMatrixCSR(A)`, `src/utils/mat_csr.jl:58-61`).
+"""
+mutable struct HIPSparseMatrixCSR <: AbstractSparseMatrix{Float64, Int32}
+    handle::Ptr{Cvoid}
+    m::Int
+    n::Int
+    nnz::Int
+    function HIPSparseMatrixCSR(A::SparseMatrixCSC{Float64}; format::Integer = 1, device::Integer = 0)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:cpdlp_matrix_create, libcpdlp), Cint,
+            (Int64, Int64, Int32, Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
+            size(A, 1), size(A, 2), 1, 64, A.colptr, A.rowval, A.nzval, format, 0, device, h))
+        M = new(h[], size(A, 1), size(A, 2), nnz(A))
+        finalizer(x -> ccall((:cpdlp_matrix_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), x.handle), M)
+        return M
+    end
+end
+
+Base.size(A::HIPSparseMatrixCSR) = (A.m, A.n)
+SparseArrays.nnz(A::HIPSparseMatrixCSR) = A.nnz
+
+# mul!(c, A, b, α, β), src/utils/mat_csr.jl:81-92 -> cpdlp_matrix_mul on device pointers
+# (here: any array type whose pointer is a HIP device pointer, e.g. an AMDGPU.ROCArray)
+function LinearAlgebra.mul!(c::AbstractVector{Float64}, A::HIPSparseMatrixCSR,
+        b::AbstractVector{Float64}, α::Number, β::Number)
+    check(ccall((:cpdlp_matrix_mul, libcpdlp), Cint,
+        (Ptr{Cvoid}, Float64, Ptr{Float64}, Float64, Ptr{Float64}, Ptr{Cvoid}),
+        A.handle, Float64(α), pointer(b), Float64(β), pointer(c), C_NULL))
+    return c
+end
+
+# ---- 2. the whole loop ------------------------------------------------------------
+"""
+    HIPEngine(; device = 0, precision = :fp64, format = :auto)
+
+Backend selector: `PDLP(...; backend = HIPEngine())` makes `solve` run on libcoolpdlp_hip.
+"""
+Base.@kwdef struct HIPEngine
+    device::Int = 0
+    precision::Symbol = :fp64
+    format::Symbol = :auto
+end
+
+function cparams(algo::Algorithm{A}, eng::HIPEngine) where {A}
+    p = CParams()
+    check(ccall((:cpdlp_default_params, libcpdlp), Cint, (Ref{CParams}, Int32), p, A === :PDLP ? 0 : 1))
+    (; preconditioning, step_size, restart, generic, termination) = algo
+    p.ruiz_iter = preconditioning.ruiz_iter
+    p.chambolle_pock_alpha = preconditioning.chambolle_pock_alpha
+    p.invnorm_scaling = step_size.invnorm_scaling
+    p.primal_weight_damping = step_size.primal_weight_damping
+    p.zero_tol = step_size.zero_tol
+    p.sufficient_decay = restart.sufficient_decay
+    p.necessary_decay = restart.necessary_decay
+    p.artificial_decay = restart.artificial_decay
+    p.check_every = generic.check_every
+    p.record_error_history = generic.record_error_history
+    p.show_progress = generic.show_progress
+    p.termination_reltol = termination.termination_reltol
+    p.max_kkt_passes = termination.max_kkt_passes
+    p.time_limit = termination.time_limit
+    p.precision = Dict(:fp64 => 0, :fp32 => 1, :mixed => 2)[eng.precision]
+    p.format = Dict(:auto => 0, :csr => 1, :ell => 2, :xslice => 3)[eng.format]
+    p.device = eng.device
+    return p
+end
+
+const STATUS = (OPTIMAL, TIME_LIMIT, ITERATION_LIMIT, STILL_RUNNING)
+
+# solve(milp, sol, algo), src/algorithms/common.jl:186-201, for backend = HIPEngine()
+function CoolPDLP.solve(milp::MILP, sol::PrimalDualSolution, algo::Algorithm{A, T, Ti, M, HIPEngine}) where {A, T, Ti, M}
+    A_ = milp.A::SparseMatrixCSC
+    c, lv, uv, lc, uc = (Vector{Float64}(v) for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))
+    lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
+        pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
+    p = cparams(algo, algo.conversion.backend)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve A_ c lv uv lc uc begin
+        check(ccall((:cpdlp_create, libcpdlp), Cint, (Ref{CLP}, Ref{CParams}, Ptr{Ptr{Cvoid}}), lp, p, h))
+    end
+    try
+        x0, y0 = Vector{Float64}(sol.x), Vector{Float64}(sol.y)
+        check(ccall((:cpdlp_solve, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), h[], x0, y0))
+        x, y = similar(x0), similar(y0)
+        check(ccall((:cpdlp_get_solution, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), h[], x, y))
+        s = Ref{CStats}()
+        check(ccall((:cpdlp_get_stats, libcpdlp), Cint, (Ptr{Cvoid}, Ref{CStats}), h[], s))
+        count = Ref{Int64}(0)
+        check(ccall((:cpdlp_get_history, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{CKKT}, Int64, Ref{Int64}),
+            h[], C_NULL, C_NULL, 0, count))
+        passes = Vector{Int64}(undef, count[])
+        errs = Vector{CKKT}(undef, count[])
+        check(ccall((:cpdlp_get_history, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{CKKT}, Int64, Ref{Int64}),
+            h[], passes, errs, count[], count))
+        tokkt(e) = KKTErrors(; primal = e.primal, primal_scale = e.primal_scale, dual = e.dual,
+            dual_scale = e.dual_scale, gap = e.gap, gap_scale = e.gap_scale)
+        stats = ConvergenceStats(Float64; err = tokkt(s[].err), time_elapsed = s[].time_elapsed,
+            kkt_passes = s[].kkt_passes, termination_status = STATUS[s[].termination_status + 1],
+            error_history = [(passes[i], tokkt(errs[i])) for i in eachindex(passes)])
+        return PrimalDualSolution(x, y), stats
+    finally
+        ccall((:cpdlp_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), h[])
+    end
+end
+
+end # module

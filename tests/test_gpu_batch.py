@@ -92,7 +92,7 @@ def test_continuous_admission_and_release(cp):
             assert np.array_equal(got[k][0].x, ref[k][0].x)
             assert got[k][1].kkt_passes == ref[k][1].kkt_passes
     g = lps[0]
-    fp = cp.batch.footprint(g.m, g.n, g.nnz)
+    fp = max(cp.batch.footprint(h.m, h.n, h.nnz) for h in lps[:2])
     with cp.BatchSolver(algo, hbm_bytes=fp + 1024) as B:
         i0 = B.add(g)
         with pytest.raises(MemoryError):
