@@ -11,7 +11,7 @@ Also reported: the roofline of the PDHG iteration (the two fused step kernels, t
 live with HIP events on the solver's stream), per-kernel times, and the CPU oracle
 (FP64, OpenMP) on a bounded sample of the same LP on this host.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|plain|batch]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|plain|batch|serve]
 """
 import argparse
 import json
@@ -133,6 +133,93 @@ def bench_batch(args):
     }
 
 
+def bench_serve(args):
+    """Serving loop: Poisson arrivals at each QPS of the sweep for `serve_duration`
+    seconds, request mix 70% small (m in [50, 2k]), 25% medium ([2k, 50k]), 5% large
+    ([50k, 500k]), n = 2m, 8 nonzeros per column.  Requests are drawn from a seeded pool
+    of distinct LPs per class.  Latency = finish - arrival to relative KKT 1e-6.  The
+    first-come-first-served baseline is the one-solve-at-a-time queue over the same
+    arrivals with each request's measured plain-engine solve time."""
+    import numpy as np
+    import coolpdlp_hip as cp
+    from coolpdlp_hip import generators as G
+    from coolpdlp_hip.serving import Server, fcfs_latencies, poisson_arrivals
+    rng = np.random.default_rng(17)
+    reltol = 1e-6
+    bounds = {"small": (50, 2_000), "medium": (2_000, 50_000), "large": (50_000, 500_000)}
+    sizes = {"small": args.pool_small, "medium": args.pool_medium, "large": args.pool_large}
+    pool = {}
+    t = time.perf_counter()
+    for cls, (lo, hi) in bounds.items():
+        ms = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=sizes[cls]))).astype(int)
+        pool[cls] = [G.request_lp(int(m), 2 * int(m), 1000 + 97 * k + len(pool)) for k, m in enumerate(ms)]
+    log(f"[bench] request pool {({k: len(v) for k, v in pool.items()})} in {time.perf_counter() - t:.1f}s")
+    # plain-engine service time of every pool LP (the FCFS baseline's server)
+    service = {}
+    for cls, lps in pool.items():
+        for k, g in enumerate(lps):
+            milp = cp.MILP.from_generated(g)
+            eng = cp.Engine(milp, cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit))
+            t = time.perf_counter()
+            sol, st = eng.solve()
+            service[(cls, k)] = time.perf_counter() - t
+            eng.close()
+            log(f"[bench] plain {cls}[{k}] m={g.m} {st.termination_status.name} passes={st.kkt_passes} "
+                f"{service[(cls, k)]:.3f}s")
+    out_points = []
+    for qps in args.qps:
+        arr = poisson_arrivals(qps, args.serve_duration, seed=int(qps))
+        u = rng.random(len(arr))
+        cls_of = np.where(u < 0.70, "small", np.where(u < 0.95, "medium", "large"))
+        picks = [(c, int(rng.integers(len(pool[c])))) for c in cls_of]
+        srv = Server(reltol=reltol, batch_bytes=args.serve_batch_gib << 30,
+                     checks_per_round=args.serve_checks, time_limit=args.time_limit)
+        t0 = time.perf_counter()
+        k = 0
+        deadline = t0 + args.serve_duration + args.serve_drain
+        last_log = t0
+        while time.perf_counter() < deadline:
+            now = time.perf_counter() - t0
+            while k < len(arr) and arr[k] <= now:
+                srv.submit(pool[picks[k][0]][picks[k][1]], t0 + arr[k])
+                k += 1
+            if k >= len(arr) and srv.pending() == 0:
+                break
+            srv.pump()
+            if time.perf_counter() - last_log > 5:
+                last_log = time.perf_counter()
+                log(f"[bench] qps={qps} t={now:.1f}s submitted={k} done={len(srv.done)} pending={srv.pending()}")
+        t_end = time.perf_counter()
+        srv.close()
+        done = srv.done
+        lat = np.array([r.t_done - r.t_arrival for r in done])
+        ok = sum(r.status == "OPTIMAL" for r in done)
+        fc = fcfs_latencies(arr, np.array([service[p] for p in picks]))
+        span = max((r.t_done for r in done), default=t0) - t0
+        pt = {"qps": qps, "submitted": int(len(arr)), "completed": len(done), "optimal": int(ok),
+              "served_lps_per_s": len(done) / max(span, 1e-9),
+              "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+              "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+              "fcfs_lps_per_s": len(arr) / float((arr + fc).max()) if len(arr) else None,
+              "fcfs_p50_s": float(np.percentile(fc, 50)) if len(fc) else None,
+              "fcfs_p99_s": float(np.percentile(fc, 99)) if len(fc) else None,
+              "wall_s": t_end - t0}
+        log(f"[bench] qps={qps}: {pt}")
+        out_points.append(pt)
+    best = max(out_points, key=lambda p: p["served_lps_per_s"])
+    return {
+        "metric": "solve time to rel-KKT 1e-6; batched LPs/s + p50/p99 vs plain PDLP, 1 GPU",
+        "value": best["served_lps_per_s"], "unit": "LPs/s", "n_gpus": 1, "steps": len(out_points),
+        "warmup": 0, "ms_per_step": 1e3 * args.serve_duration, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 bulk / f64 checks+polish",
+        "data": "synthetic (seeded request pool with known optima)",
+        "config": {"workload": "serving_poisson_mix", "qps": args.qps, "duration_s": args.serve_duration,
+                   "mix": "70/25/5 small/medium/large", "reltol": reltol,
+                   "pool": {k: len(v) for k, v in pool.items()}},
+        "points": out_points,
+    }
+
+
 def cpu_baseline(g, gpu_passes, reltol, budget_passes):
     """Oracle FP64 PDLP on this host: a bounded sample of `budget_passes` passes (plus
     their checks), projected to the GPU solve's pass count."""
@@ -177,6 +264,14 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--checks-per-round", type=int, default=20)
     ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--qps", type=float, nargs="+", default=[100.0, 1000.0])
+    ap.add_argument("--serve-duration", type=float, default=20.0)
+    ap.add_argument("--serve-drain", type=float, default=60.0)
+    ap.add_argument("--serve-checks", type=int, default=2)
+    ap.add_argument("--serve-batch-gib", type=int, default=16)
+    ap.add_argument("--pool-small", type=int, default=48)
+    ap.add_argument("--pool-medium", type=int, default=16)
+    ap.add_argument("--pool-large", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
@@ -188,6 +283,9 @@ def main():
 
     if args.workload == "batch":
         print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
         return
 
     t = time.perf_counter()

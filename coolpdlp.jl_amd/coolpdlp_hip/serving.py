@@ -1,0 +1,185 @@
+"""Serving: continuous batching of LP solve requests on one GPU.
+
+Requests that fit the batch engine (m + n <= 65536) are admitted into its HBM arena as
+room frees up (footprint accounting, cpdlp_batch_footprint) and advanced in short
+rounds of PDHG checks; every round, finished requests are harvested and their HBM
+released, and waiting requests are admitted -- so requests arriving while others are
+on the GPU join the running batch (their work overlaps on the device).  Larger
+requests run on the plain engine (one HIP stream each) in worker threads, under an
+HBM budget.  Iterates, scalings and restart state stay on the device between rounds.
+
+The first-come-first-served baseline (`fcfs_latencies`) is the one-solve-at-a-time
+queue: request k starts when both it has arrived and request k-1 has finished, and
+takes its measured plain-engine service time.
+"""
+from __future__ import annotations
+
+import threading
+import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .algorithms import Engine, PDLP
+from .batch import BatchSolver, footprint
+from .problems import MILP
+
+
+def plain_footprint(m: int, n: int, nnz: int) -> int:
+    """Estimated HBM bytes of one plain-engine solve (matrices in two layouts, FP64 and
+    FP32 values, vectors and iterates)."""
+    return int(40 * nnz + 250 * (m + n))
+
+
+@dataclass
+class Request:
+    rid: int
+    lp: object
+    t_arrival: float
+    route: str = "batch"
+    t_done: float = float("nan")
+    status: str = ""
+    objective: float = float("nan")
+    kkt_passes: int = 0
+    x: np.ndarray | None = None
+
+
+@dataclass
+class Server:
+    reltol: float = 1e-6
+    precision: str = "mixed"
+    batch_bytes: int = 16 << 30
+    plain_bytes: int = 64 << 30
+    checks_per_round: int = 2
+    plain_workers: int = 2
+    batch_limit: int = 65536           # m + n limit of the batch engine
+    time_limit: float = 600.0
+    keep_solutions: bool = False
+    done: list = field(default_factory=list)
+
+    def __post_init__(self):
+        algo = PDLP(self.precision, termination_reltol=self.reltol, time_limit=self.time_limit)
+        self.batch = BatchSolver(algo, hbm_bytes=self.batch_bytes)
+        self.waiting: deque = deque()
+        self.in_batch: dict = {}
+        self.pool = ThreadPoolExecutor(self.plain_workers)
+        self.plain_waiting: deque = deque()
+        self.plain_running = 0
+        self.plain_used = 0
+        self.lock = threading.Lock()
+        self.next_id = 0
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+        self.batch.close()
+
+    # -------------------------------------------------------------- submission
+    def submit(self, lp, t_arrival: float | None = None) -> int:
+        t = time.perf_counter() if t_arrival is None else t_arrival
+        r = Request(self.next_id, lp, t)
+        self.next_id += 1
+        if lp.m + lp.n <= self.batch_limit:
+            self.waiting.append(r)
+        else:
+            r.route = "plain"
+            self.plain_waiting.append(r)
+        return r.rid
+
+    def pending(self) -> int:
+        with self.lock:
+            return len(self.waiting) + len(self.in_batch) + len(self.plain_waiting) + self.plain_running
+
+    # -------------------------------------------------------------- plain route
+    def _plain_job(self, r: Request, fp: int):
+        try:
+            milp = MILP.from_generated(r.lp)
+            # fixed CSR format: a request's result must not depend on autotuner timing
+            algo = PDLP(self.precision, None, "csr", termination_reltol=self.reltol,
+                        time_limit=self.time_limit)
+            with Engine(milp, algo) as eng:
+                sol, st = eng.solve()
+            r.status = st.termination_status.name
+            r.kkt_passes = st.kkt_passes
+            r.objective = float(sol.x @ r.lp.c)
+            if self.keep_solutions:
+                r.x = sol.x
+        except Exception as e:   # noqa: BLE001 -- a failed request is reported, not fatal
+            r.status = f"ERROR: {e}"
+        r.t_done = time.perf_counter()
+        with self.lock:
+            self.plain_running -= 1
+            self.plain_used -= fp
+            self.done.append(r)
+
+    def _admit_plain(self):
+        while self.plain_waiting and self.plain_running < self.plain_workers:
+            r = self.plain_waiting[0]
+            fp = plain_footprint(r.lp.m, r.lp.n, r.lp.nnz)
+            with self.lock:
+                if self.plain_running > 0 and self.plain_used + fp > self.plain_bytes:
+                    return
+                self.plain_running += 1
+                self.plain_used += fp
+            self.plain_waiting.popleft()
+            self.pool.submit(self._plain_job, r, fp)
+
+    # -------------------------------------------------------------- batch route
+    def _admit_batch(self):
+        while self.waiting:
+            r = self.waiting[0]
+            try:
+                bid = self.batch.add(r.lp)
+            except MemoryError:
+                return   # no room until a running request finishes
+            self.waiting.popleft()
+            self.in_batch[bid] = r
+
+    def pump(self):
+        """One scheduling round: admit, advance the batch, harvest."""
+        self._admit_plain()
+        self._admit_batch()
+        if self.in_batch:
+            self.batch.run(self.checks_per_round)
+            now = time.perf_counter()
+            for bid in self.batch.poll():
+                r = self.in_batch.pop(bid)
+                sol, st = self.batch.get(bid)
+                self.batch.release(bid)
+                r.t_done = now
+                r.status = st.termination_status.name
+                r.kkt_passes = st.kkt_passes
+                r.objective = float(sol.x @ r.lp.c)
+                if self.keep_solutions:
+                    r.x = sol.x
+                with self.lock:
+                    self.done.append(r)
+        else:
+            time.sleep(0.0005)
+
+    def drain(self, deadline: float):
+        while self.pending() > 0 and time.perf_counter() < deadline:
+            self.pump()
+
+
+def fcfs_latencies(arrivals: np.ndarray, service: np.ndarray) -> np.ndarray:
+    """Latencies of a first-come-first-served single server: start_k = max(arrival_k,
+    finish_{k-1}), finish_k = start_k + service_k."""
+    lat = np.empty(len(arrivals))
+    free = -np.inf
+    for k, (a, s) in enumerate(zip(arrivals, service)):
+        start = max(a, free)
+        free = start + s
+        lat[k] = free - a
+    return lat
+
+
+def poisson_arrivals(qps: float, duration: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / qps)
+        if t >= duration:
+            return np.array(out)
+        out.append(t)
