@@ -242,11 +242,11 @@ def min_cost_flow_lp(nodes=500_000, out_degree=8, src_frac=0.01, seed=2,
     uniform random heads (no self loops); A = node-arc incidence (+1 at the tail,
     -1 at the head), A x = supply, 0 <= x <= capacity.  Capacities integer U{1..100},
     costs integer U{1..10}; 1% of nodes are sources and 1% sinks with magnitudes
-    U(10, 1000), the sinks rescaled so supplies sum to zero.  If the supplies exceed
-    what the graph can route (checked by a max-flow from a super source to a super
-    sink), all supplies are scaled down to 90% of the max flow, so the LP is
-    feasible.  No closed-form optimum: the objective is checked against an FP64
-    solve."""
+    U(10, 1000), the sinks rescaled so supplies sum to zero.  Each source's supply and
+    each sink's demand is then replaced by 90% of what a max-flow from a super source
+    to a super sink routes through it, so the LP is feasible by construction (a scaled
+    feasible flow).  No closed-form optimum: the objective is checked against an FP64
+    solve and, at small scale, HiGHS."""
     rng = np.random.default_rng(seed)
     E = nodes * out_degree
     tails = np.repeat(np.arange(nodes, dtype=np.int64), out_degree)
@@ -263,22 +263,30 @@ def min_cost_flow_lp(nodes=500_000, out_degree=8, src_frac=0.01, seed=2,
     supply[sinks] = -dem * (supply[sources].sum() / dem.sum())
     scale = 1.0
     if check_feasible:
+        # route the supplies with a max-flow from a super source to a super sink (in
+        # units of 1/100, integer capacities) and keep, for every source and sink, what
+        # the flow actually carries through it, times 0.9: a scaled feasible flow, so
+        # the LP is feasible by construction
         from scipy.sparse.csgraph import maximum_flow
         S, T = nodes, nodes + 1
-        big = 1 << 30
-        # integer capacities in units of 1/1 supply; floor supplies for the check
-        src_cap = np.floor(supply[sources]).astype(np.int64)
-        snk_cap = np.floor(-supply[sinks]).astype(np.int64)
+        unit = 100
         r = np.concatenate([tails, np.full(k, S), sinks])
         cc = np.concatenate([heads, sources, np.full(k, T)])
-        w = np.concatenate([cap.astype(np.int64), src_cap, snk_cap])
-        G = sp.csr_matrix((w.astype(np.int32), (r, cc)), shape=(nodes + 2, nodes + 2))
-        G.sum_duplicates()
-        flow = maximum_flow(G, S, T).flow_value
-        need = float(min(src_cap.sum(), snk_cap.sum()))
-        if flow < need:
-            scale = 0.9 * flow / supply[sources].sum()
-            supply *= scale
+        w = np.concatenate([(cap * unit).astype(np.int64),
+                            np.ceil(supply[sources] * unit).astype(np.int64),
+                            np.ceil(-supply[sinks] * unit).astype(np.int64)])
+        Gm = sp.csr_matrix((w.astype(np.int32), (r, cc)), shape=(nodes + 2, nodes + 2))
+        Gm.sum_duplicates()
+        res = maximum_flow(Gm, S, T)
+        F = res.flow.tocsr()
+        routed_src = np.asarray(F[S, sources].todense()).ravel() / unit
+        routed_snk = np.asarray(F[sinks, T].todense()).ravel() / unit
+        new = np.zeros(nodes)
+        new[sources] = 0.9 * routed_src
+        new[sinks] = -0.9 * routed_snk
+        total = supply[sources].sum()
+        scale = float(new[sources].sum() / total) if total > 0 else 1.0
+        supply = new
     cols = np.concatenate([np.arange(E), np.arange(E)])
     rows = np.concatenate([tails, heads])
     vals = np.concatenate([np.ones(E), -np.ones(E)])
@@ -287,4 +295,4 @@ def min_cost_flow_lp(nodes=500_000, out_degree=8, src_frac=0.01, seed=2,
     uc = supply.copy()
     return GeneratedLP(nodes, E, colptr, rowval, nzval, cost, np.zeros(E), cap, lc, uc,
                        name=f"min_cost_flow_{nodes}x{E}",
-                       meta=dict(seed=seed, supply_scale=scale, big=big if check_feasible else 0))
+                       meta=dict(seed=seed, supply_scale=scale))
