@@ -201,11 +201,13 @@ inline int32_t build_ell(const HostSide& S, double max_fill, std::vector<int32_t
     return w;
 }
 
-// XCD column slices of an internal side: ns sub-CSRs over all rows, slice s holding
-// the entries whose (internal) column lies in [s*W, (s+1)*W), W = ceil(cols/ns), in
-// the row's original entry order.  rowptr: ns*(rows+1) absolute offsets.  Rows
-// [0, n_long) use the wavefront path in every slice; the rest is cut into short tiles
-// per slice (<= BLOCK rows, <= TILE_NNZ nnz).
+// XCD column slices of the long rows of an internal side.  Column slice s holds the
+// internal columns [s*W, (s+1)*W), W = ceil(cols / ns); slice s's part of long row r
+// (r < n_long) is the run of that row's entries whose column lies in the slice, in the
+// row's entry order.  rowptr: ns * (n_long + 1) absolute offsets, slices stored one
+// after the other.  Each slice's long rows are cut into tiles of at most BLOCK rows and
+// TILE_NNZ entries; a row part longer than TILE_NNZ is a tile of its own (r1 = r0 + 1,
+// cnt > TILE_NNZ).  Short rows are not sliced: they stay in the side's own tiles.
 struct HostSlices {
     int32_t ns = 0, max_tiles = 0;
     std::vector<int32_t> rowptr, col, vmap, tile_off;
@@ -213,40 +215,51 @@ struct HostSlices {
 };
 
 inline void build_slices(const HostSide& S, int32_t ns, HostSlices& H) {
-    const int32_t rows = S.rows;
+    const int32_t nl = S.n_long;
     const int64_t W = std::max<int64_t>(1, (S.cols + ns - 1) / ns);
     H.ns = ns;
-    H.rowptr.assign((size_t)ns * (rows + 1), 0);
-    std::vector<int32_t> cnt((size_t)ns * rows, 0);
-    for (int32_t r = 0; r < rows; ++r)
+    H.rowptr.assign((size_t)ns * (nl + 1), 0);
+    std::vector<int32_t> cnt((size_t)ns * nl, 0);
+    for (int32_t r = 0; r < nl; ++r)
         for (int32_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k)
-            cnt[(size_t)(S.col[k] / W) * rows + r]++;
+            cnt[(size_t)(S.col[k] / W) * nl + r]++;
     int64_t off = 0;
     for (int32_t sl = 0; sl < ns; ++sl) {
-        int32_t* rp = H.rowptr.data() + (size_t)sl * (rows + 1);
-        for (int32_t r = 0; r < rows; ++r) {
+        int32_t* rp = H.rowptr.data() + (size_t)sl * (nl + 1);
+        for (int32_t r = 0; r < nl; ++r) {
             rp[r] = (int32_t)off;
-            off += cnt[(size_t)sl * rows + r];
+            off += cnt[(size_t)sl * nl + r];
         }
-        rp[rows] = (int32_t)off;
+        rp[nl] = (int32_t)off;
     }
-    H.col.resize(S.nnz);
-    H.vmap.resize(S.nnz);
+    H.col.resize(off);
+    H.vmap.resize(off);
     std::vector<int32_t> next(H.rowptr);
-    for (int32_t r = 0; r < rows; ++r)
+    for (int32_t r = 0; r < nl; ++r)
         for (int32_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k) {
             const int32_t sl = (int32_t)(S.col[k] / W);
-            const int32_t d = next[(size_t)sl * (rows + 1) + r]++;
+            const int32_t d = next[(size_t)sl * (nl + 1) + r]++;
             H.col[d] = S.col[k];
             H.vmap[d] = S.vmap[k];
         }
-    // per slice: its tiles from tile_off[s] to tile_off[s+1]
     H.tiles.clear();
     H.tile_off.assign(ns + 1, 0);
     H.max_tiles = 0;
     for (int32_t sl = 0; sl < ns; ++sl) {
         H.tile_off[sl] = (int32_t)H.tiles.size();
-        build_tiles(H.rowptr.data() + (size_t)sl * (rows + 1), S.n_long, rows, H.tiles);
+        const int32_t* rp = H.rowptr.data() + (size_t)sl * (nl + 1);
+        int32_t r = 0;
+        while (r < nl) {
+            const int32_t r0 = r;
+            if (rp[r + 1] - rp[r] > TILE_NNZ) {
+                ++r;
+            } else {
+                while (r < nl && r - r0 < BLOCK && rp[r + 1] - rp[r] <= TILE_NNZ &&
+                       rp[r + 1] - rp[r0] <= TILE_NNZ)
+                    ++r;
+            }
+            H.tiles.push_back(TileMeta{r0, r, rp[r0], rp[r] - rp[r0]});
+        }
         H.max_tiles = std::max(H.max_tiles, (int32_t)H.tiles.size() - H.tile_off[sl]);
     }
     H.tile_off[ns] = (int32_t)H.tiles.size();

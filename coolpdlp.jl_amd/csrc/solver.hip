@@ -295,7 +295,7 @@ MatView<V> DevSide::view(bool allow_slices) const {
     v.sl_val = sl_vals<V>();
     v.sl_tiles = sl_tiles.p;
     v.sl_tile_off = sl_tile_off.p;
-    v.sl_blocks = sl_blocks;
+    v.sl_max_tiles = sl_max_tiles;
     v.sl_part = sl_part.p;
     if (format == CPDLP_FMT_XSLICE && !allow_slices) v.format = CPDLP_FMT_CSR;
     return v;
@@ -313,21 +313,22 @@ const double* DevSide::sl_vals<double>() const { return sl_val64.p; }
 template <>
 const float* DevSide::sl_vals<float>() const { return sl_val32.p; }
 
-// NS = 8 column slices, one per XCD of the MI355X
+// NS = 8 column slices of the long rows, one per XCD of the MI355X
 void DevSide::build_slices(const HostSide& S, hipStream_t s) {
     constexpr int32_t NS = 8;
     HostSlices H;
     cpdlp::build_slices(S, NS, H);
     sl_ns = NS;
-    sl_blocks = ceil_div(S.n_long, LONG_ROWS_PER_BLOCK) + H.max_tiles;
+    sl_max_tiles = H.max_tiles;
+    sl_nnz = (int64_t)H.col.size();
     sl_rowptr.upload(H.rowptr, s);
-    sl_col.alloc(std::max<size_t>(H.col.size(), 1));
+    sl_col.alloc(std::max<int64_t>(sl_nnz, 1));
     sl_col.upload(H.col.data(), H.col.size(), s);
-    sl_vmap.alloc(std::max<size_t>(H.vmap.size(), 1));
+    sl_vmap.alloc(std::max<int64_t>(sl_nnz, 1));
     sl_vmap.upload(H.vmap.data(), H.vmap.size(), s);
     sl_tiles.upload(H.tiles, s);
     sl_tile_off.upload(H.tile_off, s);
-    sl_part.alloc(std::max<int64_t>((int64_t)NS * S.rows, 1));
+    sl_part.alloc(std::max<int64_t>((int64_t)NS * S.n_long, 1));
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -356,14 +357,14 @@ void DevSide::fill_values(const double* src_csc, bool want32, hipStream_t s) {
             hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
                                nnz, vmap.p, src_csc, val32.p);
     }
-    if (sl_ns > 0 && nnz > 0) {
-        if (sl_val64.n < (size_t)nnz) sl_val64.alloc(nnz);
-        hipLaunchKernelGGL(k_gather_vals<double>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
-                           nnz, sl_vmap.p, src_csc, sl_val64.p);
+    if (sl_ns > 0 && sl_nnz > 0) {
+        if (sl_val64.n < (size_t)sl_nnz) sl_val64.alloc(sl_nnz);
+        hipLaunchKernelGGL(k_gather_vals<double>, dim3(ceil_div(sl_nnz, BLOCK)), dim3(BLOCK), 0, s,
+                           sl_nnz, sl_vmap.p, src_csc, sl_val64.p);
         if (want32) {
-            if (sl_val32.n < (size_t)nnz) sl_val32.alloc(nnz);
-            hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
-                               nnz, sl_vmap.p, src_csc, sl_val32.p);
+            if (sl_val32.n < (size_t)sl_nnz) sl_val32.alloc(sl_nnz);
+            hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(sl_nnz, BLOCK)), dim3(BLOCK), 0,
+                               s, sl_nnz, sl_vmap.p, src_csc, sl_val32.p);
         }
     }
     if (ell_w > 0) {
@@ -486,10 +487,11 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         A.build_ell(L.A, s, fill);
         At.build_ell(L.At, s, fill);
     }
-    if (P.format == CPDLP_FMT_XSLICE || P.format == CPDLP_FMT_AUTO) {
+    // slices cut the long rows only: AUTO builds them for a side that has long rows
+    if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && L.A.n_long > 0))
         A.build_slices(L.A, s);
+    if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && L.At.n_long > 0))
         At.build_slices(L.At, s);
-    }
     HIP_CHECK(hipStreamSynchronize(s));
     // free the big host-side copies that are no longer needed
     std::vector<int32_t>().swap(L.A.col);
@@ -680,7 +682,7 @@ void Handle::choose_formats() {
         // format; another format replaces CSR only when it is >= 10% faster
         std::vector<int32_t> cand{CPDLP_FMT_CSR};
         if (S->ell_w > 0) cand.push_back(CPDLP_FMT_ELL);
-        if (S->sl_ns > 0) cand.push_back(CPDLP_FMT_XSLICE);
+        if (S->sl_ns > 0 && S->n_long > 0) cand.push_back(CPDLP_FMT_XSLICE);
         if (cand.size() == 1) continue;
         DevBuf<double> xin(std::max<int64_t>(S->cols, 1)), out(std::max<int64_t>(S->rows, 1));
         hipLaunchKernelGGL(k_fill, dim3(std::max(1, ceil_div(S->cols, BLOCK))), dim3(BLOCK), 0,

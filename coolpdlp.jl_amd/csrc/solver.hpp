@@ -23,8 +23,9 @@ struct DevSide {
     DevBuf<int32_t> ell_col, ell_vmap;
     DevBuf<double> ell_val64;
     DevBuf<float> ell_val32;
-    // XCD column slices (step kernels only)
-    int32_t sl_ns = 0, sl_blocks = 0;
+    // XCD column slices of the long rows (step kernels only)
+    int32_t sl_ns = 0, sl_max_tiles = 0;
+    int64_t sl_nnz = 0;
     DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_off;
     DevBuf<TileMeta> sl_tiles;
     DevBuf<double> sl_val64, sl_part;

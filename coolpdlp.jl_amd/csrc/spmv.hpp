@@ -50,21 +50,22 @@ struct MatView {
     int32_t ell_width = 0;
     const int32_t* ell_col = nullptr;
     const V* ell_val = nullptr;
-    // XCD column slices (format 3): NS sub-CSRs over all rows, one per column slice;
-    // slice s owns the blocks with blockIdx % NS == s (workgroups are dispatched to
-    // the XCDs round-robin, so each XCD's L2 holds one slice of the gathered vector)
+    // XCD column slices (format 3): the long rows are also stored cut by column slice,
+    // NS slices (one per XCD); the slice-s part of every long row is summed by a block
+    // with blockIdx % NS == s (workgroups are dispatched to the XCDs round-robin), so
+    // each XCD gathers from one slice of the vector, which stays in that XCD's L2.
     int32_t ns = 0;
-    const int32_t* sl_rowptr = nullptr;   // ns * (rows + 1), absolute offsets
+    const int32_t* sl_rowptr = nullptr;   // ns * (n_long + 1), absolute offsets
     const int32_t* sl_col = nullptr;
     const V* sl_val = nullptr;
     const TileMeta* sl_tiles = nullptr;   // tiles of slice s: sl_tiles[sl_tile_off[s] ...]
     const int32_t* sl_tile_off = nullptr; // ns + 1; slice s has off[s+1] - off[s] tiles
-    int32_t sl_blocks = 0;                // blocks per slice (long blocks + max tiles)
-    void* sl_part = nullptr;              // K * ns * rows partial sums (Acc)
+    int32_t sl_max_tiles = 0;             // most tiles of any slice
+    void* sl_part = nullptr;              // ns * n_long slice sums (Acc)
 
     int grid() const {
         if (format == CPDLP_FMT_ELL) return ceil_div(rows, BLOCK);
-        if (format == CPDLP_FMT_XSLICE) return ns * sl_blocks;
+        if (format == CPDLP_FMT_XSLICE) return ns * sl_max_tiles + n_tiles;
         return n_long_blocks + n_tiles;
     }
 };
@@ -179,58 +180,118 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X,
     if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
 }
 
-// XCD-sliced product, pass 1: block b works on slice s = b % ns and writes the
-// slice's partial row sums part[(kv * ns + s) * rows + r].
-template <class V, class Acc, int K>
-__global__ __launch_bounds__(BLOCK) void k_spmv_slices(MatView<V> A, XPack<V, K> X) {
+// Sum of a value over the SEG lanes of an aligned lane group (butterfly; fixed order).
+template <int SEG, class Acc>
+__device__ __forceinline__ Acc group_sum(Acc v) {
+#pragma unroll
+    for (int o = SEG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, SEG);
+    return v;
+}
+
+// XCD-sliced product, pass 1 (K = 1, no block partials).  Blocks [0, ns * sl_max_tiles):
+// block b sums slice s = b % ns of the long rows of tile b / ns of that slice into
+// sl_part[s * n_long + r]: the tile's entries are loaded coalesced, their products
+// formed in LDS, then every row part is summed by a group of 16 lanes (lane l takes
+// entries l, l+16, ... in order, then a butterfly); a row part longer than TILE_NNZ
+// is summed by the whole block (strided, then the four wave sums in wave order).
+// Blocks past those: the side's short tiles, summed and finished by the epilogue
+// exactly as in k_spmv_csr.  Deterministic; the long rows' order of summation differs
+// from a sequential sum (as the wavefront path's does).
+template <class V, class Acc, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, 1> X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
+    __shared__ Acc red[BLOCK / WAVE];
+    if (epi.skip()) return;
+    const int nsb = A.ns * A.sl_max_tiles;
+    if ((int)blockIdx.x >= nsb) {
+        const TileMeta tm = A.tiles[blockIdx.x - nsb];
+        const int r = tm.r0 + (int)threadIdx.x;
+        typename Epi::Pre pre{};
+        if (r < tm.r1) pre = epi.load(r);
+        Acc acc[1];
+        tile_sums<V, Acc, 1>(A.rowptr, A.col, A.val, tm, X, prod, acc);
+        double q[1] = {0.0};
+        if (r < tm.r1) epi.row(r, acc, q, pre);
+        return;
+    }
     const int s = blockIdx.x % A.ns;
-    const int j = blockIdx.x / A.ns;
-    const int32_t* rp = A.sl_rowptr + (size_t)s * (A.rows + 1);
-    Acc* part = (Acc*)A.sl_part;
-    Acc acc[K];
-    if (j < A.n_long_blocks) {
-        const int r = j * LONG_ROWS_PER_BLOCK + threadIdx.x / WAVE;
-        if (r < A.n_long) {
-            long_row_sums<V, Acc, K>(rp, A.sl_col, A.sl_val, r, X, acc);
-            if ((threadIdx.x & (WAVE - 1)) == 0)
+    const int t = A.sl_tile_off[s] + (int)blockIdx.x / A.ns;
+    if (t >= A.sl_tile_off[s + 1]) return;   // uniform per block: this slice has fewer tiles
+    const TileMeta tm = A.sl_tiles[t];
+    const int32_t* rp = A.sl_rowptr + (size_t)s * (A.n_long + 1);
+    Acc* part = (Acc*)A.sl_part + (size_t)s * A.n_long;
+    const int tid = threadIdx.x;
+    if (tm.cnt > TILE_NNZ) {   // one long row part, the whole block
+        constexpr int U = 4;
+        Acc acc = Acc(0);
+        const int k1 = tm.base + tm.cnt;
+        for (int kb = tm.base + tid; kb < k1; kb += U * BLOCK) {
+            int c[U];
+            Acc v[U], xv[U];
 #pragma unroll
-                for (int kv = 0; kv < K; ++kv) part[((size_t)kv * A.ns + s) * A.rows + r] = acc[kv];
+            for (int u = 0; u < U; ++u) {
+                const int k = min(kb + u * BLOCK, k1 - 1);
+                c[u] = A.sl_col[k];
+                v[u] = (Acc)A.sl_val[k];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[0][c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (kb + u * BLOCK < k1) acc += v[u] * xv[u];
         }
-    } else {
-        const int t = A.sl_tile_off[s] + (j - A.n_long_blocks);
-        if (t >= A.sl_tile_off[s + 1]) return;   // uniform per block: slice has fewer tiles
-        const TileMeta tm = A.sl_tiles[t];
-        tile_sums<V, Acc, K>(rp, A.sl_col, A.sl_val, tm, X, prod, acc);
-        const int r = tm.r0 + threadIdx.x;
-        if (r < tm.r1)
+        acc = wave_sum(acc);
+        if ((tid & (WAVE - 1)) == 0) red[tid / WAVE] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            Acc sum = red[0];
 #pragma unroll
-            for (int kv = 0; kv < K; ++kv) part[((size_t)kv * A.ns + s) * A.rows + r] = acc[kv];
+            for (int w = 1; w < BLOCK / WAVE; ++w) sum += red[w];
+            part[tm.r0] = sum;
+        }
+        return;
+    }
+    if (tm.cnt > 0) {   // uniform per block
+        int cc[PER_T];
+        V vv[PER_T];
+#pragma unroll
+        for (int j = 0; j < PER_T; ++j) {
+            const int e = min(tid + j * BLOCK, tm.cnt - 1);
+            cc[j] = A.sl_col[tm.base + e];
+            vv[j] = A.sl_val[tm.base + e];
+        }
+        Acc xv[PER_T];
+#pragma unroll
+        for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[0][cc[j]];
+#pragma unroll
+        for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
+    }
+    __syncthreads();
+    constexpr int SEG = 16;
+    const int g = tid / SEG, l = tid % SEG;
+    // every lane of a group runs the same trips, so the butterfly's partners are active
+    for (int r = tm.r0 + g; r < tm.r1; r += BLOCK / SEG) {
+        const int lo = rp[r] - tm.base, hi = rp[r + 1] - tm.base;
+        Acc sum = Acc(0);
+        for (int e = lo + l; e < hi; e += SEG) sum += prod[e];
+        sum = group_sum<SEG>(sum);
+        if (l == 0) part[r] = sum;
     }
 }
 
-// XCD-sliced product, pass 2: row sum = sum of the slice partials in slice order,
-// then the epilogue (thread per row, coalesced).
-template <class V, class Acc, int K, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_slices_combine(MatView<V> A, Epi epi) {
-    double q[Epi::NQ > 0 ? Epi::NQ : 1];
-#pragma unroll
-    for (int i = 0; i < (Epi::NQ > 0 ? Epi::NQ : 1); ++i) q[i] = 0.0;
+// XCD-sliced product, pass 2: long row r's sum = its slice sums added in slice order,
+// then the epilogue (thread per row).
+template <class V, class Acc, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_xslice_combine(MatView<V> A, Epi epi) {
     if (epi.skip()) return;
     const int r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r < A.rows) {
-        const typename Epi::Pre pre = epi.load(r);
-        const Acc* part = (const Acc*)A.sl_part;
-        Acc acc[K];
-#pragma unroll
-        for (int kv = 0; kv < K; ++kv) {
-            Acc sum = part[((size_t)kv * A.ns) * A.rows + r];
-            for (int s = 1; s < A.ns; ++s) sum += part[((size_t)kv * A.ns + s) * A.rows + r];
-            acc[kv] = sum;
-        }
-        epi.row(r, acc, q, pre);
-    }
-    if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
+    if (r >= A.n_long) return;
+    const typename Epi::Pre pre = epi.load(r);
+    const Acc* part = (const Acc*)A.sl_part;
+    Acc acc[1] = {part[r]};
+    for (int s = 1; s < A.ns; ++s) acc[0] += part[(size_t)s * A.n_long + r];
+    double q[1] = {0.0};
+    epi.row(r, acc, q, pre);
 }
 
 // Column-major ELL: thread per row; entry s of rows i..i+63 is contiguous, so each
@@ -279,14 +340,24 @@ template <class V, class Acc, int K, class Epi>
 inline void launch_spmv(const MatView<V>& A, const XPack<V, K>& X, const Epi& epi, hipStream_t s) {
     const int g = A.grid();
     if (g <= 0) return;
-    if (A.format == CPDLP_FMT_XSLICE) {
-        hipLaunchKernelGGL((k_spmv_slices<V, Acc, K>), dim3(g), dim3(BLOCK), 0, s, A, X);
-        hipLaunchKernelGGL((k_slices_combine<V, Acc, K, Epi>), dim3(ceil_div(A.rows, BLOCK)),
-                           dim3(BLOCK), 0, s, A, epi);
-    } else if (A.format == CPDLP_FMT_ELL)
+    if constexpr (K == 1 && Epi::NQ == 0) {
+        if (A.format == CPDLP_FMT_XSLICE) {
+            hipLaunchKernelGGL((k_spmv_xslice<V, Acc, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
+            if (A.n_long > 0)
+                hipLaunchKernelGGL((k_xslice_combine<V, Acc, Epi>), dim3(ceil_div(A.n_long, BLOCK)),
+                                   dim3(BLOCK), 0, s, A, epi);
+            return;
+        }
+    }
+    if (A.format == CPDLP_FMT_ELL)
         hipLaunchKernelGGL((k_spmv_ell<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
-    else
-        hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
+    else {
+        // the sliced format's long rows are also in the plain CSR, which serves K > 1
+        // and the block-partial reductions
+        MatView<V> v = A;
+        v.format = CPDLP_FMT_CSR;
+        hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
+    }
 }
 
 }  // namespace cpdlp

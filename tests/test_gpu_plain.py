@@ -242,3 +242,37 @@ def test_power_law_lp_formats(cp, fmt):
     obj = cp.objective_value(sol.x, milp)
     assert abs(obj - g.objective) <= 1e-4 * (1 + abs(g.objective))
     assert abs(obj - float(ref.x @ g.c)) <= 1e-4 * (1 + abs(g.objective))
+
+
+def test_min_cost_flow_mixed_vs_highs(cp):
+    """Mixed precision (FP32 bulk, FP64 checks + polish) on a small min-cost flow:
+    objective within 1e-5 relative of HiGHS's optimum, flow conservation to 1e-5."""
+    from scipy.optimize import linprog
+    g = G.min_cost_flow_lp(nodes=2000, out_degree=8, seed=5)
+    A = g.csc()
+    h = linprog(g.c, A_eq=A, b_eq=g.lc, bounds=list(zip(g.lv, g.uv)), method="highs")
+    assert h.status == 0
+    milp = cp.MILP.from_generated(g)
+    sol, st = cp.solve(milp, cp.PDLP("mixed", termination_reltol=1e-6, powm_tol=1e-6,
+                                     max_kkt_passes=2_000_000))
+    assert st.termination_status == cp.OPTIMAL
+    assert st.fp32_passes > 0
+    obj = float(sol.x @ g.c)
+    assert abs(obj - h.fun) <= 1e-5 * (1 + abs(h.fun)), (obj, h.fun)
+    assert np.max(np.abs(A @ sol.x - g.lc)) <= 1e-5 * (1 + np.max(np.abs(g.lc)))
+
+
+def test_xslice_long_rows_track_csr(cp):
+    """XCD column slices of the long rows, including row parts longer than one tile
+    (rows of 15k-30k nonzeros): the KKT history tracks the CSR run's to rounding over
+    the first checks, and the format is reported per side."""
+    g = G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4)
+    milp = cp.MILP.from_generated(g)
+    kw = dict(termination_reltol=1e-12, max_kkt_passes=1500, powm_tol=1e-6)
+    _, s_csr = cp.solve(milp, cp.PDLP("float64", None, "csr", **kw))
+    _, s_xs = cp.solve(milp, cp.PDLP("float64", None, "xslice", **kw))
+    assert (s_xs.format_A, s_xs.format_At) == (3, 3)
+    assert len(s_csr.error_history) == len(s_xs.error_history) >= 10
+    for (p1, e1), (p2, e2) in list(zip(s_csr.error_history, s_xs.error_history))[:8]:
+        assert p1 == p2
+        assert abs(e1.relative() - e2.relative()) <= 1e-7 * e1.relative(), (p1, e1, e2)
