@@ -26,6 +26,25 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(workload: str, fmt_a: str, fmt_at: str):
+    """HBM bytes per PDHG iteration measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE
+    passes, each scaled by the same counter's reading for a plain copy of known size),
+    as recorded in profiles/pdhg_steps_<workload>_hbm_traffic.md for these formats;
+    None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", f"pdhg_steps_{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    rec = {}
+    for line in open(path):
+        if line.startswith("record:"):
+            for kv in line[len("record:"):].split():
+                k, _, v = kv.partition("=")
+                rec[k] = v
+    if rec.get("workload") != workload or rec.get("A") != fmt_a or rec.get("At") != fmt_at:
+        return None, None
+    return int(rec["traffic_bytes_per_iteration"]), os.path.relpath(path, ROOT)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -390,8 +409,10 @@ def main():
         "primal_step": {"ms": ms_p, "bytes": bp, "GBs": bp / (ms_p * 1e-3) / 1e9},
         "dual_step": {"ms": ms_d, "bytes": bd, "GBs": bd / (ms_d * 1e-3) / 1e9},
     }
+    fmt = {0: "auto", 1: "csr", 2: "ell", 3: "xslice"}
+    traffic, traffic_src = measured_traffic(desc["workload"], fmt[st.format_A], fmt[st.format_At])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "pdhg_iteration (primal_step + dual_step)",
                 "bytes_per_iteration": bp + bd, "ms_per_iteration": t_iter_ms,
                 "kernels": kernels}

@@ -75,6 +75,9 @@ _SIGS = {
     "cpdlp_default_params": (C.c_int, [_P(CParams), C.c_int32]),
     "cpdlp_create": (C.c_int, [_P(CLP), _P(CParams), _P(C.c_void_p)]),
     "cpdlp_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cpdlp_initialize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cpdlp_run_checks": (C.c_int, [C.c_void_p, C.c_int64, _P(C.c_int32)]),
+    "cpdlp_get_solution_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cpdlp_get_solution": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cpdlp_get_stats": (C.c_int, [C.c_void_p, _P(CStats)]),
     "cpdlp_get_history": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,

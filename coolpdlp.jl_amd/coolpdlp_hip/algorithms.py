@@ -191,14 +191,42 @@ class Engine:
         check(lib.cpdlp_prepare(self.h))
 
     def solve(self, sol: PrimalDualSolution | None = None):
+        x0, y0 = self._start(sol)
+        check(lib.cpdlp_solve(self.h, _ptr(x0), _ptr(y0)))
+        return self.solution(), self.stats()
+
+    def _start(self, sol):
         x0 = y0 = None
         if sol is not None:
             x0 = np.ascontiguousarray(sol.x, dtype=np.float64)
             y0 = np.ascontiguousarray(sol.y, dtype=np.float64)
             if len(x0) != self.n or len(y0) != self.m:
                 raise _lib.DimensionMismatch("starting point has the wrong size")
-        check(lib.cpdlp_solve(self.h, _ptr(x0), _ptr(y0)))
-        return self.solution(), self.stats()
+        return x0, y0
+
+    def initialize(self, sol: PrimalDualSolution | None = None):
+        """initialize(milp, sol, algo), common.jl:176: everything before the loop."""
+        x0, y0 = self._start(sol)
+        check(lib.cpdlp_initialize(self.h, _ptr(x0), _ptr(y0)))
+
+    def run_checks(self, max_checks: int = 1) -> TerminationStatus:
+        """solve!'s loop for at most `max_checks` chunks of check_every steps
+        (pdlp.jl:69-94); returns the termination status after the last check."""
+        st = C.c_int32()
+        check(lib.cpdlp_run_checks(self.h, int(max_checks), C.byref(st)))
+        return TerminationStatus(st.value)
+
+    def solution_device(self, x_dev, y_dev):
+        """Unscaled solution into torch device tensors (float64, cuda): no host copy."""
+        for t, k in ((x_dev, self.n), (y_dev, self.m)):
+            if t is not None and (t.dtype.itemsize != 8 or not t.is_cuda or t.numel() < k
+                                  or not t.is_contiguous()):
+                raise _lib.ArgumentError("solution_device needs contiguous float64 cuda tensors")
+        import torch
+        torch.cuda.current_stream().synchronize()
+        check(lib.cpdlp_get_solution_device(
+            self.h, C.c_void_p(x_dev.data_ptr()) if x_dev is not None else None,
+            C.c_void_p(y_dev.data_ptr()) if y_dev is not None else None))
 
     def solution(self) -> PrimalDualSolution:
         x = np.zeros(self.n)

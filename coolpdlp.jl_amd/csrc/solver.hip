@@ -875,7 +875,7 @@ void Handle::run_chunk(IterState<V>& S, bool pdlp) {
 // Returns when the phase terminates; phase_reltol replaces termination_reltol (mixed
 // precision's FP32 phase), final = false marks a non-final phase (no duplicate entry).
 template <class V>
-void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase) {
+void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_t max_checks) {
     const bool pdlp = P.algorithm == CPDLP_ALGO_PDLP;
     const int ce = P.check_every;
     std::vector<StepCoef> coefs(ce);
@@ -893,7 +893,7 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase) {
     auto push_history = [&]() {
         if (P.record_error_history) history.emplace_back(stats.kkt_passes, stats.err);
     };
-    for (;;) {
+    for (int64_t chk = 0; chk < max_checks; ++chk) {
         // step coefficients of this chunk: tau, sigma fixed; eta_sum advances per step
         const double tau = eta / omega, sigma = eta * omega, isig = 1.0 / sigma;
         for (int k = 0; k < ce; ++k) {
@@ -1057,7 +1057,10 @@ static void upcast(IterState<float>& a, IterState<double>& b, int64_t n, int64_t
     b.cur = 0;
 }
 
-void Handle::solve(const double* x0, const double* y0) {
+// start of solve (common.jl:186-201) up to the loop: preprocess + the start point on the
+// device + fresh solver scalars.  Returns true when the early exit of common.jl:194-198
+// (no constraints and c == 0) has already produced the solution.
+bool Handle::begin(const double* x0, const double* y0) {
     HIP_CHECK(hipSetDevice(device));
     t_start = now_s();
     last_progress = t_start;
@@ -1065,9 +1068,11 @@ void Handle::solve(const double* x0, const double* y0) {
     history.clear();
     stats.err = cpdlp_kkt{NAN, NAN, NAN, NAN, NAN, NAN};
     stats.termination_status = CPDLP_STILL_RUNNING;
+    stepping = false;
     prepare();   // solve = preprocess + initialize + solve!, every call (common.jl:186-201)
     // start point (unscaled, original order) on the device
-    DevBuf<double> x0d(std::max<int64_t>(n, 1)), y0d(std::max<int64_t>(m, 1));
+    if (x0d.n != (size_t)std::max<int64_t>(n, 1)) x0d.alloc(std::max<int64_t>(n, 1));
+    if (y0d.n != (size_t)std::max<int64_t>(m, 1)) y0d.alloc(std::max<int64_t>(m, 1));
     if (x0) x0d.upload(x0, n, stream);
     else hipLaunchKernelGGL(k_fill, dim3(grid_n()), dim3(BLOCK), 0, stream, (int64_t)n, x0d.p, 0.0);
     if (y0) y0d.upload(y0, m, stream);
@@ -1083,8 +1088,7 @@ void Handle::solve(const double* x0, const double* y0) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g.exec = nullptr;
     }
-    const double t_loop = now_s();
-    const bool f32 = P.precision != CPDLP_PREC_FP64;
+    t_loop = now_s();
     // early exit for no constraints and c == 0 (common.jl:194-198)
     bool c_zero = true;
     {
@@ -1095,21 +1099,48 @@ void Handle::solve(const double* x0, const double* y0) {
         }
         for (double v : hc) if (v != 0.0) { c_zero = false; break; }
     }
-    if (m == 0 && c_zero) {
-        init_state<double>(st64, x0d.p, y0d.p);
-        std::vector<double> lvh(n), uvh(n);
-        if (n) {
-            lv_s.download(lvh.data(), n, stream);
-            uv_s.download(uvh.data(), n, stream);
-            sync();
-            std::vector<double> xs(n);
-            for (int64_t j = 0; j < n; ++j) xs[j] = proj_box(0.0, lvh[j], uvh[j]);
-            st64.x[0].upload(xs.data(), n, stream);
-            sync();
-        }
-        active64 = true;
-        stats.termination_status = CPDLP_OPTIMAL;
-    } else if (!f32) {
+    if (!(m == 0 && c_zero)) return false;
+    init_state<double>(st64, x0d.p, y0d.p);
+    std::vector<double> lvh(n), uvh(n);
+    if (n) {
+        lv_s.download(lvh.data(), n, stream);
+        uv_s.download(uvh.data(), n, stream);
+        sync();
+        std::vector<double> xs(n);
+        for (int64_t j = 0; j < n; ++j) xs[j] = proj_box(0.0, lvh[j], uvh[j]);
+        st64.x[0].upload(xs.data(), n, stream);
+        sync();
+    }
+    active64 = true;
+    stats.termination_status = CPDLP_OPTIMAL;
+    return true;
+}
+
+void Handle::finish_stats() {
+    sync();
+    loop_seconds = now_s() - t_loop;
+    stats.eta = eta;
+    stats.omega = omega;
+    stats.powm_iters = powm_iters;
+    stats.restarts = restarts;
+    stats.n_history = (int64_t)history.size();
+    stats.fp32_passes = fp32_passes;
+    stats.setup_seconds = setup_seconds;
+    stats.loop_seconds = loop_seconds;
+    stats.step_kernel_ms = step_ms;
+    stats.step_kernel_launches = step_launches;
+    stats.format_A = A.format;
+    stats.format_At = At.format;
+    solved = true;
+}
+
+void Handle::solve(const double* x0, const double* y0) {
+    if (begin(x0, y0)) {
+        finish_stats();
+        return;
+    }
+    const bool f32 = P.precision != CPDLP_PREC_FP64;
+    if (!f32) {
         init_state<double>(st64, x0d.p, y0d.p);
         active64 = true;
         loop<double>(st64, P.termination_reltol, true);
@@ -1143,42 +1174,83 @@ void Handle::solve(const double* x0, const double* y0) {
             loop<double>(st64, P.termination_reltol, true);
         }
     }
-    sync();
-    loop_seconds = now_s() - t_loop;
-    stats.eta = eta;
-    stats.omega = omega;
-    stats.powm_iters = powm_iters;
-    stats.restarts = restarts;
-    stats.n_history = (int64_t)history.size();
-    stats.fp32_passes = fp32_passes;
-    stats.setup_seconds = setup_seconds;
-    stats.loop_seconds = loop_seconds;
-    stats.step_kernel_ms = step_ms;
-    stats.step_kernel_launches = step_launches;
-    stats.format_A = A.format;
-    stats.format_At = At.format;
-    solved = true;
+    finish_stats();
+}
+
+// initialize(milp, sol, algo) (common.jl:176; pdlp.jl:45-67, pdhg.jl:40-56): everything
+// of solve before the loop; the loop then runs in pieces through run_checks.
+void Handle::initialize(const double* x0, const double* y0) {
+    if (P.precision == CPDLP_PREC_MIXED)
+        throw Error(CPDLP_ERR_ARGUMENT,
+                    "mixed precision runs its two phases through cpdlp_solve; use fp64 or fp32");
+    if (begin(x0, y0)) {
+        finish_stats();
+        return;
+    }
+    if (P.precision == CPDLP_PREC_FP64) {
+        init_state<double>(st64, x0d.p, y0d.p);
+        active64 = true;
+    } else {
+        init_state<float>(st32, x0d.p, y0d.p);
+        active64 = false;
+    }
+    stepping = true;
+    finish_stats();
+}
+
+// solve!'s loop (pdlp.jl:69-94, pdhg.jl:58-74) for at most max_checks chunks of
+// check_every steps; the reference yields once per chunk (pdlp.jl:77).
+int32_t Handle::run_checks(int64_t max_checks) {
+    HIP_CHECK(hipSetDevice(device));
+    if (!stepping) {
+        if (solved) return stats.termination_status;
+        throw Error(CPDLP_ERR_STATE, "call cpdlp_initialize first");
+    }
+    if (max_checks <= 0) return stats.termination_status;
+    if (active64)
+        loop<double>(st64, P.termination_reltol, true, max_checks);
+    else
+        loop<float>(st32, P.termination_reltol, true, max_checks);
+    if (stats.termination_status != CPDLP_STILL_RUNNING) stepping = false;
+    finish_stats();
+    return stats.termination_status;
+}
+
+// unscaled solution in the original order into device buffers (NULL skips one)
+void Handle::solution_to(double* xo, double* yo) {
+    if (!solved) throw Error(CPDLP_ERR_STATE, "no solution: call cpdlp_solve first");
+    const int32_t nn = (int32_t)n, mm = (int32_t)m;
+    if (active64) {
+        if (xo && n)
+            hipLaunchKernelGGL(k_from_internal<double>, dim3(grid_n()), dim3(BLOCK), 0, stream, nn,
+                               d_int_of_col.p, d_D2o.p, (const double*)st64.x[st64.cur].p, xo);
+        if (yo && m)
+            hipLaunchKernelGGL(k_from_internal<double>, dim3(grid_m()), dim3(BLOCK), 0, stream, mm,
+                               d_int_of_row.p, d_D1o.p, (const double*)st64.y[st64.cur].p, yo);
+    } else {
+        if (xo && n)
+            hipLaunchKernelGGL(k_from_internal<float>, dim3(grid_n()), dim3(BLOCK), 0, stream, nn,
+                               d_int_of_col.p, d_D2o.p, (const float*)st32.x[st32.cur].p, xo);
+        if (yo && m)
+            hipLaunchKernelGGL(k_from_internal<float>, dim3(grid_m()), dim3(BLOCK), 0, stream, mm,
+                               d_int_of_row.p, d_D1o.p, (const float*)st32.y[st32.cur].p, yo);
+    }
+    HIP_CHECK(hipGetLastError());
 }
 
 void Handle::get_solution(double* x, double* y) {
     HIP_CHECK(hipSetDevice(device));
-    if (!solved) throw Error(CPDLP_ERR_STATE, "no solution: call cpdlp_solve first");
     DevBuf<double> xo(std::max<int64_t>(n, 1)), yo(std::max<int64_t>(m, 1));
-    const int32_t nn = (int32_t)n, mm = (int32_t)m;
-    if (active64) {
-        hipLaunchKernelGGL(k_from_internal<double>, dim3(grid_n()), dim3(BLOCK), 0, stream, nn,
-                           d_int_of_col.p, d_D2o.p, (const double*)st64.x[st64.cur].p, xo.p);
-        hipLaunchKernelGGL(k_from_internal<double>, dim3(grid_m()), dim3(BLOCK), 0, stream, mm,
-                           d_int_of_row.p, d_D1o.p, (const double*)st64.y[st64.cur].p, yo.p);
-    } else {
-        hipLaunchKernelGGL(k_from_internal<float>, dim3(grid_n()), dim3(BLOCK), 0, stream, nn,
-                           d_int_of_col.p, d_D2o.p, (const float*)st32.x[st32.cur].p, xo.p);
-        hipLaunchKernelGGL(k_from_internal<float>, dim3(grid_m()), dim3(BLOCK), 0, stream, mm,
-                           d_int_of_row.p, d_D1o.p, (const float*)st32.y[st32.cur].p, yo.p);
-    }
-    HIP_CHECK(hipGetLastError());
+    solution_to(xo.p, yo.p);
     if (x) xo.download(x, n, stream);
     if (y) yo.download(y, m, stream);
+    sync();
+}
+
+// the reference keeps sol on the device (common.jl:200): same, into caller buffers
+void Handle::get_solution_device(double* x_dev, double* y_dev) {
+    HIP_CHECK(hipSetDevice(device));
+    solution_to(x_dev, y_dev);
     sync();
 }
 
@@ -1335,6 +1407,19 @@ int cpdlp_create(const cpdlp_lp* lp, const cpdlp_params* params, cpdlp_handle** 
 
 int cpdlp_solve(cpdlp_handle* h, const double* x0, const double* y0) {
     return guarded([&] { HANDLE_OR_THROW(h); h->h->solve(x0, y0); });
+}
+int cpdlp_initialize(cpdlp_handle* h, const double* x0, const double* y0) {
+    return guarded([&] { HANDLE_OR_THROW(h); h->h->initialize(x0, y0); });
+}
+int cpdlp_run_checks(cpdlp_handle* h, int64_t max_checks, int32_t* status) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        const int32_t st = h->h->run_checks(max_checks);
+        if (status) *status = st;
+    });
+}
+int cpdlp_get_solution_device(cpdlp_handle* h, double* x_dev, double* y_dev) {
+    return guarded([&] { HANDLE_OR_THROW(h); h->h->get_solution_device(x_dev, y_dev); });
 }
 int cpdlp_prepare(cpdlp_handle* h) {
     return guarded([&] { HANDLE_OR_THROW(h); HIP_CHECK(hipSetDevice(h->h->device)); h->h->prepare(); });

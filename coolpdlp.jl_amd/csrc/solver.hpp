@@ -84,11 +84,12 @@ struct Handle {
     DevBuf<StepCoef> d_coef;
     GraphCache graphs[4];
     // solver scalars
-    bool prepared = false, solved = false, active64 = true;
+    bool prepared = false, solved = false, active64 = true, stepping = false;
     double eta = 0, omega0 = 1, omega = 1, eta_sum = 0;
     double primal_scale = 1, dual_scale = 1, c_norm = 0, comb_norm = 0;
     int64_t powm_iters = 0, inner = 0, total = 0, restarts = 0, step_launches = 0, fp32_passes = 0;
-    double setup_seconds = 0, loop_seconds = 0, step_ms = 0, t_start = 0;
+    double setup_seconds = 0, loop_seconds = 0, step_ms = 0, t_start = 0, t_loop = 0;
+    DevBuf<double> x0d, y0d;   // the start point of the current solve
     // mixed precision hand-over (FP32 phase -> FP64 polish)
     static constexpr int STALL_CHECKS = 10;
     double best_rel = INFINITY, best_omega = 1.0, last_progress = 0.0;
@@ -120,10 +121,17 @@ struct Handle {
     template <class V>
     void run_chunk(IterState<V>& S, bool pdlp);
     template <class V>
-    void loop(IterState<V>& S, double phase_reltol, bool final_phase);
+    void loop(IterState<V>& S, double phase_reltol, bool final_phase,
+              int64_t max_checks = INT64_MAX);
     template <class V>
     void init_state(IterState<V>& S, const double* x0_dev, const double* y0_dev);
+    bool begin(const double* x0, const double* y0);
+    void finish_stats();
     void solve(const double* x0, const double* y0);
+    void initialize(const double* x0, const double* y0);
+    int32_t run_checks(int64_t max_checks);
+    void solution_to(double* xo, double* yo);
+    void get_solution_device(double* x_dev, double* y_dev);
     void progress(bool final_line);
     void get_solution(double* x, double* y);
     cpdlp_kkt kkt_host(const double* x, const double* y);

@@ -140,6 +140,25 @@ int cpdlp_create(const cpdlp_lp* lp, const cpdlp_params* params, cpdlp_handle** 
  * x0 (n) / y0 (m) are the unscaled start (NULL = zero, common.jl:207). */
 int cpdlp_solve(cpdlp_handle* h, const double* x0, const double* y0);
 
+/* initialize(milp, sol, algo), src/algorithms/common.jl:176 (pdlp.jl:45-67,
+ * pdhg.jl:40-56): what cpdlp_solve does before its loop (preprocess, step size,
+ * primal weight, the start point on the device), the loop then run in pieces by
+ * cpdlp_run_checks.  FP64 and FP32 precision; the mixed precision's two phases run
+ * through cpdlp_solve (CPDLP_ERR_ARGUMENT here). */
+int cpdlp_initialize(cpdlp_handle* h, const double* x0, const double* y0);
+
+/* solve!(state, milp, algo)'s loop, pdlp.jl:69-94 / pdhg.jl:58-74, for at most
+ * max_checks chunks of check_every steps (the reference yields once per chunk,
+ * pdlp.jl:77).  *status = the termination status after the last check
+ * (CPDLP_STILL_RUNNING while the loop has not terminated).  Running the chunks one
+ * call at a time gives the same iterates as one cpdlp_solve. */
+int cpdlp_run_checks(cpdlp_handle* h, int64_t max_checks, int32_t* status);
+
+/* The solution kept on the device, as the reference's sol (common.jl:200): unscaled x
+ * (n) and y (m), original order, into caller-owned DEVICE buffers (either may be NULL).
+ * Valid after cpdlp_solve, cpdlp_initialize or cpdlp_run_checks. */
+int cpdlp_get_solution_device(cpdlp_handle* h, double* x_dev, double* y_dev);
+
 /* get_solution + Array(...), common.jl:233-235 and src/moi.jl:215-216:
  * unscaled x (n) and y (m) into host buffers. */
 int cpdlp_get_solution(cpdlp_handle* h, double* x, double* y);

@@ -276,3 +276,39 @@ def test_xslice_long_rows_track_csr(cp):
     for (p1, e1), (p2, e2) in list(zip(s_csr.error_history, s_xs.error_history))[:8]:
         assert p1 == p2
         assert abs(e1.relative() - e2.relative()) <= 1e-7 * e1.relative(), (p1, e1, e2)
+
+
+@pytest.mark.parametrize("algo,prec", [("PDLP", "float64"), ("PDHG", "float64"), ("PDLP", "float32")])
+def test_chunked_loop_equals_solve(cp, algo, prec):
+    """initialize + run_checks one chunk at a time (the reference's solve! yielding per
+    chunk) reproduces one solve exactly: same iterates, passes and history."""
+    import torch
+    g = G.column_lp(300, 700, 6, seed=9)
+    milp = cp.MILP.from_generated(g)
+    A = cp.PDLP if algo == "PDLP" else cp.PDHG
+    algo_ = A(prec, termination_reltol=1e-5, max_kkt_passes=40_000, powm_tol=1e-6)
+    with cp.Engine(milp, algo_) as eng:
+        sol, st = eng.solve()
+        eng.initialize()
+        status, calls = cp.STILL_RUNNING, 0
+        while status == cp.STILL_RUNNING:
+            status = eng.run_checks(1)
+            calls += 1
+        st2 = eng.stats()
+        sol2 = eng.solution()
+        xd = torch.empty(g.n, dtype=torch.float64, device="cuda")
+        yd = torch.empty(g.m, dtype=torch.float64, device="cuda")
+        eng.solution_device(xd, yd)
+    assert status == st.termination_status == st2.termination_status
+    assert st2.kkt_passes == st.kkt_passes
+    assert calls == st.kkt_passes // 100
+    assert np.array_equal(sol.x, sol2.x) and np.array_equal(sol.y, sol2.y)
+    assert [p for p, _ in st.error_history] == [p for p, _ in st2.error_history]
+    assert np.array_equal(xd.cpu().numpy(), sol2.x) and np.array_equal(yd.cpu().numpy(), sol2.y)
+
+
+def test_chunked_loop_rejects_mixed(cp):
+    g = G.column_lp(50, 100, 4, seed=1)
+    with cp.Engine(cp.MILP.from_generated(g), cp.PDLP("mixed")) as eng:
+        with pytest.raises(ValueError):
+            eng.initialize()
