@@ -145,6 +145,9 @@ def bench_batch(args):
         "batch_seconds": elapsed, "rounds": rounds,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
         "optimal": int(sum(s == "OPTIMAL" for s in status)), "count": len(lps),
+        "not_optimal": [{"index": i, "status": status[i], "kkt_passes": int(passes[i]),
+                         "m": lps[i].m, "n": lps[i].n, "nnz": lps[i].nnz}
+                        for i in range(len(lps)) if status[i] != "OPTIMAL"],
         "max_objective_rel_err": float(rel_obj.max()), "median_kkt_passes": float(np.median(passes)),
         "plain_gpu": {"sample": int(len(sample)), "mean_seconds_per_lp": float(np.mean(plain_t)),
                       "projected_total_seconds": plain_total, "lps_per_s": len(lps) / plain_total},

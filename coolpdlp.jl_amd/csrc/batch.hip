@@ -21,6 +21,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <deque>
 #include <map>
@@ -33,8 +34,9 @@ namespace cpdlp {
 
 // ---------------------------------------------------------------- device structures
 struct BInst {
-    int32_t m, n, nnz, La, Lat, pad0;
+    int32_t m, n, nnz, La, Lat, n_a_tiles, n_at_tiles, pad0;
     int32_t *a_rp, *a_col, *at_rp, *at_col, *perm;
+    const TileMeta *a_tiles, *at_tiles;   // row tiles of A and At for the step kernel
     double *nz, *acsc, *atcsc, *a64, *at64;
     float *a32, *at32;
     double *c, *lv, *uv, *lc, *uc;
@@ -118,11 +120,117 @@ __device__ __forceinline__ double pnorm_fin(double acc, double p) {
     return pow(acc, 1.0 / p);
 }
 
+// ---------------------------------------------------------------- tiled products
+// Tiles of a CSR's rows for the step kernel: <= BB rows and <= BT nonzeros each (a
+// single row longer than BT is a tile of its own).
+constexpr int BT = 2048;
+constexpr int BPT = BT / BB;
+
+// One half-step over the rows of a tiled CSR.  Per tile of <= BT nonzeros, thread tid
+// owns row r0 + tid: its epilogue operands (pre) and row bounds, and the tile's
+// (col, val) pairs (coalesced, clamped indices) are loaded while the previous tile is
+// being summed; then all gathers are issued together, the products go to LDS, and each
+// thread sums its row sequentially from zero (the reference's thread-per-row order,
+// src/utils/mat_csr.jl:63-79); fin(r, sum, pre) finishes the row.  A tile of one row
+// longer than BT is summed by the whole block (strided, then the four wave sums in wave
+// order).  Ends synchronised.
+template <class V, class GX, class PRE, class FIN>
+__device__ __forceinline__ void tiled_pass(const TileMeta* tiles, int ntiles, const int32_t* rp,
+                                           const int32_t* col, const V* val, V* prod, V* red,
+                                           GX gx, PRE pre, FIN fin) {
+    using PreT = decltype(pre(0));
+    const int tid = threadIdx.x;
+    if (ntiles <= 0) return;
+    int cc[BPT];
+    V vv[BPT];
+    PreT pv{};
+    int lo = 0, hi = 0;
+    auto fetch = [&](const TileMeta& tm) {   // the loads of a tile of <= BT nonzeros
+        const int r = tm.r0 + tid;
+        if (r < tm.r1) {
+            pv = pre(r);
+            lo = rp[r] - tm.base;
+            hi = rp[r + 1] - tm.base;
+        }
+        if (tm.cnt > 0) {
+#pragma unroll
+            for (int j = 0; j < BPT; ++j) {
+                const int e = min(tid + j * BB, tm.cnt - 1);
+                cc[j] = col[tm.base + e];
+                vv[j] = val[tm.base + e];
+            }
+        }
+    };
+    TileMeta tm = tiles[0];
+    if (tm.cnt <= BT) fetch(tm);
+    for (int t = 0; t < ntiles; ++t) {
+        const bool more = t + 1 < ntiles;
+        const TileMeta tn = more ? tiles[t + 1] : TileMeta{0, 0, 0, 0};
+        if (tm.cnt > BT) {   // uniform per block
+            const int r = tm.r0;
+            const auto pb = pre(r);
+            V acc = V(0);
+            const int k1 = tm.base + tm.cnt;
+            constexpr int U = 4;
+            for (int kb = tm.base + tid; kb < k1; kb += U * BB) {
+                int c[U];
+                V v[U], xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int k = min(kb + u * BB, k1 - 1);
+                    c[u] = col[k];
+                    v[u] = val[k];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = gx(c[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (kb + u * BB < k1) acc += v[u] * xv[u];
+            }
+            acc = wave_sum(acc);
+            if ((tid & (WAVE - 1)) == 0) red[tid / WAVE] = acc;
+            __syncthreads();
+            if (tid == 0) {
+                V sum = red[0];
+#pragma unroll
+                for (int w = 1; w < BB / WAVE; ++w) sum += red[w];
+                fin(r, sum, pb);
+            }
+            if (more && tn.cnt <= BT) fetch(tn);
+            __syncthreads();
+            tm = tn;
+            continue;
+        }
+        const int r = tm.r0 + tid;
+        const bool own = r < tm.r1;
+        if (tm.cnt > 0) {   // uniform per block
+            V xv[BPT];
+#pragma unroll
+            for (int j = 0; j < BPT; ++j) xv[j] = gx(cc[j]);
+#pragma unroll
+            for (int j = 0; j < BPT; ++j) prod[tid + j * BB] = vv[j] * xv[j];
+        }
+        __syncthreads();
+        const PreT pc = pv;
+        const int l0 = lo, h0 = hi;
+        if (more && tn.cnt <= BT) fetch(tn);   // in flight while this tile is summed
+        if (own) {
+            V sum = V(0);
+            for (int e = l0; e < h0; ++e) sum += prod[e];
+            fin(r, sum, pc);
+        }
+        __syncthreads();
+        tm = tn;
+    }
+}
+
 // ---------------------------------------------------------------- setup kernel
 // preprocess + initialize for each listed instance (one workgroup each)
 __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const BInst* insts,
                                                     BState* states, BParams P) {
     __shared__ double sh[8];
+    __shared__ double prod[BT];
+    __shared__ double red[BB / WAVE];
     const int slot = list[blockIdx.x];
     const BInst I = insts[slot];
     BState* S = states + slot;
@@ -253,12 +361,12 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     double theta = 0.0, residual = 1.7976931348623157e308;
     int64_t it = 0;
     while (!(it > maxiter || residual <= tol * fabs(theta))) {
-        vcsr<double, 1>(m, I.a_rp, I.a_col, I.a64, I.La, [&](int, int c) { return x[c]; },
-                        [&](int r, const double* acc) { t[r] = acc[0]; });
-        __syncthreads();
-        vcsr<double, 1>(n, I.at_rp, I.at_col, I.at64, I.Lat, [&](int, int c) { return t[c]; },
-                        [&](int r, const double* acc) { Ax[r] = acc[0]; });
-        __syncthreads();
+        tiled_pass<double>(I.a_tiles, I.n_a_tiles, I.a_rp, I.a_col, I.a64, prod, red,
+                           [&](int c) { return x[c]; }, [&](int) { return 0; },
+                           [&](int r, double acc, int) { t[r] = acc; });
+        tiled_pass<double>(I.at_tiles, I.n_at_tiles, I.at_rp, I.at_col, I.at64, prod, red,
+                           [&](int c) { return t[c]; }, [&](int) { return 0; },
+                           [&](int r, double acc, int) { Ax[r] = acc; });
         double q[2] = {0.0, 0.0};
         for (int j = tid; j < n; j += BB) {
             q[0] += x[j] * Ax[j];
@@ -332,14 +440,24 @@ __device__ __forceinline__ double abs_of(const double* e, double w) {
     return sqrt(w * w * e[0] * e[0] + (1.0 / (w * w)) * e[2] * e[2] + e[4] * e[4]);
 }
 
-// Run up to `checks` checks on each listed instance.  LDSG: gather y / xbar from LDS
-// (dynamic shared memory of (n + m) * sizeof(V) bytes); else from global memory.
+template <class V>
+struct PrePrimal {
+    V x, c, lv, uv, a;
+};
+template <class V>
+struct PreDual {
+    V y, lc, uc, a;
+};
+
+// check_every PDHG steps (pdlp.jl:96-124 with update_average!, :126-136; pdhg.jl:76-102)
+// on each listed instance.  LDSG: y and xbar live in LDS (dynamic shared memory of
+// (n + m) * sizeof(V) bytes) and the gathers read them there; else from global memory.
 template <class V, bool LDSG>
-__global__ __launch_bounds__(BB) void k_batch_run(const int32_t* list, const BInst* insts,
-                                                  BState* states, const int32_t* timeout,
-                                                  BParams P, int32_t checks) {
+__global__ __launch_bounds__(BB) void k_batch_steps(const int32_t* list, const BInst* insts,
+                                                    BState* states, BParams P) {
     extern __shared__ __align__(16) unsigned char dyn[];
-    __shared__ double sh[32];
+    __shared__ V prod[BT];
+    __shared__ V red[BB / WAVE];
     const int slot = list[blockIdx.x];
     const BInst I = insts[slot];
     BState* S = states + slot;
@@ -349,70 +467,121 @@ __global__ __launch_bounds__(BB) void k_batch_run(const int32_t* list, const BIn
     const Iter<V> it = iter_of(I, V(0));
     V* xbs = LDSG ? (V*)dyn : nullptr;          // xbar (n)
     V* ys = LDSG ? (V*)dyn + n : nullptr;       // y (m)
-    V* xbar_g = (V*)I.sn;   // global xbar when the vectors do not fit in LDS (n doubles)
-    // per-instance scalars (identical in every thread)
-    const double eta = S->eta;
-    double omega = S->omega, eta_sum = S->eta_sum;
+    V* xbar_g = (V*)I.sn;   // global xbar when the vectors do not fit in LDS
+    const double eta = S->eta, omega = S->omega;
+    double eta_sum = S->eta_sum;
     int cur = S->cur;
-    int64_t passes = S->kkt_passes, inner = S->inner, total = S->total;
+    if (LDSG) {   // y of the current solution into LDS
+        const V* yc = cur ? it.y[1] : it.y[0];
+        for (int i = tid; i < m; i += BB) ys[i] = yc[i];
+    }
+    __syncthreads();
+    const double tau = eta / omega, sigma = eta * omega, isig = 1.0 / sigma;
+    const V tv = (V)tau, sv = (V)sigma, iv = (V)isig;
+    const bool pdlp = P.pdlp != 0;
+    // plain pointers (no runtime-indexed arrays, which would live in scratch memory)
+    V *const x0 = it.x[0], *const x1 = it.x[1], *const y0 = it.y[0], *const y1 = it.y[1];
+    V *const xa = it.xa, *const xal = it.xal, *const ya = it.ya, *const yal = it.yal;
+    const V *const cv = it.c, *const lvv = it.lv, *const uvv = it.uv, *const lcv = it.lc,
+            *const ucv = it.uc, *const aval = it.a, *const atval = it.at;
+    const TileMeta *const a_tiles = I.a_tiles, *const at_tiles = I.at_tiles;
+    const int na_t = I.n_a_tiles, nat_t = I.n_at_tiles;
+    const int32_t *const a_rp = I.a_rp, *const a_col = I.a_col, *const at_rp = I.at_rp,
+                  *const at_col = I.at_col;
+    for (int s = 0; s < P.check_every; ++s) {
+        V* const xp_ = cur ? x1 : x0;
+        V* const xq_ = cur ? x0 : x1;
+        const V* const yp = cur ? y1 : y0;
+        V* const yq_ = cur ? y0 : y1;
+        const int q = 1 - cur;
+        const double wa = eta / (eta + eta_sum), wb = eta_sum / (eta + eta_sum);
+        const V wav = (V)wa, wbv = (V)wb;
+        const bool last = s == P.check_every - 1;
+        // primal half-step over the rows of A'
+        tiled_pass<V>(
+            at_tiles, nat_t, at_rp, at_col, atval, prod, red,
+            [&](int c) { return LDSG ? ys[c] : yp[c]; },
+            [&](int j) {
+                PrePrimal<V> r;
+                r.x = xp_[j];
+                r.c = cv[j];
+                r.lv = lvv[j];
+                r.uv = uvv[j];
+                r.a = pdlp ? xa[j] : V(0);
+                return r;
+            },
+            [&](int j, V acc, const PrePrimal<V>& r) {
+                const V xp = proj_box<V>(r.x - tv * (r.c - acc), r.lv, r.uv);
+                xq_[j] = xp;
+                const V xb = V(2) * xp - r.x;
+                if (LDSG) xbs[j] = xb;
+                else xbar_g[j] = xb;
+                if (pdlp) {
+                    if (last) xal[j] = r.a;
+                    xa[j] = wav * xp + wbv * r.a;
+                }
+            });
+        // dual half-step over the rows of A
+        tiled_pass<V>(
+            a_tiles, na_t, a_rp, a_col, aval, prod, red,
+            [&](int c) { return LDSG ? xbs[c] : xbar_g[c]; },
+            [&](int i) {
+                PreDual<V> r;
+                r.y = LDSG ? ys[i] : yp[i];
+                r.lc = lcv[i];
+                r.uc = ucv[i];
+                r.a = pdlp ? ya[i] : V(0);
+                return r;
+            },
+            [&](int i, V ax, const PreDual<V>& r) {
+                const V yn = (r.y - sv * ax) - sv * proj_box<V>(iv * r.y - ax, -r.uc, -r.lc);
+                yq_[i] = yn;
+                if (LDSG) ys[i] = yn;
+                if (pdlp) {
+                    if (last) yal[i] = r.a;
+                    ya[i] = wav * yn + wbv * r.a;
+                }
+            });
+        cur = q;
+        if (pdlp) eta_sum += eta;
+    }
+    if (tid == 0) {
+        S->eta_sum = eta_sum;
+        S->cur = cur;
+        S->kkt_passes += P.check_every;
+        if (pdlp) {
+            S->inner += P.check_every;
+            S->total += P.check_every;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- check kernel
+// After each chunk of steps: the 5-candidate KKT check in FP64 (errors.jl:65-109),
+// termination (termination.jl:89-101), restart_check! / restart! and the primal-weight
+// update (pdlp.jl:138-196, restart.jl:40-57, step_size.jl:58-75), and the mixed
+// precision's hand-over to FP64.
+template <class V>
+__global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const BInst* insts,
+                                                    BState* states, const int32_t* timeout,
+                                                    BParams P) {
+    __shared__ double sh[32];
+    const int slot = list[blockIdx.x];
+    const BInst I = insts[slot];
+    BState* S = states + slot;
+    const int m = I.m, n = I.n, tid = threadIdx.x;
+    const int want_phase = std::is_same<V, double>::value ? 1 : 0;
+    if (S->setup != 1 || S->status != CPDLP_STILL_RUNNING || S->phase != want_phase) return;
+    const Iter<V> it = iter_of(I, V(0));
+    double omega = S->omega, eta_sum = S->eta_sum;
+    const int cur = S->cur;
+    const int64_t passes = S->kkt_passes, total = S->total;
+    int64_t inner = S->inner;
     const double reltol = want_phase == 0 && P.precision == CPDLP_PREC_MIXED
                               ? fmax(P.polish_reltol, P.reltol) : P.reltol;
     const bool polish_next = want_phase == 0 && P.precision == CPDLP_PREC_MIXED;
-    if (LDSG) {   // y of the current solution into LDS
-        for (int i = tid; i < m; i += BB) ys[i] = it.y[cur][i];
-    }
-    __syncthreads();
     int32_t status = CPDLP_STILL_RUNNING;
-    for (int chk = 0; chk < checks && status == CPDLP_STILL_RUNNING; ++chk) {
-        const double tau = eta / omega, sigma = eta * omega, isig = 1.0 / sigma;
-        const V tv = (V)tau, sv = (V)sigma, iv = (V)isig;
-        for (int s = 0; s < P.check_every; ++s) {
-            const int p = cur, q = 1 - cur;
-            const double wa = eta / (eta + eta_sum), wb = eta_sum / (eta + eta_sum);
-            const V wav = (V)wa, wbv = (V)wb;
-            const bool last = s == P.check_every - 1;
-            const V* yp = it.y[p];
-            // primal half-step over the rows of A'
-            vcsr<V, 1>(n, I.at_rp, I.at_col, it.at, I.Lat,
-                       [&](int, int c) { return LDSG ? ys[c] : yp[c]; },
-                       [&](int j, const V* acc) {
-                           const V xj = it.x[p][j];
-                           const V xp = proj_box<V>(xj - tv * (it.c[j] - acc[0]), it.lv[j], it.uv[j]);
-                           it.x[q][j] = xp;
-                           const V xb = V(2) * xp - xj;
-                           if (LDSG) xbs[j] = xb;
-                           else xbar_g[j] = xb;
-                           if (P.pdlp) {
-                               const V a = it.xa[j];
-                               if (last) it.xal[j] = a;
-                               it.xa[j] = wav * xp + wbv * a;
-                           }
-                       });
-            __syncthreads();
-            // dual half-step over the rows of A
-            vcsr<V, 1>(m, I.a_rp, I.a_col, it.a, I.La,
-                       [&](int, int c) { return LDSG ? xbs[c] : xbar_g[c]; },
-                       [&](int i, const V* acc) {
-                           const V yi = LDSG ? ys[i] : yp[i];
-                           const V ax = acc[0];
-                           const V yn = (yi - sv * ax) - sv * proj_box<V>(iv * yi - ax, -it.uc[i], -it.lc[i]);
-                           it.y[q][i] = yn;
-                           if (LDSG) ys[i] = yn;
-                           if (P.pdlp) {
-                               const V a = it.ya[i];
-                               if (last) it.yal[i] = a;
-                               it.ya[i] = wav * yn + wbv * a;
-                           }
-                       });
-            __syncthreads();
-            cur = q;
-            if (P.pdlp) {
-                eta_sum += eta;
-                ++inner;
-                ++total;
-            }
-            ++passes;
-        }
+    {
         // ---- check: KKT errors (FP64) of sol, avg, sol_last, avg_last, restart
         const int c = cur, l = 1 - cur;
         constexpr int K = 5;
@@ -484,18 +653,19 @@ __global__ __launch_bounds__(BB) void k_batch_run(const int32_t* list, const BIn
         if (tid == 0) {
             for (int f = 0; f < 6; ++f) S->err[f] = e[0][f];
         }
-        int restart = 0, from_avg = 0, switch_phase = 0, improve = 0;
+        int restart = 0, from_avg = 0, switch_phase = 0, improve = 0, blowup = 0, reached = 0;
         if (status == CPDLP_STILL_RUNNING && polish_next) {
-            // FP32 phase: keep the best iterate; hand over on stall / blow-up
+            // FP32 phase: keep the best iterate; hand over on blow-up (at once, restart
+            // from the best) or on stall (after this check's restart decision, continuing)
             if (rel < S->best_rel) improve = 1;
-            else if (S->stall + 1 >= STALL || rel > 100.0 * S->best_rel) switch_phase = 1;
+            else if (rel > 100.0 * S->best_rel) switch_phase = blowup = 1;
+            else if (S->stall + 1 >= STALL) switch_phase = 1;
         }
         if (status == CPDLP_OPTIMAL && polish_next && P.reltol < reltol) {
-            improve = 1;          // the terminating sol is the hand-over point
-            switch_phase = 1;
+            switch_phase = reached = 1;   // continue in FP64 from this state
             status = CPDLP_STILL_RUNNING;
         }
-        if (status == CPDLP_STILL_RUNNING && !switch_phase && P.pdlp) {
+        if (status == CPDLP_STILL_RUNNING && !blowup && !reached && P.pdlp) {
             // restart_check! + should_restart (pdlp.jl:138-170, restart.jl:40-57)
             from_avg = !(abs_of(e[0], omega) < abs_of(e[1], omega));
             const double* cand = from_avg ? e[1] : e[0];
@@ -540,32 +710,55 @@ __global__ __launch_bounds__(BB) void k_batch_run(const int32_t* list, const BIn
                 it.y[c][i] = s0;
                 it.ya[i] = V(0);
                 it.yr[i] = s0;
-                if (LDSG) ys[i] = s0;
             }
             if (tid == 0) S->restarts += 1;
         }
         __syncthreads();
         if (switch_phase) {
-            // FP64 polish from the best FP32 iterate: sol = avg = restart = best
-            for (int j = tid; j < n; j += BB) {
-                const double v = (double)I.xb32[j];
-                I.x64[0][j] = I.xa64[j] = I.xr64[j] = v;
-                I.x64[1][j] = I.xal64[j] = 0.0;
-            }
-            for (int i = tid; i < m; i += BB) {
-                const double v = (double)I.yb32[i];
-                I.y64[0][i] = I.ya64[i] = I.yr64[i] = v;
-                I.y64[1][i] = I.yal64[i] = 0.0;
+            if (blowup) {
+                // FP64 phase from the best FP32 iterate: sol = avg = restart = best
+                for (int j = tid; j < n; j += BB) {
+                    const double v = (double)I.xb32[j];
+                    I.x64[0][j] = I.xa64[j] = I.xr64[j] = v;
+                    I.x64[1][j] = I.xal64[j] = 0.0;
+                }
+                for (int i = tid; i < m; i += BB) {
+                    const double v = (double)I.yb32[i];
+                    I.y64[0][i] = I.ya64[i] = I.yr64[i] = v;
+                    I.y64[1][i] = I.yal64[i] = 0.0;
+                }
+            } else {
+                // continuation: every iterate of the state in FP64, same parity
+                for (int j = tid; j < n; j += BB) {
+                    I.x64[0][j] = (double)I.x32[0][j];
+                    I.x64[1][j] = (double)I.x32[1][j];
+                    I.xa64[j] = (double)I.xa32[j];
+                    I.xal64[j] = (double)I.xal32[j];
+                    I.xr64[j] = (double)I.xr32[j];
+                }
+                for (int i = tid; i < m; i += BB) {
+                    I.y64[0][i] = (double)I.y32[0][i];
+                    I.y64[1][i] = (double)I.y32[1][i];
+                    I.ya64[i] = (double)I.ya32[i];
+                    I.yal64[i] = (double)I.yal32[i];
+                    I.yr64[i] = (double)I.yr32[i];
+                }
             }
             __syncthreads();
             if (tid == 0) {
                 S->phase = 1;
                 S->fp32_passes = passes;
-                S->omega = S->best_omega;
-                S->eta_sum = 0.0;
-                S->inner = 0;
-                S->total = 0;
-                S->cur = 0;
+                if (blowup) {
+                    S->omega = S->best_omega;
+                    S->eta_sum = 0.0;
+                    S->inner = 0;
+                    S->total = 0;
+                    S->cur = 0;
+                } else {
+                    S->omega = omega;
+                    S->eta_sum = eta_sum;
+                    S->inner = inner;
+                }
                 S->kkt_passes = passes;
                 S->status = CPDLP_STILL_RUNNING;
             }
@@ -575,10 +768,7 @@ __global__ __launch_bounds__(BB) void k_batch_run(const int32_t* list, const BIn
     if (tid == 0) {
         S->omega = omega;
         S->eta_sum = eta_sum;
-        S->cur = cur;
-        S->kkt_passes = passes;
         S->inner = inner;
-        S->total = total;
         S->status = status;
     }
 }
@@ -605,11 +795,21 @@ static double bnow() {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+static_assert(BT == TILE_NNZ && BB == BLOCK, "batch tiles are built by build_tiles");
+
+// most tiles build_tiles can make: a tile closes at BB rows, at the end, or because the
+// next row would pass BT nonzeros -- then it and the next tile hold > BT together, so
+// there are fewer than nnz / (BT / 2) of those
+static int64_t max_tiles(int64_t rows, int64_t nnz) {
+    return (rows + BB - 1) / BB + (nnz + BT / 2 - 1) / (BT / 2) + 1;
+}
+
 // bytes of one instance in the arena (same order as carve())
 static size_t footprint(int64_t m, int64_t n, int64_t nnz) {
     size_t b = 0;
     auto add = [&](int64_t count, size_t sz) { b += align256((size_t)std::max<int64_t>(count, 1) * sz); };
     add(m + 1, 4); add(nnz, 4); add(n + 1, 4); add(nnz, 4); add(nnz, 4);       // ints
+    add(max_tiles(m, nnz), sizeof(TileMeta)); add(max_tiles(n, nnz), sizeof(TileMeta));
     for (int k = 0; k < 5; ++k) add(nnz, 8);                                    // nz, acsc, atcsc, a64, at64
     add(nnz, 4); add(nnz, 4);                                                   // a32, at32
     for (int k = 0; k < 3; ++k) add(n, 8);                                      // c, lv, uv
@@ -642,6 +842,8 @@ static BInst carve(char* base, int64_t m, int64_t n, int64_t nnz) {
     I.at_rp = (int32_t*)take(n + 1, 4);
     I.at_col = (int32_t*)take(nnz, 4);
     I.perm = (int32_t*)take(nnz, 4);
+    I.a_tiles = (const TileMeta*)take(max_tiles(m, nnz), sizeof(TileMeta));
+    I.at_tiles = (const TileMeta*)take(max_tiles(n, nnz), sizeof(TileMeta));
     I.nz = (double*)take(nnz, 8);
     I.acsc = (double*)take(nnz, 8);
     I.atcsc = (double*)take(nnz, 8);
@@ -786,7 +988,12 @@ struct Batch {
         arena.init((size_t)bytes);
         int lds = 0;
         HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-        lds_max = lds > 4096 ? (size_t)lds - 4096 : 0;
+        // room for the step kernel's static LDS (BT products in FP64 + the reductions)
+        constexpr size_t static_lds = BT * sizeof(double) + 4096;
+        lds_max = (size_t)lds > static_lds ? (size_t)lds - static_lds : 0;
+        // instances whose y and xbar need more LDS than this gather from global memory
+        // (L2) instead: more workgroups fit on a CU
+        if (const char* e = std::getenv("CPDLP_BATCH_LDS_BYTES")) lds_max = std::min<size_t>(lds_max, std::strtoull(e, nullptr, 10));
         max_slots = 1 << 16;
         d_inst.alloc(max_slots);
         d_state.alloc(max_slots);
@@ -844,6 +1051,13 @@ struct Batch {
         BInst I = carve(arena_mem.p + off, m, n, nnz);
         I.La = lanes_for(nnz, m);
         I.Lat = lanes_for(nnz, n);
+        std::vector<TileMeta> ta, tat;
+        build_tiles(rp.data(), 0, (int32_t)m, ta);
+        build_tiles(cp.data(), 0, (int32_t)n, tat);
+        if ((int64_t)ta.size() > max_tiles(m, nnz) || (int64_t)tat.size() > max_tiles(n, nnz))
+            throw Error(CPDLP_ERR_STATE, "internal: tile count above its bound");
+        I.n_a_tiles = (int32_t)ta.size();
+        I.n_at_tiles = (int32_t)tat.size();
         const hipStream_t s = streams[0];
         auto up = [&](void* dst, const void* src, size_t b) {
             if (b) HIP_CHECK(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, s));
@@ -853,6 +1067,8 @@ struct Batch {
         up(I.at_rp, cp.data(), (n + 1) * 4);
         up(I.at_col, rv.data(), nnz * 4);
         up(I.perm, perm.data(), nnz * 4);
+        up((void*)I.a_tiles, ta.data(), ta.size() * sizeof(TileMeta));
+        up((void*)I.at_tiles, tat.data(), tat.size() * sizeof(TileMeta));
         up(I.nz, lp.nzval, nnz * 8);
         up(I.c, lp.c, n * 8);
         up(I.lv, lp.lv, n * 8);
@@ -934,6 +1150,13 @@ struct Batch {
                         : lds <= std::min<size_t>(98304, lds_max) ? 2 : 3;
             cls[ph][c].push_back(h.slot);
         }
+        // largest instances first: workgroups are dispatched in list order, so the long
+        // ones start early and the short ones fill the CUs at the end of each launch
+        for (auto& ph_cls : cls)
+            for (auto& v : ph_cls)
+                std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
+                    return h_inst[a].nnz + h_inst[a].m + h_inst[a].n > h_inst[b].nnz + h_inst[b].m + h_inst[b].n;
+                });
         lists.clear();
         std::vector<std::pair<int, int>> spans;   // (offset, count) into lists per launch
         for (int ph = 0; ph < 2; ++ph)
@@ -949,31 +1172,45 @@ struct Batch {
             HIP_CHECK(hipEventRecord(ready, streams[0]));
             const size_t lds_bytes[4] = {std::min<size_t>(16384, lds_max), std::min<size_t>(40960, lds_max),
                                          std::min<size_t>(98304, lds_max), 0};
+            hipStream_t gs[8];
             int si = 0;
-            for (int ph = 0; ph < 2; ++ph)
-                for (int c = 0; c < 4; ++c) {
-                    const auto sp = spans[ph * 4 + c];
-                    if (sp.second == 0) continue;
-                    const hipStream_t s = streams[si++ % 4];
-                    HIP_CHECK(hipStreamWaitEvent(s, ready, 0));
-                    const int32_t* L = d_list.p + sp.first;
-                    if (ph == 0) {
-                        if (c < 3)
-                            hipLaunchKernelGGL((k_batch_run<float, true>), dim3(sp.second), dim3(BB), lds_bytes[c], s,
-                                               L, d_inst.p, d_state.p, d_timeout.p, bp, checks);
-                        else
-                            hipLaunchKernelGGL((k_batch_run<float, false>), dim3(sp.second), dim3(BB), 0, s, L,
-                                               d_inst.p, d_state.p, d_timeout.p, bp, checks);
-                    } else {
-                        if (c < 3)
-                            hipLaunchKernelGGL((k_batch_run<double, true>), dim3(sp.second), dim3(BB), lds_bytes[c], s,
-                                               L, d_inst.p, d_state.p, d_timeout.p, bp, checks);
-                        else
-                            hipLaunchKernelGGL((k_batch_run<double, false>), dim3(sp.second), dim3(BB), 0, s, L,
-                                               d_inst.p, d_state.p, d_timeout.p, bp, checks);
+            for (int g = 0; g < 8; ++g) {
+                gs[g] = nullptr;
+                if (spans[g].second == 0) continue;
+                gs[g] = streams[si++ % 4];
+                HIP_CHECK(hipStreamWaitEvent(gs[g], ready, 0));
+            }
+            // per check: the steps of every group, each followed by its check
+            for (int chk = 0; chk < checks; ++chk)
+                for (int ph = 0; ph < 2; ++ph)
+                    for (int c = 0; c < 4; ++c) {
+                        const int g = ph * 4 + c;
+                        const auto sp = spans[g];
+                        if (sp.second == 0) continue;
+                        const hipStream_t s = gs[g];
+                        const int32_t* L = d_list.p + sp.first;
+                        const dim3 grid(sp.second), blk(BB);
+                        if (ph == 0) {
+                            if (c < 3)
+                                hipLaunchKernelGGL((k_batch_steps<float, true>), grid, blk, lds_bytes[c], s, L,
+                                                   d_inst.p, d_state.p, bp);
+                            else
+                                hipLaunchKernelGGL((k_batch_steps<float, false>), grid, blk, 0, s, L, d_inst.p,
+                                                   d_state.p, bp);
+                            hipLaunchKernelGGL(k_batch_check<float>, grid, blk, 0, s, L, d_inst.p, d_state.p,
+                                               d_timeout.p, bp);
+                        } else {
+                            if (c < 3)
+                                hipLaunchKernelGGL((k_batch_steps<double, true>), grid, blk, lds_bytes[c], s, L,
+                                                   d_inst.p, d_state.p, bp);
+                            else
+                                hipLaunchKernelGGL((k_batch_steps<double, false>), grid, blk, 0, s, L, d_inst.p,
+                                                   d_state.p, bp);
+                            hipLaunchKernelGGL(k_batch_check<double>, grid, blk, 0, s, L, d_inst.p, d_state.p,
+                                               d_timeout.p, bp);
+                        }
+                        HIP_CHECK(hipGetLastError());
                     }
-                    HIP_CHECK(hipGetLastError());
-                }
             sync_all();
             (void)hipEventDestroy(ready);
         }

@@ -940,10 +940,13 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_
         push_history();
         stats.termination_status = term_status(stats.err);
         if (P.show_progress) progress(false);
+        bool switch_after = false;
         if (stats.termination_status == CPDLP_STILL_RUNNING && !final_phase) {
-            // mixed precision, FP32 phase: keep the best iterate; hand over to FP64 when
-            // the relative KKT error has not improved for STALL_CHECKS checks or has
-            // grown 100x past its best (FP32 rounding floor reached)
+            // mixed precision, FP32 phase: hand over to FP64 when the relative KKT error
+            // has not improved for STALL_CHECKS checks (after this check's restart
+            // decision; the FP64 phase continues the trajectory), or at once when it has
+            // grown 100x past its best (the FP64 phase then restarts from the best
+            // iterate, kept here)
             const double rel = relative(stats.err);
             if (rel < best_rel) {
                 best_rel = rel;
@@ -951,9 +954,13 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_
                 stall = 0;
                 if (n) HIP_CHECK(hipMemcpyAsync(S.tmp_n.p, S.x[c].p, n * sizeof(V), hipMemcpyDeviceToDevice, stream));
                 if (m) HIP_CHECK(hipMemcpyAsync(S.tmp_m.p, S.y[c].p, m * sizeof(V), hipMemcpyDeviceToDevice, stream));
-            } else if (++stall >= STALL_CHECKS || rel > 100.0 * best_rel) {
+            } else if (rel > 100.0 * best_rel) {
                 phase_switch = true;
+                blowup = true;
                 return;
+            } else if (++stall >= STALL_CHECKS) {
+                phase_switch = true;
+                switch_after = true;
             }
         }
         if (stats.termination_status != CPDLP_STILL_RUNNING) {
@@ -965,7 +972,10 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_
             }
             return;
         }
-        if (!pdlp) continue;
+        if (!pdlp) {
+            if (switch_after) return;
+            continue;
+        }
         // restart_check!, pdlp.jl:138-170 and should_restart, restart.jl:40-57
         const cpdlp_kkt e_s = stats.err, e_a = kkt_from_sums(sums, K, 1);
         const cpdlp_kkt e_l = kkt_from_sums(sums, K, 2), e_al = kkt_from_sums(sums, K, 3);
@@ -997,6 +1007,7 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_
             inner = 0;
             restarts += 1;
         }
+        if (switch_after) return;
     }
 }
 
@@ -1134,6 +1145,24 @@ void Handle::finish_stats() {
     solved = true;
 }
 
+// FP32 phase -> FP64 phase by continuation: every iterate of the state (sol, sol_last,
+// avg, avg_last, restart) converted, same buffer parity; omega, eta_sum and the restart
+// counters carry over unchanged
+static void continue_fp64(IterState<float>& a, IterState<double>& b, int64_t n, int64_t m, hipStream_t s) {
+    const int gn = std::max(1, ceil_div(n, BLOCK)), gm = std::max(1, ceil_div(m, BLOCK));
+    const std::pair<DevBuf<float>*, DevBuf<double>*> xs[] = {
+        {&a.x[0], &b.x[0]}, {&a.x[1], &b.x[1]}, {&a.xavg, &b.xavg}, {&a.xavgl, &b.xavgl}, {&a.xrst, &b.xrst}};
+    const std::pair<DevBuf<float>*, DevBuf<double>*> ys[] = {
+        {&a.y[0], &b.y[0]}, {&a.y[1], &b.y[1]}, {&a.yavg, &b.yavg}, {&a.yavgl, &b.yavgl}, {&a.yrst, &b.yrst}};
+    for (const auto& pr : xs)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gn), dim3(BLOCK), 0, s, n, (const float*)pr.first->p,
+                           pr.second->p);
+    for (const auto& pr : ys)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gm), dim3(BLOCK), 0, s, m, (const float*)pr.first->p,
+                           pr.second->p);
+    b.cur = a.cur;
+}
+
 void Handle::solve(const double* x0, const double* y0) {
     if (begin(x0, y0)) {
         finish_stats();
@@ -1153,22 +1182,21 @@ void Handle::solve(const double* x0, const double* y0) {
         best_omega = omega;
         stall = 0;
         phase_switch = false;
+        blowup = false;
         loop<float>(st32, r32, !polish);
         fp32_passes = stats.kkt_passes;
         const bool reached = stats.termination_status == CPDLP_OPTIMAL && r32 > P.termination_reltol;
         if (polish && (reached || phase_switch)) {
-            if (reached) {   // the terminating check's sol is the hand-over point
-                const int c = st32.cur;
-                if (n) HIP_CHECK(hipMemcpyAsync(st32.tmp_n.p, st32.x[c].p, n * 4, hipMemcpyDeviceToDevice, stream));
-                if (m) HIP_CHECK(hipMemcpyAsync(st32.tmp_m.p, st32.y[c].p, m * 4, hipMemcpyDeviceToDevice, stream));
-            } else {
-                omega = best_omega;
-            }
             st64.alloc(n, m);
-            upcast(st32, st64, n, m, stream);
-            eta_sum = 0.0;
-            inner = 0;
-            total = 0;
+            if (blowup) {   // restart from the best FP32 iterate with its primal weight
+                omega = best_omega;
+                upcast(st32, st64, n, m, stream);
+                eta_sum = 0.0;
+                inner = 0;
+                total = 0;
+            } else {        // continue: the whole PDLP state in FP64, same scalars
+                continue_fp64(st32, st64, n, m, stream);
+            }
             active64 = true;
             stats.termination_status = CPDLP_STILL_RUNNING;
             loop<double>(st64, P.termination_reltol, true);

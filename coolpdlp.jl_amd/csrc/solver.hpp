@@ -94,7 +94,7 @@ struct Handle {
     static constexpr int STALL_CHECKS = 10;
     double best_rel = INFINITY, best_omega = 1.0, last_progress = 0.0;
     int stall = 0;
-    bool phase_switch = false;
+    bool phase_switch = false, blowup = false;
     IterState<double> st64;
     IterState<float> st32;
     cpdlp_stats stats{};

@@ -219,4 +219,46 @@ function CoolPDLP.solve(milp::MILP, sol::PrimalDualSolution, algo::Algorithm{A, 
     end
 end
 
+# ---- 3. the loop in pieces (initialize / solve! yielding per chunk) --------------
+"""
+    HIPState
+
+Handle of a solve run chunk by chunk: `initialize` (common.jl:176) -> `cpdlp_initialize`,
+`solve!` (pdlp.jl:69-94) -> `cpdlp_run_checks` one chunk of `check_every` steps at a time,
+yielding in between as the reference does (pdlp.jl:77).  FP64 / FP32 only.
+"""
+mutable struct HIPState
+    handle::Ptr{Cvoid}
+    m::Int
+    n::Int
+end
+
+function hip_initialize(milp::MILP, sol::PrimalDualSolution, algo::Algorithm{A, T, Ti, M, HIPEngine}) where {A, T, Ti, M}
+    A_ = milp.A::SparseMatrixCSC
+    c, lv, uv, lc, uc = (Vector{Float64}(v) for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))
+    lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
+        pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
+    p = cparams(algo, algo.conversion.backend)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve A_ c lv uv lc uc begin
+        check(ccall((:cpdlp_create, libcpdlp), Cint, (Ref{CLP}, Ref{CParams}, Ptr{Ptr{Cvoid}}), lp, p, h))
+    end
+    state = HIPState(h[], size(A_, 1), size(A_, 2))
+    finalizer(s -> ccall((:cpdlp_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), s.handle), state)
+    x0, y0 = Vector{Float64}(sol.x), Vector{Float64}(sol.y)
+    check(ccall((:cpdlp_initialize, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+        state.handle, x0, y0))
+    return state
+end
+
+function hip_solve!(state::HIPState)
+    status = Ref{Int32}(3)   # STILL_RUNNING
+    while status[] == 3
+        check(ccall((:cpdlp_run_checks, libcpdlp), Cint, (Ptr{Cvoid}, Int64, Ref{Int32}),
+            state.handle, 1, status))
+        yield()
+    end
+    return STATUS[status[] + 1]
+end
+
 end # module
