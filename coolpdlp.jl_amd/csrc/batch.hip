@@ -214,11 +214,7 @@ __device__ __forceinline__ void tiled_pass(const TileMeta* tiles, int ntiles, co
         const PreT pc = pv;
         const int l0 = lo, h0 = hi;
         if (more && tn.cnt <= BT) fetch(tn);   // in flight while this tile is summed
-        if (own) {
-            V sum = V(0);
-            for (int e = l0; e < h0; ++e) sum += prod[e];
-            fin(r, sum, pc);
-        }
+        if (own) fin(r, seq_sum(prod, l0, h0), pc);
         __syncthreads();
         tm = tn;
     }

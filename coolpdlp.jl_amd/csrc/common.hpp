@@ -147,6 +147,32 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// Sequential sum p[lo] + p[lo+1] + ... + p[hi-1] from zero, in that order (bit-identical
+// to the plain loop), with the LDS reads of each group of 8 issued together so that their
+// latencies overlap instead of adding up.
+template <class T>
+__device__ __forceinline__ T seq_sum(const T* p, int lo, int hi) {
+    T sum = T(0);
+    int e = lo;
+    for (; e + 8 <= hi; e += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[e + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum += v[u];
+    }
+    if (e + 4 <= hi) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = p[e + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sum += v[u];
+        e += 4;
+    }
+    for (; e < hi; ++e) sum += p[e];
+    return sum;
+}
+
 // Deterministic block reduction of NQ doubles per thread; thread q < NQ of the block
 // returns the block total of quantity q (others: unspecified).  Fixed order: butterfly
 // within each wave, then the 4 wave sums in wave order.
