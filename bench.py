@@ -118,6 +118,7 @@ def bench_batch(args):
     rel_obj = np.array([abs(float(sol.x @ g.c) - g.objective) / (1 + abs(g.objective))
                         for g, (sol, st) in zip(lps, res)])
     passes = np.array([st.kkt_passes for _, st in res])
+    powm = np.array([st.powm_iters for _, st in res])
     # plain one-at-a-time GPU path on a stratified sample (sorted by size, evenly spaced)
     order = np.argsort([g.nnz for g in lps])
     sample = order[np.linspace(0, len(lps) - 1, args.plain_sample).astype(int)]
@@ -149,6 +150,8 @@ def bench_batch(args):
                          "m": lps[i].m, "n": lps[i].n, "nnz": lps[i].nnz}
                         for i in range(len(lps)) if status[i] != "OPTIMAL"],
         "max_objective_rel_err": float(rel_obj.max()), "median_kkt_passes": float(np.median(passes)),
+        "mean_kkt_passes": float(np.mean(passes)), "fp32_pass_share": float(np.sum([st.fp32_passes for _, st in res]) / max(1, passes.sum())),
+        "powm_iters_median": float(np.median(powm)), "powm_iters_max": int(powm.max()),
         "plain_gpu": {"sample": int(len(sample)), "mean_seconds_per_lp": float(np.mean(plain_t)),
                       "projected_total_seconds": plain_total, "lps_per_s": len(lps) / plain_total},
         "speedup_vs_plain": plain_total / elapsed,
@@ -272,6 +275,21 @@ def bench_serve(args):
         lat = np.array([r.t_done - r.t_arrival for r in done])
         ok = sum(r.status == "OPTIMAL" for r in done)
         fc = fcfs_latencies(arr, np.array([service[p] for p in picks]))
+        cls_done = {r.rid: picks[r.rid][0] for r in done}
+        rs = np.array(srv.round_seconds) if srv.round_seconds else np.zeros(1)
+        waits = {c: float(np.median([r.t_start - r.t_arrival for r in done if cls_done[r.rid] == c] or [0.0]))
+                 for c in ("small", "medium", "large")}
+        runs = {c: float(np.median([r.t_done - r.t_start for r in done if cls_done[r.rid] == c] or [0.0]))
+                for c in ("small", "medium", "large")}
+        per_class = {}
+        for c in ("small", "medium", "large"):
+            ls = np.array([r.t_done - r.t_arrival for r in done if cls_done[r.rid] == c])
+            fs = np.array([fc[k] for k in range(len(arr)) if picks[k][0] == c])
+            per_class[c] = {"submitted": int(sum(p[0] == c for p in picks)), "completed": int(len(ls)),
+                            "p50_s": float(np.percentile(ls, 50)) if len(ls) else None,
+                            "p99_s": float(np.percentile(ls, 99)) if len(ls) else None,
+                            "fcfs_p50_s": float(np.percentile(fs, 50)) if len(fs) else None,
+                            "fcfs_p99_s": float(np.percentile(fs, 99)) if len(fs) else None}
         span = max((r.t_done for r in done), default=t0) - t0
         pt = {"qps": qps, "submitted": int(len(arr)), "completed": len(done), "optimal": int(ok),
               "served_lps_per_s": len(done) / max(span, 1e-9),
@@ -280,7 +298,9 @@ def bench_serve(args):
               "fcfs_lps_per_s": len(arr) / float((arr + fc).max()) if len(arr) else None,
               "fcfs_p50_s": float(np.percentile(fc, 50)) if len(fc) else None,
               "fcfs_p99_s": float(np.percentile(fc, 99)) if len(fc) else None,
-              "wall_s": t_end - t0}
+              "wall_s": t_end - t0, "per_class": per_class,
+              "batch_rounds": srv.rounds, "round_s_median": float(np.median(rs)), "round_s_max": float(rs.max()),
+              "median_wait_to_start_s": waits, "median_start_to_done_s": runs}
         log(f"[bench] qps={qps}: {pt}")
         out_points.append(pt)
     best = max(out_points, key=lambda p: p["served_lps_per_s"])
@@ -344,7 +364,7 @@ def main():
     ap.add_argument("--qps", type=float, nargs="+", default=[100.0, 1000.0])
     ap.add_argument("--serve-duration", type=float, default=20.0)
     ap.add_argument("--serve-drain", type=float, default=60.0)
-    ap.add_argument("--serve-checks", type=int, default=2)
+    ap.add_argument("--serve-checks", type=int, default=20)
     ap.add_argument("--serve-batch-gib", type=int, default=16)
     ap.add_argument("--pool-small", type=int, default=48)
     ap.add_argument("--pool-medium", type=int, default=16)

@@ -1,10 +1,12 @@
 """Serving: continuous batching of LP solve requests on one GPU.
 
 Requests that fit the batch engine (m + n <= 65536) are admitted into its HBM arena as
-room frees up (footprint accounting, cpdlp_batch_footprint) and advanced in short
-rounds of PDHG checks; every round, finished requests are harvested and their HBM
-released, and waiting requests are admitted -- so requests arriving while others are
-on the GPU join the running batch (their work overlaps on the device).  Larger
+room frees up (footprint accounting, cpdlp_batch_footprint) and advanced in rounds of
+PDHG checks; every round, finished requests are harvested and their HBM released, and
+waiting requests are admitted -- so requests arriving while others are on the GPU join
+the running batch (their work overlaps on the device).  Rounds are work-budgeted by the
+batch engine: the largest instance runs one check per round and smaller ones up to
+`checks_per_round`, so a small request is not held back by a large neighbour.  Larger
 requests run on the plain engine (one HIP stream each) in worker threads, under an
 HBM budget.  Iterates, scalings and restart state stay on the device between rounds.
 
@@ -40,6 +42,7 @@ class Request:
     t_arrival: float
     route: str = "batch"
     t_done: float = float("nan")
+    t_start: float = float("nan")      # admitted to the batch arena / taken by a plain worker
     status: str = ""
     objective: float = float("nan")
     kkt_passes: int = 0
@@ -52,15 +55,17 @@ class Server:
     precision: str = "mixed"
     batch_bytes: int = 16 << 30
     plain_bytes: int = 64 << 30
-    checks_per_round: int = 2
+    checks_per_round: int = 20         # checks of the smallest instances per round
     plain_workers: int = 2
+    batch_streams: int = 2             # + plain_workers streams: within 4 hardware queues
     batch_limit: int = 65536           # m + n limit of the batch engine
     time_limit: float = 600.0
     keep_solutions: bool = False
     done: list = field(default_factory=list)
 
     def __post_init__(self):
-        algo = PDLP(self.precision, termination_reltol=self.reltol, time_limit=self.time_limit)
+        algo = PDLP(self.precision, termination_reltol=self.reltol, time_limit=self.time_limit,
+                    batch_streams=self.batch_streams)
         self.batch = BatchSolver(algo, hbm_bytes=self.batch_bytes)
         self.waiting: deque = deque()
         self.in_batch: dict = {}
@@ -70,6 +75,8 @@ class Server:
         self.plain_used = 0
         self.lock = threading.Lock()
         self.next_id = 0
+        self.rounds = 0
+        self.round_seconds = []            # wall time of each batch round (run + harvest)
 
     def close(self):
         self.pool.shutdown(wait=True)
@@ -114,6 +121,10 @@ class Server:
             self.done.append(r)
 
     def _admit_plain(self):
+        # shortest job first (by nonzeros): a large request does not hold back the
+        # medium ones queued behind it
+        if len(self.plain_waiting) > 1:
+            self.plain_waiting = deque(sorted(self.plain_waiting, key=lambda q: q.lp.nnz))
         while self.plain_waiting and self.plain_running < self.plain_workers:
             r = self.plain_waiting[0]
             fp = plain_footprint(r.lp.m, r.lp.n, r.lp.nnz)
@@ -123,6 +134,7 @@ class Server:
                 self.plain_running += 1
                 self.plain_used += fp
             self.plain_waiting.popleft()
+            r.t_start = time.perf_counter()
             self.pool.submit(self._plain_job, r, fp)
 
     # -------------------------------------------------------------- batch route
@@ -134,6 +146,7 @@ class Server:
             except MemoryError:
                 return   # no room until a running request finishes
             self.waiting.popleft()
+            r.t_start = time.perf_counter()
             self.in_batch[bid] = r
 
     def pump(self):
@@ -141,8 +154,11 @@ class Server:
         self._admit_plain()
         self._admit_batch()
         if self.in_batch:
+            t0 = time.perf_counter()
             self.batch.run(self.checks_per_round)
             now = time.perf_counter()
+            self.rounds += 1
+            self.round_seconds.append(now - t0)
             for bid in self.batch.poll():
                 r = self.in_batch.pop(bid)
                 sol, st = self.batch.get(bid)

@@ -954,7 +954,7 @@ struct BHost {
 struct Batch {
     cpdlp_params P;
     int device = 0;
-    hipStream_t streams[4] = {};
+    std::vector<hipStream_t> streams;
     DevBuf<char> arena_mem;
     Arena arena;
     int32_t max_slots = 0;
@@ -978,6 +978,7 @@ struct Batch {
         if (P.check_every <= 0) throw Error(CPDLP_ERR_ARGUMENT, "check_every must be positive");
         device = P.device;
         HIP_CHECK(hipSetDevice(device));
+        streams.assign((size_t)std::min(8, std::max(1, P.batch_streams)), nullptr);
         for (auto& s : streams) HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         if (bytes <= 0) throw Error(CPDLP_ERR_ARGUMENT, "arena size must be positive");
         arena_mem.alloc((size_t)bytes);
@@ -1173,19 +1174,37 @@ struct Batch {
             for (int g = 0; g < 8; ++g) {
                 gs[g] = nullptr;
                 if (spans[g].second == 0) continue;
-                gs[g] = streams[si++ % 4];
+                gs[g] = streams[si++ % streams.size()];
                 HIP_CHECK(hipStreamWaitEvent(gs[g], ready, 0));
             }
-            // per check: the steps of every group, each followed by its check
+            // Work-budgeted round: an instance of work w (nonzeros + rows + columns, x2 in
+            // FP64) runs floor(budget / w) checks, at least 1 and at most `checks`; budget
+            // = the largest w, so the largest instance runs one check and small ones run
+            // up to `checks` in the same time.  Lists are sorted by decreasing w, so the
+            // instances of check k are a suffix of their group's list.
+            auto work = [&](int32_t slot, int ph) {
+                const BInst& I = h_inst[slot];
+                return (double)(I.nnz + I.m + I.n) * (ph ? 2.0 : 1.0);
+            };
+            double budget = 0.0;
+            for (int ph = 0; ph < 2; ++ph)
+                for (int c = 0; c < 4; ++c)
+                    if (!cls[ph][c].empty()) budget = std::max(budget, work(cls[ph][c].front(), ph));
+            int start[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             for (int chk = 0; chk < checks; ++chk)
                 for (int ph = 0; ph < 2; ++ph)
                     for (int c = 0; c < 4; ++c) {
                         const int g = ph * 4 + c;
                         const auto sp = spans[g];
                         if (sp.second == 0) continue;
+                        const std::vector<int32_t>& v = cls[ph][c];
+                        while (chk > 0 && start[g] < sp.second && work(v[start[g]], ph) * (chk + 1) > budget)
+                            ++start[g];
+                        const int cnt = sp.second - start[g];
+                        if (cnt <= 0) continue;
                         const hipStream_t s = gs[g];
-                        const int32_t* L = d_list.p + sp.first;
-                        const dim3 grid(sp.second), blk(BB);
+                        const int32_t* L = d_list.p + sp.first + start[g];
+                        const dim3 grid(cnt), blk(BB);
                         if (ph == 0) {
                             if (c < 3)
                                 hipLaunchKernelGGL((k_batch_steps<float, true>), grid, blk, lds_bytes[c], s, L,

@@ -1410,6 +1410,7 @@ int cpdlp_default_params(cpdlp_params* p, int32_t algorithm) {
         p->polish_reltol = 1e-5;
         p->use_graph = 1;
         p->long_row = 64;
+        p->batch_streams = 4;
     });
 }
 

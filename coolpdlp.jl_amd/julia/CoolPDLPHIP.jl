@@ -68,6 +68,7 @@ mutable struct CParams
     polish_reltol::Float64
     use_graph::Int32
     long_row::Int32
+    batch_streams::Int32
     CParams() = new()
 end
 

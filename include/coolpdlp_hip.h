@@ -98,6 +98,7 @@ typedef struct cpdlp_params {
     double polish_reltol;           /* MIXED: FP32 phase stops at max(this, reltol) */
     int32_t use_graph;              /* capture the check_every-step chunk in a hipGraph */
     int32_t long_row;               /* rows longer than this use a wavefront per row (64) */
+    int32_t batch_streams;          /* HIP streams of a batch (cpdlp_batch_*), 1..8 (4) */
 } cpdlp_params;
 
 /* KKTErrors, src/components/errors.jl:8-21 */
