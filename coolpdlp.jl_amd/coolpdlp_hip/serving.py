@@ -1,14 +1,16 @@
 """Serving: continuous batching of LP solve requests on one GPU.
 
-Requests that fit the batch engine (m + n <= 65536) are admitted into its HBM arena as
+Small requests (m + n <= batch_limit) are admitted into the batch engine's HBM arena as
 room frees up (footprint accounting, cpdlp_batch_footprint) and advanced in rounds of
 PDHG checks; every round, finished requests are harvested and their HBM released, and
 waiting requests are admitted -- so requests arriving while others are on the GPU join
 the running batch (their work overlaps on the device).  Rounds are work-budgeted by the
-batch engine: the largest instance runs one check per round and smaller ones up to
-`checks_per_round`, so a small request is not held back by a large neighbour.  Larger
-requests run on the plain engine (one HIP stream each) in worker threads, under an
-HBM budget.  Iterates, scalings and restart state stay on the device between rounds.
+batch engine: a large instance runs fewer checks per round than small ones, so a small
+request is not held back by a larger neighbour.  Larger requests run on the plain
+engine (one HIP stream each) in worker threads, in lanes by size with their own queues,
+under an HBM budget: one workgroup per LP is the right shape for many small LPs, while
+a medium or large LP solves much faster on the whole GPU.  Iterates, scalings and
+restart state stay on the device between rounds.
 
 The first-come-first-served baseline (`fcfs_latencies`) is the one-solve-at-a-time
 queue: request k starts when both it has arrived and request k-1 has finished, and
@@ -50,15 +52,32 @@ class Request:
 
 
 @dataclass
+class Lane:
+    """A plain-engine route: its own queue (shortest job first) and worker threads, for
+    requests of at most `max_nnz` nonzeros, so that a long solve in one lane never holds
+    back the requests of another."""
+    max_nnz: float
+    workers: int
+    waiting: deque = field(default_factory=deque)
+    running: int = 0
+    pool: ThreadPoolExecutor | None = None
+
+
+@dataclass
 class Server:
     reltol: float = 1e-6
     precision: str = "mixed"
     batch_bytes: int = 16 << 30
     plain_bytes: int = 64 << 30
     checks_per_round: int = 20         # checks of the smallest instances per round
-    plain_workers: int = 2
-    batch_streams: int = 2             # + plain_workers streams: within 4 hardware queues
-    batch_limit: int = 65536           # m + n limit of the batch engine
+    batch_streams: int = 2             # + one stream per plain worker: 4 hardware queues
+    batch_limit: int = 8192            # m + n limit of the batch route
+    # a request for a plain lane that already has `overflow_queue` x workers waiting goes
+    # to the batch engine instead when it fits there (m + n <= 65536): under load the
+    # batch's throughput beats the lane's queue
+    overflow_queue: int = 2
+    # plain lanes (max nonzeros, workers): medium requests, then everything larger
+    lanes: tuple = ((1_000_000, 1), (float("inf"), 1))
     time_limit: float = 600.0
     keep_solutions: bool = False
     done: list = field(default_factory=list)
@@ -69,9 +88,9 @@ class Server:
         self.batch = BatchSolver(algo, hbm_bytes=self.batch_bytes)
         self.waiting: deque = deque()
         self.in_batch: dict = {}
-        self.pool = ThreadPoolExecutor(self.plain_workers)
-        self.plain_waiting: deque = deque()
-        self.plain_running = 0
+        self.plain_lanes = [Lane(float(mx), int(w)) for mx, w in self.lanes]
+        for ln in self.plain_lanes:
+            ln.pool = ThreadPoolExecutor(ln.workers)
         self.plain_used = 0
         self.lock = threading.Lock()
         self.next_id = 0
@@ -79,7 +98,8 @@ class Server:
         self.round_seconds = []            # wall time of each batch round (run + harvest)
 
     def close(self):
-        self.pool.shutdown(wait=True)
+        for ln in self.plain_lanes:
+            ln.pool.shutdown(wait=True)
         self.batch.close()
 
     # -------------------------------------------------------------- submission
@@ -90,16 +110,21 @@ class Server:
         if lp.m + lp.n <= self.batch_limit:
             self.waiting.append(r)
         else:
-            r.route = "plain"
-            self.plain_waiting.append(r)
+            lane = next((ln for ln in self.plain_lanes if lp.nnz <= ln.max_nnz), self.plain_lanes[-1])
+            if lp.m + lp.n <= 65536 and len(lane.waiting) >= self.overflow_queue * lane.workers:
+                self.waiting.append(r)
+            else:
+                r.route = "plain"
+                lane.waiting.append(r)
         return r.rid
 
     def pending(self) -> int:
         with self.lock:
-            return len(self.waiting) + len(self.in_batch) + len(self.plain_waiting) + self.plain_running
+            return (len(self.waiting) + len(self.in_batch)
+                    + sum(len(ln.waiting) + ln.running for ln in self.plain_lanes))
 
     # -------------------------------------------------------------- plain route
-    def _plain_job(self, r: Request, fp: int):
+    def _plain_job(self, r: Request, fp: int, lane: Lane):
         try:
             milp = MILP.from_generated(r.lp)
             # fixed CSR format: a request's result must not depend on autotuner timing
@@ -116,26 +141,27 @@ class Server:
             r.status = f"ERROR: {e}"
         r.t_done = time.perf_counter()
         with self.lock:
-            self.plain_running -= 1
+            lane.running -= 1
             self.plain_used -= fp
             self.done.append(r)
 
     def _admit_plain(self):
-        # shortest job first (by nonzeros): a large request does not hold back the
-        # medium ones queued behind it
-        if len(self.plain_waiting) > 1:
-            self.plain_waiting = deque(sorted(self.plain_waiting, key=lambda q: q.lp.nnz))
-        while self.plain_waiting and self.plain_running < self.plain_workers:
-            r = self.plain_waiting[0]
-            fp = plain_footprint(r.lp.m, r.lp.n, r.lp.nnz)
-            with self.lock:
-                if self.plain_running > 0 and self.plain_used + fp > self.plain_bytes:
-                    return
-                self.plain_running += 1
-                self.plain_used += fp
-            self.plain_waiting.popleft()
-            r.t_start = time.perf_counter()
-            self.pool.submit(self._plain_job, r, fp)
+        for lane in self.plain_lanes:
+            # shortest job first (by nonzeros) within the lane
+            if len(lane.waiting) > 1:
+                lane.waiting = deque(sorted(lane.waiting, key=lambda q: q.lp.nnz))
+            while lane.waiting and lane.running < lane.workers:
+                r = lane.waiting[0]
+                fp = plain_footprint(r.lp.m, r.lp.n, r.lp.nnz)
+                with self.lock:
+                    busy = sum(ln.running for ln in self.plain_lanes)
+                    if busy > 0 and self.plain_used + fp > self.plain_bytes:
+                        break
+                    lane.running += 1
+                    self.plain_used += fp
+                lane.waiting.popleft()
+                r.t_start = time.perf_counter()
+                lane.pool.submit(self._plain_job, r, fp, lane)
 
     # -------------------------------------------------------------- batch route
     def _admit_batch(self):

@@ -1179,17 +1179,22 @@ struct Batch {
             }
             // Work-budgeted round: an instance of work w (nonzeros + rows + columns, x2 in
             // FP64) runs floor(budget / w) checks, at least 1 and at most `checks`; budget
-            // = the largest w, so the largest instance runs one check and small ones run
-            // up to `checks` in the same time.  Lists are sorted by decreasing w, so the
+            // = max(largest w, checks * smallest w): a lone large instance runs one check
+            // while small ones run up to `checks` in the same time, and instances of
+            // similar size all run `checks`.  Lists are sorted by decreasing w, so the
             // instances of check k are a suffix of their group's list.
             auto work = [&](int32_t slot, int ph) {
                 const BInst& I = h_inst[slot];
                 return (double)(I.nnz + I.m + I.n) * (ph ? 2.0 : 1.0);
             };
-            double budget = 0.0;
+            double wmax = 0.0, wmin = INFINITY;
             for (int ph = 0; ph < 2; ++ph)
                 for (int c = 0; c < 4; ++c)
-                    if (!cls[ph][c].empty()) budget = std::max(budget, work(cls[ph][c].front(), ph));
+                    if (!cls[ph][c].empty()) {
+                        wmax = std::max(wmax, work(cls[ph][c].front(), ph));
+                        wmin = std::min(wmin, work(cls[ph][c].back(), ph));
+                    }
+            const double budget = std::max(wmax, (double)checks * wmin);
             int start[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             for (int chk = 0; chk < checks; ++chk)
                 for (int ph = 0; ph < 2; ++ph)
