@@ -369,8 +369,8 @@ def main():
     ap.add_argument("--serve-drain", type=float, default=60.0)
     ap.add_argument("--serve-checks", type=int, default=20)
     ap.add_argument("--serve-batch-gib", type=int, default=16)
-    ap.add_argument("--serve-medium-workers", type=int, default=1)
-    ap.add_argument("--serve-overflow", type=int, default=2)
+    ap.add_argument("--serve-medium-workers", type=int, default=2)
+    ap.add_argument("--serve-overflow", type=int, default=0)
     ap.add_argument("--pool-small", type=int, default=48)
     ap.add_argument("--pool-medium", type=int, default=16)
     ap.add_argument("--pool-large", type=int, default=3)

@@ -70,14 +70,14 @@ class Server:
     batch_bytes: int = 16 << 30
     plain_bytes: int = 64 << 30
     checks_per_round: int = 20         # checks of the smallest instances per round
-    batch_streams: int = 2             # + one stream per plain worker: 4 hardware queues
+    batch_streams: int = 1             # + one stream per plain worker: 4 hardware queues
     batch_limit: int = 8192            # m + n limit of the batch route
-    # a request for a plain lane that already has `overflow_queue` x workers waiting goes
-    # to the batch engine instead when it fits there (m + n <= 65536): under load the
-    # batch's throughput beats the lane's queue
-    overflow_queue: int = 2
+    # when > 0: a request for a plain lane that already has `overflow_queue` x workers
+    # waiting goes to the batch engine instead if it fits there (m + n <= 65536).  Off by
+    # default: medium LPs in the batch lengthen the rounds of the small ones.
+    overflow_queue: int = 0
     # plain lanes (max nonzeros, workers): medium requests, then everything larger
-    lanes: tuple = ((1_000_000, 1), (float("inf"), 1))
+    lanes: tuple = ((1_000_000, 2), (float("inf"), 1))
     time_limit: float = 600.0
     keep_solutions: bool = False
     done: list = field(default_factory=list)
@@ -111,7 +111,8 @@ class Server:
             self.waiting.append(r)
         else:
             lane = next((ln for ln in self.plain_lanes if lp.nnz <= ln.max_nnz), self.plain_lanes[-1])
-            if lp.m + lp.n <= 65536 and len(lane.waiting) >= self.overflow_queue * lane.workers:
+            if (self.overflow_queue > 0 and lp.m + lp.n <= 65536
+                    and len(lane.waiting) >= self.overflow_queue * lane.workers):
                 self.waiting.append(r)
             else:
                 r.route = "plain"

@@ -36,6 +36,7 @@ namespace cpdlp {
 struct BInst {
     int32_t m, n, nnz, La, Lat, n_a_tiles, n_at_tiles, pad0;
     int32_t *a_rp, *a_col, *at_rp, *at_col, *perm;
+    int32_t* ecol;                        // column of each CSC entry (flat scaling passes)
     const TileMeta *a_tiles, *at_tiles;   // row tiles of A and At for the step kernel
     double *nz, *acsc, *atcsc, *a64, *at64;
     float *a32, *at32;
@@ -269,9 +270,44 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
         }
         __syncthreads();
     };
+    // Ruiz's infinity norms in one flat pass over the entries: exact maxima (order-free,
+    // so the same values as a sequential max) via atomicMax on the bit patterns of |a|,
+    // which order like the values for non-negative doubles; NaN entries are skipped as
+    // the sequential max skips them
+    auto inf_norms = [&]() {
+        unsigned long long* b1 = (unsigned long long*)d1;
+        unsigned long long* b2 = (unsigned long long*)d2;
+        for (int i = tid; i < m; i += BB) b1[i] = 0ull;
+        for (int j = tid; j < n; j += BB) b2[j] = 0ull;
+        __syncthreads();
+        for (int k = tid; k < nnz; k += BB) {
+            const double a = fabs(I.acsc[k]), at = fabs(I.atcsc[k]);
+            if (a == a) atomicMax(&b2[I.ecol[k]], (unsigned long long)__double_as_longlong(a));
+            if (at == at) atomicMax(&b1[I.at_col[k]], (unsigned long long)__double_as_longlong(at));
+        }
+        __syncthreads();
+        for (int i = tid; i < m; i += BB) {
+            const double v = __longlong_as_double((long long)b1[i]);
+            d1[i] = v == 0.0 ? 1.0 : 1.0 / sqrt(v);
+        }
+        for (int j = tid; j < n; j += BB) {
+            const double v = __longlong_as_double((long long)b2[j]);
+            d2[j] = v == 0.0 ? 1.0 : 1.0 / sqrt(v);
+        }
+        __syncthreads();
+    };
+    // (d1_i * a) * d2_j into A, (d2_j * a') * d1_i into A', entry by entry
+    auto apply_flat = [&](const double* D1v, const double* D2v) {
+        for (int k = tid; k < nnz; k += BB) {
+            const double di = D1v[I.at_col[k]], dj = D2v[I.ecol[k]];
+            I.acsc[k] = (di * I.acsc[k]) * dj;
+            I.atcsc[k] = (dj * I.atcsc[k]) * di;
+        }
+        __syncthreads();
+    };
     for (int it = 0; it < P.ruiz_iter; ++it) {   // ruiz_preconditioner, :117-125
-        norms(INFINITY, INFINITY);
-        apply(I.acsc, I.atcsc, d1, d2);
+        inf_norms();
+        apply_flat(d1, d2);
         for (int i = tid; i < m; i += BB) I.D1[i] = d1[i] * I.D1[i];
         for (int j = tid; j < n; j += BB) I.D2[j] = I.D2[j] * d2[j];
         __syncthreads();
@@ -805,6 +841,7 @@ static size_t footprint(int64_t m, int64_t n, int64_t nnz) {
     size_t b = 0;
     auto add = [&](int64_t count, size_t sz) { b += align256((size_t)std::max<int64_t>(count, 1) * sz); };
     add(m + 1, 4); add(nnz, 4); add(n + 1, 4); add(nnz, 4); add(nnz, 4);       // ints
+    add(nnz, 4);                                                                // ecol
     add(max_tiles(m, nnz), sizeof(TileMeta)); add(max_tiles(n, nnz), sizeof(TileMeta));
     for (int k = 0; k < 5; ++k) add(nnz, 8);                                    // nz, acsc, atcsc, a64, at64
     add(nnz, 4); add(nnz, 4);                                                   // a32, at32
@@ -838,6 +875,7 @@ static BInst carve(char* base, int64_t m, int64_t n, int64_t nnz) {
     I.at_rp = (int32_t*)take(n + 1, 4);
     I.at_col = (int32_t*)take(nnz, 4);
     I.perm = (int32_t*)take(nnz, 4);
+    I.ecol = (int32_t*)take(nnz, 4);
     I.a_tiles = (const TileMeta*)take(max_tiles(m, nnz), sizeof(TileMeta));
     I.at_tiles = (const TileMeta*)take(max_tiles(n, nnz), sizeof(TileMeta));
     I.nz = (double*)take(nnz, 8);
@@ -1004,6 +1042,8 @@ struct Batch {
     ~Batch() {
         (void)hipSetDevice(device);
         for (auto& s : streams)
+            if (s) (void)hipStreamSynchronize(s);   // buffers go back to the block cache
+        for (auto& s : streams)
             if (s) (void)hipStreamDestroy(s);
     }
 
@@ -1064,6 +1104,10 @@ struct Batch {
         up(I.at_rp, cp.data(), (n + 1) * 4);
         up(I.at_col, rv.data(), nnz * 4);
         up(I.perm, perm.data(), nnz * 4);
+        std::vector<int32_t> ecol(nnz);
+        for (int64_t j = 0; j < n; ++j)
+            for (int32_t k = cp[j]; k < cp[j + 1]; ++k) ecol[k] = (int32_t)j;
+        up(I.ecol, ecol.data(), nnz * 4);
         up((void*)I.a_tiles, ta.data(), ta.size() * sizeof(TileMeta));
         up((void*)I.at_tiles, tat.data(), tat.size() * sizeof(TileMeta));
         up(I.nz, lp.nzval, nnz * 8);

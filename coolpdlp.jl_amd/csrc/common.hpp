@@ -8,6 +8,9 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/coolpdlp_hip.h"
@@ -51,6 +54,80 @@ int guarded(F&& f) {
 }
 
 // ---------------------------------------------------------------- device buffers
+// Process-wide cache of device blocks.  hipFree synchronises the whole device, which
+// would stall every other stream (a served request's teardown would wait for all
+// running solves), so released blocks go to a per-device free list and are handed out
+// again; hipFree runs only to make room when hipMalloc fails.  Sizes are rounded to
+// powers of two below 1 MiB and to 2 MiB multiples above.  Every buffer here is
+// released only after the work that uses it has completed (its stream was
+// synchronised), so a reused block is never still in use.
+struct BlockCache {
+    std::mutex mu;
+    std::map<int, std::multimap<size_t, void*>> free_;   // device -> size -> block
+    std::unordered_map<void*, std::pair<int, size_t>> live;   // block -> (device, size)
+};
+inline BlockCache& block_cache() {
+    static BlockCache* c = new BlockCache();   // never destroyed: used until exit
+    return *c;
+}
+inline size_t block_round(size_t b) {
+    if (b <= 256) return 256;
+    if (b < (size_t(1) << 20)) {
+        size_t r = 256;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    const size_t q = size_t(2) << 20;
+    return (b + q - 1) / q * q;
+}
+inline void block_flush(int dev) {   // caller holds the lock
+    BlockCache& c = block_cache();
+    auto& fl = c.free_[dev];
+    for (auto& kv : fl) (void)hipFree(kv.second);
+    fl.clear();
+}
+inline void* dev_alloc(size_t bytes) {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    const size_t sz = block_round(bytes);
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    auto& fl = c.free_[dev];
+    auto it = fl.lower_bound(sz);
+    if (it != fl.end() && it->first <= 2 * sz) {
+        void* p = it->second;
+        c.live[p] = {dev, it->first};
+        fl.erase(it);
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        block_flush(dev);
+        e = hipMalloc(&p, sz);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(CPDLP_ERR_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                          hipGetErrorString(e));
+    }
+    c.live[p] = {dev, sz};
+    return p;
+}
+inline void dev_free(void* p) {
+    if (!p) return;
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.live.find(p);
+    if (it == c.live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    c.free_[it->second.first].emplace(it->second.second, p);
+    c.live.erase(it);
+}
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -67,18 +144,12 @@ struct DevBuf {
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
-        n = count;
         if (count == 0) return;
-        hipError_t e = hipMalloc(&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            n = 0;
-            throw Error(CPDLP_ERR_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) +
-                                              " bytes failed: " + hipGetErrorString(e));
-        }
+        p = (T*)dev_alloc(count * sizeof(T));
+        n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        dev_free(p);
         p = nullptr;
         n = 0;
     }

@@ -36,6 +36,9 @@ struct cpdlp_matrix {
 
     ~cpdlp_matrix() {
         (void)hipSetDevice(device);
+        // products may still be queued on the caller's streams: wait for the device
+        // before the buffers go back to the block cache
+        (void)hipDeviceSynchronize();
         if (stream) (void)hipStreamDestroy(stream);
     }
 

@@ -503,6 +503,7 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
 
 Handle::~Handle() {
     (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);   // buffers go back to the block cache
     for (auto& g : graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (ev0) (void)hipEventDestroy(ev0);
