@@ -396,14 +396,15 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
         tiled_pass<double>(I.a_tiles, I.n_a_tiles, I.a_rp, I.a_col, I.a64, prod, red,
                            [&](int c) { return x[c]; }, [&](int) { return 0; },
                            [&](int r, double acc, int) { t[r] = acc; });
-        tiled_pass<double>(I.at_tiles, I.n_at_tiles, I.at_rp, I.at_col, I.at64, prod, red,
-                           [&](int c) { return t[c]; }, [&](int) { return 0; },
-                           [&](int r, double acc, int) { Ax[r] = acc; });
+        // Ax = A't, with x.Ax and Ax.Ax accumulated by the row's owner as it is formed
         double q[2] = {0.0, 0.0};
-        for (int j = tid; j < n; j += BB) {
-            q[0] += x[j] * Ax[j];
-            q[1] += Ax[j] * Ax[j];
-        }
+        tiled_pass<double>(I.at_tiles, I.n_at_tiles, I.at_rp, I.at_col, I.at64, prod, red,
+                           [&](int c) { return t[c]; }, [&](int j) { return x[j]; },
+                           [&](int r, double acc, double xr) {
+                               Ax[r] = acc;
+                               q[0] += xr * acc;
+                               q[1] += acc * acc;
+                           });
         bsum<2>(q, sh);
         theta = sh[0];
         const double inv = 1.0 / sqrt(sh[1]);
@@ -1223,10 +1224,11 @@ struct Batch {
             }
             // Work-budgeted round: an instance of work w (nonzeros + rows + columns, x2 in
             // FP64) runs floor(budget / w) checks, at least 1 and at most `checks`; budget
-            // = max(largest w, checks * smallest w): a lone large instance runs one check
-            // while small ones run up to `checks` in the same time, and instances of
-            // similar size all run `checks`.  Lists are sorted by decreasing w, so the
-            // instances of check k are a suffix of their group's list.
+            // = max(largest w * max(1, checks / 4), checks * smallest w): the largest
+            // instance runs a quarter of `checks` while the smaller ones run up to
+            // `checks` in the same time, and instances of similar size all run `checks`.
+            // Lists are sorted by decreasing w, so the instances of check k are a suffix
+            // of their group's list.
             auto work = [&](int32_t slot, int ph) {
                 const BInst& I = h_inst[slot];
                 return (double)(I.nnz + I.m + I.n) * (ph ? 2.0 : 1.0);
@@ -1238,7 +1240,7 @@ struct Batch {
                         wmax = std::max(wmax, work(cls[ph][c].front(), ph));
                         wmin = std::min(wmin, work(cls[ph][c].back(), ph));
                     }
-            const double budget = std::max(wmax, (double)checks * wmin);
+            const double budget = std::max(wmax * std::max(1, checks / 4), (double)checks * wmin);
             int start[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             for (int chk = 0; chk < checks; ++chk)
                 for (int ph = 0; ph < 2; ++ph)

@@ -8,7 +8,7 @@
  *
  * The reference (CoolPDLP.jl) is pure Julia and has no FFI of its own.  Each entry
  * point below names the reference function or plug-in method it replaces; the
- * Julia-side ccall binding a maintainer would add is in INTEGRATION.md and
+ * Julia-side ccall binding a maintainer would add is
  * coolpdlp.jl_amd/julia/CoolPDLPHIP.jl.  Paths are relative to the reference repo.
  *
  * Threading: one handle owns one HIP stream and its device memory; calls on
