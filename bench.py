@@ -70,9 +70,9 @@ def make_workload(name):
         return g, 1e-6, desc
     if name == "plain":
         g = G.small_dense_lp()
-        desc = dict(workload="plain_lp_m2000_n5000", m=g.m, n=g.n, nnz=g.nnz, reltol=1e-6,
+        desc = dict(workload="plain_lp_m2000_n5000", m=g.m, n=g.n, nnz=g.nnz, reltol=1e-4,
                     algorithm="PDLP", precision="fp64")
-        return g, 1e-6, desc
+        return g, 1e-4, desc
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -346,6 +346,17 @@ def cpu_baseline(g, gpu_passes, reltol, budget_passes):
     loop = max(wall - r.stats.setup_seconds, 1e-9)
     per_pass = loop / max(r.stats.kkt_passes, 1)
     projected = r.stats.setup_seconds + per_pass * gpu_passes
+    if projected <= 30.0:   # small enough: time the whole CPU solve instead
+        p = O.default_params(termination_reltol=reltol, time_limit=1e9)
+        t = time.perf_counter()
+        r = O.solve(g, p, hist_cap=16)
+        wall = time.perf_counter() - t
+        return {
+            "value": 1.0 / wall, "unit": "LPs/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle FP64 PDLP, the whole solve: {r.status} after {r.stats.kkt_passes} "
+                       f"passes in {wall:.2f}s"),
+            "status_at_sample": r.status,
+        }
     return {
         "value": 1.0 / projected, "unit": "LPs/s", "cores": threads, "kind": "port",
         "sample": (f"oracle FP64 PDLP, {r.stats.kkt_passes} passes incl. their checks "
