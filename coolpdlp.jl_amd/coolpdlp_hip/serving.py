@@ -107,16 +107,12 @@ class Server:
         t = time.perf_counter() if t_arrival is None else t_arrival
         r = Request(self.next_id, lp, t)
         self.next_id += 1
-        if lp.m + lp.n <= self.batch_limit:
+        k = route(lp.m, lp.n, lp.nnz, self.batch_limit, self.plain_lanes, self.overflow_queue)
+        if k < 0:
             self.waiting.append(r)
         else:
-            lane = next((ln for ln in self.plain_lanes if lp.nnz <= ln.max_nnz), self.plain_lanes[-1])
-            if (self.overflow_queue > 0 and lp.m + lp.n <= 65536
-                    and len(lane.waiting) >= self.overflow_queue * lane.workers):
-                self.waiting.append(r)
-            else:
-                r.route = "plain"
-                lane.waiting.append(r)
+            r.route = "plain"
+            self.plain_lanes[k].waiting.append(r)
         return r.rid
 
     def pending(self) -> int:
@@ -204,6 +200,20 @@ class Server:
     def drain(self, deadline: float):
         while self.pending() > 0 and time.perf_counter() < deadline:
             self.pump()
+
+
+def route(m: int, n: int, nnz: int, batch_limit: int, lanes, overflow_queue: int = 0) -> int:
+    """Where a request goes: -1 for the batch engine, else the index of its plain lane
+    (the first whose max_nnz admits it, the last lane otherwise).  With overflow_queue > 0
+    a request whose lane already has overflow_queue x workers waiting goes to the batch
+    engine instead when it fits there (m + n <= 65536)."""
+    if m + n <= batch_limit:
+        return -1
+    k = next((i for i, ln in enumerate(lanes) if nnz <= ln.max_nnz), len(lanes) - 1)
+    ln = lanes[k]
+    if overflow_queue > 0 and m + n <= 65536 and len(ln.waiting) >= overflow_queue * ln.workers:
+        return -1
+    return k
 
 
 def fcfs_latencies(arrivals: np.ndarray, service: np.ndarray) -> np.ndarray:

@@ -41,3 +41,17 @@ def test_min_cost_flow_small_is_balanced_and_feasible():
     assert np.allclose(np.asarray(A.sum(axis=0)).ravel(), 0)   # +1 / -1 per arc
     h = linprog(g.c, A_eq=A, b_eq=g.lc, bounds=list(zip(g.lv, g.uv)), method="highs")
     assert h.status == 0
+
+
+def test_serving_routes_by_size():
+    from collections import deque
+    from coolpdlp_hip.serving import Lane, route
+    lanes = [Lane(1_000_000, 2), Lane(float("inf"), 1)]
+    assert route(1000, 2000, 16_000, 8192, lanes) == -1            # small: batch engine
+    assert route(3000, 6000, 48_000, 8192, lanes) == 0             # medium: first lane
+    assert route(200_000, 400_000, 3_200_000, 8192, lanes) == 1    # large: last lane
+    # overflow: only when asked for, only when the lane is backed up, only if it fits
+    lanes[0].waiting = deque(range(4))
+    assert route(3000, 6000, 48_000, 8192, lanes) == 0
+    assert route(3000, 6000, 48_000, 8192, lanes, overflow_queue=2) == -1
+    assert route(30_000, 60_000, 480_000, 8192, lanes, overflow_queue=2) == 0
