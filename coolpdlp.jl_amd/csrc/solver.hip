@@ -328,7 +328,7 @@ void DevSide::build_slices(const HostSide& S, hipStream_t s) {
     sl_vmap.upload(H.vmap.data(), H.vmap.size(), s);
     sl_tiles.upload(H.tiles, s);
     sl_tile_off.upload(H.tile_off, s);
-    sl_part.alloc(std::max<int64_t>((int64_t)NS * S.n_long, 1));
+    sl_part.alloc(std::max<int64_t>((int64_t)SL_KMAX * NS * S.n_long, 1));
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -624,7 +624,7 @@ void Handle::power_method(double tol, int64_t maxiter) {
     st.upload(&h0, 1, s);
     const int gn = grid_n();
     const int nbr = std::min(gn, 1024);
-    const int gA = A.view<V>(false).grid(), gAt = At.view<V>(false).grid();
+    const int gA = A.view<V>().partial_blocks(), gAt = At.view<V>().partial_blocks();
     ensure_partials((size_t)std::max({gA, gAt, nbr}) * 2 + 16);
     hipLaunchKernelGGL(k_powm_start<V>, dim3(gn), dim3(BLOCK), 0, s, nn, d_col_of_int.p,
                        P.powm_seed, x.p);
@@ -633,7 +633,7 @@ void Handle::power_method(double tol, int64_t maxiter) {
                        d_sums.p, (const PowState*)nullptr);
     hipLaunchKernelGGL(k_scale_by_inv_sqrt<V>, dim3(gn), dim3(BLOCK), 0, s, nn, x.p,
                        (const double*)d_sums.p);
-    const MatView<V> vA = A.view<V>(false), vAt = At.view<V>(false);
+    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
     PowState hs{};
     for (;;) {
         for (int c = 0; c < 32; ++c) {
@@ -742,8 +742,8 @@ template <class V, int K>
 void Handle::kkt_launch(const V* const* X, const V* const* Y, const V* xr, const V* yr, bool dist,
                         double* host_sums) {
     const hipStream_t s = stream;
-    const MatView<V> vA = A.view<V>(false), vAt = At.view<V>(false);
-    const int gA = vA.grid(), gAt = vAt.grid();
+    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
+    const int gA = vA.partial_blocks(), gAt = vAt.partial_blocks();
     constexpr int NQR = EpiKktRows<V, K>::NQ, NQC = EpiKktCols<V, K>::NQ;
     ensure_partials((size_t)std::max(gA * NQR, gAt * NQC) + 16);
     EpiKktRows<V, K> er;

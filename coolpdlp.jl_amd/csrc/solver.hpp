@@ -37,8 +37,7 @@ struct DevSide {
     const V* ell_vals() const;
     template <class V>
     const V* sl_vals() const;
-    // allow_slices = false: the KKT / power-method products (K > 1 or block-partial
-    // reductions) use the unsliced CSR instead of the XCD slices
+    // allow_slices = false: the plain CSR copy instead of the XCD slices
     template <class V>
     MatView<V> view(bool allow_slices = true) const;
     void upload_structure(const HostSide& S, hipStream_t s);

@@ -72,7 +72,15 @@ struct MatView {
         if (format == CPDLP_FMT_XSLICE) return ns * sl_max_tiles + n_tiles;
         return n_long_blocks + n_tiles;
     }
+    // blocks that write an epilogue's block partials (NQ per block): the sliced format's
+    // short tiles, then its combine blocks
+    int partial_blocks() const {
+        if (format == CPDLP_FMT_XSLICE) return n_tiles + ceil_div(n_long, BLOCK);
+        return grid();
+    }
 };
+
+constexpr int SL_KMAX = 5;   // most vectors per product in the sliced format
 
 template <class V, int K>
 struct XPack {
@@ -190,30 +198,36 @@ __device__ __forceinline__ Acc group_sum(Acc v) {
     return v;
 }
 
-// XCD-sliced product, pass 1 (K = 1, no block partials).  Blocks [0, ns * sl_max_tiles):
-// block b sums slice s = b % ns of the long rows of tile b / ns of that slice into
-// sl_part[s * n_long + r]: the tile's entries are loaded coalesced, their products
-// formed in LDS, then every row part is summed by a group of 16 lanes (lane l takes
-// entries l, l+16, ... in order, then a butterfly); a row part longer than TILE_NNZ
-// is summed by the whole block (strided, then the four wave sums in wave order).
-// Blocks past those: the side's short tiles, summed and finished by the epilogue
-// exactly as in k_spmv_csr.  Deterministic; the long rows' order of summation differs
-// from a sequential sum (as the wavefront path's does).
-template <class V, class Acc, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, 1> X, Epi epi) {
+// XCD-sliced product, pass 1.  Blocks [0, ns * sl_max_tiles): block b sums slice
+// s = b % ns of the long rows of tile b / ns of that slice, for each of the K vectors,
+// into sl_part[(kv * ns + s) * n_long + r]: the tile's entries are loaded coalesced
+// once, the products of vector kv formed in LDS, and every row part summed by a group of
+// 16 lanes (lane l takes entries l, l+16, ... in order, then a butterfly); a row part
+// longer than TILE_NNZ is summed by the whole block (strided, then the four wave sums
+// in wave order).  Blocks past those: the side's short tiles, summed and finished by the
+// epilogue exactly as in k_spmv_csr, block partials at index b - ns * sl_max_tiles.
+// Deterministic; the long rows' order of summation differs from a sequential sum (as
+// the wavefront path's does).
+template <class V, class Acc, int K, class Epi>
+__global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K> X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
-    __shared__ Acc red[BLOCK / WAVE];
+    __shared__ Acc red[K][BLOCK / WAVE];
     if (epi.skip()) return;
     const int nsb = A.ns * A.sl_max_tiles;
+    const int tid = threadIdx.x;
     if ((int)blockIdx.x >= nsb) {
+        double q[Epi::NQ > 0 ? Epi::NQ : 1];
+#pragma unroll
+        for (int i = 0; i < (Epi::NQ > 0 ? Epi::NQ : 1); ++i) q[i] = 0.0;
         const TileMeta tm = A.tiles[blockIdx.x - nsb];
-        const int r = tm.r0 + (int)threadIdx.x;
+        const int r = tm.r0 + tid;
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);
-        Acc acc[1];
-        tile_sums<V, Acc, 1>(A.rowptr, A.col, A.val, tm, X, prod, acc);
-        double q[1] = {0.0};
+        Acc acc[K];
+        tile_sums<V, Acc, K>(A.rowptr, A.col, A.val, tm, X, prod, acc);
         if (r < tm.r1) epi.row(r, acc, q, pre);
+        if constexpr (Epi::NQ > 0)
+            block_sum<Epi::NQ>(q, epi.partials + (size_t)(blockIdx.x - nsb) * Epi::NQ);
         return;
     }
     const int s = blockIdx.x % A.ns;
@@ -221,15 +235,17 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, 1>
     if (t >= A.sl_tile_off[s + 1]) return;   // uniform per block: this slice has fewer tiles
     const TileMeta tm = A.sl_tiles[t];
     const int32_t* rp = A.sl_rowptr + (size_t)s * (A.n_long + 1);
-    Acc* part = (Acc*)A.sl_part + (size_t)s * A.n_long;
-    const int tid = threadIdx.x;
+    Acc* part = (Acc*)A.sl_part;
+    auto part_at = [&](int kv, int r) -> Acc& { return part[((size_t)kv * A.ns + s) * A.n_long + r]; };
     if (tm.cnt > TILE_NNZ) {   // one long row part, the whole block
         constexpr int U = 4;
-        Acc acc = Acc(0);
+        Acc acc[K];
+#pragma unroll
+        for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
         const int k1 = tm.base + tm.cnt;
         for (int kb = tm.base + tid; kb < k1; kb += U * BLOCK) {
             int c[U];
-            Acc v[U], xv[U];
+            Acc v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = min(kb + u * BLOCK, k1 - 1);
@@ -237,25 +253,44 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, 1>
                 v[u] = (Acc)__builtin_nontemporal_load(A.sl_val + k);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[0][c[u]];
+            for (int kv = 0; kv < K; ++kv) {
+                Acc xv[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (kb + u * BLOCK < k1) acc += v[u] * xv[u];
+                for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[kv][c[u]];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (kb + u * BLOCK < k1) acc[kv] += v[u] * xv[u];
+            }
         }
-        acc = wave_sum(acc);
-        if ((tid & (WAVE - 1)) == 0) red[tid / WAVE] = acc;
-        __syncthreads();
-        if (tid == 0) {
-            Acc sum = red[0];
 #pragma unroll
-            for (int w = 1; w < BLOCK / WAVE; ++w) sum += red[w];
-            part[tm.r0] = sum;
+        for (int kv = 0; kv < K; ++kv) {
+            const Acc w = wave_sum(acc[kv]);
+            if ((tid & (WAVE - 1)) == 0) red[kv][tid / WAVE] = w;
+        }
+        __syncthreads();
+        if (tid < K) {
+            Acc sum = red[tid][0];
+#pragma unroll
+            for (int w = 1; w < BLOCK / WAVE; ++w) sum += red[tid][w];
+            part_at(tid, tm.r0) = sum;
         }
         return;
     }
-    if (tm.cnt > 0) {   // uniform per block
-        int cc[PER_T];
-        V vv[PER_T];
+    constexpr int SEG = 16;
+    const int g = tid / SEG, l = tid % SEG;
+    // every lane of a group runs the same trips, so the butterfly's partners are active
+    auto row_parts = [&](int kv) {
+        for (int r = tm.r0 + g; r < tm.r1; r += BLOCK / SEG) {
+            const int lo = rp[r] - tm.base, hi = rp[r + 1] - tm.base;
+            Acc sum = Acc(0);
+            for (int e = lo + l; e < hi; e += SEG) sum += prod[e];
+            sum = group_sum<SEG>(sum);
+            if (l == 0) part_at(kv, r) = sum;
+        }
+    };
+    int cc[PER_T];
+    V vv[PER_T];
+    if (tm.cnt > 0) {   // uniform per block; loads and the first vector's gathers together
 #pragma unroll
         for (int j = 0; j < PER_T; ++j) {
             const int e = min(tid + j * BLOCK, tm.cnt - 1);
@@ -269,31 +304,45 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, 1>
         for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
     }
     __syncthreads();
-    constexpr int SEG = 16;
-    const int g = tid / SEG, l = tid % SEG;
-    // every lane of a group runs the same trips, so the butterfly's partners are active
-    for (int r = tm.r0 + g; r < tm.r1; r += BLOCK / SEG) {
-        const int lo = rp[r] - tm.base, hi = rp[r + 1] - tm.base;
-        Acc sum = Acc(0);
-        for (int e = lo + l; e < hi; e += SEG) sum += prod[e];
-        sum = group_sum<SEG>(sum);
-        if (l == 0) part[r] = sum;
+    row_parts(0);
+#pragma unroll
+    for (int kv = 1; kv < K; ++kv) {   // prod is reused by the next vector
+        __syncthreads();
+        if (tm.cnt > 0) {
+            Acc xv[PER_T];
+#pragma unroll
+            for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
+#pragma unroll
+            for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
+        }
+        __syncthreads();
+        row_parts(kv);
     }
 }
 
 // XCD-sliced product, pass 2: long row r's sum = its slice sums added in slice order,
-// then the epilogue (thread per row).
-template <class V, class Acc, class Epi>
+// then the epilogue (thread per row); block partials at index n_tiles + block.
+template <class V, class Acc, int K, class Epi>
 __global__ __launch_bounds__(BLOCK) void k_xslice_combine(MatView<V> A, Epi epi) {
     if (epi.skip()) return;
+    double q[Epi::NQ > 0 ? Epi::NQ : 1];
+#pragma unroll
+    for (int i = 0; i < (Epi::NQ > 0 ? Epi::NQ : 1); ++i) q[i] = 0.0;
     const int r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r >= A.n_long) return;
-    const typename Epi::Pre pre = epi.load(r);
-    const Acc* part = (const Acc*)A.sl_part;
-    Acc acc[1] = {part[r]};
-    for (int s = 1; s < A.ns; ++s) acc[0] += part[(size_t)s * A.n_long + r];
-    double q[1] = {0.0};
-    epi.row(r, acc, q, pre);
+    if (r < A.n_long) {
+        const typename Epi::Pre pre = epi.load(r);
+        const Acc* part = (const Acc*)A.sl_part;
+        Acc acc[K];
+#pragma unroll
+        for (int kv = 0; kv < K; ++kv) {
+            Acc sum = part[((size_t)kv * A.ns) * A.n_long + r];
+            for (int s = 1; s < A.ns; ++s) sum += part[((size_t)kv * A.ns + s) * A.n_long + r];
+            acc[kv] = sum;
+        }
+        epi.row(r, acc, q, pre);
+    }
+    if constexpr (Epi::NQ > 0)
+        block_sum<Epi::NQ>(q, epi.partials + (size_t)(A.n_tiles + blockIdx.x) * Epi::NQ);
 }
 
 // Column-major ELL: thread per row; entry s of rows i..i+63 is contiguous, so each
@@ -342,11 +391,11 @@ template <class V, class Acc, int K, class Epi>
 inline void launch_spmv(const MatView<V>& A, const XPack<V, K>& X, const Epi& epi, hipStream_t s) {
     const int g = A.grid();
     if (g <= 0) return;
-    if constexpr (K == 1 && Epi::NQ == 0) {
+    if constexpr (K <= SL_KMAX) {
         if (A.format == CPDLP_FMT_XSLICE) {
-            hipLaunchKernelGGL((k_spmv_xslice<V, Acc, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
+            hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
             if (A.n_long > 0)
-                hipLaunchKernelGGL((k_xslice_combine<V, Acc, Epi>), dim3(ceil_div(A.n_long, BLOCK)),
+                hipLaunchKernelGGL((k_xslice_combine<V, Acc, K, Epi>), dim3(ceil_div(A.n_long, BLOCK)),
                                    dim3(BLOCK), 0, s, A, epi);
             return;
         }
