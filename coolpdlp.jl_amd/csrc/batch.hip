@@ -986,6 +986,7 @@ struct BHost {
     int32_t slot, m, n, nnz;
     size_t off, bytes;
     bool finished = false, polled = false;
+    bool ready = false;   // its setup kernel has completed and its state is on the host
     double t_start = 0.0, t_done = 0.0;
     int32_t status = CPDLP_STILL_RUNNING;
 };
@@ -999,13 +1000,20 @@ struct Batch {
     int32_t max_slots = 0;
     DevBuf<BInst> d_inst;
     DevBuf<BState> d_state;
-    DevBuf<int32_t> d_timeout, d_list;
+    DevBuf<int32_t> d_timeout, d_list, d_setup_list;
+    // setup runs on its own stream, asynchronously: rounds go on for the instances that
+    // are ready while new ones are set up
+    hipStream_t setup_stream = nullptr;
+    hipEvent_t setup_done = nullptr;
+    bool setup_in_flight = false;
+    std::vector<int64_t> setup_ids;
     std::vector<BInst> h_inst;
     std::vector<BState> h_state;
     std::vector<int32_t> h_timeout, free_slots;
     std::map<int64_t, BHost> inst;   // by id
     std::deque<int64_t> done;
     std::vector<int32_t> pending_setup;
+    std::vector<int64_t> pending_ids;
     int64_t next_id = 0;
     int32_t hi_slot = 0;   // slots [0, hi_slot) may be in use
     size_t lds_max = 65536;  // dynamic LDS a workgroup may take (device limit minus static use)
@@ -1019,6 +1027,8 @@ struct Batch {
         HIP_CHECK(hipSetDevice(device));
         streams.assign((size_t)std::min(8, std::max(1, P.batch_streams)), nullptr);
         for (auto& s : streams) HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&setup_stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&setup_done, hipEventDisableTiming));
         if (bytes <= 0) throw Error(CPDLP_ERR_ARGUMENT, "arena size must be positive");
         arena_mem.alloc((size_t)bytes);
         arena.init((size_t)bytes);
@@ -1035,6 +1045,7 @@ struct Batch {
         d_state.alloc(max_slots);
         d_timeout.alloc(max_slots);
         d_list.alloc(max_slots);
+        d_setup_list.alloc(max_slots);
         h_inst.resize(max_slots);
         h_state.resize(max_slots);
         h_timeout.assign(max_slots, 0);
@@ -1044,8 +1055,11 @@ struct Batch {
         (void)hipSetDevice(device);
         for (auto& s : streams)
             if (s) (void)hipStreamSynchronize(s);   // buffers go back to the block cache
+        if (setup_stream) (void)hipStreamSynchronize(setup_stream);
         for (auto& s : streams)
             if (s) (void)hipStreamDestroy(s);
+        if (setup_stream) (void)hipStreamDestroy(setup_stream);
+        if (setup_done) (void)hipEventDestroy(setup_done);
     }
 
     BParams params() const {
@@ -1141,6 +1155,7 @@ struct Batch {
         h.t_start = bnow();
         inst[h.id] = h;
         pending_setup.push_back(slot);
+        pending_ids.push_back(h.id);
         return h.id;
     }
 
@@ -1153,16 +1168,44 @@ struct Batch {
         if (checks <= 0) throw Error(CPDLP_ERR_ARGUMENT, "max_checks must be positive");
         const BParams bp = params();
         std::vector<int32_t> lists;
-        // 1. setup of newly admitted instances
-        if (!pending_setup.empty()) {
-            lists = pending_setup;
-            HIP_CHECK(hipMemcpyAsync(d_list.p, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, streams[0]));
-            hipLaunchKernelGGL(k_batch_setup, dim3((int)lists.size()), dim3(BB), 0, streams[0], d_list.p,
-                               d_inst.p, d_state.p, bp);
+        // 1. setup of newly admitted instances, on the setup stream; a round never waits
+        //    for it unless nothing else is ready to run
+        bool newly_ready = false;
+        auto mark_ready = [&]() {
+            for (int64_t id : setup_ids) {
+                auto it = inst.find(id);
+                if (it != inst.end()) it->second.ready = true;
+            }
+            setup_ids.clear();
+            setup_in_flight = false;
+            newly_ready = true;
+        };
+        if (setup_in_flight && hipEventQuery(setup_done) == hipSuccess) mark_ready();
+        if (!setup_in_flight && !pending_setup.empty()) {
+            HIP_CHECK(hipMemcpyAsync(d_setup_list.p, pending_setup.data(), pending_setup.size() * 4,
+                                     hipMemcpyHostToDevice, setup_stream));
+            hipLaunchKernelGGL(k_batch_setup, dim3((int)pending_setup.size()), dim3(BB), 0, setup_stream,
+                               d_setup_list.p, d_inst.p, d_state.p, bp);
             HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipEventRecord(setup_done, setup_stream));
+            setup_ids = pending_ids;
+            pending_setup.clear();
+            pending_ids.clear();
+            setup_in_flight = true;
+        }
+        bool any_ready = false;
+        for (auto& kv : inst)
+            if (kv.second.ready && !kv.second.finished) {
+                any_ready = true;
+                break;
+            }
+        if (!any_ready && setup_in_flight) {
+            HIP_CHECK(hipEventSynchronize(setup_done));
+            mark_ready();
+        }
+        if (newly_ready && hi_slot > 0) {   // their set-up states (phase, eta, ...) to the host
             d_state.download(h_state.data(), hi_slot, streams[0]);
             HIP_CHECK(hipStreamSynchronize(streams[0]));
-            pending_setup.clear();
         }
         // 2. time limits (host clock per instance)
         const double t = bnow();
@@ -1183,7 +1226,7 @@ struct Batch {
         std::vector<int32_t> cls[2][4];
         for (auto& kv : inst) {
             const BHost& h = kv.second;
-            if (h.finished) continue;
+            if (h.finished || !h.ready) continue;
             const BState& st = h_state[h.slot];
             const int ph = st.phase;
             const size_t lds = (size_t)(h.n + h.m) * (ph ? 8 : 4);
@@ -1290,6 +1333,10 @@ struct Batch {
         for (auto& kv : inst) {
             BHost& h = kv.second;
             if (h.finished) continue;
+            if (!h.ready) {   // being set up (its state on the host is not valid yet)
+                ++running;
+                continue;
+            }
             const BState& st = h_state[h.slot];
             if (st.status != CPDLP_STILL_RUNNING) {
                 h.finished = true;
@@ -1313,7 +1360,7 @@ struct Batch {
         HIP_CHECK(hipSetDevice(device));
         BHost& h = get_host(id);
         if (x || y) {
-            if (!h_state[h.slot].setup) throw Error(CPDLP_ERR_STATE, "instance not set up yet: run the batch first");
+            if (!h.ready) throw Error(CPDLP_ERR_STATE, "instance not set up yet: run the batch first");
             DevBuf<double> xo(std::max<int32_t>(h.n, 1)), yo(std::max<int32_t>(h.m, 1));
             hipLaunchKernelGGL(k_batch_unscale, dim3(64), dim3(256), 0, streams[0], d_inst.p, d_state.p, h.slot,
                                xo.p, yo.p);
