@@ -635,7 +635,11 @@ void Handle::power_method(double tol, int64_t maxiter) {
                        (const double*)d_sums.p);
     const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
     PowState hs{};
-    for (;;) {
+    // 32 iterations per host check; every kernel reads the device PowState (a converged
+    // state makes the rest of the chunk a no-op), so the chunk is captured once in a
+    // hipGraph and replayed (no memset nodes)
+    hipGraphExec_t chunk = nullptr;
+    auto launch_chunk = [&]() {
         for (int c = 0; c < 32; ++c) {
             EpiStore<V> e1;
             e1.out = t.p;
@@ -656,11 +660,27 @@ void Handle::power_method(double tol, int64_t maxiter) {
                                d_partials2.p, d_sums.p + 8, (const PowState*)st.p);
             hipLaunchKernelGGL(k_pow_fin2, dim3(1), dim3(1), 0, s, st.p, (const double*)d_sums.p + 8);
         }
+    };
+    for (;;) {
+        if (P.use_graph) {
+            if (!chunk) {
+                hipGraph_t graph;
+                HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                launch_chunk();
+                HIP_CHECK(hipStreamEndCapture(s, &graph));
+                HIP_CHECK(hipGraphInstantiate(&chunk, graph, nullptr, nullptr, 0));
+                HIP_CHECK(hipGraphDestroy(graph));
+            }
+            HIP_CHECK(hipGraphLaunch(chunk, s));
+        } else {
+            launch_chunk();
+        }
         HIP_CHECK(hipGetLastError());
         st.download(&hs, 1, s);
         sync();
         if (hs.done) break;
     }
+    if (chunk) (void)hipGraphExecDestroy(chunk);
     powm_iters = hs.iter;
     eta = P.invnorm_scaling * (1.0 / std::sqrt(hs.theta));
 }

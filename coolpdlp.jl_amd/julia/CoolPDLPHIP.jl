@@ -262,4 +262,60 @@ function hip_solve!(state::HIPState)
     return STATUS[status[] + 1]
 end
 
+# ---- 4. many LPs at once (cpdlp_batch_*; no counterpart in the reference) ----------
+"""
+    solve(milps::Vector{<:MILP}, algo; hbm_bytes, checks_per_round = 20)
+
+Solve independent LPs together on one GPU (one workgroup per LP, `m + n <= 65536` each),
+returning `[(PrimalDualSolution, ConvergenceStats)]` in input order.
+"""
+function CoolPDLP.solve(milps::Vector{<:MILP}, algo::Algorithm{A, T, Ti, M, HIPEngine};
+        hbm_bytes::Integer = 0, checks_per_round::Integer = 20) where {A, T, Ti, M}
+    p = cparams(algo, algo.conversion.backend)
+    lps = map(milps) do milp
+        A_ = milp.A::SparseMatrixCSC
+        (A_, (Vector{Float64}(v) for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))...)
+    end
+    if hbm_bytes == 0
+        hbm_bytes = sum(lps) do (A_, _...)
+            ccall((:cpdlp_batch_footprint, libcpdlp), Int64, (Int64, Int64, Int64),
+                size(A_, 1), size(A_, 2), nnz(A_))
+        end + (64 << 20)
+    end
+    b = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:cpdlp_batch_create, libcpdlp), Cint, (Ref{CParams}, Int64, Ptr{Ptr{Cvoid}}), p, hbm_bytes, b))
+    try
+        ids = map(lps) do (A_, c, lv, uv, lc, uc)
+            lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
+                pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
+            id = Ref{Int64}(0)
+            GC.@preserve A_ c lv uv lc uc begin
+                check(ccall((:cpdlp_batch_add, libcpdlp), Cint, (Ptr{Cvoid}, Ref{CLP}, Ref{Int64}), b[], lp, id))
+            end
+            id[]
+        end
+        running = Ref{Int64}(length(ids))
+        while running[] > 0
+            check(ccall((:cpdlp_batch_run, libcpdlp), Cint, (Ptr{Cvoid}, Int32, Ref{Int64}),
+                b[], checks_per_round, running))
+            yield()
+        end
+        return map(zip(ids, lps)) do (id, (A_, _...))
+            x, y = zeros(size(A_, 2)), zeros(size(A_, 1))
+            s = Ref{CStats}()
+            check(ccall((:cpdlp_batch_get, libcpdlp), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ref{CStats}),
+                b[], id, x, y, s))
+            check(ccall((:cpdlp_batch_release, libcpdlp), Cint, (Ptr{Cvoid}, Int64), b[], id))
+            e = s[].err
+            err = KKTErrors(; primal = e.primal, primal_scale = e.primal_scale, dual = e.dual,
+                dual_scale = e.dual_scale, gap = e.gap, gap_scale = e.gap_scale)
+            stats = ConvergenceStats(Float64; err, time_elapsed = s[].time_elapsed,
+                kkt_passes = s[].kkt_passes, termination_status = STATUS[s[].termination_status + 1])
+            (PrimalDualSolution(x, y), stats)
+        end
+    finally
+        ccall((:cpdlp_batch_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), b[])
+    end
+end
+
 end # module
