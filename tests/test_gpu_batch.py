@@ -111,3 +111,32 @@ def test_batch_limits(cp):
         res = B.solve_all(lps)
     for sol, st in res:
         assert st.termination_status == cp.ITERATION_LIMIT and st.kkt_passes == 300
+
+
+def test_batch_long_rows_and_columns(cp):
+    """Rows and columns longer than one tile (2,048 nonzeros) take the whole-block path of
+    the batch engine: same status and objective as the plain engine and the closed form."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(17)
+    m, n = 600, 4000
+    A = sp.random(m, n, density=0.002, format="lil", random_state=rng)
+    A[0, :] = rng.standard_normal(n)            # one dense row (4,000 nonzeros)
+    A[:, 1] = rng.standard_normal((m, 1))       # one dense column (600 nonzeros)
+    A[5, :2500] = rng.standard_normal(2500)     # a row just over one tile
+    A = sp.csc_matrix(A)
+    x0 = np.where(rng.random(n) < 0.6, rng.random(n), 0.0)
+    y0 = rng.standard_normal(m)
+    s0 = np.where(x0 > 0, 0.0, rng.random(n))
+    b = A @ x0
+    g = G.GeneratedLP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.copy(),
+                      A.T @ y0 + s0, np.zeros(n), np.full(n, np.inf), b, b)
+    obj = float(g.c @ x0)
+    milp = cp.MILP.from_generated(g)
+    sol_p, st_p = cp.solve(milp, cp.PDLP("float64", termination_reltol=1e-6, max_kkt_passes=400_000))
+    with cp.BatchSolver(cp.PDLP("float64", termination_reltol=1e-6, max_kkt_passes=400_000),
+                        hbm_bytes=64 << 20) as B:
+        (sol_b, st_b), = B.solve_all([g])
+    assert st_p.termination_status == st_b.termination_status == cp.OPTIMAL
+    assert abs(st_b.eta - st_p.eta) <= 1e-9 * st_p.eta
+    for sol in (sol_p, sol_b):
+        assert abs(float(g.c @ sol.x) - obj) <= 1e-4 * (1 + abs(obj))

@@ -312,3 +312,40 @@ def test_chunked_loop_rejects_mixed(cp):
     with cp.Engine(cp.MILP.from_generated(g), cp.PDLP("mixed")) as eng:
         with pytest.raises(ValueError):
             eng.initialize()
+
+
+# ------------------------------------------------------------------ full size
+@pytest.fixture(scope="module")
+def large_lp():
+    return G.power_law_lp()   # m=1,000,000, n=2,000,000, nnz=20,000,027 (the benchmark LP)
+
+
+def test_full_size_csr_and_scaling_bit_exact(cp, large_lp):
+    """At the benchmark LP's real shape: CSR(A) construction and the Ruiz + Chambolle-Pock
+    scaling (D1, D2 and both scaled value arrays) equal the oracle's bit for bit."""
+    g = large_lp
+    milp = cp.MILP.from_generated(g)
+    with cp.Engine(milp, cp.PDLP("float64", None, "csr", termination_reltol=1e-6)) as eng:
+        rp, col = eng.csr()
+        orp, ocol, _ = O.csc_to_csr(g.m, g.n, g.colptr, g.rowval, g.nzval)
+        assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+        del orp, ocol, rp, col
+        D1, D2, a, at = eng.scaling()
+    oD1, oD2, oa, oat = O.scaling(g.m, g.n, g.colptr, g.rowval, g.nzval, 10, 1.0)
+    assert np.array_equal(D1, oD1) and np.array_equal(D2, oD2)
+    assert np.array_equal(a, oa) and np.array_equal(at, oat)
+
+
+def test_full_size_xslice_tracks_csr(cp, large_lp):
+    """At the benchmark LP's real shape: the XCD-slice format (steps, KKT check and power
+    method) follows the CSR run's KKT history to rounding over the first checks."""
+    milp = cp.MILP.from_generated(large_lp)
+    kw = dict(termination_reltol=1e-12, max_kkt_passes=500, powm_tol=1e-4)
+    _, s_csr = cp.solve(milp, cp.PDLP("float64", None, "csr", **kw))
+    _, s_xs = cp.solve(milp, cp.PDLP("float64", None, "xslice", **kw))
+    assert s_xs.format_A == 3 and s_csr.format_A == 1
+    assert abs(s_xs.eta - s_csr.eta) <= 1e-10 * s_csr.eta
+    assert len(s_csr.error_history) == len(s_xs.error_history) >= 5
+    for (p1, e1), (p2, e2) in zip(s_csr.error_history, s_xs.error_history):
+        assert p1 == p2
+        assert abs(e1.relative() - e2.relative()) <= 1e-8 * e1.relative(), (p1, e1, e2)
