@@ -1,22 +1,23 @@
 // batch.hip -- many independent LPs solved together (batched and served solves).
 //
-// One 256-thread workgroup owns one instance for the whole launch and runs the
-// reference's PDLP loop on it (src/algorithms/pdlp.jl:69-196): check_every PDHG steps,
-// the 5-candidate KKT check (src/components/errors.jl:65-109) in FP64, termination
-// (src/components/termination.jl:89-101), the restart rule (src/components/restart.jl:
-// 40-57) and the primal-weight update (src/components/step_size.jl:58-75) -- all on the
-// device.  Per-instance preprocessing (Ruiz + Chambolle-Pock scaling,
-// src/components/preconditioning.jl:117-155; the power method, src/utils/linalg.jl:86-96)
-// also runs on the device, one workgroup per instance.
+// One 256-thread workgroup per instance runs the reference's PDLP loop on it
+// (src/algorithms/pdlp.jl:69-196), in three kernels: k_batch_setup (Ruiz +
+// Chambolle-Pock scaling, src/components/preconditioning.jl:117-155, and the power
+// method, src/utils/linalg.jl:86-96), k_batch_steps (check_every PDHG steps) and
+// k_batch_check (the 5-candidate KKT check in FP64, src/components/errors.jl:65-109;
+// termination, src/components/termination.jl:89-101; the restart rule,
+// src/components/restart.jl:40-57; the primal-weight update,
+// src/components/step_size.jl:58-75) -- all decisions on the device.
 //
-// Each step gathers y (primal half-step) and xbar (dual half-step) from LDS: the
-// workgroup keeps both vectors of its instance in LDS, so the random gathers of the
-// sparse products never leave the CU.  Matrix values and indices stream from HBM with
-// L lanes per row (L chosen per instance from its mean row length).
+// The step kernel walks row tiles (<= 256 rows, <= 2048 nonzeros): the tile's entries
+// are loaded coalesced, all gathers issued together, the products summed per row in
+// LDS in the reference's order.  y (primal half-step) and xbar (dual half-step) are
+// gathered from LDS when the instance's vectors fit the LDS budget, else from L2.
 //
 // Precision MIXED: FP32 matrix, iterates and products (FP32 accumulation); KKT sums,
 // restart decisions, omega and eta in FP64; when the FP32 phase reaches polish_reltol
-// (or stops improving) the instance continues in FP64 from its best FP32 iterate.
+// or stops improving, the instance continues in FP64 from the same state (every
+// iterate converted; a diverging FP32 phase restarts from its best iterate instead).
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -72,7 +73,8 @@ __device__ __forceinline__ void bsum(double (&q)[NQ], double* sh) {
     __syncthreads();
 }
 
-// rows [0, rows) of a CSR with L lanes per row (L a power of two <= 64): lane l of a
+// rows [0, rows) of a CSR with L lanes per row (L a power of two <= 64), used by the
+// K-candidate KKT check (once per check_every steps): lane l of a
 // group takes entries l, l+L, ... then a butterfly over the group; f(r, acc) runs on
 // lane 0 of the group.  Every thread runs the same number of outer iterations, so all
 // lanes of a group take part in the shuffles.
@@ -1222,17 +1224,20 @@ struct Batch {
             d_timeout.upload(h_timeout.data(), hi_slot, streams[0]);
             HIP_CHECK(hipStreamSynchronize(streams[0]));
         }
-        // 3. one launch per (phase, LDS class)
-        std::vector<int32_t> cls[2][4];
+        // 3. one launch per (phase, LDS class); an instance's y and xbar take
+        //    (n + m) * sizeof(V) bytes of LDS, in the smallest class that holds them
+        constexpr int NCLS = 5;
+        const size_t lds_bytes[NCLS] = {std::min<size_t>(16384, lds_max), std::min<size_t>(40960, lds_max),
+                                        std::min<size_t>(98304, lds_max), lds_max, 0};
+        std::vector<int32_t> cls[2][NCLS];
         for (auto& kv : inst) {
             const BHost& h = kv.second;
             if (h.finished || !h.ready) continue;
             const BState& st = h_state[h.slot];
             const int ph = st.phase;
             const size_t lds = (size_t)(h.n + h.m) * (ph ? 8 : 4);
-            const int c = lds <= std::min<size_t>(16384, lds_max) ? 0
-                        : lds <= std::min<size_t>(40960, lds_max) ? 1
-                        : lds <= std::min<size_t>(98304, lds_max) ? 2 : 3;
+            int c = 0;
+            while (c < NCLS - 1 && lds > lds_bytes[c]) ++c;   // last class: gathers from L2
             cls[ph][c].push_back(h.slot);
         }
         // largest instances first: workgroups are dispatched in list order, so the long
@@ -1245,7 +1250,7 @@ struct Batch {
         lists.clear();
         std::vector<std::pair<int, int>> spans;   // (offset, count) into lists per launch
         for (int ph = 0; ph < 2; ++ph)
-            for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < NCLS; ++c) {
                 spans.emplace_back((int)lists.size(), (int)cls[ph][c].size());
                 lists.insert(lists.end(), cls[ph][c].begin(), cls[ph][c].end());
             }
@@ -1255,11 +1260,9 @@ struct Batch {
             hipEvent_t ready;
             HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
             HIP_CHECK(hipEventRecord(ready, streams[0]));
-            const size_t lds_bytes[4] = {std::min<size_t>(16384, lds_max), std::min<size_t>(40960, lds_max),
-                                         std::min<size_t>(98304, lds_max), 0};
-            hipStream_t gs[8];
+            hipStream_t gs[2 * NCLS];
             int si = 0;
-            for (int g = 0; g < 8; ++g) {
+            for (int g = 0; g < 2 * NCLS; ++g) {
                 gs[g] = nullptr;
                 if (spans[g].second == 0) continue;
                 gs[g] = streams[si++ % streams.size()];
@@ -1278,17 +1281,17 @@ struct Batch {
             };
             double wmax = 0.0, wmin = INFINITY;
             for (int ph = 0; ph < 2; ++ph)
-                for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < NCLS; ++c)
                     if (!cls[ph][c].empty()) {
                         wmax = std::max(wmax, work(cls[ph][c].front(), ph));
                         wmin = std::min(wmin, work(cls[ph][c].back(), ph));
                     }
             const double budget = std::max(wmax * std::max(1, checks / 4), (double)checks * wmin);
-            int start[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            int start[2 * NCLS] = {};
             for (int chk = 0; chk < checks; ++chk)
                 for (int ph = 0; ph < 2; ++ph)
-                    for (int c = 0; c < 4; ++c) {
-                        const int g = ph * 4 + c;
+                    for (int c = 0; c < NCLS; ++c) {
+                        const int g = ph * NCLS + c;
                         const auto sp = spans[g];
                         if (sp.second == 0) continue;
                         const std::vector<int32_t>& v = cls[ph][c];
@@ -1300,7 +1303,7 @@ struct Batch {
                         const int32_t* L = d_list.p + sp.first + start[g];
                         const dim3 grid(cnt), blk(BB);
                         if (ph == 0) {
-                            if (c < 3)
+                            if (c < NCLS - 1)
                                 hipLaunchKernelGGL((k_batch_steps<float, true>), grid, blk, lds_bytes[c], s, L,
                                                    d_inst.p, d_state.p, bp);
                             else
@@ -1309,7 +1312,7 @@ struct Batch {
                             hipLaunchKernelGGL(k_batch_check<float>, grid, blk, 0, s, L, d_inst.p, d_state.p,
                                                d_timeout.p, bp);
                         } else {
-                            if (c < 3)
+                            if (c < NCLS - 1)
                                 hipLaunchKernelGGL((k_batch_steps<double, true>), grid, blk, lds_bytes[c], s, L,
                                                    d_inst.p, d_state.p, bp);
                             else
