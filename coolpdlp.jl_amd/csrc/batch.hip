@@ -223,6 +223,83 @@ __device__ __forceinline__ void tiled_pass(const TileMeta* tiles, int ntiles, co
     }
 }
 
+// K products over the rows of a tiled CSR, accumulated in FP64 (the KKT check): per tile
+// the (col, val) pairs are loaded coalesced once, then for each vector kv the gathers
+// gx(kv, col) are issued together, the products go to LDS and each thread sums its row
+// sequentially from zero (the reference's order); fin(r, acc) gets the K row sums.  A
+// tile of one row longer than BT is summed by the whole block (strided, then the wave
+// sums in wave order).  Ends synchronised.
+template <int K, class V, class GX, class FIN>
+__device__ __forceinline__ void tiled_pass_k(const TileMeta* tiles, int ntiles, const int32_t* rp,
+                                             const int32_t* col, const V* val, double* prod,
+                                             double (*red)[BB / WAVE], GX gx, FIN fin) {
+    const int tid = threadIdx.x;
+    for (int t = 0; t < ntiles; ++t) {
+        const TileMeta tm = tiles[t];
+        if (tm.cnt > BT) {   // uniform per block
+            double acc[K];
+#pragma unroll
+            for (int kv = 0; kv < K; ++kv) acc[kv] = 0.0;
+            const int k1 = tm.base + tm.cnt;
+            for (int k = tm.base + tid; k < k1; k += BB) {
+                const double v = (double)val[k];
+                const int c = col[k];
+#pragma unroll
+                for (int kv = 0; kv < K; ++kv) acc[kv] += v * gx(kv, c);
+            }
+#pragma unroll
+            for (int kv = 0; kv < K; ++kv) {
+                const double w = wave_sum(acc[kv]);
+                if ((tid & (WAVE - 1)) == 0) red[kv][tid / WAVE] = w;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double sum[K];
+#pragma unroll
+                for (int kv = 0; kv < K; ++kv) {
+                    sum[kv] = red[kv][0];
+                    for (int w = 1; w < BB / WAVE; ++w) sum[kv] += red[kv][w];
+                }
+                fin(tm.r0, sum);
+            }
+            __syncthreads();
+            continue;
+        }
+        const int r = tm.r0 + tid;
+        const bool own = r < tm.r1;
+        int lo = 0, hi = 0;
+        if (own) {
+            lo = rp[r] - tm.base;
+            hi = rp[r + 1] - tm.base;
+        }
+        int cc[BPT];
+        double vv[BPT];
+        if (tm.cnt > 0) {
+#pragma unroll
+            for (int j = 0; j < BPT; ++j) {
+                const int e = min(tid + j * BB, tm.cnt - 1);
+                cc[j] = col[tm.base + e];
+                vv[j] = (double)val[tm.base + e];
+            }
+        }
+        double acc[K];
+#pragma unroll
+        for (int kv = 0; kv < K; ++kv) {
+            if (tm.cnt > 0) {
+                double xv[BPT];
+#pragma unroll
+                for (int j = 0; j < BPT; ++j) xv[j] = gx(kv, cc[j]);
+#pragma unroll
+                for (int j = 0; j < BPT; ++j) prod[tid + j * BB] = vv[j] * xv[j];
+            }
+            __syncthreads();
+            acc[kv] = own ? seq_sum(prod, lo, hi) : 0.0;
+            __syncthreads();
+        }
+        if (own) fin(r, acc);
+    }
+}
+
 // ---------------------------------------------------------------- setup kernel
 // preprocess + initialize for each listed instance (one workgroup each)
 __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const BInst* insts,
@@ -601,6 +678,8 @@ __global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const B
                                                     BState* states, const int32_t* timeout,
                                                     BParams P) {
     __shared__ double sh[32];
+    __shared__ double kprod[BT];
+    __shared__ double kred[5][BB / WAVE];
     const int slot = list[blockIdx.x];
     const BInst I = insts[slot];
     BState* S = states + slot;
@@ -620,17 +699,24 @@ __global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const B
         // ---- check: KKT errors (FP64) of sol, avg, sol_last, avg_last, restart
         const int c = cur, l = 1 - cur;
         constexpr int K = 5;
-        const V* X[K] = {it.x[c], it.xa, it.x[l], it.xal, it.xr};
-        const V* Y[K] = {it.y[c], it.ya, it.y[l], it.yal, it.yr};
+        // selects, not it.x[c]: a runtime index into the pointer pair sends it to scratch
+        V* const xc = c ? it.x[1] : it.x[0];
+        V* const yc = c ? it.y[1] : it.y[0];
+        V* const xl = l ? it.x[1] : it.x[0];
+        V* const yl = l ? it.y[1] : it.y[0];
+        const V* X[K] = {xc, it.xa, xl, it.xal, it.xr};
+        const V* Y[K] = {yc, it.ya, yl, it.yal, it.yr};
         const int KK = P.pdlp ? K : 1;
         double qa[2 * K + 2], qb[3 * K + 2];
         for (int t2 = 0; t2 < 2 * K + 2; ++t2) qa[t2] = 0.0;
         for (int t2 = 0; t2 < 3 * K + 2; ++t2) qb[t2] = 0.0;
-        vcsr<double, K>(m, I.a_rp, I.a_col, it.a, I.La,
+        tiled_pass_k<K>(I.a_tiles, I.n_a_tiles, I.a_rp, I.a_col, it.a, kprod, kred,
                         [&](int kv, int col) { return kv < KK ? (double)X[kv][col] : 0.0; },
                         [&](int i, const double* acc) {
                             const double lo = I.lcs[i], up = I.ucs[i], id = I.iD1[i];
-                            for (int k = 0; k < KK; ++k) {
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                if (k >= KK) break;
                                 const double pd = id * (acc[k] - proj_box(acc[k], lo, up));
                                 qa[2 * k] += pd * pd;
                                 const double yk = (double)Y[k][i];
@@ -643,11 +729,13 @@ __global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const B
                                 qa[2 * K + 1] += d1 * d1;
                             }
                         });
-        vcsr<double, K>(n, I.at_rp, I.at_col, it.at, I.Lat,
+        tiled_pass_k<K>(I.at_tiles, I.n_at_tiles, I.at_rp, I.at_col, it.at, kprod, kred,
                         [&](int kv, int col) { return kv < KK ? (double)Y[kv][col] : 0.0; },
                         [&](int j, const double* acc) {
                             const double cj = I.cs[j], lo = I.lvs[j], up = I.uvs[j], id = I.iD2[j];
-                            for (int k = 0; k < KK; ++k) {
+#pragma unroll
+                            for (int k = 0; k < K; ++k) {
+                                if (k >= KK) break;
                                 const double g = cj - acc[k];
                                 const double r = proj_multiplier(g, lo, up);
                                 const double dd = id * (g - r);
@@ -671,7 +759,9 @@ __global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const B
         for (int t2 = 0; t2 < 3 * K + 2; ++t2) rb[t2] = sh[t2];
         __syncthreads();
         double e[K][6];
-        for (int k = 0; k < KK; ++k) {   // errors.jl:77-98
+#pragma unroll
+        for (int k = 0; k < K; ++k) {   // errors.jl:77-98
+            if (k >= KK) break;
             e[k][0] = sqrt(ra[2 * k]);
             e[k][1] = S->primal_scale;
             e[k][2] = sqrt(rb[3 * k]);
@@ -702,17 +792,19 @@ __global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const B
         }
         if (status == CPDLP_STILL_RUNNING && !blowup && !reached && P.pdlp) {
             // restart_check! + should_restart (pdlp.jl:138-170, restart.jl:40-57)
-            from_avg = !(abs_of(e[0], omega) < abs_of(e[1], omega));
-            const double* cand = from_avg ? e[1] : e[0];
-            const double* cl = abs_of(e[2], omega) < abs_of(e[3], omega) ? e[2] : e[3];
-            const double ac = abs_of(cand, omega), ar = abs_of(e[4], omega), acl = abs_of(cl, omega);
+            // candidates compared by value (no pointer into e: keeps e in registers)
+            const double a0 = abs_of(e[0], omega), a1 = abs_of(e[1], omega);
+            const double a2 = abs_of(e[2], omega), a3 = abs_of(e[3], omega);
+            from_avg = !(a0 < a1);
+            const double ac = from_avg ? a1 : a0, ar = abs_of(e[4], omega), acl = a2 < a3 ? a2 : a3;
             const bool sufficient = ac <= P.suff * ar;
             const bool necessary = ac <= P.nec * ar;
             const bool no_local = ac > acl;
             const bool long_inner = (double)inner >= P.art * (double)total;
             if (sufficient || (necessary && no_local) || long_inner) {
                 restart = 1;
-                const double dx = sqrt(rb[3 * K + from_avg]), dy = sqrt(ra[2 * K + from_avg]);
+                const double dx = sqrt(from_avg ? rb[3 * K + 1] : rb[3 * K]);
+                const double dy = sqrt(from_avg ? ra[2 * K + 1] : ra[2 * K]);
                 if (dx > P.zero_tol && dy > P.zero_tol)
                     omega = exp(P.damping * log(dy / dx) + (1.0 - P.damping) * log(omega));
                 eta_sum = 0.0;
@@ -733,16 +825,16 @@ __global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const B
         }
         if (restart) {   // restart!: sol <- candidate; avg <- 0; restart <- sol
             for (int j = tid; j < n; j += BB) {
-                V s0 = it.x[c][j];
+                V s0 = xc[j];
                 if (from_avg) s0 = it.xa[j];
-                it.x[c][j] = s0;
+                xc[j] = s0;
                 it.xa[j] = V(0);
                 it.xr[j] = s0;
             }
             for (int i = tid; i < m; i += BB) {
-                V s0 = it.y[c][i];
+                V s0 = yc[i];
                 if (from_avg) s0 = it.ya[i];
-                it.y[c][i] = s0;
+                yc[i] = s0;
                 it.ya[i] = V(0);
                 it.yr[i] = s0;
             }
@@ -815,10 +907,14 @@ __global__ void k_batch_unscale(const BInst* insts, const BState* states, int32_
     const BState* S = states + slot;
     const int c = S->cur;
     const bool f64 = S->phase == 1;
+    const double* x64 = c ? I.x64[1] : I.x64[0];
+    const double* y64 = c ? I.y64[1] : I.y64[0];
+    const float* x32 = c ? I.x32[1] : I.x32[0];
+    const float* y32 = c ? I.y32[1] : I.y32[0];
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < I.n; j += gridDim.x * blockDim.x)
-        x[j] = I.D2[j] * (f64 ? I.x64[c][j] : (double)I.x32[c][j]);
+        x[j] = I.D2[j] * (f64 ? x64[j] : (double)x32[j]);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < I.m; i += gridDim.x * blockDim.x)
-        y[i] = I.D1[i] * (f64 ? I.y64[c][i] : (double)I.y32[c][i]);
+        y[i] = I.D1[i] * (f64 ? y64[i] : (double)y32[i]);
 }
 
 // ================================================================ host side
