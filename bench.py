@@ -24,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "coolpdlp.jl_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0
+FORMATS = ["auto", "csr", "ell", "xslice", "xslice_all"]   # index = CPDLP_FMT_*
 
 
 def measured_traffic(workload: str, fmt_a: str, fmt_at: str):
@@ -382,7 +383,7 @@ def main():
     ap.add_argument("--max-passes", type=int, default=10**7)
     # PMC calibration: also run a plain 512 MiB device-to-device copy kernel 5 times
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--format", default="auto", choices=["auto", "csr", "ell", "xslice"])
+    ap.add_argument("--format", default="auto", choices=FORMATS)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--checks-per-round", type=int, default=20)
     ap.add_argument("--plain-sample", type=int, default=16)
@@ -460,7 +461,7 @@ def main():
         "primal_step": {"ms": ms_p, "bytes": bp, "GBs": bp / (ms_p * 1e-3) / 1e9},
         "dual_step": {"ms": ms_d, "bytes": bd, "GBs": bd / (ms_d * 1e-3) / 1e9},
     }
-    fmt = {0: "auto", 1: "csr", 2: "ell", 3: "xslice"}
+    fmt = FORMATS
     traffic, traffic_src = measured_traffic(desc["workload"], fmt[st.format_A], fmt[st.format_At])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
@@ -495,8 +496,7 @@ def main():
         "relative_kkt": st.err.relative(), "restarts": st.restarts,
         "objective": obj, "objective_closed_form": g.objective,
         "objective_rel_err": abs(obj - g.objective) / (1 + abs(g.objective)),
-        "formats": {"A": ["auto", "csr", "ell", "xslice"][st.format_A],
-                    "At": ["auto", "csr", "ell", "xslice"][st.format_At]},
+        "formats": {"A": FORMATS[st.format_A], "At": FORMATS[st.format_At]},
         "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
         "powm_tol": args.powm_tol,
         "roofline": roofline, "cpu_baseline": cpu,

@@ -16,7 +16,7 @@ ERR_DIMENSION, ERR_ARGUMENT, ERR_MEMORY, ERR_HIP, ERR_STATE = 1, 2, 3, 4, 5
 
 ALGO_PDLP, ALGO_PDHG = 0, 1
 PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 2
-FMT_AUTO, FMT_CSR, FMT_ELL, FMT_XSLICE = 0, 1, 2, 3
+FMT_AUTO, FMT_CSR, FMT_ELL, FMT_XSLICE, FMT_XSLICE_ALL = 0, 1, 2, 3, 4
 
 
 class DimensionMismatch(ValueError):

@@ -111,7 +111,8 @@ class GPUSparseMatrixELL(_GPUSparseMatrix):
 
 def format_of(matrix_type) -> int:
     """Matrix type of Algorithm(T, Ti, M): None / "auto" lets the engine's autotuner pick
-    per matrix (CSR, ELL or XCD column slices); the others force one format."""
+    per matrix (CSR, ELL, XCD column slices of the long rows or of every row); the others
+    force one format."""
     if matrix_type is None or matrix_type in ("auto", "AUTO"):
         return _lib.FMT_AUTO
     if matrix_type in ("csr", "CSR"):
@@ -120,6 +121,8 @@ def format_of(matrix_type) -> int:
         return _lib.FMT_ELL
     if matrix_type in ("xslice", "XSLICE"):
         return _lib.FMT_XSLICE
+    if matrix_type in ("xslice_all", "XSLICE_ALL"):
+        return _lib.FMT_XSLICE_ALL
     if isinstance(matrix_type, type) and issubclass(matrix_type, _GPUSparseMatrix):
         return matrix_type._format
     if matrix_type in ("SparseMatrixCSC",):

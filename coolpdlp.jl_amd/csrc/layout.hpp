@@ -201,66 +201,96 @@ inline int32_t build_ell(const HostSide& S, double max_fill, std::vector<int32_t
     return w;
 }
 
-// XCD column slices of the long rows of an internal side.  Column slice s holds the
-// internal columns [s*W, (s+1)*W), W = ceil(cols / ns); slice s's part of long row r
-// (r < n_long) is the run of that row's entries whose column lies in the slice, in the
-// row's entry order.  rowptr: ns * (n_long + 1) absolute offsets, slices stored one
-// after the other.  Each slice's long rows are cut into tiles of at most BLOCK rows and
-// TILE_NNZ entries; a row part longer than TILE_NNZ is a tile of its own (r1 = r0 + 1,
-// cnt > TILE_NNZ).  Short rows are not sliced: they stay in the side's own tiles.
+// XCD column slices of the first `nrows` rows of an internal side (its long rows, or all
+// of its rows).  Column slice s holds a contiguous range of internal columns (ranges
+// balanced by entry count, slice_of_columns); slice s's part of row r (r < nrows) is the
+// run of that row's entries whose column lies in the slice, in the row's entry order.  rowptr:
+// ns * (nrows + 1) absolute offsets, slices stored one after the other.  Each slice's
+// rows are cut into tiles of at most TILE_NNZ entries and BLOCK rows (long rows) or
+// SL_RPT * BLOCK rows (short rows, whose parts hold a few entries) that never span
+// the long/short boundary (so the long rows' tiles are a prefix of every slice's tiles,
+// tile_n_long[s] of them); a row part longer than TILE_NNZ is a tile of its own
+// (r1 = r0 + 1, cnt > TILE_NNZ).  Rows past nrows are not sliced.
 struct HostSlices {
-    int32_t ns = 0, max_tiles = 0;
-    std::vector<int32_t> rowptr, col, vmap, tile_off;
+    int32_t ns = 0, rows = 0, max_tiles = 0, max_tiles_long = 0;
+    std::vector<int32_t> rowptr, col, vmap, tile_off, tile_n_long;
     std::vector<TileMeta> tiles;
 };
 
-inline void build_slices(const HostSide& S, int32_t ns, HostSlices& H) {
-    const int32_t nl = S.n_long;
-    const int64_t W = std::max<int64_t>(1, (S.cols + ns - 1) / ns);
+// Column slice boundaries: contiguous column ranges holding about nnz / ns entries of
+// rows [0, nrows) each (a skewed column distribution -- e.g. the other side's long rows,
+// which come first -- would otherwise load one XCD with most of the work).
+inline std::vector<int8_t> slice_of_columns(const HostSide& S, int32_t ns, int32_t nrows) {
+    std::vector<int64_t> per_col(S.cols, 0);
+    for (int64_t k = 0; k < S.rowptr[nrows]; ++k) per_col[S.col[k]]++;
+    const int64_t total = S.rowptr[nrows];
+    std::vector<int8_t> sl(S.cols, 0);
+    int64_t acc = 0;
+    int32_t cur = 0;
+    for (int32_t j = 0; j < S.cols; ++j) {
+        // move to the next slice once this one holds its share
+        while (cur < ns - 1 && acc * ns >= (int64_t)(cur + 1) * total) ++cur;
+        sl[j] = (int8_t)cur;
+        acc += per_col[j];
+    }
+    return sl;
+}
+
+inline void build_slices(const HostSide& S, int32_t ns, int32_t nrows, HostSlices& H) {
+    const int32_t nr = nrows, nl = std::min(S.n_long, nrows);
+    const std::vector<int8_t> slice_of = slice_of_columns(S, ns, nr);
     H.ns = ns;
-    H.rowptr.assign((size_t)ns * (nl + 1), 0);
-    std::vector<int32_t> cnt((size_t)ns * nl, 0);
-    for (int32_t r = 0; r < nl; ++r)
+    H.rows = nr;
+    H.rowptr.assign((size_t)ns * (nr + 1), 0);
+    std::vector<int32_t> cnt((size_t)ns * nr, 0);
+    for (int32_t r = 0; r < nr; ++r)
         for (int32_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k)
-            cnt[(size_t)(S.col[k] / W) * nl + r]++;
+            cnt[(size_t)slice_of[S.col[k]] * nr + r]++;
     int64_t off = 0;
     for (int32_t sl = 0; sl < ns; ++sl) {
-        int32_t* rp = H.rowptr.data() + (size_t)sl * (nl + 1);
-        for (int32_t r = 0; r < nl; ++r) {
+        int32_t* rp = H.rowptr.data() + (size_t)sl * (nr + 1);
+        for (int32_t r = 0; r < nr; ++r) {
             rp[r] = (int32_t)off;
-            off += cnt[(size_t)sl * nl + r];
+            off += cnt[(size_t)sl * nr + r];
         }
-        rp[nl] = (int32_t)off;
+        rp[nr] = (int32_t)off;
     }
+    std::vector<int32_t>().swap(cnt);
     H.col.resize(off);
     H.vmap.resize(off);
     std::vector<int32_t> next(H.rowptr);
-    for (int32_t r = 0; r < nl; ++r)
+    for (int32_t r = 0; r < nr; ++r)
         for (int32_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k) {
-            const int32_t sl = (int32_t)(S.col[k] / W);
-            const int32_t d = next[(size_t)sl * (nl + 1) + r]++;
+            const int32_t sl = slice_of[S.col[k]];
+            const int32_t d = next[(size_t)sl * (nr + 1) + r]++;
             H.col[d] = S.col[k];
             H.vmap[d] = S.vmap[k];
         }
     H.tiles.clear();
     H.tile_off.assign(ns + 1, 0);
-    H.max_tiles = 0;
+    H.tile_n_long.assign(ns, 0);
+    H.max_tiles = H.max_tiles_long = 0;
     for (int32_t sl = 0; sl < ns; ++sl) {
         H.tile_off[sl] = (int32_t)H.tiles.size();
-        const int32_t* rp = H.rowptr.data() + (size_t)sl * (nl + 1);
+        const int32_t* rp = H.rowptr.data() + (size_t)sl * (nr + 1);
         int32_t r = 0;
-        while (r < nl) {
-            const int32_t r0 = r;
+        while (r < nr) {
+            if (r == nl) H.tile_n_long[sl] = (int32_t)H.tiles.size() - H.tile_off[sl];
+            const int32_t r0 = r, lim = r0 < nl ? nl : nr;
+            // short rows' parts are summed a thread per row, SL_RPT rows per thread
+            const int32_t max_rows = r0 < nl ? BLOCK : SL_RPT * BLOCK;
             if (rp[r + 1] - rp[r] > TILE_NNZ) {
                 ++r;
             } else {
-                while (r < nl && r - r0 < BLOCK && rp[r + 1] - rp[r] <= TILE_NNZ &&
+                while (r < lim && r - r0 < max_rows && rp[r + 1] - rp[r] <= TILE_NNZ &&
                        rp[r + 1] - rp[r0] <= TILE_NNZ)
                     ++r;
             }
             H.tiles.push_back(TileMeta{r0, r, rp[r0], rp[r] - rp[r0]});
         }
+        if (nl == nr) H.tile_n_long[sl] = (int32_t)H.tiles.size() - H.tile_off[sl];
         H.max_tiles = std::max(H.max_tiles, (int32_t)H.tiles.size() - H.tile_off[sl]);
+        H.max_tiles_long = std::max(H.max_tiles_long, H.tile_n_long[sl]);
     }
     H.tile_off[ns] = (int32_t)H.tiles.size();
 }

@@ -290,14 +290,20 @@ MatView<V> DevSide::view(bool allow_slices) const {
     v.ell_col = ell_col.p;
     v.ell_val = ell_vals<V>();
     v.ns = sl_ns;
+    v.sl_all = format == CPDLP_FMT_XSLICE_ALL;
+    v.sl_rows = v.sl_all ? rows : n_long;
+    v.sl_stride = sl_built + 1;
     v.sl_rowptr = sl_rowptr.p;
     v.sl_col = sl_col.p;
     v.sl_val = sl_vals<V>();
     v.sl_tiles = sl_tiles.p;
     v.sl_tile_off = sl_tile_off.p;
-    v.sl_max_tiles = sl_max_tiles;
+    v.sl_tile_n = v.sl_all ? sl_tile_n.p : sl_tile_n_long.p;
+    v.sl_max_tiles = v.sl_all ? sl_max_tiles : sl_max_tiles_long;
     v.sl_part = sl_part.p;
-    if (format == CPDLP_FMT_XSLICE && !allow_slices) v.format = CPDLP_FMT_CSR;
+    // both slice variants run the sliced kernels
+    if (format == CPDLP_FMT_XSLICE_ALL) v.format = CPDLP_FMT_XSLICE;
+    if (v.format == CPDLP_FMT_XSLICE && !allow_slices) v.format = CPDLP_FMT_CSR;
     return v;
 }
 template <>
@@ -313,13 +319,15 @@ const double* DevSide::sl_vals<double>() const { return sl_val64.p; }
 template <>
 const float* DevSide::sl_vals<float>() const { return sl_val32.p; }
 
-// NS = 8 column slices of the long rows, one per XCD of the MI355X
-void DevSide::build_slices(const HostSide& S, hipStream_t s) {
+// NS = 8 column slices of rows [0, nrows), one per XCD of the MI355X
+void DevSide::build_slices(const HostSide& S, int32_t nrows, hipStream_t s) {
     constexpr int32_t NS = 8;
     HostSlices H;
-    cpdlp::build_slices(S, NS, H);
+    cpdlp::build_slices(S, NS, nrows, H);
     sl_ns = NS;
+    sl_built = nrows;
     sl_max_tiles = H.max_tiles;
+    sl_max_tiles_long = H.max_tiles_long;
     sl_nnz = (int64_t)H.col.size();
     sl_rowptr.upload(H.rowptr, s);
     sl_col.alloc(std::max<int64_t>(sl_nnz, 1));
@@ -328,7 +336,11 @@ void DevSide::build_slices(const HostSide& S, hipStream_t s) {
     sl_vmap.upload(H.vmap.data(), H.vmap.size(), s);
     sl_tiles.upload(H.tiles, s);
     sl_tile_off.upload(H.tile_off, s);
-    sl_part.alloc(std::max<int64_t>((int64_t)SL_KMAX * NS * S.n_long, 1));
+    std::vector<int32_t> tn(NS);
+    for (int32_t k = 0; k < NS; ++k) tn[k] = H.tile_off[k + 1] - H.tile_off[k];
+    sl_tile_n.upload(tn, s);
+    sl_tile_n_long.upload(H.tile_n_long, s);
+    sl_part.alloc(std::max<int64_t>((int64_t)SL_KMAX * NS * nrows, 1));
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
@@ -430,6 +442,8 @@ static int64_t wrap_cube(int64_t n) {
 Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
     if (P.check_every <= 0) throw Error(CPDLP_ERR_ARGUMENT, "check_every must be positive");
     if (P.precision < 0 || P.precision > 2) throw Error(CPDLP_ERR_ARGUMENT, "unknown precision");
+    if (P.format < CPDLP_FMT_AUTO || P.format > CPDLP_FMT_XSLICE_ALL)
+        throw Error(CPDLP_ERR_ARGUMENT, "unknown format");
     if (P.algorithm != CPDLP_ALGO_PDLP && P.algorithm != CPDLP_ALGO_PDHG)
         throw Error(CPDLP_ERR_ARGUMENT, "unknown algorithm");
     if (P.long_row <= 0) P.long_row = 64;
@@ -487,11 +501,18 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         A.build_ell(L.A, s, fill);
         At.build_ell(L.At, s, fill);
     }
-    // slices cut the long rows only: AUTO builds them for a side that has long rows
-    if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && L.A.n_long > 0))
-        A.build_slices(L.A, s);
-    if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && L.At.n_long > 0))
-        At.build_slices(L.At, s);
+    // XCD slices: XSLICE cuts the long rows, XSLICE_ALL every row.  AUTO builds slices
+    // of every row for a large side whose gathered vector outgrows one XCD's 4 MiB L2
+    // (the tuner then times both variants), else of the long rows when it has any.
+    const size_t elem = P.precision == CPDLP_PREC_FP64 ? 8 : 4;
+    for (auto side : {std::make_pair(&A, &L.A), std::make_pair(&At, &L.At)}) {
+        const HostSide& H = *side.second;
+        const bool big = (size_t)H.cols * elem > ((size_t)4 << 20) && H.nnz >= ((int64_t)4 << 20);
+        if (P.format == CPDLP_FMT_XSLICE_ALL || (P.format == CPDLP_FMT_AUTO && big))
+            side.first->build_slices(H, H.rows, s);
+        else if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && H.n_long > 0))
+            side.first->build_slices(H, H.n_long, s);
+    }
     HIP_CHECK(hipStreamSynchronize(s));
     // free the big host-side copies that are no longer needed
     std::vector<int32_t>().swap(L.A.col);
@@ -695,8 +716,8 @@ void Handle::choose_formats() {
             if (S->ell_w > 0) S->format = CPDLP_FMT_ELL;
             continue;
         }
-        if (P.format == CPDLP_FMT_XSLICE) {
-            if (S->sl_ns > 0) S->format = CPDLP_FMT_XSLICE;
+        if (P.format == CPDLP_FMT_XSLICE || P.format == CPDLP_FMT_XSLICE_ALL) {
+            if (S->sl_ns > 0) S->format = P.format;
             continue;
         }
         // AUTO: time the plain product (K = 1, store epilogue) of every available
@@ -704,6 +725,7 @@ void Handle::choose_formats() {
         std::vector<int32_t> cand{CPDLP_FMT_CSR};
         if (S->ell_w > 0) cand.push_back(CPDLP_FMT_ELL);
         if (S->sl_ns > 0 && S->n_long > 0) cand.push_back(CPDLP_FMT_XSLICE);
+        if (S->slices_all()) cand.push_back(CPDLP_FMT_XSLICE_ALL);
         if (cand.size() == 1) continue;
         DevBuf<double> xin(std::max<int64_t>(S->cols, 1)), out(std::max<int64_t>(S->rows, 1));
         hipLaunchKernelGGL(k_fill, dim3(std::max(1, ceil_div(S->cols, BLOCK))), dim3(BLOCK), 0,

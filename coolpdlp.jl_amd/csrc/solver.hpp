@@ -23,10 +23,10 @@ struct DevSide {
     DevBuf<int32_t> ell_col, ell_vmap;
     DevBuf<double> ell_val64;
     DevBuf<float> ell_val32;
-    // XCD column slices of the long rows (step kernels only)
-    int32_t sl_ns = 0, sl_max_tiles = 0;
+    // XCD column slices of rows [0, sl_built) (the long rows, or all rows)
+    int32_t sl_ns = 0, sl_built = 0, sl_max_tiles = 0, sl_max_tiles_long = 0;
     int64_t sl_nnz = 0;
-    DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_off;
+    DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_off, sl_tile_n, sl_tile_n_long;
     DevBuf<TileMeta> sl_tiles;
     DevBuf<double> sl_val64, sl_part;
     DevBuf<float> sl_val32;
@@ -41,7 +41,8 @@ struct DevSide {
     template <class V>
     MatView<V> view(bool allow_slices = true) const;
     void upload_structure(const HostSide& S, hipStream_t s);
-    void build_slices(const HostSide& S, hipStream_t s);
+    void build_slices(const HostSide& S, int32_t nrows, hipStream_t s);
+    bool slices_all() const { return sl_ns > 0 && sl_built == rows; }
     void build_ell(const HostSide& S, hipStream_t s, double max_fill);
     void fill_values(const double* src_csc, bool want32, hipStream_t s);
 };

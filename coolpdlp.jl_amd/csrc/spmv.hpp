@@ -54,33 +54,41 @@ struct MatView {
     int32_t ell_width = 0;
     const int32_t* ell_col = nullptr;
     const V* ell_val = nullptr;
-    // XCD column slices (format 3): the long rows are also stored cut by column slice,
-    // NS slices (one per XCD); the slice-s part of every long row is summed by a block
-    // with blockIdx % NS == s (workgroups are dispatched to the XCDs round-robin), so
-    // each XCD gathers from one slice of the vector, which stays in that XCD's L2.
+    // XCD column slices (format 3): the first sl_rows rows (the long rows, or every
+    // row when sl_all) are also stored cut by column slice, NS slices (one per XCD); the
+    // slice-s part of every such row is summed by a block with blockIdx % NS == s
+    // (workgroups are dispatched to the XCDs round-robin), so each XCD gathers from one
+    // slice of the vector, which stays in that XCD's L2.
     int32_t ns = 0;
-    const int32_t* sl_rowptr = nullptr;   // ns * (n_long + 1), absolute offsets
+    int32_t sl_rows = 0;                  // rows [0, sl_rows) are sliced
+    bool sl_all = false;                  // sl_rows == rows: no short-tile blocks
+    int32_t sl_stride = 0;                // per-slice rowptr length (built rows + 1)
+    const int32_t* sl_rowptr = nullptr;   // ns * sl_stride, absolute offsets
     const int32_t* sl_col = nullptr;
     const V* sl_val = nullptr;
     const TileMeta* sl_tiles = nullptr;   // tiles of slice s: sl_tiles[sl_tile_off[s] ...]
-    const int32_t* sl_tile_off = nullptr; // ns + 1; slice s has off[s+1] - off[s] tiles
-    int32_t sl_max_tiles = 0;             // most tiles of any slice
-    void* sl_part = nullptr;              // ns * n_long slice sums (Acc)
+    const int32_t* sl_tile_off = nullptr; // ns + 1: first tile of each slice
+    const int32_t* sl_tile_n = nullptr;   // ns: tiles of each slice covering [0, sl_rows)
+    int32_t sl_max_tiles = 0;             // most of any slice
+    void* sl_part = nullptr;              // ns * sl_rows slice sums (Acc) per vector
 
+    // the sliced format's blocks that run the side's own short tiles
+    __host__ __device__ int sl_short_tiles() const { return sl_all ? 0 : n_tiles; }
     int grid() const {
         if (format == CPDLP_FMT_ELL) return ceil_div(rows, BLOCK);
-        if (format == CPDLP_FMT_XSLICE) return ns * sl_max_tiles + n_tiles;
+        if (format == CPDLP_FMT_XSLICE) return ns * sl_max_tiles + sl_short_tiles();
         return n_long_blocks + n_tiles;
     }
     // blocks that write an epilogue's block partials (NQ per block): the sliced format's
     // short tiles, then its combine blocks
     int partial_blocks() const {
-        if (format == CPDLP_FMT_XSLICE) return n_tiles + ceil_div(n_long, BLOCK);
+        if (format == CPDLP_FMT_XSLICE) return sl_short_tiles() + ceil_div(sl_rows, BLOCK);
         return grid();
     }
 };
 
 constexpr int SL_KMAX = 5;   // most vectors per product in the sliced format
+constexpr int SL_RPT = 8;    // short rows per thread in a sliced tile
 
 template <class V, int K>
 struct XPack {
@@ -199,15 +207,16 @@ __device__ __forceinline__ Acc group_sum(Acc v) {
 }
 
 // XCD-sliced product, pass 1.  Blocks [0, ns * sl_max_tiles): block b sums slice
-// s = b % ns of the long rows of tile b / ns of that slice, for each of the K vectors,
-// into sl_part[(kv * ns + s) * n_long + r]: the tile's entries are loaded coalesced
-// once, the products of vector kv formed in LDS, and every row part summed by a group of
-// 16 lanes (lane l takes entries l, l+16, ... in order, then a butterfly); a row part
-// longer than TILE_NNZ is summed by the whole block (strided, then the four wave sums
-// in wave order).  Blocks past those: the side's short tiles, summed and finished by the
-// epilogue exactly as in k_spmv_csr, block partials at index b - ns * sl_max_tiles.
-// Deterministic; the long rows' order of summation differs from a sequential sum (as
-// the wavefront path's does).
+// s = b % ns of the rows of tile b / ns of that slice, for each of the K vectors,
+// into sl_part[(kv * ns + s) * sl_rows + r]: the tile's entries are loaded coalesced
+// once and the products of vector kv formed in LDS.  A long row's part is summed by a
+// group of 16 lanes (lane l takes entries l, l+16, ... in order, then a butterfly); a
+// short row's part (a few entries) by one thread, sequentially; a row part longer than
+// TILE_NNZ by the whole block (strided, then the four wave sums in wave order).  Blocks
+// past those (when the short rows are not sliced): the side's short tiles, summed and
+// finished by the epilogue exactly as in k_spmv_csr, block partials at index
+// b - ns * sl_max_tiles.  Deterministic; the sliced rows' order of summation differs
+// from a sequential sum (as the wavefront path's does).
 template <class V, class Acc, int K, class Epi>
 __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K> X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
@@ -231,12 +240,12 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K>
         return;
     }
     const int s = blockIdx.x % A.ns;
-    const int t = A.sl_tile_off[s] + (int)blockIdx.x / A.ns;
-    if (t >= A.sl_tile_off[s + 1]) return;   // uniform per block: this slice has fewer tiles
-    const TileMeta tm = A.sl_tiles[t];
-    const int32_t* rp = A.sl_rowptr + (size_t)s * (A.n_long + 1);
+    const int tl = (int)blockIdx.x / A.ns;
+    if (tl >= A.sl_tile_n[s]) return;   // uniform per block: this slice has fewer tiles
+    const TileMeta tm = A.sl_tiles[A.sl_tile_off[s] + tl];
+    const int32_t* rp = A.sl_rowptr + (size_t)s * A.sl_stride;
     Acc* part = (Acc*)A.sl_part;
-    auto part_at = [&](int kv, int r) -> Acc& { return part[((size_t)kv * A.ns + s) * A.n_long + r]; };
+    auto part_at = [&](int kv, int r) -> Acc& { return part[((size_t)kv * A.ns + s) * A.sl_rows + r]; };
     if (tm.cnt > TILE_NNZ) {   // one long row part, the whole block
         constexpr int U = 4;
         Acc acc[K];
@@ -278,8 +287,29 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K>
     }
     constexpr int SEG = 16;
     const int g = tid / SEG, l = tid % SEG;
+    // short rows' tiles (uniform per block: tiles never span the long/short boundary):
+    // thread tid sums the parts of rows r0 + tid + j * BLOCK, j < SL_RPT
+    const bool per_thread = tm.r0 >= A.n_long;
+    int tlo[SL_RPT], thi[SL_RPT];
+#pragma unroll
+    for (int j = 0; j < SL_RPT; ++j) {
+        const int rt = tm.r0 + tid + j * BLOCK;
+        tlo[j] = thi[j] = 0;
+        if (per_thread && rt < tm.r1) {
+            tlo[j] = rp[rt] - tm.base;
+            thi[j] = rp[rt + 1] - tm.base;
+        }
+    }
     // every lane of a group runs the same trips, so the butterfly's partners are active
     auto row_parts = [&](int kv) {
+        if (per_thread) {
+#pragma unroll
+            for (int j = 0; j < SL_RPT; ++j) {
+                const int rt = tm.r0 + tid + j * BLOCK;
+                if (rt < tm.r1) part_at(kv, rt) = seq_sum(prod, tlo[j], thi[j]);
+            }
+            return;
+        }
         for (int r = tm.r0 + g; r < tm.r1; r += BLOCK / SEG) {
             const int lo = rp[r] - tm.base, hi = rp[r + 1] - tm.base;
             Acc sum = Acc(0);
@@ -320,8 +350,8 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K>
     }
 }
 
-// XCD-sliced product, pass 2: long row r's sum = its slice sums added in slice order,
-// then the epilogue (thread per row); block partials at index n_tiles + block.
+// XCD-sliced product, pass 2: sliced row r's sum = its slice sums added in slice order,
+// then the epilogue (thread per row); block partials at index sl_short_tiles() + block.
 template <class V, class Acc, int K, class Epi>
 __global__ __launch_bounds__(BLOCK) void k_xslice_combine(MatView<V> A, Epi epi) {
     if (epi.skip()) return;
@@ -329,20 +359,20 @@ __global__ __launch_bounds__(BLOCK) void k_xslice_combine(MatView<V> A, Epi epi)
 #pragma unroll
     for (int i = 0; i < (Epi::NQ > 0 ? Epi::NQ : 1); ++i) q[i] = 0.0;
     const int r = blockIdx.x * BLOCK + threadIdx.x;
-    if (r < A.n_long) {
+    if (r < A.sl_rows) {
         const typename Epi::Pre pre = epi.load(r);
         const Acc* part = (const Acc*)A.sl_part;
         Acc acc[K];
 #pragma unroll
         for (int kv = 0; kv < K; ++kv) {
-            Acc sum = part[((size_t)kv * A.ns) * A.n_long + r];
-            for (int s = 1; s < A.ns; ++s) sum += part[((size_t)kv * A.ns + s) * A.n_long + r];
+            Acc sum = part[((size_t)kv * A.ns) * A.sl_rows + r];
+            for (int s = 1; s < A.ns; ++s) sum += part[((size_t)kv * A.ns + s) * A.sl_rows + r];
             acc[kv] = sum;
         }
         epi.row(r, acc, q, pre);
     }
     if constexpr (Epi::NQ > 0)
-        block_sum<Epi::NQ>(q, epi.partials + (size_t)(A.n_tiles + blockIdx.x) * Epi::NQ);
+        block_sum<Epi::NQ>(q, epi.partials + (size_t)(A.sl_short_tiles() + blockIdx.x) * Epi::NQ);
 }
 
 // Column-major ELL: thread per row; entry s of rows i..i+63 is contiguous, so each
@@ -394,8 +424,8 @@ inline void launch_spmv(const MatView<V>& A, const XPack<V, K>& X, const Epi& ep
     if constexpr (K <= SL_KMAX) {
         if (A.format == CPDLP_FMT_XSLICE) {
             hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
-            if (A.n_long > 0)
-                hipLaunchKernelGGL((k_xslice_combine<V, Acc, K, Epi>), dim3(ceil_div(A.n_long, BLOCK)),
+            if (A.sl_rows > 0)
+                hipLaunchKernelGGL((k_xslice_combine<V, Acc, K, Epi>), dim3(ceil_div(A.sl_rows, BLOCK)),
                                    dim3(BLOCK), 0, s, A, epi);
             return;
         }

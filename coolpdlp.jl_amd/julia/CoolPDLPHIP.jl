@@ -177,7 +177,7 @@ function cparams(algo::Algorithm{A}, eng::HIPEngine) where {A}
     p.max_kkt_passes = termination.max_kkt_passes
     p.time_limit = termination.time_limit
     p.precision = Dict(:fp64 => 0, :fp32 => 1, :mixed => 2)[eng.precision]
-    p.format = Dict(:auto => 0, :csr => 1, :ell => 2, :xslice => 3)[eng.format]
+    p.format = Dict(:auto => 0, :csr => 1, :ell => 2, :xslice => 3, :xslice_all => 4)[eng.format]
     p.device = eng.device
     return p
 end

@@ -48,7 +48,8 @@ extern "C" {
 #define CPDLP_FMT_AUTO 0        /* autotuned per matrix */
 #define CPDLP_FMT_CSR 1         /* GPUSparseMatrixCSR, src/utils/mat_csr.jl */
 #define CPDLP_FMT_ELL 2         /* GPUSparseMatrixELL, src/utils/mat_ell.jl */
-#define CPDLP_FMT_XSLICE 3      /* CSR cut into 8 column slices, one per XCD's L2 */
+#define CPDLP_FMT_XSLICE 3      /* long rows cut into 8 column slices, one per XCD's L2 */
+#define CPDLP_FMT_XSLICE_ALL 4  /* every row cut into 8 column slices */
 
 /* ---- problem input --------------------------------------------------------------
  * MILP(; c, lv, uv, A, lc, uc), src/problems/milp.jl:55-115, with A given as the

@@ -143,15 +143,16 @@ def test_termination_not_skipped(cp):
 
 @pytest.mark.parametrize("algo,use_graph,fmt", [
     ("PDLP", 0, "csr"), ("PDLP", 1, "csr"), ("PDLP", 1, "ell"), ("PDLP", 1, "xslice"),
-    ("PDLP", 1, "auto"), ("PDHG", 0, "csr"), ("PDHG", 1, "auto")])
+    ("PDLP", 1, "xslice_all"), ("PDLP", 1, "auto"), ("PDHG", 0, "csr"), ("PDHG", 1, "auto")])
 def test_solve_matches_oracle(cp, algo, use_graph, fmt):
     g = G.column_lp(400, 1000, 6, seed=21)
     milp = cp.MILP.from_generated(g)
     A = cp.PDLP if algo == "PDLP" else cp.PDHG
     reltol = 1e-6 if algo == "PDLP" else 1e-4
     sol, st = cp.solve(milp, A("float64", None, fmt, termination_reltol=reltol, use_graph=use_graph))
-    if fmt in ("ell", "xslice"):
-        assert (st.format_A, st.format_At) == ((2, 2) if fmt == "ell" else (3, 3))
+    code = {"ell": 2, "xslice": 3, "xslice_all": 4}
+    if fmt in code:
+        assert (st.format_A, st.format_At) == (code[fmt], code[fmt])
     ref = O.solve(g, O.default_params(algo, termination_reltol=reltol))
     assert st.termination_status.name == ref.status == "OPTIMAL"
     assert abs(st.kkt_passes - ref.stats.kkt_passes) <= 200
@@ -229,7 +230,7 @@ def test_float32_and_mixed(cp):
     assert abs(cp.objective_value(sol.x, milp) - g.objective) <= 1e-2 * (1 + abs(g.objective))
 
 
-@pytest.mark.parametrize("fmt", ["csr", "xslice", "auto"])
+@pytest.mark.parametrize("fmt", ["csr", "xslice", "xslice_all", "auto"])
 def test_power_law_lp_formats(cp, fmt):
     """Long rows (wavefront path) + XCD slices on a small power-law LP: same status and
     objective as the oracle; the slice format's row sums differ only in rounding."""
@@ -262,16 +263,18 @@ def test_min_cost_flow_mixed_vs_highs(cp):
     assert np.max(np.abs(A @ sol.x - g.lc)) <= 1e-5 * (1 + np.max(np.abs(g.lc)))
 
 
-def test_xslice_long_rows_track_csr(cp):
-    """XCD column slices of the long rows, including row parts longer than one tile
-    (rows of 15k-30k nonzeros): the KKT history tracks the CSR run's to rounding over
-    the first checks, and the format is reported per side."""
+@pytest.mark.parametrize("fmt,code", [("xslice", 3), ("xslice_all", 4)])
+def test_xslice_long_rows_track_csr(cp, fmt, code):
+    """XCD column slices of the long rows (xslice) or of every row (xslice_all),
+    including row parts longer than one tile (rows of 15k-30k nonzeros): the KKT history
+    tracks the CSR run's to rounding over the first checks, and the format is reported
+    per side."""
     g = G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4)
     milp = cp.MILP.from_generated(g)
     kw = dict(termination_reltol=1e-12, max_kkt_passes=1500, powm_tol=1e-6)
     _, s_csr = cp.solve(milp, cp.PDLP("float64", None, "csr", **kw))
-    _, s_xs = cp.solve(milp, cp.PDLP("float64", None, "xslice", **kw))
-    assert (s_xs.format_A, s_xs.format_At) == (3, 3)
+    _, s_xs = cp.solve(milp, cp.PDLP("float64", None, fmt, **kw))
+    assert (s_xs.format_A, s_xs.format_At) == (code, code)
     assert len(s_csr.error_history) == len(s_xs.error_history) >= 10
     for (p1, e1), (p2, e2) in list(zip(s_csr.error_history, s_xs.error_history))[:8]:
         assert p1 == p2
@@ -336,14 +339,15 @@ def test_full_size_csr_and_scaling_bit_exact(cp, large_lp):
     assert np.array_equal(a, oa) and np.array_equal(at, oat)
 
 
-def test_full_size_xslice_tracks_csr(cp, large_lp):
-    """At the benchmark LP's real shape: the XCD-slice format (steps, KKT check and power
-    method) follows the CSR run's KKT history to rounding over the first checks."""
+@pytest.mark.parametrize("fmt,code", [("xslice", 3), ("xslice_all", 4)])
+def test_full_size_xslice_tracks_csr(cp, large_lp, fmt, code):
+    """At the benchmark LP's real shape: the XCD-slice formats (steps, KKT check and power
+    method) follow the CSR run's KKT history to rounding over the first checks."""
     milp = cp.MILP.from_generated(large_lp)
     kw = dict(termination_reltol=1e-12, max_kkt_passes=500, powm_tol=1e-4)
     _, s_csr = cp.solve(milp, cp.PDLP("float64", None, "csr", **kw))
-    _, s_xs = cp.solve(milp, cp.PDLP("float64", None, "xslice", **kw))
-    assert s_xs.format_A == 3 and s_csr.format_A == 1
+    _, s_xs = cp.solve(milp, cp.PDLP("float64", None, fmt, **kw))
+    assert s_xs.format_A == code and s_csr.format_A == 1
     assert abs(s_xs.eta - s_csr.eta) <= 1e-10 * s_csr.eta
     assert len(s_csr.error_history) == len(s_xs.error_history) >= 5
     for (p1, e1), (p2, e2) in zip(s_csr.error_history, s_xs.error_history):
