@@ -1098,13 +1098,19 @@ struct Batch {
     int32_t max_slots = 0;
     DevBuf<BInst> d_inst;
     DevBuf<BState> d_state;
-    DevBuf<int32_t> d_timeout, d_list, d_setup_list;
+    DevBuf<int32_t> d_timeout, d_list;
     // setup runs on its own stream, asynchronously: rounds go on for the instances that
-    // are ready while new ones are set up
+    // are ready while new ones are set up.  An admission's setup is split by size into
+    // launches run smallest first, so the small instances become ready long before the
+    // largest one's power method ends.
+    struct SetupChunk {
+        hipEvent_t ev = nullptr;
+        std::vector<int32_t> slots;   // host copy stays alive until the launch completes
+        std::vector<int64_t> ids;
+        DevBuf<int32_t> list;
+    };
     hipStream_t setup_stream = nullptr;
-    hipEvent_t setup_done = nullptr;
-    bool setup_in_flight = false;
-    std::vector<int64_t> setup_ids;
+    std::deque<SetupChunk> setups;   // in flight, in launch order
     std::vector<BInst> h_inst;
     std::vector<BState> h_state;
     std::vector<int32_t> h_timeout, free_slots;
@@ -1126,7 +1132,6 @@ struct Batch {
         streams.assign((size_t)std::min(8, std::max(1, P.batch_streams)), nullptr);
         for (auto& s : streams) HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&setup_stream, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreateWithFlags(&setup_done, hipEventDisableTiming));
         if (bytes <= 0) throw Error(CPDLP_ERR_ARGUMENT, "arena size must be positive");
         arena_mem.alloc((size_t)bytes);
         arena.init((size_t)bytes);
@@ -1143,7 +1148,6 @@ struct Batch {
         d_state.alloc(max_slots);
         d_timeout.alloc(max_slots);
         d_list.alloc(max_slots);
-        d_setup_list.alloc(max_slots);
         h_inst.resize(max_slots);
         h_state.resize(max_slots);
         h_timeout.assign(max_slots, 0);
@@ -1157,7 +1161,8 @@ struct Batch {
         for (auto& s : streams)
             if (s) (void)hipStreamDestroy(s);
         if (setup_stream) (void)hipStreamDestroy(setup_stream);
-        if (setup_done) (void)hipEventDestroy(setup_done);
+        for (auto& c : setups)
+            if (c.ev) (void)hipEventDestroy(c.ev);
     }
 
     BParams params() const {
@@ -1269,27 +1274,57 @@ struct Batch {
         // 1. setup of newly admitted instances, on the setup stream; a round never waits
         //    for it unless nothing else is ready to run
         bool newly_ready = false;
-        auto mark_ready = [&]() {
-            for (int64_t id : setup_ids) {
-                auto it = inst.find(id);
-                if (it != inst.end()) it->second.ready = true;
+        auto retire_done = [&](bool wait_first) {   // completed launches, in order
+            while (!setups.empty()) {
+                SetupChunk& c = setups.front();
+                if (wait_first) {
+                    HIP_CHECK(hipEventSynchronize(c.ev));
+                    wait_first = false;
+                } else if (hipEventQuery(c.ev) != hipSuccess) {
+                    break;
+                }
+                for (int64_t id : c.ids) {
+                    auto it = inst.find(id);
+                    if (it != inst.end()) it->second.ready = true;
+                }
+                (void)hipEventDestroy(c.ev);
+                setups.pop_front();
+                newly_ready = true;
             }
-            setup_ids.clear();
-            setup_in_flight = false;
-            newly_ready = true;
         };
-        if (setup_in_flight && hipEventQuery(setup_done) == hipSuccess) mark_ready();
-        if (!setup_in_flight && !pending_setup.empty()) {
-            HIP_CHECK(hipMemcpyAsync(d_setup_list.p, pending_setup.data(), pending_setup.size() * 4,
-                                     hipMemcpyHostToDevice, setup_stream));
-            hipLaunchKernelGGL(k_batch_setup, dim3((int)pending_setup.size()), dim3(BB), 0, setup_stream,
-                               d_setup_list.p, d_inst.p, d_state.p, bp);
-            HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipEventRecord(setup_done, setup_stream));
-            setup_ids = pending_ids;
+        retire_done(false);
+        if (setups.empty() && !pending_setup.empty()) {
+            // smallest first, in launches whose largest instance has at most 1/64, 1/8 and
+            // all of the largest one's work (nonzeros + rows + columns)
+            std::vector<size_t> order(pending_setup.size());
+            for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+            auto w = [&](size_t i) {
+                const BInst& I = h_inst[pending_setup[i]];
+                return (double)I.nnz + I.m + I.n;
+            };
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return w(a) < w(b); });
+            const double wmax = w(order.back());
+            size_t k = 0;
+            for (const double frac : {1.0 / 64, 1.0 / 8, 1.0}) {
+                SetupChunk c;
+                while (k < order.size() && (frac == 1.0 || w(order[k]) <= frac * wmax)) {
+                    c.slots.push_back(pending_setup[order[k]]);
+                    c.ids.push_back(pending_ids[order[k]]);
+                    ++k;
+                }
+                if (c.slots.empty()) continue;
+                c.list.alloc(c.slots.size());
+                HIP_CHECK(hipMemcpyAsync(c.list.p, c.slots.data(), c.slots.size() * 4,
+                                         hipMemcpyHostToDevice, setup_stream));
+                hipLaunchKernelGGL(k_batch_setup, dim3((int)c.slots.size()), dim3(BB), 0, setup_stream,
+                                   c.list.p, d_inst.p, d_state.p, bp);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
+                HIP_CHECK(hipEventRecord(c.ev, setup_stream));
+                setups.push_back(std::move(c));
+            }
             pending_setup.clear();
             pending_ids.clear();
-            setup_in_flight = true;
         }
         bool any_ready = false;
         for (auto& kv : inst)
@@ -1297,10 +1332,7 @@ struct Batch {
                 any_ready = true;
                 break;
             }
-        if (!any_ready && setup_in_flight) {
-            HIP_CHECK(hipEventSynchronize(setup_done));
-            mark_ready();
-        }
+        if (!any_ready && !setups.empty()) retire_done(true);
         if (newly_ready && hi_slot > 0) {   // their set-up states (phase, eta, ...) to the host
             d_state.download(h_state.data(), hi_slot, streams[0]);
             HIP_CHECK(hipStreamSynchronize(streams[0]));
