@@ -223,6 +223,101 @@ __device__ __forceinline__ void tiled_pass(const TileMeta* tiles, int ntiles, co
     }
 }
 
+// tiled_pass for a block of NSUB * BB threads: quarter (sub-block) s sums the tiles
+// s, s + NSUB, ... side by side with the others, each exactly as tiled_pass sums a tile
+// (own LDS products prod[s], own wave sums red[s]), so the row sums are the same bits.
+// Every trip has the two barriers of either tile kind, so sub-blocks with a long row, a
+// short tile or no tile at all stay in step.  Ends synchronised.
+template <int NSUB, class V, class GX, class PRE, class FIN>
+__device__ __forceinline__ void tiled_pass_wide(const TileMeta* tiles, int ntiles, const int32_t* rp,
+                                                const int32_t* col, const V* val, V* prod_all,
+                                                V* red_all, GX gx, PRE pre, FIN fin) {
+    using PreT = decltype(pre(0));
+    const int sub = threadIdx.x / BB, tid = threadIdx.x % BB;
+    V* const prod = prod_all + sub * BT;
+    V* const red = red_all + sub * (BB / WAVE);
+    if (ntiles <= 0) return;
+    int cc[BPT];
+    V vv[BPT];
+    PreT pv{};
+    int lo = 0, hi = 0;
+    auto fetch = [&](const TileMeta& tm) {
+        const int r = tm.r0 + tid;
+        if (r < tm.r1) {
+            pv = pre(r);
+            lo = rp[r] - tm.base;
+            hi = rp[r + 1] - tm.base;
+        }
+        if (tm.cnt > 0) {
+#pragma unroll
+            for (int j = 0; j < BPT; ++j) {
+                const int e = min(tid + j * BB, tm.cnt - 1);
+                cc[j] = col[tm.base + e];
+                vv[j] = val[tm.base + e];
+            }
+        }
+    };
+    TileMeta tm = sub < ntiles ? tiles[sub] : TileMeta{0, 0, 0, 0};
+    if (tm.cnt <= BT) fetch(tm);
+    const int trips = (ntiles + NSUB - 1) / NSUB;
+    for (int k = 0; k < trips; ++k) {
+        const int tnext = (k + 1) * NSUB + sub;
+        const bool more = tnext < ntiles;
+        const TileMeta tn = more ? tiles[tnext] : TileMeta{0, 0, 0, 0};
+        if (tm.cnt > BT) {   // uniform per sub-block
+            const int r = tm.r0;
+            const auto pb = pre(r);
+            V acc = V(0);
+            const int k1 = tm.base + tm.cnt;
+            constexpr int U = 4;
+            for (int kb = tm.base + tid; kb < k1; kb += U * BB) {
+                int c[U];
+                V v[U], xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int kk = min(kb + u * BB, k1 - 1);
+                    c[u] = col[kk];
+                    v[u] = val[kk];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = gx(c[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (kb + u * BB < k1) acc += v[u] * xv[u];
+            }
+            acc = wave_sum(acc);
+            if ((tid & (WAVE - 1)) == 0) red[tid / WAVE] = acc;
+            __syncthreads();
+            if (tid == 0) {
+                V sum = red[0];
+#pragma unroll
+                for (int w = 1; w < BB / WAVE; ++w) sum += red[w];
+                fin(r, sum, pb);
+            }
+            if (more && tn.cnt <= BT) fetch(tn);
+            __syncthreads();
+            tm = tn;
+            continue;
+        }
+        const int r = tm.r0 + tid;
+        const bool own = r < tm.r1;
+        if (tm.cnt > 0) {   // uniform per sub-block
+            V xv[BPT];
+#pragma unroll
+            for (int j = 0; j < BPT; ++j) xv[j] = gx(cc[j]);
+#pragma unroll
+            for (int j = 0; j < BPT; ++j) prod[tid + j * BB] = vv[j] * xv[j];
+        }
+        __syncthreads();
+        const PreT pc = pv;
+        const int l0 = lo, h0 = hi;
+        if (more && tn.cnt <= BT) fetch(tn);
+        if (own) fin(r, seq_sum(prod, l0, h0), pc);
+        __syncthreads();
+        tm = tn;
+    }
+}
+
 // K products over the rows of a tiled CSR, accumulated in FP64 (the KKT check): per tile
 // the (col, val) pairs are loaded coalesced once, then for each vector kv the gathers
 // gx(kv, col) are issued together, the products go to LDS and each thread sums its row
@@ -564,12 +659,15 @@ struct PreDual {
 // check_every PDHG steps (pdlp.jl:96-124 with update_average!, :126-136; pdhg.jl:76-102)
 // on each listed instance.  LDSG: y and xbar live in LDS (dynamic shared memory of
 // (n + m) * sizeof(V) bytes) and the gathers read them there; else from global memory.
-template <class V, bool LDSG>
-__global__ __launch_bounds__(BB) void k_batch_steps(const int32_t* list, const BInst* insts,
-                                                    BState* states, BParams P) {
+// NSUB > 1: a block of NSUB * BB threads per instance, NSUB tiles at a time (large
+// instances, whose single workgroup would otherwise set the batch's tail).
+template <class V, bool LDSG, int NSUB = 1>
+__global__ __launch_bounds__(NSUB * BB) void k_batch_steps(const int32_t* list, const BInst* insts,
+                                                           BState* states, BParams P) {
+    constexpr int NT = NSUB * BB;
     extern __shared__ __align__(16) unsigned char dyn[];
-    __shared__ V prod[BT];
-    __shared__ V red[BB / WAVE];
+    __shared__ V prod[NSUB * BT];
+    __shared__ V red[NSUB * (BB / WAVE)];
     const int slot = list[blockIdx.x];
     const BInst I = insts[slot];
     BState* S = states + slot;
@@ -585,7 +683,7 @@ __global__ __launch_bounds__(BB) void k_batch_steps(const int32_t* list, const B
     int cur = S->cur;
     if (LDSG) {   // y of the current solution into LDS
         const V* yc = cur ? it.y[1] : it.y[0];
-        for (int i = tid; i < m; i += BB) ys[i] = yc[i];
+        for (int i = tid; i < m; i += NT) ys[i] = yc[i];
     }
     __syncthreads();
     const double tau = eta / omega, sigma = eta * omega, isig = 1.0 / sigma;
@@ -609,8 +707,12 @@ __global__ __launch_bounds__(BB) void k_batch_steps(const int32_t* list, const B
         const double wa = eta / (eta + eta_sum), wb = eta_sum / (eta + eta_sum);
         const V wav = (V)wa, wbv = (V)wb;
         const bool last = s == P.check_every - 1;
+        auto pass = [&](auto... a) {
+            if constexpr (NSUB == 1) tiled_pass<V>(a...);
+            else tiled_pass_wide<NSUB, V>(a...);
+        };
         // primal half-step over the rows of A'
-        tiled_pass<V>(
+        pass(
             at_tiles, nat_t, at_rp, at_col, atval, prod, red,
             [&](int c) { return LDSG ? ys[c] : yp[c]; },
             [&](int j) {
@@ -634,7 +736,7 @@ __global__ __launch_bounds__(BB) void k_batch_steps(const int32_t* list, const B
                 }
             });
         // dual half-step over the rows of A
-        tiled_pass<V>(
+        pass(
             a_tiles, na_t, a_rp, a_col, aval, prod, red,
             [&](int c) { return LDSG ? xbs[c] : xbar_g[c]; },
             [&](int i) {
@@ -1121,6 +1223,11 @@ struct Batch {
     int64_t next_id = 0;
     int32_t hi_slot = 0;   // slots [0, hi_slot) may be in use
     size_t lds_max = 65536;  // dynamic LDS a workgroup may take (device limit minus static use)
+    // wide workgroups (WIDE * BB threads) for instances of at least wide_nnz nonzeros;
+    // their kernel's static LDS is WIDE times larger, so their budget is per phase
+    static constexpr int WIDE = 4;
+    int32_t wide_nnz = 16384;
+    size_t wide_lds_max[2] = {0, 0};
 
     Batch(const cpdlp_params& p, int64_t bytes) : P(p) {
         int ndev = 0;
@@ -1143,6 +1250,11 @@ struct Batch {
         // instances whose y and xbar need more LDS than this gather from global memory
         // (L2) instead: more workgroups fit on a CU
         if (const char* e = std::getenv("CPDLP_BATCH_LDS_BYTES")) lds_max = std::min<size_t>(lds_max, std::strtoull(e, nullptr, 10));
+        for (int ph = 0; ph < 2; ++ph) {
+            const size_t st = (size_t)WIDE * (BT + BB / WAVE) * (ph ? 8 : 4) + 4096;
+            wide_lds_max[ph] = std::min(lds_max, (size_t)lds > st ? (size_t)lds - st : 0);
+        }
+        if (const char* e = std::getenv("CPDLP_BATCH_WIDE_NNZ")) wide_nnz = (int32_t)std::strtol(e, nullptr, 10);
         max_slots = 1 << 16;
         d_inst.alloc(max_slots);
         d_state.alloc(max_slots);
@@ -1352,12 +1464,17 @@ struct Batch {
             d_timeout.upload(h_timeout.data(), hi_slot, streams[0]);
             HIP_CHECK(hipStreamSynchronize(streams[0]));
         }
-        // 3. one launch per (phase, LDS class); an instance's y and xbar take
-        //    (n + m) * sizeof(V) bytes of LDS, in the smallest class that holds them
-        constexpr int NCLS = 5;
-        const size_t lds_bytes[NCLS] = {std::min<size_t>(16384, lds_max), std::min<size_t>(40960, lds_max),
-                                        std::min<size_t>(98304, lds_max), lds_max, 0};
+        // 3. one launch per (phase, class); an instance's y and xbar take
+        //    (n + m) * sizeof(V) bytes of LDS, in the smallest class that holds them.
+        //    Classes [0, NLDS + 1): one workgroup of BB threads per instance, LDS budgets
+        //    lds_bytes, then gathers from L2; classes NCLS - 2 and NCLS - 1: instances with
+        //    at least wide_nnz nonzeros, WIDE * BB threads each, vectors in LDS when they
+        //    fit the wide kernel's budget, else gathers from L2.
+        constexpr int NLDS = 4, NCLS = NLDS + 3, CW_LDS = NCLS - 2, CW_GLB = NCLS - 1;
+        const size_t lds_bytes[NLDS + 1] = {std::min<size_t>(16384, lds_max), std::min<size_t>(40960, lds_max),
+                                            std::min<size_t>(98304, lds_max), lds_max, 0};
         std::vector<int32_t> cls[2][NCLS];
+        size_t wide_lds[2] = {0, 0};   // dynamic LDS of each phase's wide LDS launch
         for (auto& kv : inst) {
             const BHost& h = kv.second;
             if (h.finished || !h.ready) continue;
@@ -1365,7 +1482,12 @@ struct Batch {
             const int ph = st.phase;
             const size_t lds = (size_t)(h.n + h.m) * (ph ? 8 : 4);
             int c = 0;
-            while (c < NCLS - 1 && lds > lds_bytes[c]) ++c;   // last class: gathers from L2
+            if (h.nnz >= wide_nnz) {
+                c = lds <= wide_lds_max[ph] ? CW_LDS : CW_GLB;
+                if (c == CW_LDS) wide_lds[ph] = std::max(wide_lds[ph], lds);
+            } else {
+                while (c < NLDS && lds > lds_bytes[c]) ++c;   // class NLDS: gathers from L2
+            }
             cls[ph][c].push_back(h.slot);
         }
         // largest instances first: workgroups are dispatched in list order, so the long
@@ -1429,23 +1551,35 @@ struct Batch {
                         if (cnt <= 0) continue;
                         const hipStream_t s = gs[g];
                         const int32_t* L = d_list.p + sp.first + start[g];
-                        const dim3 grid(cnt), blk(BB);
+                        const dim3 grid(cnt), blk(BB), wblk(WIDE * BB);
                         if (ph == 0) {
-                            if (c < NCLS - 1)
+                            if (c < NLDS)
                                 hipLaunchKernelGGL((k_batch_steps<float, true>), grid, blk, lds_bytes[c], s, L,
                                                    d_inst.p, d_state.p, bp);
-                            else
+                            else if (c == NLDS)
                                 hipLaunchKernelGGL((k_batch_steps<float, false>), grid, blk, 0, s, L, d_inst.p,
                                                    d_state.p, bp);
+                            else if (c == CW_LDS)
+                                hipLaunchKernelGGL((k_batch_steps<float, true, WIDE>), grid, wblk, wide_lds[0], s,
+                                                   L, d_inst.p, d_state.p, bp);
+                            else
+                                hipLaunchKernelGGL((k_batch_steps<float, false, WIDE>), grid, wblk, 0, s, L,
+                                                   d_inst.p, d_state.p, bp);
                             hipLaunchKernelGGL(k_batch_check<float>, grid, blk, 0, s, L, d_inst.p, d_state.p,
                                                d_timeout.p, bp);
                         } else {
-                            if (c < NCLS - 1)
+                            if (c < NLDS)
                                 hipLaunchKernelGGL((k_batch_steps<double, true>), grid, blk, lds_bytes[c], s, L,
                                                    d_inst.p, d_state.p, bp);
-                            else
+                            else if (c == NLDS)
                                 hipLaunchKernelGGL((k_batch_steps<double, false>), grid, blk, 0, s, L, d_inst.p,
                                                    d_state.p, bp);
+                            else if (c == CW_LDS)
+                                hipLaunchKernelGGL((k_batch_steps<double, true, WIDE>), grid, wblk, wide_lds[1], s,
+                                                   L, d_inst.p, d_state.p, bp);
+                            else
+                                hipLaunchKernelGGL((k_batch_steps<double, false, WIDE>), grid, wblk, 0, s, L,
+                                                   d_inst.p, d_state.p, bp);
                             hipLaunchKernelGGL(k_batch_check<double>, grid, blk, 0, s, L, d_inst.p, d_state.p,
                                                d_timeout.p, bp);
                         }

@@ -140,3 +140,34 @@ def test_batch_long_rows_and_columns(cp):
     assert abs(st_b.eta - st_p.eta) <= 1e-9 * st_p.eta
     for sol in (sol_p, sol_b):
         assert abs(float(g.c @ sol.x) - obj) <= 1e-4 * (1 + abs(obj))
+
+
+def test_wide_workgroups_bit_identical(cp, monkeypatch):
+    """Large instances get workgroups of 4 x 256 threads summing four tiles side by side.
+    Forcing every instance onto them (CPDLP_BATCH_WIDE_NNZ=1) gives the same bits as the
+    256-thread kernels: same passes, same iterates, in FP32 and FP64, for tiles of short
+    rows and for rows longer than a tile (the whole-sub-block path)."""
+    import scipy.sparse as sp
+    lps = _mix(10, seed=21)
+    rng = np.random.default_rng(5)
+    m, n = 400, 3000
+    A = sp.random(m, n, density=0.003, format="lil", random_state=rng)
+    A[0, :] = rng.standard_normal(n)            # a row of 3,000 nonzeros
+    A[:, 2] = rng.standard_normal((m, 1))
+    A = sp.csc_matrix(A)
+    x0 = np.where(rng.random(n) < 0.6, rng.random(n), 0.0)
+    y0 = rng.standard_normal(m)
+    s0 = np.where(x0 > 0, 0.0, rng.random(n))
+    b = A @ x0
+    lps.append(G.GeneratedLP(m, n, A.indptr.astype(np.int64), A.indices.astype(np.int32),
+                             A.data.copy(), A.T @ y0 + s0, np.zeros(n), np.full(n, np.inf), b, b))
+    algo = cp.PDLP("mixed", termination_reltol=1e-6, max_kkt_passes=200_000)
+    with cp.BatchSolver(algo, hbm_bytes=256 << 20) as B:
+        narrow = B.solve_all(lps, checks_per_round=6)
+    monkeypatch.setenv("CPDLP_BATCH_WIDE_NNZ", "1")
+    with cp.BatchSolver(algo, hbm_bytes=256 << 20) as B:
+        wide = B.solve_all(lps, checks_per_round=6)
+    for (s1, t1), (s2, t2) in zip(narrow, wide):
+        assert t1.termination_status == t2.termination_status == cp.OPTIMAL
+        assert t1.kkt_passes == t2.kkt_passes and t1.fp32_passes == t2.fp32_passes
+        assert np.array_equal(s1.x, s2.x) and np.array_equal(s1.y, s2.y)
