@@ -66,10 +66,12 @@ struct BParams {
 constexpr int BB = 256;   // threads per instance workgroup
 constexpr int STALL = 10;
 
-// block-wide sum of NQ doubles, result broadcast through sh[NQ] (deterministic order)
-template <int NQ>
+// block-wide sum of NQ doubles, result broadcast through sh[NQ] (deterministic order).
+// For a block of NT > BB threads whose threads past BB hold zeros, the result is the
+// 256-thread block's bit for bit: their waves add +0.0 to a sum that started at +0.0.
+template <int NQ, int NT = BB>
 __device__ __forceinline__ void bsum(double (&q)[NQ], double* sh) {
-    block_sum<NQ>(q, sh);
+    block_sum<NQ, NT>(q, sh);
     __syncthreads();
 }
 
@@ -396,12 +398,18 @@ __device__ __forceinline__ void tiled_pass_k(const TileMeta* tiles, int ntiles, 
 }
 
 // ---------------------------------------------------------------- setup kernel
-// preprocess + initialize for each listed instance (one workgroup each)
-__global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const BInst* insts,
-                                                    BState* states, BParams P) {
+// preprocess + initialize for each listed instance (one workgroup each).  NSUB > 1: a
+// block of NSUB * BB threads (large instances).  Element-wise and per-row work is spread
+// over all threads; the power method's products run NSUB tiles side by side (the same
+// row sums); every sum over a vector is accumulated by the first BB threads in the
+// 256-thread block's order -- so the result is the same bits for any NSUB.
+template <int NSUB = 1>
+__global__ __launch_bounds__(NSUB * BB) void k_batch_setup(const int32_t* list, const BInst* insts,
+                                                           BState* states, BParams P) {
+    constexpr int NT = NSUB * BB;
     __shared__ double sh[8];
-    __shared__ double prod[BT];
-    __shared__ double red[BB / WAVE];
+    __shared__ double prod[NSUB * BT];
+    __shared__ double red[NSUB * (BB / WAVE)];
     const int slot = list[blockIdx.x];
     const BInst I = insts[slot];
     BState* S = states + slot;
@@ -409,21 +417,21 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     double* d1 = I.sm;
     double* d2 = I.sn;
     // identity + working copies (CSC order: A values and A' values)
-    for (int i = tid; i < m; i += BB) I.D1[i] = 1.0;
-    for (int j = tid; j < n; j += BB) I.D2[j] = 1.0;
-    for (int k = tid; k < nnz; k += BB) {
+    for (int i = tid; i < m; i += NT) I.D1[i] = 1.0;
+    for (int j = tid; j < n; j += NT) I.D2[j] = 1.0;
+    for (int k = tid; k < nnz; k += NT) {
         I.acsc[k] = I.nz[k];
         I.atcsc[k] = I.nz[k];
     }
     __syncthreads();
     auto norms = [&](double p_row, double p_col) {
-        for (int j = tid; j < n; j += BB) {   // columns of A = rows of CSR(A')
+        for (int j = tid; j < n; j += NT) {   // columns of A = rows of CSR(A')
             double acc = 0.0;
             for (int k = I.at_rp[j]; k < I.at_rp[j + 1]; ++k) acc = pnorm_acc(acc, I.acsc[k], p_col);
             const double v = pnorm_fin(acc, p_col);
             d2[j] = v == 0.0 ? 1.0 : 1.0 / sqrt(v);
         }
-        for (int i = tid; i < m; i += BB) {   // rows of A, from the A' values
+        for (int i = tid; i < m; i += NT) {   // rows of A, from the A' values
             double acc = 0.0;
             for (int k = I.a_rp[i]; k < I.a_rp[i + 1]; ++k) acc = pnorm_acc(acc, I.atcsc[I.perm[k]], p_row);
             const double v = pnorm_fin(acc, p_row);
@@ -433,7 +441,7 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     };
     // (d1_i * a) * d2_j into A, (d2_j * a') * d1_i into A'  (preconditioning.jl:37-43)
     auto apply = [&](const double* ain, const double* atin, const double* D1, const double* D2) {
-        for (int j = tid; j < n; j += BB) {
+        for (int j = tid; j < n; j += NT) {
             const double dj = D2[j];
             for (int k = I.at_rp[j]; k < I.at_rp[j + 1]; ++k) {
                 const double di = D1[I.at_col[k]];
@@ -451,20 +459,20 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     auto inf_norms = [&]() {
         unsigned long long* b1 = (unsigned long long*)d1;
         unsigned long long* b2 = (unsigned long long*)d2;
-        for (int i = tid; i < m; i += BB) b1[i] = 0ull;
-        for (int j = tid; j < n; j += BB) b2[j] = 0ull;
+        for (int i = tid; i < m; i += NT) b1[i] = 0ull;
+        for (int j = tid; j < n; j += NT) b2[j] = 0ull;
         __syncthreads();
-        for (int k = tid; k < nnz; k += BB) {
+        for (int k = tid; k < nnz; k += NT) {
             const double a = fabs(I.acsc[k]), at = fabs(I.atcsc[k]);
             if (a == a) atomicMax(&b2[I.ecol[k]], (unsigned long long)__double_as_longlong(a));
             if (at == at) atomicMax(&b1[I.at_col[k]], (unsigned long long)__double_as_longlong(at));
         }
         __syncthreads();
-        for (int i = tid; i < m; i += BB) {
+        for (int i = tid; i < m; i += NT) {
             const double v = __longlong_as_double((long long)b1[i]);
             d1[i] = v == 0.0 ? 1.0 : 1.0 / sqrt(v);
         }
-        for (int j = tid; j < n; j += BB) {
+        for (int j = tid; j < n; j += NT) {
             const double v = __longlong_as_double((long long)b2[j]);
             d2[j] = v == 0.0 ? 1.0 : 1.0 / sqrt(v);
         }
@@ -472,7 +480,7 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     };
     // (d1_i * a) * d2_j into A, (d2_j * a') * d1_i into A', entry by entry
     auto apply_flat = [&](const double* D1v, const double* D2v) {
-        for (int k = tid; k < nnz; k += BB) {
+        for (int k = tid; k < nnz; k += NT) {
             const double di = D1v[I.at_col[k]], dj = D2v[I.ecol[k]];
             I.acsc[k] = (di * I.acsc[k]) * dj;
             I.atcsc[k] = (dj * I.atcsc[k]) * di;
@@ -482,24 +490,24 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     for (int it = 0; it < P.ruiz_iter; ++it) {   // ruiz_preconditioner, :117-125
         inf_norms();
         apply_flat(d1, d2);
-        for (int i = tid; i < m; i += BB) I.D1[i] = d1[i] * I.D1[i];
-        for (int j = tid; j < n; j += BB) I.D2[j] = I.D2[j] * d2[j];
+        for (int i = tid; i < m; i += NT) I.D1[i] = d1[i] * I.D1[i];
+        for (int j = tid; j < n; j += NT) I.D2[j] = I.D2[j] * d2[j];
         __syncthreads();
     }
     apply(I.nz, I.nz, I.D1, I.D2);   // cons_r from the originals
     norms(2.0 - P.cp_alpha, P.cp_alpha);
-    for (int i = tid; i < m; i += BB) I.D1[i] = I.D1[i] * d1[i];
-    for (int j = tid; j < n; j += BB) I.D2[j] = d2[j] * I.D2[j];
+    for (int i = tid; i < m; i += NT) I.D1[i] = I.D1[i] * d1[i];
+    for (int j = tid; j < n; j += NT) I.D2[j] = d2[j] * I.D2[j];
     __syncthreads();
     apply(I.nz, I.nz, I.D1, I.D2);   // A_p = D1*A*D2, At_p = D2*At*D1
-    for (int k = tid; k < nnz; k += BB) {
+    for (int k = tid; k < nnz; k += NT) {
         const double a = I.acsc[I.perm[k]];   // CSR(A) order
         I.a64[k] = a;
         I.a32[k] = (float)a;
         I.at64[k] = I.atcsc[k];
         I.at32[k] = (float)I.atcsc[k];
     }
-    for (int j = tid; j < n; j += BB) {
+    for (int j = tid; j < n; j += NT) {
         const double d = I.D2[j];
         I.cs[j] = d * I.c[j];
         I.lvs[j] = I.lv[j] / d;
@@ -509,7 +517,7 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
         I.lv32[j] = (float)I.lvs[j];
         I.uv32[j] = (float)I.uvs[j];
     }
-    for (int i = tid; i < m; i += BB) {
+    for (int i = tid; i < m; i += NT) {
         const double d = I.D1[i];
         I.lcs[i] = d * I.lc[i];
         I.ucs[i] = d * I.uc[i];
@@ -521,18 +529,18 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     // constant KKT scales and the primal_weight_init norms
     {
         double q[4] = {0, 0, 0, 0};
-        for (int i = tid; i < m; i += BB) {
+        for (int i = tid; i < m && tid < BB; i += BB) {
             const double cb = combine(I.lcs[i], I.ucs[i]);
             const double v = I.iD1[i] * cb;
             q[0] += v * v;
             q[2] += cb * cb;
         }
-        for (int j = tid; j < n; j += BB) {
+        for (int j = tid; j < n && tid < BB; j += BB) {
             const double v = I.iD2[j] * I.cs[j];
             q[1] += v * v;
             q[3] += I.cs[j] * I.cs[j];
         }
-        bsum<4>(q, sh);
+        bsum<4, NT>(q, sh);
     }
     const double primal_scale = 1.0 + sqrt(sh[0]), dual_scale = 1.0 + sqrt(sh[1]);
     const double comb_norm = sqrt(sh[2]), c_norm = sqrt(sh[3]);
@@ -541,7 +549,7 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     double* x = I.x64[0];
     double* Ax = I.x64[1];
     double* t = I.y64[0];
-    for (int j = tid; j < n; j += BB) {
+    for (int j = tid; j < n; j += NT) {
         double acc = 0.0;
         for (int r = 0; r < 12; ++r) {
             const uint64_t h = b_splitmix64(P.seed * 0x100000001B3ull ^ b_splitmix64((uint64_t)j * 16u + (uint64_t)r));
@@ -552,13 +560,13 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     __syncthreads();
     {
         double q[1] = {0.0};
-        for (int j = tid; j < n; j += BB) q[0] += x[j] * x[j];
-        bsum<1>(q, sh);
+        for (int j = tid; j < n && tid < BB; j += BB) q[0] += x[j] * x[j];
+        bsum<1, NT>(q, sh);
     }
     {
         const double inv = 1.0 / sqrt(sh[0]);
         __syncthreads();
-        for (int j = tid; j < n; j += BB) x[j] = x[j] * inv;
+        for (int j = tid; j < n; j += NT) x[j] = x[j] * inv;
     }
     __syncthreads();
     const uint64_t u = (uint64_t)n;
@@ -567,29 +575,34 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     double theta = 0.0, residual = 1.7976931348623157e308;
     int64_t it = 0;
     while (!(it > maxiter || residual <= tol * fabs(theta))) {
-        tiled_pass<double>(I.a_tiles, I.n_a_tiles, I.a_rp, I.a_col, I.a64, prod, red,
-                           [&](int c) { return x[c]; }, [&](int) { return 0; },
-                           [&](int r, double acc, int) { t[r] = acc; });
-        // Ax = A't, with x.Ax and Ax.Ax accumulated by the row's owner as it is formed
+        auto pass = [&](auto... a) {
+            if constexpr (NSUB == 1) tiled_pass<double>(a...);
+            else tiled_pass_wide<NSUB, double>(a...);
+        };
+        pass(I.a_tiles, I.n_a_tiles, I.a_rp, I.a_col, I.a64, prod, red,
+             [&](int c) { return x[c]; }, [&](int) { return 0; },
+             [&](int r, double acc, int) { t[r] = acc; });
+        // Ax = A't; then x.Ax and Ax.Ax
+        pass(I.at_tiles, I.n_at_tiles, I.at_rp, I.at_col, I.at64, prod, red,
+             [&](int c) { return t[c]; }, [&](int) { return 0; },
+             [&](int r, double acc, int) { Ax[r] = acc; });
         double q[2] = {0.0, 0.0};
-        tiled_pass<double>(I.at_tiles, I.n_at_tiles, I.at_rp, I.at_col, I.at64, prod, red,
-                           [&](int c) { return t[c]; }, [&](int j) { return x[j]; },
-                           [&](int r, double acc, double xr) {
-                               Ax[r] = acc;
-                               q[0] += xr * acc;
-                               q[1] += acc * acc;
-                           });
-        bsum<2>(q, sh);
+        for (int j = tid; j < n && tid < BB; j += BB) {
+            const double a = Ax[j];
+            q[0] += x[j] * a;
+            q[1] += a * a;
+        }
+        bsum<2, NT>(q, sh);
         theta = sh[0];
         const double inv = 1.0 / sqrt(sh[1]);
         __syncthreads();
         double q2[1] = {0.0};
-        for (int j = tid; j < n; j += BB) {
+        for (int j = tid; j < n && tid < BB; j += BB) {
             const double r = Ax[j] + (-theta) * x[j];
             q2[0] += r * r;
             x[j] = Ax[j] * inv;
         }
-        bsum<1>(q2, sh);
+        bsum<1, NT>(q2, sh);
         residual = sqrt(sh[0]);
         __syncthreads();
         ++it;
@@ -598,11 +611,11 @@ __global__ __launch_bounds__(BB) void k_batch_setup(const int32_t* list, const B
     double omega = 1.0;
     if (P.pdlp && c_norm > P.zero_tol && comb_norm > P.zero_tol) omega = c_norm / comb_norm;
     // initial iterates (zero start): sol = 0, sol_last = 0, avg = sol, avg_last = 0, restart = sol
-    for (int j = tid; j < n; j += BB) {
+    for (int j = tid; j < n; j += NT) {
         I.x32[0][j] = I.x32[1][j] = I.xa32[j] = I.xal32[j] = I.xr32[j] = I.xb32[j] = 0.f;
         I.x64[0][j] = I.x64[1][j] = I.xa64[j] = I.xal64[j] = I.xr64[j] = 0.0;
     }
-    for (int i = tid; i < m; i += BB) {
+    for (int i = tid; i < m; i += NT) {
         I.y32[0][i] = I.y32[1][i] = I.ya32[i] = I.yal32[i] = I.yr32[i] = I.yb32[i] = 0.f;
         I.y64[0][i] = I.y64[1][i] = I.ya64[i] = I.yal64[i] = I.yr64[i] = 0.0;
     }
@@ -1228,6 +1241,7 @@ struct Batch {
     static constexpr int WIDE = 4;
     int32_t wide_nnz = 16384;
     size_t wide_lds_max[2] = {0, 0};
+    int cus = 256;   // compute units of the device
 
     Batch(const cpdlp_params& p, int64_t bytes) : P(p) {
         int ndev = 0;
@@ -1255,6 +1269,7 @@ struct Batch {
             wide_lds_max[ph] = std::min(lds_max, (size_t)lds > st ? (size_t)lds - st : 0);
         }
         if (const char* e = std::getenv("CPDLP_BATCH_WIDE_NNZ")) wide_nnz = (int32_t)std::strtol(e, nullptr, 10);
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
         max_slots = 1 << 16;
         d_inst.alloc(max_slots);
         d_state.alloc(max_slots);
@@ -1428,8 +1443,16 @@ struct Batch {
                 c.list.alloc(c.slots.size());
                 HIP_CHECK(hipMemcpyAsync(c.list.p, c.slots.data(), c.slots.size() * 4,
                                          hipMemcpyHostToDevice, setup_stream));
-                hipLaunchKernelGGL(k_batch_setup, dim3((int)c.slots.size()), dim3(BB), 0, setup_stream,
-                                   c.list.p, d_inst.p, d_state.p, bp);
+                // wide workgroups for the large instances, which are at the end of the list
+                int n_narrow = 0;
+                while (n_narrow < (int)c.slots.size() && h_inst[c.slots[n_narrow]].nnz < wide_nnz) ++n_narrow;
+                const int n_wide = (int)c.slots.size() - n_narrow;
+                if (n_narrow > 0)
+                    hipLaunchKernelGGL(k_batch_setup<1>, dim3(n_narrow), dim3(BB), 0, setup_stream, c.list.p,
+                                       d_inst.p, d_state.p, bp);
+                if (n_wide > 0)
+                    hipLaunchKernelGGL(k_batch_setup<WIDE>, dim3(n_wide), dim3(WIDE * BB), 0, setup_stream,
+                                       c.list.p + n_narrow, d_inst.p, d_state.p, bp);
                 HIP_CHECK(hipGetLastError());
                 HIP_CHECK(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
                 HIP_CHECK(hipEventRecord(c.ev, setup_stream));
@@ -1475,6 +1498,11 @@ struct Batch {
                                             std::min<size_t>(98304, lds_max), lds_max, 0};
         std::vector<int32_t> cls[2][NCLS];
         size_t wide_lds[2] = {0, 0};   // dynamic LDS of each phase's wide LDS launch
+        // once fewer than two instances per CU are left, the GPU has room to spare: half
+        // the threshold then (the results are the same bits either way)
+        int64_t n_active = 0;
+        for (auto& kv : inst) n_active += !kv.second.finished && kv.second.ready;
+        const int32_t wide_min = n_active <= 2 * (int64_t)cus ? wide_nnz / 2 : wide_nnz;
         for (auto& kv : inst) {
             const BHost& h = kv.second;
             if (h.finished || !h.ready) continue;
@@ -1482,7 +1510,7 @@ struct Batch {
             const int ph = st.phase;
             const size_t lds = (size_t)(h.n + h.m) * (ph ? 8 : 4);
             int c = 0;
-            if (h.nnz >= wide_nnz) {
+            if (h.nnz >= wide_min) {
                 c = lds <= wide_lds_max[ph] ? CW_LDS : CW_GLB;
                 if (c == CW_LDS) wide_lds[ph] = std::max(wide_lds[ph], lds);
             } else {

@@ -246,10 +246,10 @@ __device__ __forceinline__ T seq_sum(const T* p, int lo, int hi) {
 
 // Deterministic block reduction of NQ doubles per thread; thread q < NQ of the block
 // returns the block total of quantity q (others: unspecified).  Fixed order: butterfly
-// within each wave, then the 4 wave sums in wave order.
-template <int NQ>
+// within each wave, then the THREADS / 64 wave sums in wave order.
+template <int NQ, int THREADS = BLOCK>
 __device__ __forceinline__ void block_sum(double (&q)[NQ], double* out_partial) {
-    __shared__ double red[BLOCK / WAVE][NQ > 0 ? NQ : 1];
+    __shared__ double red[THREADS / WAVE][NQ > 0 ? NQ : 1];
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
@@ -260,7 +260,7 @@ __device__ __forceinline__ void block_sum(double (&q)[NQ], double* out_partial) 
     if ((int)threadIdx.x < NQ) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < BLOCK / WAVE; ++w) s += red[w][threadIdx.x];
+        for (int w = 0; w < THREADS / WAVE; ++w) s += red[w][threadIdx.x];
         out_partial[threadIdx.x] = s;
     }
 }
