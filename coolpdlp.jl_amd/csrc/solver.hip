@@ -762,12 +762,14 @@ void Handle::prepare() {
     if (d_sums.n == 0) d_sums.alloc(64);
     scale();
     choose_formats();
-    // power method in T: Float32 semantics only for the all-FP32 precision
+    // power method in T: Float32 semantics for the all-FP32 precision.  Mixed precision
+    // runs it on the FP32 matrix too when the tolerance is coarse enough (>= 1e-4
+    // relative) for FP32 products to reach it, else in FP64.
     const bool t32 = P.precision == CPDLP_PREC_FP32;
     const double eps = t32 ? (double)FLT_EPSILON : DBL_EPSILON;
     const double tol = std::isnan(P.powm_tol) ? eps * (double)wrap_cube(n) : P.powm_tol;
     const int64_t maxiter = P.powm_maxiter < 0 ? n : P.powm_maxiter;
-    if (t32)
+    if (t32 || (P.precision == CPDLP_PREC_MIXED && tol >= 1e-4))
         power_method<float>(tol, maxiter);
     else
         power_method<double>(tol, maxiter);

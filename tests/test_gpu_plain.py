@@ -245,16 +245,18 @@ def test_power_law_lp_formats(cp, fmt):
     assert abs(obj - float(ref.x @ g.c)) <= 1e-4 * (1 + abs(g.objective))
 
 
-def test_min_cost_flow_mixed_vs_highs(cp):
+@pytest.mark.parametrize("powm_tol", [1e-6, 1e-4])
+def test_min_cost_flow_mixed_vs_highs(cp, powm_tol):
     """Mixed precision (FP32 bulk, FP64 checks + polish) on a small min-cost flow:
-    objective within 1e-5 relative of HiGHS's optimum, flow conservation to 1e-5."""
+    objective within 1e-5 relative of HiGHS's optimum, flow conservation to 1e-5.  With
+    powm_tol 1e-4 the power method runs on the FP32 matrix."""
     from scipy.optimize import linprog
     g = G.min_cost_flow_lp(nodes=2000, out_degree=8, seed=5)
     A = g.csc()
     h = linprog(g.c, A_eq=A, b_eq=g.lc, bounds=list(zip(g.lv, g.uv)), method="highs")
     assert h.status == 0
     milp = cp.MILP.from_generated(g)
-    sol, st = cp.solve(milp, cp.PDLP("mixed", termination_reltol=1e-6, powm_tol=1e-6,
+    sol, st = cp.solve(milp, cp.PDLP("mixed", termination_reltol=1e-6, powm_tol=powm_tol,
                                      max_kkt_passes=2_000_000))
     assert st.termination_status == cp.OPTIMAL
     assert st.fp32_passes > 0
