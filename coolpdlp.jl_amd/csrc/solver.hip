@@ -175,15 +175,40 @@ __global__ void k_scale_sums(int32_t m, int32_t n, const double* lc, const doubl
     block_sum<4>(q, partials + (size_t)blockIdx.x * 4);
 }
 
-// deterministic final reduction of partials[nb][nq] -> out[nq] (one block)
+// power-method step finished by the reduction that produces its sums (POW_FIN1: theta =
+// x.Ax and |Ax| from the two sums; POW_FIN2: residual, iteration count and convergence)
+enum : int32_t { POW_NONE = 0, POW_FIN1 = 1, POW_FIN2 = 2 };
+__device__ __forceinline__ void pow_fin(PowState* st, int32_t fin, const double* sums) {
+    if (fin == POW_FIN1) {
+        st->theta = sums[0];
+        st->nrm = sqrt(sums[1]);
+    } else if (fin == POW_FIN2) {
+        st->residual = sqrt(sums[0]);
+        st->iter += 1;
+        st->done = (st->iter > st->maxiter) || (st->residual <= st->tol * fabs(st->theta));
+    }
+}
+
+// deterministic final reduction of partials[nb][nq] -> out[nq] (one block); each thread
+// adds its partials b, b + BLOCK, ... in that order, eight loads in flight at a time.
+// With st: a no-op once the power method has converged, and pow_fin(fin) at the end.
 __global__ __launch_bounds__(BLOCK) void k_reduce_partials(int32_t nb, int32_t nq,
                                                            const double* partials, double* out,
-                                                           const PowState* st) {
+                                                           PowState* st, int32_t fin) {
     if (st && st->done) return;
     __shared__ double red[BLOCK / WAVE];
+    double first[2] = {0.0, 0.0};
     for (int q = 0; q < nq; ++q) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += BLOCK) s += partials[(size_t)b * nq + q];
+        int b = threadIdx.x;
+        for (; b + 7 * BLOCK < nb; b += 8 * BLOCK) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = partials[(size_t)(b + u * BLOCK) * nq + q];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; b < nb; b += BLOCK) s += partials[(size_t)b * nq + q];
         s = wave_sum(s);
         if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = s;
         __syncthreads();
@@ -191,9 +216,11 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_partials(int32_t nb, int32_t n
             double t = 0.0;
             for (int w = 0; w < BLOCK / WAVE; ++w) t += red[w];
             out[q] = t;
+            if (q < 2) first[q] = t;
         }
         __syncthreads();
     }
+    if (st && threadIdx.x == 0) pow_fin(st, fin, first);
 }
 
 // ---------------------------------------------------------------- power method kernels
@@ -232,11 +259,6 @@ __global__ void k_scale_by_inv_sqrt(int32_t n, V* x, const double* sumsq) {
     x[s] = x[s] * inv;
 }
 // theta and |Ax| from the sums (one thread)
-__global__ void k_pow_fin1(PowState* st, const double* sums) {
-    if (st->done) return;
-    st->theta = sums[0];
-    st->nrm = sqrt(sums[1]);
-}
 // residual sum of (Ax - theta x)^2 and x <- Ax * inv(|Ax|)
 template <class V>
 __global__ void k_pow_update(int32_t n, V* x, const V* Ax, const PowState* st, double* partials) {
@@ -251,12 +273,6 @@ __global__ void k_pow_update(int32_t n, V* x, const V* Ax, const PowState* st, d
         x[s] = a * inv;
     }
     block_sum<1>(q, partials + blockIdx.x);
-}
-__global__ void k_pow_fin2(PowState* st, const double* sums) {
-    if (st->done) return;
-    st->residual = sqrt(sums[0]);
-    st->iter += 1;
-    st->done = (st->iter > st->maxiter) || (st->residual <= st->tol * fabs(st->theta));
 }
 
 // ---------------------------------------------------------------- restart!
@@ -540,7 +556,7 @@ void Handle::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
 // reduce nb blocks of nq partials on the device and copy the nq sums to the host
 void Handle::reduce_to_host(int nb, int nq, const double* partials, double* host_out) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, stream, nb, nq, partials,
-                       d_sums.p, (const PowState*)nullptr);
+                       d_sums.p, (PowState*)nullptr, POW_NONE);
     d_sums.download(host_out, nq, stream);
     sync();
 }
@@ -651,7 +667,7 @@ void Handle::power_method(double tol, int64_t maxiter) {
                        P.powm_seed, x.p);
     hipLaunchKernelGGL(k_sumsq<V>, dim3(nbr), dim3(BLOCK), 0, s, nn, (const V*)x.p, d_partials.p);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, nbr, 1, d_partials.p,
-                       d_sums.p, (const PowState*)nullptr);
+                       d_sums.p, (PowState*)nullptr, POW_NONE);
     hipLaunchKernelGGL(k_scale_by_inv_sqrt<V>, dim3(gn), dim3(BLOCK), 0, s, nn, x.p,
                        (const double*)d_sums.p);
     const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
@@ -673,13 +689,11 @@ void Handle::power_method(double tol, int64_t maxiter) {
             e2.st = st.p;
             launch_spmv<V, V, 1>(vAt, XPack<V, 1>{{t.p}}, e2, s);
             hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gAt, 2, d_partials.p,
-                               d_sums.p, (const PowState*)st.p);
-            hipLaunchKernelGGL(k_pow_fin1, dim3(1), dim3(1), 0, s, st.p, (const double*)d_sums.p);
+                               d_sums.p, st.p, POW_FIN1);
             hipLaunchKernelGGL(k_pow_update<V>, dim3(nbr), dim3(BLOCK), 0, s, nn, x.p,
                                (const V*)Ax.p, (const PowState*)st.p, d_partials2.p);
             hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, nbr, 1,
-                               d_partials2.p, d_sums.p + 8, (const PowState*)st.p);
-            hipLaunchKernelGGL(k_pow_fin2, dim3(1), dim3(1), 0, s, st.p, (const double*)d_sums.p + 8);
+                               d_partials2.p, d_sums.p + 8, st.p, POW_FIN2);
         }
     };
     for (;;) {
@@ -815,12 +829,12 @@ void Handle::kkt_launch(const V* const* X, const V* const* Y, const V* xr, const
     launch_spmv<V, double, K>(vAt, yp, ec, s);
     if (gA > 0)
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gA, NQR, d_partials.p,
-                           d_sums.p, (const PowState*)nullptr);
+                           d_sums.p, (PowState*)nullptr, POW_NONE);
     else
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQR, d_sums.p, 0.0);
     if (gAt > 0)
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gAt, NQC, d_partials2.p,
-                           d_sums.p + NQR, (const PowState*)nullptr);
+                           d_sums.p + NQR, (PowState*)nullptr, POW_NONE);
     else
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQC, d_sums.p + NQR, 0.0);
     HIP_CHECK(hipGetLastError());
