@@ -267,7 +267,7 @@ def bench_serve(args):
         srv = Server(reltol=reltol, batch_bytes=args.serve_batch_gib << 30,
                      checks_per_round=args.serve_checks, time_limit=args.time_limit,
                      lanes=((1_000_000, args.serve_medium_workers), (float("inf"), 1)),
-                     overflow_queue=args.serve_overflow)
+                     overflow_queue=args.serve_overflow, batch_limit=args.serve_batch_limit)
         t0 = time.perf_counter()
         k = 0
         deadline = t0 + args.serve_duration + args.serve_drain
@@ -326,6 +326,7 @@ def bench_serve(args):
         "data": "synthetic (seeded request pool with known optima)",
         "config": {"workload": "serving_poisson_mix", "qps": args.qps, "duration_s": args.serve_duration,
                    "medium_workers": args.serve_medium_workers, "overflow_queue": args.serve_overflow,
+                   "batch_limit": args.serve_batch_limit,
                    "mix": "70/25/5 small/medium/large", "reltol": reltol,
                    "pool": {k: len(v) for k, v in pool.items()}},
         "points": out_points,
@@ -394,6 +395,7 @@ def main():
     ap.add_argument("--serve-batch-gib", type=int, default=16)
     ap.add_argument("--serve-medium-workers", type=int, default=2)
     ap.add_argument("--serve-overflow", type=int, default=0)
+    ap.add_argument("--serve-batch-limit", type=int, default=8192)
     ap.add_argument("--pool-small", type=int, default=48)
     ap.add_argument("--pool-medium", type=int, default=16)
     ap.add_argument("--pool-large", type=int, default=3)
