@@ -1,0 +1,155 @@
+// Host-only checks of the device layouts built by csrc/layout.hpp (no GPU needed):
+// partition_rows, build_tiles and build_slices on random matrices with long and short
+// rows.  Prints "ok" and exits 0, or prints the first violated invariant and exits 1.
+// Built and run by tests/test_layout.py.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+
+#include "../coolpdlp.jl_amd/csrc/layout.hpp"
+
+using namespace cpdlp;
+
+static void fail(const char* what, long a, long b) {
+    std::printf("FAIL %s (%ld, %ld)\n", what, a, b);
+    std::exit(1);
+}
+
+// a side whose first n_long rows are long (65..maxlong entries), the rest 0..64
+static HostSide make_side(std::mt19937& rng, int rows, int cols, int n_long, int maxlong, bool skew) {
+    HostSide S;
+    S.rows = rows;
+    S.cols = cols;
+    S.n_long = n_long;
+    S.rowptr.assign(rows + 1, 0);
+    std::vector<int> len(rows);
+    for (int r = 0; r < rows; ++r)
+        len[r] = std::min(cols, r < n_long ? 65 + (int)(rng() % (maxlong - 64)) : (int)(rng() % 65));
+    for (int r = 0; r < rows; ++r) S.rowptr[r + 1] = S.rowptr[r] + len[r];
+    S.nnz = S.rowptr[rows];
+    S.col.resize(S.nnz);
+    S.vmap.resize(S.nnz);
+    std::vector<int> pick(cols);
+    for (int r = 0; r < rows; ++r) {
+        for (int j = 0; j < cols; ++j) pick[j] = j;
+        // skew: the first tenth of the columns is drawn 20x as often (a hot head)
+        if (skew) {
+            for (int k = 0; k < len[r]; ++k) {
+                const int lim = (rng() % 3 == 0) ? cols : std::max(1, cols / 10);
+                std::swap(pick[k], pick[k + rng() % (lim > k ? lim - k : 1)]);
+            }
+        } else {
+            std::shuffle(pick.begin(), pick.end(), rng);
+        }
+        std::sort(pick.begin(), pick.begin() + len[r]);
+        for (int k = 0; k < len[r]; ++k) {
+            S.col[S.rowptr[r] + k] = pick[k];
+            S.vmap[S.rowptr[r] + k] = S.rowptr[r] + k;
+        }
+    }
+    return S;
+}
+
+static void check_slices(const HostSide& S, int nr) {
+    const int ns = 8, nl = std::min(S.n_long, nr);
+    HostSlices H;
+    build_slices(S, ns, nr, H);
+    const std::vector<int8_t> so = slice_of_columns(S, ns, nr);
+    // contiguous column ranges, in slice order
+    for (int j = 1; j < S.cols; ++j)
+        if (so[j] < so[j - 1]) fail("slice ranges not contiguous", j, so[j]);
+    // balance: a slice holds at most its share plus the largest column's count
+    std::vector<long> per_col(S.cols, 0), per_slice(ns, 0);
+    for (long k = 0; k < S.rowptr[nr]; ++k) per_col[S.col[k]]++;
+    long maxcol = 0;
+    for (int j = 0; j < S.cols; ++j) {
+        per_slice[so[j]] += per_col[j];
+        maxcol = std::max(maxcol, per_col[j]);
+    }
+    const long share = (S.rowptr[nr] + ns - 1) / ns;
+    for (int s = 0; s < ns; ++s)
+        if (per_slice[s] > share + maxcol) fail("slice over its share", s, per_slice[s]);
+    // every row's parts, in slice order, are the row's entries in their order
+    for (int r = 0; r < nr; ++r) {
+        long at = S.rowptr[r];
+        for (int s = 0; s < ns; ++s) {
+            const int32_t* rp = H.rowptr.data() + (size_t)s * (nr + 1);
+            for (int k = rp[r]; k < rp[r + 1]; ++k, ++at) {
+                if (at >= S.rowptr[r + 1]) fail("row part too long", r, s);
+                if (H.vmap[k] != S.vmap[at] || H.col[k] != S.col[at]) fail("row part order", r, k);
+                if (so[H.col[k]] != s) fail("entry in the wrong slice", r, k);
+            }
+        }
+        if (at != S.rowptr[r + 1]) fail("row part missing entries", r, at);
+    }
+    // tiles: cover [0, nr) per slice, limits, never span the long/short boundary, and the
+    // long rows' tiles are the first tile_n_long of the slice
+    int maxt = 0, maxl = 0;
+    for (int s = 0; s < ns; ++s) {
+        const int32_t* rp = H.rowptr.data() + (size_t)s * (nr + 1);
+        int expect = 0;
+        for (int t = H.tile_off[s]; t < H.tile_off[s + 1]; ++t) {
+            const TileMeta& tm = H.tiles[t];
+            if (tm.r0 != expect || tm.r1 <= tm.r0 || tm.r1 > nr) fail("tile cover", s, t);
+            const int cap = tm.r0 < nl ? BLOCK : SL_RPT * BLOCK;
+            if (tm.r1 - tm.r0 > cap) fail("tile rows", s, t);
+            if (tm.base != rp[tm.r0] || tm.cnt != rp[tm.r1] - rp[tm.r0]) fail("tile base", s, t);
+            if (tm.cnt > TILE_NNZ && tm.r1 - tm.r0 != 1) fail("oversized tile of several rows", s, t);
+            if (tm.r0 < nl && tm.r1 > nl) fail("tile spans the long/short boundary", s, t);
+            const bool in_long_prefix = t - H.tile_off[s] < H.tile_n_long[s];
+            if (in_long_prefix != (tm.r0 < nl)) fail("long tile count", s, t);
+            expect = tm.r1;
+        }
+        if (expect != nr) fail("tiles end early", s, expect);
+        maxt = std::max(maxt, H.tile_off[s + 1] - H.tile_off[s]);
+        maxl = std::max(maxl, H.tile_n_long[s]);
+    }
+    if (maxt != H.max_tiles || maxl != H.max_tiles_long) fail("max tiles", maxt, maxl);
+}
+
+static void check_partition_and_tiles(std::mt19937& rng) {
+    // partition_rows: long rows first, longest first (stable), then the short rows in order
+    const int rows = 5000;
+    std::vector<int32_t> ptr(rows + 1, 0);
+    for (int r = 0; r < rows; ++r) ptr[r + 1] = ptr[r] + (int)(rng() % 10 == 0 ? 60 + rng() % 300 : rng() % 40);
+    std::vector<int32_t> of_int, int_of;
+    int32_t n_long = 0;
+    partition_rows(ptr, 64, of_int, int_of, n_long);
+    auto L = [&](int i) { return ptr[i + 1] - ptr[i]; };
+    for (int r = 0; r < rows; ++r) {
+        if (int_of[of_int[r]] != r) fail("partition not a permutation", r, of_int[r]);
+        if ((r < n_long) != (L(of_int[r]) > 64)) fail("long rows not first", r, L(of_int[r]));
+        if (r > 0 && r < n_long && L(of_int[r]) > L(of_int[r - 1])) fail("long rows not longest first", r, 0);
+        if (r > 0 && r < n_long && L(of_int[r]) == L(of_int[r - 1]) && of_int[r] < of_int[r - 1])
+            fail("long rows not stable", r, 0);
+        if (r > n_long && of_int[r] < of_int[r - 1]) fail("short rows out of order", r, 0);
+    }
+    // build_tiles over the short rows: <= BLOCK rows, <= TILE_NNZ entries, contiguous
+    std::vector<int32_t> rp(rows + 1, 0);
+    for (int r = 0; r < rows; ++r) rp[r + 1] = rp[r] + L(of_int[r]);
+    std::vector<TileMeta> tiles;
+    build_tiles(rp.data(), n_long, rows, tiles);
+    int expect = n_long;
+    for (size_t t = 0; t < tiles.size(); ++t) {
+        const TileMeta& tm = tiles[t];
+        if (tm.r0 != expect || tm.r1 - tm.r0 > BLOCK || tm.cnt > TILE_NNZ) fail("short tile", (long)t, tm.r0);
+        if (tm.base != rp[tm.r0] || tm.cnt != rp[tm.r1] - rp[tm.r0]) fail("short tile base", (long)t, 0);
+        expect = tm.r1;
+    }
+    if (expect != rows) fail("short tiles end early", expect, rows);
+}
+
+int main() {
+    std::mt19937 rng(12345);
+    check_partition_and_tiles(rng);
+    for (int trial = 0; trial < 6; ++trial) {
+        const int rows = 2000 + 700 * trial, cols = 3000 + 1100 * trial;
+        const int n_long = trial == 0 ? 0 : 20 + 15 * trial;
+        const int maxlong = trial == 5 ? 9000 : 2500;   // trial 5: row parts longer than a tile
+        const HostSide S = make_side(rng, rows, cols, n_long, maxlong, trial % 2 == 1);
+        check_slices(S, S.n_long);   // xslice: long rows
+        check_slices(S, S.rows);     // xslice_all: every row
+    }
+    std::printf("ok\n");
+    return 0;
+}
