@@ -9,11 +9,13 @@
 //     MILP.At, src/problems/milp.jl:60);
 //   * the internal layouts used by the kernels: rows of A (resp. A') longer than
 //     `long_row` are moved to the front by a stable partition, and column indices are
-//     relabelled through the other side's partition.  Entries inside each row keep
-//     their original order, so the sequential row sums are unchanged.
+//     relabelled through the other side's partition; the short rows follow in
+//     first-touch order (build_layout).  Entries inside each row keep their original
+//     order, so the sequential row sums are unchanged.
 #pragma once
 #include <algorithm>
 #include <cstdint>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -97,10 +99,10 @@ inline void csc_to_csr(int64_t m, int64_t n, const std::vector<int32_t>& cp,
 
 // partition: long rows (length > long_row) first, longest first (ties by index), so a
 // block of four wavefronts gets rows of similar length and the longest start first;
-// the short rows follow in their original order
+// the short rows follow in their original order, or stably ordered by key[] when given
 inline void partition_rows(const std::vector<int32_t>& ptr, int32_t long_row,
                            std::vector<int32_t>& of_int, std::vector<int32_t>& int_of,
-                           int32_t& n_long) {
+                           int32_t& n_long, const std::vector<int32_t>* key = nullptr) {
     const int64_t rows = (int64_t)ptr.size() - 1;
     of_int.clear();
     of_int.reserve(rows);
@@ -112,6 +114,9 @@ inline void partition_rows(const std::vector<int32_t>& ptr, int32_t long_row,
     n_long = (int32_t)of_int.size();
     for (int64_t i = 0; i < rows; ++i)
         if (ptr[i + 1] - ptr[i] <= long_row) of_int.push_back((int32_t)i);
+    if (key)
+        std::stable_sort(of_int.begin() + n_long, of_int.end(),
+                         [&](int32_t a, int32_t b) { return (*key)[a] < (*key)[b]; });
     int_of.assign(rows, 0);
     for (int64_t r = 0; r < rows; ++r) int_of[of_int[r]] = (int32_t)r;
 }
@@ -174,8 +179,27 @@ inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
     L.rowval = std::move(rv);
     csc_to_csr(m, n, L.colptr, L.rowval, L.csr_rowptr, L.csr_col, L.perm);
     int32_t nlA = 0, nlAt = 0;
-    partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA);
+    // First-touch order of the short rows, in two rounds.  Round 1: A's short rows by the
+    // first internal row of A' (column of A) that holds them; round 2: A's short columns
+    // by the first (new) row of A that holds them, then A's rows again.  Each step kernel
+    // sweeps its rows in order (each XCD a contiguous range of tiles), so the first
+    // gather of most entries of y and x comes in sequence and shares a line with its
+    // neighbours in that XCD's L2.  Long rows and columns stay first, longest first.
+    auto first_touch = [](int64_t outer, const std::vector<int32_t>& ptr, const std::vector<int32_t>& idx,
+                          const std::vector<int32_t>& order_of_outer, int64_t inner) {
+        std::vector<int32_t> first(inner, std::numeric_limits<int32_t>::max());
+        for (int64_t o = 0; o < outer; ++o)
+            for (int32_t k = ptr[o]; k < ptr[o + 1]; ++k)
+                first[idx[k]] = std::min(first[idx[k]], order_of_outer[o]);
+        return first;
+    };
     partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt);
+    std::vector<int32_t> key = first_touch(n, L.colptr, L.rowval, L.int_of_col, m);
+    partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, &key);
+    key = first_touch(m, L.csr_rowptr, L.csr_col, L.int_of_row, n);
+    partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt, &key);
+    key = first_touch(n, L.colptr, L.rowval, L.int_of_col, m);
+    partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, &key);
     build_side(L.A, (int32_t)m, (int32_t)n, L.csr_rowptr, L.csr_col, &L.perm, L.row_of_int,
                &L.int_of_col, nlA);
     build_side(L.At, (int32_t)n, (int32_t)m, L.colptr, L.rowval, nullptr, L.col_of_int,

@@ -74,10 +74,15 @@ struct MatView {
 
     // the sliced format's blocks that run the side's own short tiles
     __host__ __device__ int sl_short_tiles() const { return sl_all ? 0 : n_tiles; }
+    // CSR: the short tiles start at a block index that is a multiple of 8 (blocks
+    // [n_long_blocks, csr_short0()) are idle), and XCD x takes the contiguous tiles
+    // [x * C, (x + 1) * C), C = ceil(n_tiles / 8), in order
+    __host__ __device__ int csr_short0() const { return (n_long_blocks + 7) & ~7; }
+    __host__ __device__ int csr_per_xcd() const { return (n_tiles + 7) / 8; }
     int grid() const {
         if (format == CPDLP_FMT_ELL) return ceil_div(rows, BLOCK);
         if (format == CPDLP_FMT_XSLICE) return ns * sl_max_tiles + sl_short_tiles();
-        return n_long_blocks + n_tiles;
+        return csr_short0() + 8 * csr_per_xcd();
     }
     // blocks that write an epilogue's block partials (NQ per block): the sliced format's
     // short tiles, then its combine blocks
@@ -187,8 +192,13 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X,
             long_row_sums<V, Acc, K>(A.rowptr, A.col, A.val, r, X, acc);
             if ((threadIdx.x & (WAVE - 1)) == 0) epi.row(r, acc, q, pre);
         }
-    } else {
-        const TileMeta tm = A.tiles[blockIdx.x - A.n_long_blocks];
+    } else if ((int)blockIdx.x >= A.csr_short0()) {
+        // XCD blockIdx % 8 walks its own contiguous range of tiles, so entries first
+        // touched by neighbouring tiles share lines in that XCD's L2
+        const int b = (int)blockIdx.x - A.csr_short0();
+        const int t = (b % 8) * A.csr_per_xcd() + b / 8;
+        TileMeta tm{0, 0, 0, 0};
+        if (t < A.n_tiles) tm = A.tiles[t];
         const int r = tm.r0 + (int)threadIdx.x;
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);   // issued before the gathers

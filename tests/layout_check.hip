@@ -1,7 +1,8 @@
 // Host-only checks of the device layouts built by csrc/layout.hpp (no GPU needed):
-// partition_rows, build_tiles and build_slices on random matrices with long and short
-// rows.  Prints "ok" and exits 0, or prints the first violated invariant and exits 1.
-// Built and run by tests/test_layout.py.
+// partition_rows, build_tiles, build_layout and build_slices on random matrices with
+// long and short rows.  Prints "ok" and exits 0, or prints the first violated invariant
+// and exits 1.  Built and run by tests/test_layout.py.
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -139,9 +140,71 @@ static void check_partition_and_tiles(std::mt19937& rng) {
     if (expect != rows) fail("short tiles end early", expect, rows);
 }
 
+// build_layout: both orders are permutations with the long rows (columns) first; A's
+// short rows are in first-touch order of the final column order; every internal row is
+// its source row with columns relabelled, entry order kept (so row sums are unchanged)
+static void check_build_layout(std::mt19937& rng) {
+    const int m = 3000, n = 7000;
+    std::vector<std::vector<int32_t>> colrows(n);
+    for (int j = 0; j < n; ++j) {
+        const int len = j % 97 == 0 ? 80 + (int)(rng() % 200) : (int)(rng() % 12);   // a few long columns
+        std::vector<int32_t> rs;
+        for (int k = 0; k < len; ++k) rs.push_back((int32_t)(rng() % (k % 5 == 0 ? 50 : m)));   // hot rows
+        std::sort(rs.begin(), rs.end());
+        rs.erase(std::unique(rs.begin(), rs.end()), rs.end());
+        colrows[j] = rs;
+    }
+    std::vector<int32_t> cp(n + 1, 0), rv;
+    for (int j = 0; j < n; ++j) {
+        rv.insert(rv.end(), colrows[j].begin(), colrows[j].end());
+        cp[j + 1] = (int32_t)rv.size();
+    }
+    const std::vector<int32_t> cp0 = cp, rv0 = rv;
+    const HostLayout L = build_layout(m, n, std::move(cp), std::move(rv), 64);
+    auto is_perm = [](const std::vector<int32_t>& of, const std::vector<int32_t>& inv, int len) {
+        if ((int)of.size() != len || (int)inv.size() != len) return false;
+        for (int i = 0; i < len; ++i)
+            if (of[i] < 0 || of[i] >= len || inv[of[i]] != i) return false;
+        return true;
+    };
+    if (!is_perm(L.row_of_int, L.int_of_row, m)) fail("row order not a permutation", 0, 0);
+    if (!is_perm(L.col_of_int, L.int_of_col, n)) fail("column order not a permutation", 0, 0);
+    auto rlen = [&](int i) { return L.csr_rowptr[i + 1] - L.csr_rowptr[i]; };
+    auto clen = [&](int j) { return cp0[j + 1] - cp0[j]; };
+    for (int r = 0; r < m; ++r)
+        if ((r < L.A.n_long) != (rlen(L.row_of_int[r]) > 64)) fail("long rows not first", r, 0);
+    for (int c = 0; c < n; ++c)
+        if ((c < L.At.n_long) != (clen(L.col_of_int[c]) > 64)) fail("long columns not first", c, 0);
+    // first touch of each row in the final column order, nondecreasing over the short rows
+    std::vector<int32_t> first(m, INT32_MAX);
+    for (int j = 0; j < n; ++j)
+        for (int k = cp0[j]; k < cp0[j + 1]; ++k) first[rv0[k]] = std::min(first[rv0[k]], L.int_of_col[j]);
+    for (int r = L.A.n_long + 1; r < m; ++r)
+        if (first[L.row_of_int[r]] < first[L.row_of_int[r - 1]]) fail("short rows not in first-touch order", r, 0);
+    // internal rows of A and A' are the source rows, relabelled, in entry order
+    for (int r = 0; r < m; ++r) {
+        const int i = L.row_of_int[r];
+        if (L.A.rowptr[r + 1] - L.A.rowptr[r] != rlen(i)) fail("A row length", r, i);
+        for (int k = 0; k < rlen(i); ++k) {
+            const int src = L.csr_rowptr[i] + k;
+            if (L.A.col[L.A.rowptr[r] + k] != L.int_of_col[L.csr_col[src]]) fail("A row entry", r, k);
+            if (L.A.vmap[L.A.rowptr[r] + k] != L.perm[src]) fail("A value map", r, k);
+        }
+    }
+    for (int c = 0; c < n; ++c) {
+        const int j = L.col_of_int[c];
+        if (L.At.rowptr[c + 1] - L.At.rowptr[c] != clen(j)) fail("A' row length", c, j);
+        for (int k = 0; k < clen(j); ++k) {
+            if (L.At.col[L.At.rowptr[c] + k] != L.int_of_row[rv0[cp0[j] + k]]) fail("A' row entry", c, k);
+            if (L.At.vmap[L.At.rowptr[c] + k] != cp0[j] + k) fail("A' value map", c, k);
+        }
+    }
+}
+
 int main() {
     std::mt19937 rng(12345);
     check_partition_and_tiles(rng);
+    check_build_layout(rng);
     for (int trial = 0; trial < 6; ++trial) {
         const int rows = 2000 + 700 * trial, cols = 3000 + 1100 * trial;
         const int n_long = trial == 0 ? 0 : 20 + 15 * trial;
