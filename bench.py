@@ -385,6 +385,8 @@ def main():
     # PMC calibration: also run a plain 512 MiB device-to-device copy kernel 5 times
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--format", default="auto", choices=FORMATS)
+    ap.add_argument("--formats", default=None,
+                    help="A,At formats for an auto run (sets CPDLP_FORMATS; for counter runs)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--checks-per-round", type=int, default=20)
     ap.add_argument("--plain-sample", type=int, default=16)
@@ -407,6 +409,8 @@ def main():
               "iterates per half-step.", file=sys.stderr)
         sys.exit(2)
 
+    if args.formats:
+        os.environ["CPDLP_FORMATS"] = args.formats
     import coolpdlp_hip as cp
     import torch
 

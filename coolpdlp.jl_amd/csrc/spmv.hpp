@@ -72,13 +72,19 @@ struct MatView {
     int32_t sl_max_tiles = 0;             // most of any slice
     void* sl_part = nullptr;              // ns * sl_rows slice sums (Acc) per vector
 
-    // the sliced format's blocks that run the side's own short tiles
-    __host__ __device__ int sl_short_tiles() const { return sl_all ? 0 : n_tiles; }
-    // CSR: the short tiles start at a block index that is a multiple of 8 (blocks
+    // CSR short tiles: they start at a block index that is a multiple of 8 (blocks
     // [n_long_blocks, csr_short0()) are idle), and XCD x takes the contiguous tiles
-    // [x * C, (x + 1) * C), C = ceil(n_tiles / 8), in order
+    // [x * C, (x + 1) * C), C = ceil(n_tiles / 8), in order.  (The sliced format keeps
+    // its short tiles interleaved over the XCDs: there each XCD's L2 holds its column
+    // slice, and a contiguous range of short rows measured 26% slower on the dual step.)
     __host__ __device__ int csr_short0() const { return (n_long_blocks + 7) & ~7; }
     __host__ __device__ int csr_per_xcd() const { return (n_tiles + 7) / 8; }
+    __host__ __device__ int short_tile_of(int b) const {   // b: block index from the first short block
+        const int t = (b % 8) * csr_per_xcd() + b / 8;
+        return t < n_tiles ? t : -1;
+    }
+    // the sliced format's blocks that run the side's own short tiles
+    __host__ __device__ int sl_short_tiles() const { return sl_all ? 0 : n_tiles; }
     int grid() const {
         if (format == CPDLP_FMT_ELL) return ceil_div(rows, BLOCK);
         if (format == CPDLP_FMT_XSLICE) return ns * sl_max_tiles + sl_short_tiles();
@@ -195,10 +201,9 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X,
     } else if ((int)blockIdx.x >= A.csr_short0()) {
         // XCD blockIdx % 8 walks its own contiguous range of tiles, so entries first
         // touched by neighbouring tiles share lines in that XCD's L2
-        const int b = (int)blockIdx.x - A.csr_short0();
-        const int t = (b % 8) * A.csr_per_xcd() + b / 8;
+        const int t = A.short_tile_of((int)blockIdx.x - A.csr_short0());
         TileMeta tm{0, 0, 0, 0};
-        if (t < A.n_tiles) tm = A.tiles[t];
+        if (t >= 0) tm = A.tiles[t];
         const int r = tm.r0 + (int)threadIdx.x;
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);   // issued before the gathers
