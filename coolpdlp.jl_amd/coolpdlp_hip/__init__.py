@@ -12,6 +12,8 @@ from .batch import BatchSolver
 from .matrices import GPUSparseMatrixCSR, GPUSparseMatrixELL
 from .problems import (MILP, PrimalDualSolution, is_feasible, nbcons, nbcons_eq, nbcons_ineq,
                        nbvar, nbvar_cont, nbvar_int, objective_value)
+from . import moi
+from .moi import Optimizer
 
 __all__ = [
     "GPUSparseMatrixCSR", "GPUSparseMatrixELL", "MILP", "nbvar", "nbvar_int", "nbvar_cont",
@@ -19,6 +21,7 @@ __all__ = [
     "Algorithm", "is_feasible", "objective_value", "TerminationStatus", "OPTIMAL",
     "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING", "KKTErrors", "ConvergenceStats",
     "Engine", "BatchSolver", "relative", "absolute", "DimensionMismatch", "ArgumentError", "HIPError",
+    "Optimizer", "moi",
 ]
 
 
