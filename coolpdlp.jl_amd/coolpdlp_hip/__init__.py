@@ -14,6 +14,7 @@ from .problems import (MILP, PrimalDualSolution, is_feasible, nbcons, nbcons_eq,
                        nbvar, nbvar_cont, nbvar_int, objective_value)
 from . import moi
 from .moi import Optimizer
+from .qps import QPSData, read_mps, write_mps
 
 __all__ = [
     "GPUSparseMatrixCSR", "GPUSparseMatrixELL", "MILP", "nbvar", "nbvar_int", "nbvar_cont",
@@ -21,7 +22,7 @@ __all__ = [
     "Algorithm", "is_feasible", "objective_value", "TerminationStatus", "OPTIMAL",
     "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING", "KKTErrors", "ConvergenceStats",
     "Engine", "BatchSolver", "relative", "absolute", "DimensionMismatch", "ArgumentError", "HIPError",
-    "Optimizer", "moi",
+    "Optimizer", "moi", "QPSData", "read_mps", "write_mps",
 ]
 
 

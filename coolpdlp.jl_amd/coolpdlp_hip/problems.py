@@ -71,6 +71,13 @@ class MILP:
         A = sp.csc_matrix((g.nzval, g.rowval, g.colptr), shape=(g.m, g.n))
         return cls(c=g.c, lv=g.lv, uv=g.uv, A=A, lc=g.lc, uc=g.uc, name=g.name, **kw)
 
+    @classmethod
+    def from_qps(cls, qps, **kw):
+        """MILP(qps::QPSData; kwargs...), src/problems/milp.jl:122-137, for a QPSData from
+        coolpdlp_hip.qps.read_mps."""
+        from .qps import milp_from_qps
+        return milp_from_qps(qps, **kw)
+
     def __repr__(self):
         return (f"MILP instance {self.name} from dataset {self.dataset}: {nbvar(self)} variables "
                 f"({nbvar_int(self)} integer), {nbcons(self)} constraints "
