@@ -4,16 +4,47 @@ expected bounds follow from the MPS format's definition (worked by hand in the c
 a write -> read round trip that must be exact, and the reference's MOI test LP
 (test/moi.jl:44-52, objective 205.0) read from fixed-format MPS and solved by the oracle."""
 import io
+import json
+import os
 
 import numpy as np
 import pytest
 
 import oracle as O
-from coolpdlp_hip import MILP, generators as G, nbcons, nbvar, nbvar_int
+from coolpdlp_hip import MILP, generators as G, is_feasible, nbcons, nbvar, nbvar_int, objective_value
 from coolpdlp_hip._lib import ArgumentError
 from coolpdlp_hip.qps import VTYPE_BINARY, VTYPE_CONTINUOUS, VTYPE_INTEGER, read_mps, write_mps
 
 INF = np.inf
+SUITE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mps")
+
+
+def suite():
+    """(name, MILP, HiGHS objective) for the Netlib-style MPS fixtures, smallest first by
+    variable count, as test/algorithms/all.jl:12-15 sorts Netlib."""
+    with open(os.path.join(SUITE_DIR, "highs.json")) as f:
+        rec = json.load(f)["instances"]
+    out = []
+    for name in sorted(rec):
+        path = os.path.join(SUITE_DIR, name + ".mps")
+        milp = MILP.from_qps(read_mps(path), dataset="netlike", path=path)
+        out.append((name, milp, rec[name]["objective"]))
+    return sorted(out, key=lambda t: nbvar(t[1]))
+
+
+def generated(milp):
+    """The MILP as the oracle takes it."""
+    A = milp.A
+    return G.GeneratedLP(A.shape[0], A.shape[1], A.indptr.astype(np.int64), A.indices.astype(np.int32),
+                         A.data.copy(), milp.c, milp.lv, milp.uv, milp.lc, milp.uc)
+
+
+def row_scale(milp):
+    """1 + the largest finite row bound: the relative KKT error bounds the residual by
+    about this factor times the tolerance."""
+    b = np.concatenate([milp.lc, milp.uc])
+    b = b[np.isfinite(b)]
+    return 1.0 + (np.max(np.abs(b)) if len(b) else 0.0)
 
 LP205_FIXED = """\
 NAME          LP205
@@ -114,10 +145,7 @@ def test_lp205_fields_and_solution():
     # the MILP's name comes from the keywords (here derived from the path), not from the
     # file's NAME card, as in the reference
     assert milp.var_names == ["X", "Y"] and milp.name == "lp205" and milp.dataset == "hand"
-    A = milp.A
-    g = G.GeneratedLP(2, 2, A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.copy(),
-                      milp.c, milp.lv, milp.uv, milp.lc, milp.uc)
-    r = O.solve(g, O.default_params(termination_reltol=1e-8))
+    r = O.solve(generated(milp), O.default_params(termination_reltol=1e-8))
     assert r.status == "OPTIMAL"
     assert abs(float(milp.c @ r.x) - 205.0) <= 1e-4
     assert np.allclose(r.x, [15.0, 1.25], atol=1e-4)
@@ -188,3 +216,39 @@ def test_refusals():
         read_mps(io.StringIO(bad_row))
     with pytest.raises(ArgumentError):
         write_mps(io.StringIO(), c=[1.0], lv=[0.0], uv=[1.0], A=np.ones((1, 1)), lc=[-INF], uc=[INF])
+
+
+def test_suite_reads_back_the_lps_highs_solved():
+    """HiGHS on the LPs read from the fixtures reproduces the objectives it found on the
+    in-memory LPs before they were written (tests/golden/make_mps_suite.py)."""
+    from scipy.optimize import linprog
+    import scipy.sparse as sp
+    cases = suite()
+    assert [n for n, _, _ in cases][:3] == ["netlike_free_fixed", "netlike_ranges", "netlike_badscale"]
+    for name, milp, obj in cases:
+        A = sp.csr_matrix(milp.A)
+        eq = milp.lc == milp.uc
+        up, lo = ~eq & np.isfinite(milp.uc), ~eq & np.isfinite(milp.lc)
+        h = linprog(milp.c, A_ub=sp.vstack([A[up], -A[lo]]), b_ub=np.concatenate([milp.uc[up], -milp.lc[lo]]),
+                    A_eq=A[eq], b_eq=milp.lc[eq], bounds=list(zip(milp.lv, milp.uv)), method="highs")
+        assert h.status == 0, name
+        assert abs(h.fun - obj) <= 1e-9 * (1 + abs(obj)), name
+
+
+@pytest.mark.parametrize("algo", ["PDLP", "PDHG"])
+def test_suite_oracle_vs_highs(algo):
+    """test_optimizer, test/algorithms/all.jl:20-62, on the oracle: PDLP at reltol 1e-5 with
+    objective rtol 1e-2 and cons_tol 1e-2; PDHG at reltol 1e-1 with objective rtol 1e-1.
+    PDHG's cons_tol 1e-1 is scaled by 1 + the largest row bound: these right-hand sides
+    reach 80, where a relative KKT error of 1e-1 allows residuals above 0.1."""
+    for name, milp, obj in suite():
+        reltol = 1e-5 if algo == "PDLP" else 1e-1
+        r = O.solve(generated(milp), O.default_params(algo, termination_reltol=reltol,
+                                                      max_kkt_passes=10 ** 7, time_limit=1e9))
+        assert r.status == "OPTIMAL", name
+        if algo == "PDLP":
+            assert is_feasible(r.x, milp, cons_tol=1e-2, int_tol=INF, verbose=False), name
+            assert abs(objective_value(r.x, milp) - obj) <= 1e-2 * abs(obj), name
+        else:
+            assert is_feasible(r.x, milp, cons_tol=1e-1 * row_scale(milp), int_tol=INF, verbose=False), name
+            assert abs(objective_value(r.x, milp) - obj) <= 1e-1 * abs(obj), name
