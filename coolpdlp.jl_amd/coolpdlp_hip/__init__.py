@@ -15,6 +15,7 @@ from .problems import (MILP, PrimalDualSolution, is_feasible, nbcons, nbcons_eq,
 from . import moi
 from .moi import Optimizer
 from .qps import QPSData, read_mps, write_mps
+from . import permutation
 
 __all__ = [
     "GPUSparseMatrixCSR", "GPUSparseMatrixELL", "MILP", "nbvar", "nbvar_int", "nbvar_cont",
