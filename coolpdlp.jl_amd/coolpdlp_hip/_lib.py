@@ -96,6 +96,7 @@ _SIGS = {
     "cpdlp_version": (C.c_char_p, []),
     "cpdlp_batch_create": (C.c_int, [_P(CParams), C.c_int64, _P(C.c_void_p)]),
     "cpdlp_batch_add": (C.c_int, [C.c_void_p, _P(CLP), _P(C.c_int64)]),
+    "cpdlp_batch_add_start": (C.c_int, [C.c_void_p, _P(CLP), C.c_void_p, C.c_void_p, _P(C.c_int64)]),
     "cpdlp_batch_footprint": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64]),
     "cpdlp_batch_run": (C.c_int, [C.c_void_p, C.c_int32, _P(C.c_int64)]),
     "cpdlp_batch_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _P(C.c_int64)]),

@@ -47,9 +47,11 @@ class BatchSolver:
     def __exit__(self, *a):
         self.close()
 
-    def add(self, lp) -> int:
+    def add(self, lp, sol: PrimalDualSolution | None = None) -> int:
         """Admit an LP (a MILP or a generated LP with CSC arrays); returns its id.
-        Raises MemoryError when the arena is full (retry after release)."""
+        `sol` is an optional unscaled start, as in solve(milp, sol, algo)
+        (src/algorithms/common.jl:186-209).  Raises MemoryError when the arena is full
+        (retry after release)."""
         if hasattr(lp, "A"):   # MILP
             A = lp.A
             m, n = A.shape
@@ -65,7 +67,15 @@ class BatchSolver:
         c_lp.colptr, c_lp.rowval, c_lp.nzval = [a.ctypes.data_as(C.c_void_p) for a in arrs]
         c_lp.c, c_lp.lv, c_lp.uv, c_lp.lc, c_lp.uc = [v.ctypes.data_as(C.c_void_p) for v in vecs]
         i = C.c_int64()
-        check(lib.cpdlp_batch_add(self.h, C.byref(c_lp), C.byref(i)))
+        if sol is None:
+            check(lib.cpdlp_batch_add(self.h, C.byref(c_lp), C.byref(i)))
+        else:
+            x0 = np.ascontiguousarray(sol.x, dtype=np.float64)
+            y0 = np.ascontiguousarray(sol.y, dtype=np.float64)
+            if x0.shape != (n,) or y0.shape != (m,):
+                raise _lib.DimensionMismatch("start point does not match the LP's size")
+            check(lib.cpdlp_batch_add_start(self.h, C.byref(c_lp), x0.ctypes.data_as(C.c_void_p),
+                                            y0.ctypes.data_as(C.c_void_p), C.byref(i)))
         self.shapes[i.value] = (m, n)
         return i.value
 
@@ -103,9 +113,13 @@ class BatchSolver:
         check(lib.cpdlp_batch_memory(self.h, C.byref(used), C.byref(size)))
         return used.value, size.value
 
-    def solve_all(self, lps, checks_per_round: int = 20):
-        """Solve a list of LPs together; returns [(solution, stats)] in input order."""
-        ids = [self.add(lp) for lp in lps]
+    def solve_all(self, lps, checks_per_round: int = 20, sols=None):
+        """Solve a list of LPs together (optionally from start points `sols`, one per LP or
+        None); returns [(solution, stats)] in input order."""
+        sols = [None] * len(lps) if sols is None else list(sols)
+        if len(sols) != len(lps):
+            raise _lib.DimensionMismatch("one start point (or None) per LP")
+        ids = [self.add(lp, s) for lp, s in zip(lps, sols)]
         while self.run(checks_per_round) > 0:
             pass
         out = [self.get(i) for i in ids]

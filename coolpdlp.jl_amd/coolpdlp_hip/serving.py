@@ -49,6 +49,8 @@ class Request:
     objective: float = float("nan")
     kkt_passes: int = 0
     x: np.ndarray | None = None
+    y: np.ndarray | None = None
+    sol: object = None                 # optional start point (PrimalDualSolution)
 
 
 @dataclass
@@ -103,9 +105,12 @@ class Server:
         self.batch.close()
 
     # -------------------------------------------------------------- submission
-    def submit(self, lp, t_arrival: float | None = None) -> int:
+    def submit(self, lp, t_arrival: float | None = None, sol=None) -> int:
+        """Queue an LP; `sol` is an optional start point, as in solve(milp, sol, algo)
+        (e.g. the solution of an earlier request for a nearby LP, kept with
+        keep_solutions)."""
         t = time.perf_counter() if t_arrival is None else t_arrival
-        r = Request(self.next_id, lp, t)
+        r = Request(self.next_id, lp, t, sol=sol)
         self.next_id += 1
         k = route(lp.m, lp.n, lp.nnz, self.batch_limit, self.plain_lanes, self.overflow_queue)
         if k < 0:
@@ -128,12 +133,12 @@ class Server:
             algo = PDLP(self.precision, None, "csr", termination_reltol=self.reltol,
                         time_limit=self.time_limit)
             with Engine(milp, algo) as eng:
-                sol, st = eng.solve()
+                sol, st = eng.solve(r.sol)
             r.status = st.termination_status.name
             r.kkt_passes = st.kkt_passes
             r.objective = float(sol.x @ r.lp.c)
             if self.keep_solutions:
-                r.x = sol.x
+                r.x, r.y = sol.x, sol.y
         except Exception as e:   # noqa: BLE001 -- a failed request is reported, not fatal
             r.status = f"ERROR: {e}"
         r.t_done = time.perf_counter()
@@ -165,7 +170,7 @@ class Server:
         while self.waiting:
             r = self.waiting[0]
             try:
-                bid = self.batch.add(r.lp)
+                bid = self.batch.add(r.lp, r.sol)
             except MemoryError:
                 return   # no room until a running request finishes
             self.waiting.popleft()
@@ -191,7 +196,7 @@ class Server:
                 r.kkt_passes = st.kkt_passes
                 r.objective = float(sol.x @ r.lp.c)
                 if self.keep_solutions:
-                    r.x = sol.x
+                    r.x, r.y = sol.x, sol.y
                 with self.lock:
                     self.done.append(r)
         else:

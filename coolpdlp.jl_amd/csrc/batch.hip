@@ -35,7 +35,8 @@ namespace cpdlp {
 
 // ---------------------------------------------------------------- device structures
 struct BInst {
-    int32_t m, n, nnz, La, Lat, n_a_tiles, n_at_tiles, pad0;
+    int32_t m, n, nnz, La, Lat, n_a_tiles, n_at_tiles;
+    int32_t warm;                         // 1: xr64 / yr64 hold the caller's unscaled start
     int32_t *a_rp, *a_col, *at_rp, *at_col, *perm;
     int32_t* ecol;                        // column of each CSC entry (flat scaling passes)
     const TileMeta *a_tiles, *at_tiles;   // row tiles of A and At for the step kernel
@@ -610,14 +611,25 @@ __global__ __launch_bounds__(NSUB * BB) void k_batch_setup(const int32_t* list, 
     const double eta = P.invnorm * (1.0 / sqrt(theta));
     double omega = 1.0;
     if (P.pdlp && c_norm > P.zero_tol && comb_norm > P.zero_tol) omega = c_norm / comb_norm;
-    // initial iterates (zero start): sol = 0, sol_last = 0, avg = sol, avg_last = 0, restart = sol
+    // initial iterates: sol = precondition(sol_init) = (x0 / D2, y0 / D1), or zero
+    // without a start (common.jl:200-207); sol_last = 0, avg = sol, avg_last = 0,
+    // restart = sol (pdlp.jl:51-54).  The start was uploaded into the restart buffers,
+    // which nothing above touches; each thread reads its own entries before writing them.
     for (int j = tid; j < n; j += NT) {
-        I.x32[0][j] = I.x32[1][j] = I.xa32[j] = I.xal32[j] = I.xr32[j] = I.xb32[j] = 0.f;
-        I.x64[0][j] = I.x64[1][j] = I.xa64[j] = I.xal64[j] = I.xr64[j] = 0.0;
+        const double xs = I.warm ? I.xr64[j] / I.D2[j] : 0.0;
+        const float xf = (float)xs;
+        I.x32[0][j] = I.xa32[j] = I.xr32[j] = xf;
+        I.x32[1][j] = I.xal32[j] = I.xb32[j] = 0.f;
+        I.x64[0][j] = I.xa64[j] = I.xr64[j] = xs;
+        I.x64[1][j] = I.xal64[j] = 0.0;
     }
     for (int i = tid; i < m; i += NT) {
-        I.y32[0][i] = I.y32[1][i] = I.ya32[i] = I.yal32[i] = I.yr32[i] = I.yb32[i] = 0.f;
-        I.y64[0][i] = I.y64[1][i] = I.ya64[i] = I.yal64[i] = I.yr64[i] = 0.0;
+        const double ys = I.warm ? I.yr64[i] / I.D1[i] : 0.0;
+        const float yf = (float)ys;
+        I.y32[0][i] = I.ya32[i] = I.yr32[i] = yf;
+        I.y32[1][i] = I.yal32[i] = I.yb32[i] = 0.f;
+        I.y64[0][i] = I.ya64[i] = I.yr64[i] = ys;
+        I.y64[1][i] = I.yal64[i] = 0.0;
     }
     if (tid == 0) {
         S->eta = eta;
@@ -1314,7 +1326,7 @@ struct Batch {
         return b;
     }
 
-    int64_t add(const cpdlp_lp& lp) {
+    int64_t add(const cpdlp_lp& lp, const double* x0 = nullptr, const double* y0 = nullptr) {
         HIP_CHECK(hipSetDevice(device));
         std::vector<int32_t> cp, rv, rp, col, perm;
         read_csc(lp.m, lp.n, lp.index_base, lp.index_bits, lp.colptr, lp.rowval, cp, rv);
@@ -1361,6 +1373,14 @@ struct Batch {
         up(I.uv, lp.uv, n * 8);
         up(I.lc, lp.lc, m * 8);
         up(I.uc, lp.uc, m * 8);
+        // a start goes into the restart buffers, where the setup kernel scales it
+        std::vector<double> zeros;
+        if (x0 || y0) {
+            I.warm = 1;
+            if (!x0 || !y0) zeros.assign((size_t)std::max(m, n), 0.0);
+            up(I.xr64, x0 ? x0 : zeros.data(), n * 8);
+            up(I.yr64, y0 ? y0 : zeros.data(), m * 8);
+        }
         const int32_t slot = free_slots.back();
         free_slots.pop_back();
         hi_slot = std::max(hi_slot, slot + 1);
@@ -1718,10 +1738,15 @@ int cpdlp_batch_create(const cpdlp_params* params, int64_t hbm_bytes, cpdlp_batc
     if (!(bp) || !(bp)->b) throw Error(CPDLP_ERR_ARGUMENT, "NULL batch");
 
 int cpdlp_batch_add(cpdlp_batch* b, const cpdlp_lp* lp, int64_t* id) {
+    return cpdlp_batch_add_start(b, lp, nullptr, nullptr, id);
+}
+
+int cpdlp_batch_add_start(cpdlp_batch* b, const cpdlp_lp* lp, const double* x0, const double* y0,
+                          int64_t* id) {
     return guarded([&] {
         BATCH_OR_THROW(b);
         if (!lp || !id) throw Error(CPDLP_ERR_ARGUMENT, "NULL argument");
-        *id = b->b->add(*lp);
+        *id = b->b->add(*lp, x0, y0);
     });
 }
 

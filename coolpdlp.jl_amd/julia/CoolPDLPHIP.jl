@@ -264,13 +264,17 @@ end
 
 # ---- 4. many LPs at once (cpdlp_batch_*; no counterpart in the reference) ----------
 """
-    solve(milps::Vector{<:MILP}, algo; hbm_bytes, checks_per_round = 20)
+    solve(milps::Vector{<:MILP}, algo; hbm_bytes, checks_per_round = 20, sols = nothing)
 
 Solve independent LPs together on one GPU (one workgroup per LP, `m + n <= 65536` each),
-returning `[(PrimalDualSolution, ConvergenceStats)]` in input order.
+returning `[(PrimalDualSolution, ConvergenceStats)]` in input order.  `sols` optionally
+holds one start point per LP (`nothing` entries start from zero), as `sol` in
+`solve(milp, sol, algo)`.
 """
 function CoolPDLP.solve(milps::Vector{<:MILP}, algo::Algorithm{A, T, Ti, M, HIPEngine};
-        hbm_bytes::Integer = 0, checks_per_round::Integer = 20) where {A, T, Ti, M}
+        hbm_bytes::Integer = 0, checks_per_round::Integer = 20, sols = nothing) where {A, T, Ti, M}
+    starts = sols === nothing ? fill(nothing, length(milps)) : sols
+    length(starts) == length(milps) || throw(DimensionMismatch("one start point (or nothing) per LP"))
     p = cparams(algo, algo.conversion.backend)
     lps = map(milps) do milp
         A_ = milp.A::SparseMatrixCSC
@@ -285,12 +289,18 @@ function CoolPDLP.solve(milps::Vector{<:MILP}, algo::Algorithm{A, T, Ti, M, HIPE
     b = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:cpdlp_batch_create, libcpdlp), Cint, (Ref{CParams}, Int64, Ptr{Ptr{Cvoid}}), p, hbm_bytes, b))
     try
-        ids = map(lps) do (A_, c, lv, uv, lc, uc)
+        ids = map(zip(lps, starts)) do ((A_, c, lv, uv, lc, uc), sol)
             lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
                 pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
             id = Ref{Int64}(0)
-            GC.@preserve A_ c lv uv lc uc begin
-                check(ccall((:cpdlp_batch_add, libcpdlp), Cint, (Ptr{Cvoid}, Ref{CLP}, Ref{Int64}), b[], lp, id))
+            x0 = sol === nothing ? Float64[] : Vector{Float64}(Array(sol.x))
+            y0 = sol === nothing ? Float64[] : Vector{Float64}(Array(sol.y))
+            sol === nothing || (length(x0) == size(A_, 2) && length(y0) == size(A_, 1)) ||
+                throw(DimensionMismatch("start point does not match the LP's size"))
+            GC.@preserve A_ c lv uv lc uc x0 y0 begin
+                check(ccall((:cpdlp_batch_add_start, libcpdlp), Cint,
+                    (Ptr{Cvoid}, Ref{CLP}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}), b[], lp,
+                    sol === nothing ? C_NULL : pointer(x0), sol === nothing ? C_NULL : pointer(y0), id))
             end
             id[]
         end

@@ -226,6 +226,13 @@ int cpdlp_batch_create(const cpdlp_params* params, int64_t hbm_bytes, cpdlp_batc
  * (and admits nothing) when the arena lacks room: the caller retries after a release. */
 int cpdlp_batch_add(cpdlp_batch* b, const cpdlp_lp* lp, int64_t* id);
 
+/* cpdlp_batch_add with a start point: x0 (n) / y0 (m) are the unscaled start, as the
+ * sol_init of solve(milp, sol, algo) (src/algorithms/common.jl:186-209); either may be
+ * NULL (zero).  The instance starts from (x0 / D2, y0 / D1), with avg = restart = start,
+ * as cpdlp_solve does. */
+int cpdlp_batch_add_start(cpdlp_batch* b, const cpdlp_lp* lp, const double* x0, const double* y0,
+                          int64_t* id);
+
 /* HBM footprint an LP of this size would take in the arena. */
 int64_t cpdlp_batch_footprint(int64_t m, int64_t n, int64_t nnz);
 

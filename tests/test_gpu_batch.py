@@ -55,6 +55,43 @@ def test_batch_fp64_agrees_with_plain_engine(cp):
     assert abs(float(sol_b.x @ g.c) - float(sol_p.x @ g.c)) <= 1e-5 * (1 + abs(g.objective))
 
 
+def test_batch_warm_start(cp):
+    """cpdlp_batch_add_start: a start point as sol in solve(milp, sol, algo)
+    (src/algorithms/common.jl:186-209).  The re-solve of an LP after a change of cost
+    from the old optimum agrees with the plain engine's warm start; a cold instance in the
+    same batch keeps its bits; a start at the optimum stops within the first checks."""
+    import dataclasses
+    g = G.column_lp(200, 450, 6, seed=33)
+    milp = cp.MILP.from_generated(g)
+    rng = np.random.default_rng(1)
+    g_old = dataclasses.replace(g, c=g.c * (1 + 0.01 * rng.standard_normal(g.n)))
+    start, _ = cp.solve(cp.MILP.from_generated(g_old), cp.PDLP(termination_reltol=1e-8))
+    sol_p, st_p = cp.solve(milp, start, cp.PDLP(termination_reltol=1e-7))
+    algo = cp.PDLP("float64", termination_reltol=1e-7)
+    with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
+        (sol_w, st_w), (sol_c, st_c) = B.solve_all([g, g], sols=[start, None])
+    with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
+        (sol_a, st_a), = B.solve_all([g])
+    assert st_p.termination_status == st_w.termination_status == cp.OPTIMAL
+    assert abs(st_w.kkt_passes - st_p.kkt_passes) <= 300
+    assert abs(float(sol_w.x @ g.c) - float(sol_p.x @ g.c)) <= 1e-5 * (1 + abs(g.objective))
+    assert abs(float(sol_w.x @ g.c) - g.objective) <= 1e-5 * (1 + abs(g.objective))
+    # the cold instance beside the warm one is the cold instance alone, bit for bit
+    assert np.array_equal(sol_c.x, sol_a.x) and np.array_equal(sol_c.y, sol_a.y)
+    assert st_c.kkt_passes == st_a.kkt_passes
+    # from an optimum at 1e-9, a re-solve at 1e-6 stops within the first checks, in FP64
+    # and in mixed precision
+    best, _ = cp.solve(milp, cp.PDLP(termination_reltol=1e-9))
+    for prec in ("float64", "mixed"):
+        with cp.BatchSolver(cp.PDLP(prec, termination_reltol=1e-6), hbm_bytes=64 << 20) as B:
+            (sol_o, st_o), = B.solve_all([g], sols=[best])
+        assert st_o.termination_status == cp.OPTIMAL and st_o.kkt_passes <= 300, (prec, st_o.kkt_passes)
+        assert st_o.kkt_passes < st_a.kkt_passes
+    with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
+        with pytest.raises(cp.DimensionMismatch):
+            B.add(g, cp.PrimalDualSolution(np.zeros(g.n - 1), np.zeros(g.m)))
+
+
 def test_alone_equals_in_mix_and_round_length_free(cp):
     lps = _mix(12, seed=9)
     target = lps[5]

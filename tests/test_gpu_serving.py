@@ -49,3 +49,26 @@ def test_serving_alone_equals_in_mix(cp):
     assert np.array_equal(sol.x, r.x)
     assert abs(r.objective - big.objective) <= 2e-5 * (1 + abs(big.objective))
 
+
+
+@pytest.mark.parametrize("batch_limit", [8000, 0])
+def test_serving_warm_start_resubmission(cp, batch_limit):
+    """A request submitted with the solution of an earlier one as its start (the re-solve
+    of a nearby LP) takes at most half the passes of the cold solve, on the batch route (batch_limit 8000)
+    and on the plain route (batch_limit 0)."""
+    from coolpdlp_hip.serving import Server
+    g = G.request_lp(400, 800, 91)
+    srv = Server(reltol=1e-6, batch_bytes=64 << 20, plain_bytes=2 << 30, batch_limit=batch_limit,
+                 keep_solutions=True)
+    srv.submit(g)
+    srv.drain(time.perf_counter() + 120)
+    first, = srv.done
+    assert first.status == "OPTIMAL"
+    srv.submit(g, sol=cp.PrimalDualSolution(first.x, first.y))
+    srv.drain(time.perf_counter() + 120)
+    srv.close()
+    second = srv.done[1]
+    assert second.route == ("batch" if batch_limit else "plain")
+    assert second.status == "OPTIMAL"
+    assert second.kkt_passes <= first.kkt_passes // 2
+    assert abs(second.objective - g.objective) <= 2e-5 * (1 + abs(g.objective))
