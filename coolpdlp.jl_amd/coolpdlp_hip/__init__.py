@@ -5,7 +5,8 @@ PrimalDualSolution, solve / PDHG / PDLP / Algorithm, is_feasible, objective_valu
 All computation runs in libcoolpdlp_hip.so (hand-written HIP kernels for gfx950).
 """
 from ._lib import ArgumentError, DimensionMismatch, HIPError
-from .algorithms import (ITERATION_LIMIT, OPTIMAL, STILL_RUNNING, TIME_LIMIT, Algorithm,
+from .algorithms import (DUAL_INFEASIBLE, ITERATION_LIMIT, NUMERICAL_ERROR, OPTIMAL,
+                         PRIMAL_INFEASIBLE, STILL_RUNNING, TIME_LIMIT, Algorithm,
                          ConvergenceStats, Engine, KKTErrors, PDHG, PDLP, TerminationStatus,
                          absolute, relative, solve)
 from .batch import BatchSolver
@@ -21,7 +22,8 @@ __all__ = [
     "GPUSparseMatrixCSR", "GPUSparseMatrixELL", "MILP", "nbvar", "nbvar_int", "nbvar_cont",
     "nbcons", "nbcons_eq", "nbcons_ineq", "PrimalDualSolution", "solve", "PDHG", "PDLP",
     "Algorithm", "is_feasible", "objective_value", "TerminationStatus", "OPTIMAL",
-    "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING", "KKTErrors", "ConvergenceStats",
+    "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING", "PRIMAL_INFEASIBLE", "DUAL_INFEASIBLE",
+    "NUMERICAL_ERROR", "KKTErrors", "ConvergenceStats",
     "Engine", "BatchSolver", "relative", "absolute", "DimensionMismatch", "ArgumentError", "HIPError",
     "Optimizer", "moi", "QPSData", "read_mps", "write_mps",
 ]

@@ -51,7 +51,8 @@ class CParams(C.Structure):
         ("format", C.c_int32), ("device", C.c_int32), ("powm_tol", C.c_double),
         ("powm_maxiter", C.c_int64), ("powm_seed", C.c_uint64),
         ("polish_reltol", C.c_double), ("use_graph", C.c_int32), ("long_row", C.c_int32),
-        ("batch_streams", C.c_int32),
+        ("batch_streams", C.c_int32), ("detect_infeasibility", C.c_int32),
+        ("infeasibility_tol", C.c_double),
     ]
 
 

@@ -19,17 +19,24 @@ from .problems import MILP, PrimalDualSolution
 
 
 class TerminationStatus(enum.IntEnum):
-    """src/components/termination.jl:13 (same order)"""
+    """src/components/termination.jl:13 (same order), plus three statuses that only the
+    opt-in detect_infeasibility=True produces (the reference has no such check)."""
     OPTIMAL = 0
     TIME_LIMIT = 1
     ITERATION_LIMIT = 2
     STILL_RUNNING = 3
+    PRIMAL_INFEASIBLE = 4     # a dual ray (Farkas certificate) was found
+    DUAL_INFEASIBLE = 5       # a primal ray was found
+    NUMERICAL_ERROR = 6       # the KKT errors became NaN or infinite
 
 
 OPTIMAL = TerminationStatus.OPTIMAL
 TIME_LIMIT = TerminationStatus.TIME_LIMIT
 ITERATION_LIMIT = TerminationStatus.ITERATION_LIMIT
 STILL_RUNNING = TerminationStatus.STILL_RUNNING
+PRIMAL_INFEASIBLE = TerminationStatus.PRIMAL_INFEASIBLE
+DUAL_INFEASIBLE = TerminationStatus.DUAL_INFEASIBLE
+NUMERICAL_ERROR = TerminationStatus.NUMERICAL_ERROR
 
 
 @dataclass

@@ -36,6 +36,10 @@ class TerminationStatusCode(enum.Enum):
     TIME_LIMIT = 2
     ITERATION_LIMIT = 3
     OTHER_ERROR = 4
+    # only with the opt-in "detect_infeasibility" attribute
+    INFEASIBLE = 5
+    DUAL_INFEASIBLE = 6
+    NUMERICAL_ERROR = 7
 
 
 class ResultStatusCode(enum.Enum):
@@ -142,6 +146,13 @@ _STATUS = {
                         ResultStatusCode.UNKNOWN_RESULT_STATUS),
     "STILL_RUNNING": (TerminationStatusCode.OTHER_ERROR, ResultStatusCode.NO_SOLUTION,
                       ResultStatusCode.NO_SOLUTION),
+    # the certificates themselves are not returned: no solution is reported
+    "PRIMAL_INFEASIBLE": (TerminationStatusCode.INFEASIBLE, ResultStatusCode.NO_SOLUTION,
+                          ResultStatusCode.NO_SOLUTION),
+    "DUAL_INFEASIBLE": (TerminationStatusCode.DUAL_INFEASIBLE, ResultStatusCode.NO_SOLUTION,
+                        ResultStatusCode.NO_SOLUTION),
+    "NUMERICAL_ERROR": (TerminationStatusCode.NUMERICAL_ERROR, ResultStatusCode.NO_SOLUTION,
+                        ResultStatusCode.NO_SOLUTION),
 }
 
 

@@ -1260,6 +1260,8 @@ struct Batch {
         HIP_CHECK(hipGetDeviceCount(&ndev));
         if (P.device < 0 || P.device >= ndev) throw Error(CPDLP_ERR_ARGUMENT, "invalid device ordinal");
         if (P.check_every <= 0) throw Error(CPDLP_ERR_ARGUMENT, "check_every must be positive");
+        if (P.detect_infeasibility)
+            throw Error(CPDLP_ERR_ARGUMENT, "detect_infeasibility is not supported by the batch engine");
         device = P.device;
         HIP_CHECK(hipSetDevice(device));
         streams.assign((size_t)std::min(8, std::max(1, P.batch_streams)), nullptr);

@@ -183,6 +183,86 @@ struct EpiKktCols {
     }
 };
 
+// ---------------------------------------------------------------- infeasibility rays
+// Opt-in detection (no counterpart in the reference).  Three candidate rays per side:
+// k = 0, the current iterate minus the restart point; k = 1, the current iterate;
+// k = 2, the current iterate minus the previous one.  The products run over
+// {current, restart, previous} and give every candidate by linearity.  Sums are in
+// original units -- x = D2 x_s, y = D1 y_s, A x = inv(D1) A_s x_s, A'y = inv(D2) A_s'y_s --
+// and a bound times a ray component is the same number in either space.
+__device__ __forceinline__ double ray_of(int k, double cur, double rst, double last) {
+    return k == 0 ? cur - rst : k == 1 ? cur : cur - last;
+}
+
+// rows of A: for the primal ray d, |violation of A d in the recession cone of
+// [lc, uc]|^2 (A d = 0 on rows with two finite bounds, >= 0 with lc only, <= 0 with uc
+// only); for the dual ray e, the Farkas objective lc e+ - uc e- over finite bounds and
+// |sign violation|^2 (e+ where lc = -inf, e- where uc = +inf).
+template <class V>
+struct EpiRayRows {
+    static constexpr int NQ = 9;   // cone violation [3], objective [3], sign violation [3]
+    double* partials;
+    const V *y, *y_rst, *y_last;
+    const double *lc, *uc, *invD1;
+    struct Pre {
+        double l, u, id, y, yr, yl;
+    };
+    __device__ bool skip() const { return false; }
+    __device__ __forceinline__ Pre load(int i) const {
+        return Pre{lc[i], uc[i], invD1[i], (double)y[i], (double)y_rst[i], (double)y_last[i]};
+    }
+    template <class Acc>
+    __device__ __forceinline__ void row(int, const Acc* acc, double* q, const Pre& p) const {
+        const bool lf = isfinite(p.l), uf = isfinite(p.u);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double v = p.id * ray_of(k, (double)acc[0], (double)acc[1], (double)acc[2]);
+            const double cv = lf && uf ? v : lf ? fmin(v, 0.0) : uf ? fmax(v, 0.0) : 0.0;
+            q[k] += cv * cv;
+            const double e = ray_of(k, p.y, p.yr, p.yl);
+            const double ep = pos_part(e), en = neg_part(e);
+            q[3 + k] += (lf ? p.l * ep : 0.0) - (uf ? p.u * en : 0.0);
+            const double sv = ((lf ? 0.0 : ep) + (uf ? 0.0 : en)) / p.id;
+            q[6 + k] += sv * sv;
+        }
+    }
+};
+
+// rows of A' (= columns j): for the dual ray e with r = -A'e, the Farkas objective
+// lv r+ - uv r- over finite bounds and |sign violation|^2 (r+ where lv = -inf, r- where
+// uv = +inf); for the primal ray d, c'd and |violation of d in the recession cone of
+// [lv, uv]|^2.
+template <class V>
+struct EpiRayCols {
+    static constexpr int NQ = 12;   // objective [3], sign violation [3], c'd [3], cone violation [3]
+    double* partials;
+    const V *x, *x_rst, *x_last;
+    const double *c, *lv, *uv, *invD2;
+    struct Pre {
+        double c, l, u, id, x, xr, xl;
+    };
+    __device__ bool skip() const { return false; }
+    __device__ __forceinline__ Pre load(int j) const {
+        return Pre{c[j], lv[j], uv[j], invD2[j], (double)x[j], (double)x_rst[j], (double)x_last[j]};
+    }
+    template <class Acc>
+    __device__ __forceinline__ void row(int, const Acc* acc, double* q, const Pre& p) const {
+        const bool lf = isfinite(p.l), uf = isfinite(p.u);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double r = -ray_of(k, (double)acc[0], (double)acc[1], (double)acc[2]);
+            const double rp = pos_part(r), rn = neg_part(r);
+            q[k] += (lf ? p.l * rp : 0.0) - (uf ? p.u * rn : 0.0);
+            const double sv = p.id * ((lf ? 0.0 : rp) + (uf ? 0.0 : rn));
+            q[3 + k] += sv * sv;
+            const double d = ray_of(k, p.x, p.xr, p.xl);
+            q[6 + k] += p.c * d;
+            const double cv = (lf && uf ? d : lf ? fmin(d, 0.0) : uf ? fmax(d, 0.0) : 0.0) / p.id;
+            q[9 + k] += cv * cv;
+        }
+    }
+};
+
 // ---------------------------------------------------------------- power method
 struct PowState {
     double theta, nrm, residual, tol;

@@ -464,6 +464,8 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         throw Error(CPDLP_ERR_ARGUMENT, "unknown format");
     if (P.algorithm != CPDLP_ALGO_PDLP && P.algorithm != CPDLP_ALGO_PDHG)
         throw Error(CPDLP_ERR_ARGUMENT, "unknown algorithm");
+    if (P.detect_infeasibility && !(P.infeasibility_tol > 0.0))
+        throw Error(CPDLP_ERR_ARGUMENT, "infeasibility_tol must be positive");
     if (P.long_row <= 0) P.long_row = 64;
     if (P.long_row > TILE_NNZ) P.long_row = TILE_NNZ;
     m = lp.m;
@@ -876,6 +878,65 @@ cpdlp_kkt Handle::kkt_from_sums(const double* sums, int K, int k) const {
     return e;
 }
 
+// Opt-in infeasibility test at a check (no counterpart in the reference): the three
+// candidate rays of EpiRayRows / EpiRayCols, in order.  A dual ray e with a positive Farkas
+// objective and sign violation at most infeasibility_tol times that objective certifies
+// primal infeasibility; a primal ray d with c'd < 0 and cone violation at most
+// infeasibility_tol * |c'd| certifies dual infeasibility.  Returns a status, or
+// CPDLP_STILL_RUNNING when neither qualifies.
+template <class V>
+int32_t Handle::ray_check(IterState<V>& S) {
+    const hipStream_t s = stream;
+    const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
+    const int gA = vA.partial_blocks(), gAt = vAt.partial_blocks();
+    constexpr int NQR = EpiRayRows<V>::NQ, NQC = EpiRayCols<V>::NQ;
+    ensure_partials((size_t)std::max(gA * NQR, gAt * NQC) + 16);
+    const int c = S.cur, l = 1 - S.cur;
+    EpiRayRows<V> er;
+    er.partials = d_partials.p;
+    er.y = S.y[c].p;
+    er.y_rst = S.yrst.p;
+    er.y_last = S.y[l].p;
+    er.lc = lc_s.p;
+    er.uc = uc_s.p;
+    er.invD1 = invD1_s.p;
+    launch_spmv<V, double, 3>(vA, XPack<V, 3>{{S.x[c].p, S.xrst.p, S.x[l].p}}, er, s);
+    EpiRayCols<V> ec;
+    ec.partials = d_partials2.p;
+    ec.x = S.x[c].p;
+    ec.x_rst = S.xrst.p;
+    ec.x_last = S.x[l].p;
+    ec.c = c_s.p;
+    ec.lv = lv_s.p;
+    ec.uv = uv_s.p;
+    ec.invD2 = invD2_s.p;
+    launch_spmv<V, double, 3>(vAt, XPack<V, 3>{{S.y[c].p, S.yrst.p, S.y[l].p}}, ec, s);
+    if (gA > 0)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gA, NQR, d_partials.p,
+                           d_sums.p, (PowState*)nullptr, POW_NONE);
+    else
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQR, d_sums.p, 0.0);
+    if (gAt > 0)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gAt, NQC, d_partials2.p,
+                           d_sums.p + NQR, (PowState*)nullptr, POW_NONE);
+    else
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQC, d_sums.p + NQR, 0.0);
+    HIP_CHECK(hipGetLastError());
+    double q[NQR + NQC];
+    d_sums.download(q, NQR + NQC, s);
+    sync();
+    const double* r = q;
+    const double* cl = q + NQR;
+    const double tol = P.infeasibility_tol;
+    for (int k = 0; k < 3; ++k) {
+        const double obj = r[3 + k] + cl[k];
+        if (obj > 0.0 && std::sqrt(r[6 + k] + cl[3 + k]) <= tol * obj) return CPDLP_PRIMAL_INFEASIBLE;
+        const double cd = cl[6 + k];
+        if (cd < 0.0 && std::sqrt(r[k] + cl[9 + k]) <= tol * -cd) return CPDLP_DUAL_INFEASIBLE;
+    }
+    return CPDLP_STILL_RUNNING;
+}
+
 static double relative(const cpdlp_kkt& e) {   // errors.jl:55-58
     return std::max(std::max(e.primal / e.primal_scale, e.dual / e.dual_scale), e.gap / e.gap_scale);
 }
@@ -1017,6 +1078,15 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_
         stats.err = kkt_from_sums(sums, K, 0);
         push_history();
         stats.termination_status = term_status(stats.err);
+        if (P.detect_infeasibility && stats.termination_status != CPDLP_OPTIMAL) {
+            // opt-in: NaN / infinite errors, then the ray certificates
+            const int32_t st = std::isfinite(relative(stats.err)) ? ray_check<V>(S) : CPDLP_NUMERICAL_ERROR;
+            if (st != CPDLP_STILL_RUNNING) {
+                stats.termination_status = st;
+                if (P.show_progress) progress(true);
+                return;
+            }
+        }
         if (P.show_progress) progress(false);
         bool switch_after = false;
         if (stats.termination_status == CPDLP_STILL_RUNNING && !final_phase) {
@@ -1489,6 +1559,8 @@ int cpdlp_default_params(cpdlp_params* p, int32_t algorithm) {
         p->use_graph = 1;
         p->long_row = 64;
         p->batch_streams = 4;
+        p->detect_infeasibility = 0;
+        p->infeasibility_tol = 1e-8;
     });
 }
 

@@ -117,6 +117,8 @@ struct Handle {
                     double* host_sums);
     cpdlp_kkt kkt_from_sums(const double* sums, int K, int k) const;
     template <class V>
+    int32_t ray_check(IterState<V>& S);
+    template <class V>
     void launch_steps(IterState<V>& S, int nsteps, bool pdlp);
     template <class V>
     void run_chunk(IterState<V>& S, bool pdlp);

@@ -69,6 +69,8 @@ mutable struct CParams
     use_graph::Int32
     long_row::Int32
     batch_streams::Int32
+    detect_infeasibility::Int32   # left at 0: the reference's TerminationStatus has no infeasible values
+    infeasibility_tol::Float64
     CParams() = new()
 end
 

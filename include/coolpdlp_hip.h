@@ -37,6 +37,11 @@ extern "C" {
 #define CPDLP_TIME_LIMIT 1
 #define CPDLP_ITERATION_LIMIT 2
 #define CPDLP_STILL_RUNNING 3
+/* Only with cpdlp_params.detect_infeasibility = 1 (no counterpart in the reference,
+ * whose TerminationStatus stops at STILL_RUNNING): */
+#define CPDLP_PRIMAL_INFEASIBLE 4  /* a dual ray (Farkas certificate) was found */
+#define CPDLP_DUAL_INFEASIBLE 5    /* a primal ray: the LP is unbounded or infeasible */
+#define CPDLP_NUMERICAL_ERROR 6    /* the KKT errors became NaN or infinite */
 
 #define CPDLP_ALGO_PDLP 0       /* Algorithm{:PDLP}, src/algorithms/pdlp.jl:6-11 */
 #define CPDLP_ALGO_PDHG 1       /* Algorithm{:PDHG}, src/algorithms/pdhg.jl:6-16 */
@@ -100,6 +105,10 @@ typedef struct cpdlp_params {
     int32_t use_graph;              /* capture the check_every-step chunk in a hipGraph */
     int32_t long_row;               /* rows longer than this use a wavefront per row (64) */
     int32_t batch_streams;          /* HIP streams of a batch (cpdlp_batch_*), 1..8 (4) */
+    int32_t detect_infeasibility;   /* 0; 1: test ray certificates at every check that is
+                                       not OPTIMAL, and stop on NaN errors (plain engine) */
+    double infeasibility_tol;       /* 1e-8: a ray qualifies when its violation is at most
+                                       this times its objective */
 } cpdlp_params;
 
 /* KKTErrors, src/components/errors.jl:8-21 */

@@ -66,6 +66,9 @@ void orc_default_params(orc_params* p, int32_t algorithm) {
     p->powm_tol = NAN;
     p->powm_maxiter = -1;
     p->powm_seed = 0;
+    p->detect_infeasibility = 0;
+    p->ray_candidates = 7;
+    p->infeasibility_tol = 1e-8;
 }
 
 /* ---------------------------------------------------------------- helpers */
@@ -457,6 +460,77 @@ static void record(orc_stats* st, orc_hist* hist, int64_t cap, int rec) {
     st->n_history++;
 }
 
+/* Opt-in infeasibility test (no counterpart in the reference), on the ray
+ * (dx, dy) in scaled space.  In original units x = D2 x_s, y = D1 y_s, A x = A_s x_s / D1,
+ * A'y = A_s'y_s / D2; a bound times a ray component is the same in either space.
+ * Dual ray e = dy, r = -A'e: Farkas objective lc e+ - uc e- + lv r+ - uv r- over finite
+ * bounds, with sign violations e+ (lc = -inf), e- (uc = +inf), r+ (lv = -inf),
+ * r- (uv = +inf); primal infeasible when objective > 0 and |violations| <= tol * objective.
+ * Primal ray d = dx: c'd < 0 and |violation of A d in the recession cone of [lc, uc] and
+ * of d in that of [lv, uv]| <= tol * |c'd| gives dual infeasible.  Returns 4, 5 or 3. */
+static int32_t ray_test(const orc_lp* L, const double* dx, const double* dy, double tol) {
+    int64_t m = L->m, n = L->n;
+    double* Ad = (double*)malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+    double* Ate = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    orc_spmv_csr(m, L->a_rowptr, L->a_col, L->a_val, 1.0, dx, 0.0, Ad);
+    orc_spmv_csr(n, L->at_rowptr, L->at_col, L->at_val, 1.0, dy, 0.0, Ate);
+    double obj = 0.0, sviol = 0.0, cd = 0.0, cviol = 0.0;
+    for (int64_t i = 0; i < m; ++i) {
+        int lf = isfinite(L->lc[i]), uf = isfinite(L->uc[i]);
+        double e = dy[i], ep = pos_part(e), en = neg_part(e);
+        obj += (lf ? L->lc[i] * ep : 0.0) - (uf ? L->uc[i] * en : 0.0);
+        double sv = ((lf ? 0.0 : ep) + (uf ? 0.0 : en)) * L->D1[i];
+        sviol += sv * sv;
+        double v = Ad[i] / L->D1[i];
+        double cv = lf && uf ? v : lf ? fmin(v, 0.0) : uf ? fmax(v, 0.0) : 0.0;
+        cviol += cv * cv;
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        int lf = isfinite(L->lv[j]), uf = isfinite(L->uv[j]);
+        double r = -Ate[j], rp = pos_part(r), rn = neg_part(r);
+        obj += (lf ? L->lv[j] * rp : 0.0) - (uf ? L->uv[j] * rn : 0.0);
+        double sv = ((lf ? 0.0 : rp) + (uf ? 0.0 : rn)) / L->D2[j];
+        sviol += sv * sv;
+        double d = dx[j];
+        cd += L->c[j] * d;
+        double cv = (lf && uf ? d : lf ? fmin(d, 0.0) : uf ? fmax(d, 0.0) : 0.0) * L->D2[j];
+        cviol += cv * cv;
+    }
+    free(Ad);
+    free(Ate);
+    if (obj > 0.0 && sqrt(sviol) <= tol * obj) return 4;
+    if (cd < 0.0 && sqrt(cviol) <= tol * -cd) return 5;
+    return 3;
+}
+
+int32_t orc_ray_test(int64_t m, int64_t n, const int64_t* a_rowptr, const int32_t* a_col,
+                     const double* a_val, const int64_t* at_rowptr, const int32_t* at_col,
+                     const double* at_val, const double* c, const double* lv, const double* uv,
+                     const double* lc, const double* uc, const double* D1, const double* D2,
+                     const double* dx, const double* dy, double tol) {
+    orc_lp L = {m, n, a_rowptr, a_col, a_val, at_rowptr, at_col, at_val, c, lv, uv, lc, uc, D1, D2, 0, 0, 0};
+    return ray_test(&L, dx, dy, tol);
+}
+
+static int32_t infeasibility_status(const orc_lp* L, const orc_params* p, const orc_kkt* err,
+                                    const pd* sol, const pd* rst, const pd* last) {
+    if (!isfinite(relative(err))) return 6;
+    int64_t m = L->m, n = L->n;
+    double* dx = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    double* dy = (double*)malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+    int32_t st = 3;
+    for (int k = 0; k < 3 && st == 3; ++k) {
+        if (!(p->ray_candidates >> k & 1)) continue;
+        const pd* o = k == 0 ? rst : last;
+        for (int64_t j = 0; j < n; ++j) dx[j] = k == 1 ? sol->x[j] : sol->x[j] - o->x[j];
+        for (int64_t i = 0; i < m; ++i) dy[i] = k == 1 ? sol->y[i] : sol->y[i] - o->y[i];
+        st = ray_test(L, dx, dy, p->infeasibility_tol);
+    }
+    free(dx);
+    free(dy);
+    return st;
+}
+
 /* termination_status, termination.jl:89-101 */
 static int32_t term_status(const orc_stats* st, const orc_params* p) {
     if (relative(&st->err) <= p->termination_reltol) return 0;
@@ -512,8 +586,9 @@ int orc_solve(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval
     int pdlp = p->algorithm == 0;
     if (pdlp) {                                       /* pdlp.jl:51-54 */
         memcpy(avg.x, sol.x, (size_t)n * sizeof(double)); memcpy(avg.y, sol.y, (size_t)m * sizeof(double));
-        memcpy(rst.x, sol.x, (size_t)n * sizeof(double)); memcpy(rst.y, sol.y, (size_t)m * sizeof(double));
     }
+    /* restart = sol (PDHG keeps it as the start point: the infeasibility test's origin) */
+    memcpy(rst.x, sol.x, (size_t)n * sizeof(double)); memcpy(rst.y, sol.y, (size_t)m * sizeof(double));
 
     /* ---- initialize: η (step_size.jl:22-27) and ω (:29-40; PDHG ω = 1, pdhg.jl:48) */
     double tol = isnan(p->powm_tol) ? powm_default_tol(n) : p->powm_tol;
@@ -583,6 +658,10 @@ int orc_solve(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval
             kkt_errors(&L, sol.x, sol.y, &st->err);
             record(st, hist, hist_cap, p->record_error_history);
             st->termination_status = term_status(st, p);
+            if (p->detect_infeasibility && st->termination_status != 0) {
+                int32_t inf = infeasibility_status(&L, p, &st->err, &sol, &rst, &sol_last);
+                if (inf != 3) { st->termination_status = inf; goto finish; }
+            }
             if (st->termination_status != 3) { terminated = 1; break; }
             if (!pdlp) continue;
             /* restart_check!, pdlp.jl:138-170 */

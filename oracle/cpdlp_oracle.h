@@ -35,6 +35,12 @@ typedef struct {
     double powm_tol;                /* NaN: eps(T) * n^3 (Int64 wrap) */
     int64_t powm_maxiter;           /* -1: n */
     uint64_t powm_seed;             /* seed of the power-method start vector */
+    /* opt-in infeasibility detection (no counterpart in the reference; the engine's
+     * detect_infeasibility / infeasibility_tol): status 4 PRIMAL_INFEASIBLE,
+     * 5 DUAL_INFEASIBLE, 6 NUMERICAL_ERROR */
+    int32_t detect_infeasibility;   /* 0 */
+    int32_t ray_candidates;         /* bit 0: z - z_restart, bit 1: z, bit 2: z - z_last */
+    double infeasibility_tol;       /* 1e-8 */
 } orc_params;
 
 /* KKTErrors, src/components/errors.jl:8-21 */
@@ -52,7 +58,7 @@ typedef struct {
     orc_kkt err;
     double time_elapsed;
     int64_t kkt_passes;
-    int32_t termination_status;     /* 0 OPTIMAL 1 TIME_LIMIT 2 ITERATION_LIMIT 3 STILL_RUNNING */
+    int32_t termination_status;     /* 0 OPTIMAL 1 TIME_LIMIT 2 ITERATION_LIMIT 3 STILL_RUNNING (4-6 above) */
     int32_t pad;
     double eta, omega;
     int64_t powm_iters;
@@ -62,6 +68,14 @@ typedef struct {
 } orc_stats;
 
 void orc_default_params(orc_params* p, int32_t algorithm);
+
+/* The opt-in infeasibility test on one ray (dx, dy) of the scaled LP (A_s in CSR, A_s'
+ * in CSR, scaled vectors, D1 / D2): 4 primal infeasible, 5 dual infeasible, 3 neither. */
+int32_t orc_ray_test(int64_t m, int64_t n, const int64_t* a_rowptr, const int32_t* a_col,
+                     const double* a_val, const int64_t* at_rowptr, const int32_t* at_col,
+                     const double* at_val, const double* c, const double* lv, const double* uv,
+                     const double* lc, const double* uc, const double* D1, const double* D2,
+                     const double* dx, const double* dy, double tol);
 void orc_set_threads(int32_t nthreads);
 
 /* CSC (m x n) -> CSR (m x n) by stable counting sort: sparse(transpose(A)),

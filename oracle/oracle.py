@@ -29,7 +29,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcpdlp_oracle.so")
 
-STATUS_NAMES = ["OPTIMAL", "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING"]
+STATUS_NAMES = ["OPTIMAL", "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING", "PRIMAL_INFEASIBLE",
+                "DUAL_INFEASIBLE", "NUMERICAL_ERROR"]
 
 
 class Params(C.Structure):
@@ -42,6 +43,8 @@ class Params(C.Structure):
         ("record_error_history", C.c_int32), ("termination_reltol", C.c_double),
         ("max_kkt_passes", C.c_int64), ("time_limit", C.c_double),
         ("powm_tol", C.c_double), ("powm_maxiter", C.c_int64), ("powm_seed", C.c_uint64),
+        ("detect_infeasibility", C.c_int32), ("ray_candidates", C.c_int32),
+        ("infeasibility_tol", C.c_double),
     ]
 
 
@@ -101,6 +104,9 @@ def lib():
         L.orc_solve.argtypes = [C.c_int64, C.c_int64, i64, i32, d, d, d, d, d, d, d, d,
                                 P(Params), d, d, P(Stats), P(Hist), C.c_int64]
         L.orc_solve.restype = C.c_int
+        L.orc_ray_test.argtypes = [C.c_int64, C.c_int64, i64, i32, d, i64, i32, d,
+                                   d, d, d, d, d, d, d, d, d, C.c_double]
+        L.orc_ray_test.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -188,6 +194,18 @@ def kkt_errors(m, n, A_csr, At_csr, c, lv, uv, lc, uc, D1, D2, x, y) -> KKT:
                          _p(tr, C.c_int64), _p(tc, C.c_int32), _p(tv, C.c_double),
                          *[_p(v, C.c_double) for v in vs], C.byref(out))
     return out
+
+
+def ray_test(m, n, colptr, rowval, nzval, c, lv, uv, lc, uc, D1, D2, dx, dy, tol=1e-8) -> str:
+    """The opt-in infeasibility test on one ray (dx, dy) of an LP given in scaled form
+    (CSC A_s and scaled vectors) with its scaling D1, D2."""
+    A_csr = csc_to_csr(m, n, colptr, rowval, nzval)
+    colptr, rowval, nzval = _i64(colptr), _i32(rowval), _f64(nzval)
+    vs = [_f64(v) for v in (c, lv, uv, lc, uc, D1, D2, dx, dy)]
+    st = lib().orc_ray_test(m, n, _p(A_csr[0], C.c_int64), _p(A_csr[1], C.c_int32),
+                            _p(A_csr[2], C.c_double), _p(colptr, C.c_int64), _p(rowval, C.c_int32),
+                            _p(nzval, C.c_double), *[_p(v, C.c_double) for v in vs], float(tol))
+    return STATUS_NAMES[st]
 
 
 def powm_start(n, seed=0):
