@@ -82,11 +82,14 @@ class Server:
     lanes: tuple = ((1_000_000, 2), (float("inf"), 1))
     time_limit: float = 600.0
     keep_solutions: bool = False
+    # opt-in: infeasible / unbounded / NaN requests stop at the check that proves it
+    # (PRIMAL_INFEASIBLE, DUAL_INFEASIBLE, NUMERICAL_ERROR) instead of at a limit
+    detect_infeasibility: bool = False
     done: list = field(default_factory=list)
 
     def __post_init__(self):
         algo = PDLP(self.precision, termination_reltol=self.reltol, time_limit=self.time_limit,
-                    batch_streams=self.batch_streams)
+                    batch_streams=self.batch_streams, detect_infeasibility=int(self.detect_infeasibility))
         self.batch = BatchSolver(algo, hbm_bytes=self.batch_bytes)
         self.waiting: deque = deque()
         self.in_batch: dict = {}
@@ -131,7 +134,7 @@ class Server:
             milp = MILP.from_generated(r.lp)
             # fixed CSR format: a request's result must not depend on autotuner timing
             algo = PDLP(self.precision, None, "csr", termination_reltol=self.reltol,
-                        time_limit=self.time_limit)
+                        time_limit=self.time_limit, detect_infeasibility=int(self.detect_infeasibility))
             with Engine(milp, algo) as eng:
                 sol, st = eng.solve(r.sol)
             r.status = st.termination_status.name

@@ -62,6 +62,7 @@ struct BParams {
     double powm_tol;   // NaN: eps(Float64) * n^3 rule
     int64_t max_kkt_passes, powm_maxiter;
     uint64_t seed;
+    double inf_tol;    // opt-in infeasibility detection (k_batch_check<V, true>)
 };
 
 constexpr int BB = 256;   // threads per instance workgroup
@@ -795,12 +796,89 @@ __global__ __launch_bounds__(NSUB * BB) void k_batch_steps(const int32_t* list, 
     }
 }
 
+// ---------------------------------------------------------------- infeasibility rays
+// The plain engine's EpiRayRows / EpiRayCols on one instance: products over
+// {current, restart, previous} give the three candidate rays by linearity; sums in
+// original units.  Returns PRIMAL_INFEASIBLE, DUAL_INFEASIBLE or STILL_RUNNING (block-
+// uniform).  Ends synchronised.
+template <class V>
+__device__ int32_t batch_rays(const BInst& I, const Iter<V>& it, int cur, double tol, double* kprod,
+                              double (*kred)[BB / WAVE], double* sh) {
+    const V* const xc = cur ? it.x[1] : it.x[0];
+    const V* const yc = cur ? it.y[1] : it.y[0];
+    const V* const xl = cur ? it.x[0] : it.x[1];
+    const V* const yl = cur ? it.y[0] : it.y[1];
+    auto ray = [](int k, double a, double r, double l) { return k == 0 ? a - r : k == 1 ? a : a - l; };
+    double qa[9], qb[12];
+    for (int t = 0; t < 9; ++t) qa[t] = 0.0;
+    for (int t = 0; t < 12; ++t) qb[t] = 0.0;
+    tiled_pass_k<3>(I.a_tiles, I.n_a_tiles, I.a_rp, I.a_col, it.a, kprod, kred,
+                    [&](int kv, int col) {
+                        return (double)(kv == 0 ? xc[col] : kv == 1 ? it.xr[col] : xl[col]);
+                    },
+                    [&](int i, const double* acc) {
+                        const double lo = I.lcs[i], up = I.ucs[i], id = I.iD1[i];
+                        const bool lf = isfinite(lo), uf = isfinite(up);
+                        const double y = (double)yc[i], yr = (double)it.yr[i], yo = (double)yl[i];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const double v = id * ray(k, acc[0], acc[1], acc[2]);
+                            const double cv = lf && uf ? v : lf ? fmin(v, 0.0) : uf ? fmax(v, 0.0) : 0.0;
+                            qa[k] += cv * cv;
+                            const double e = ray(k, y, yr, yo);
+                            const double ep = pos_part(e), en = neg_part(e);
+                            qa[3 + k] += (lf ? lo * ep : 0.0) - (uf ? up * en : 0.0);
+                            const double sv = ((lf ? 0.0 : ep) + (uf ? 0.0 : en)) / id;
+                            qa[6 + k] += sv * sv;
+                        }
+                    });
+    tiled_pass_k<3>(I.at_tiles, I.n_at_tiles, I.at_rp, I.at_col, it.at, kprod, kred,
+                    [&](int kv, int col) {
+                        return (double)(kv == 0 ? yc[col] : kv == 1 ? it.yr[col] : yl[col]);
+                    },
+                    [&](int j, const double* acc) {
+                        const double cj = I.cs[j], lo = I.lvs[j], up = I.uvs[j], id = I.iD2[j];
+                        const bool lf = isfinite(lo), uf = isfinite(up);
+                        const double x = (double)xc[j], xr = (double)it.xr[j], xo = (double)xl[j];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const double r = -ray(k, acc[0], acc[1], acc[2]);
+                            const double rp = pos_part(r), rn = neg_part(r);
+                            qb[k] += (lf ? lo * rp : 0.0) - (uf ? up * rn : 0.0);
+                            const double sv = id * ((lf ? 0.0 : rp) + (uf ? 0.0 : rn));
+                            qb[3 + k] += sv * sv;
+                            const double d = ray(k, x, xr, xo);
+                            qb[6 + k] += cj * d;
+                            const double cv = (lf && uf ? d : lf ? fmin(d, 0.0) : uf ? fmax(d, 0.0) : 0.0) / id;
+                            qb[9 + k] += cv * cv;
+                        }
+                    });
+    bsum<9>(qa, sh);
+    double ra[9];
+    for (int t = 0; t < 9; ++t) ra[t] = sh[t];
+    __syncthreads();
+    bsum<12>(qb, sh);
+    double rb[12];
+    for (int t = 0; t < 12; ++t) rb[t] = sh[t];
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) {
+        const double obj = ra[3 + k] + rb[k];
+        if (obj > 0.0 && sqrt(ra[6 + k] + rb[3 + k]) <= tol * obj) return CPDLP_PRIMAL_INFEASIBLE;
+        const double cd = rb[6 + k];
+        if (cd < 0.0 && sqrt(ra[k] + rb[9 + k]) <= tol * -cd) return CPDLP_DUAL_INFEASIBLE;
+    }
+    return CPDLP_STILL_RUNNING;
+}
+
 // ---------------------------------------------------------------- check kernel
 // After each chunk of steps: the 5-candidate KKT check in FP64 (errors.jl:65-109),
 // termination (termination.jl:89-101), restart_check! / restart! and the primal-weight
 // update (pdlp.jl:138-196, restart.jl:40-57, step_size.jl:58-75), and the mixed
-// precision's hand-over to FP64.
-template <class V>
+// precision's hand-over to FP64.  RAYS (opt-in detect_infeasibility): at a check that is
+// not OPTIMAL, NaN / infinite errors stop the instance, and three candidate rays -- the
+// iterate minus the restart point, the iterate, the iterate minus the previous one --
+// are tested as a Farkas dual ray and as a primal ray, as the plain engine's ray_check.
+template <class V, bool RAYS = false>
 __global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const BInst* insts,
                                                     BState* states, const int32_t* timeout,
                                                     BParams P) {
@@ -906,6 +984,13 @@ __global__ __launch_bounds__(BB) void k_batch_check(const int32_t* list, const B
             for (int f = 0; f < 6; ++f) S->err[f] = e[0][f];
         }
         int restart = 0, from_avg = 0, switch_phase = 0, improve = 0, blowup = 0, reached = 0;
+        if constexpr (RAYS) {
+            if (status != CPDLP_OPTIMAL) {   // uniform: rel and status come from block sums
+                const int32_t rs = isfinite(rel) ? batch_rays(I, it, cur, P.inf_tol, kprod, kred, sh)
+                                                 : CPDLP_NUMERICAL_ERROR;
+                if (rs != CPDLP_STILL_RUNNING) status = rs;
+            }
+        }
         if (status == CPDLP_STILL_RUNNING && polish_next) {
             // FP32 phase: keep the best iterate; hand over on blow-up (at once, restart
             // from the best) or on stall (after this check's restart decision, continuing)
@@ -1260,8 +1345,8 @@ struct Batch {
         HIP_CHECK(hipGetDeviceCount(&ndev));
         if (P.device < 0 || P.device >= ndev) throw Error(CPDLP_ERR_ARGUMENT, "invalid device ordinal");
         if (P.check_every <= 0) throw Error(CPDLP_ERR_ARGUMENT, "check_every must be positive");
-        if (P.detect_infeasibility)
-            throw Error(CPDLP_ERR_ARGUMENT, "detect_infeasibility is not supported by the batch engine");
+        if (P.detect_infeasibility && !(P.infeasibility_tol > 0.0))
+            throw Error(CPDLP_ERR_ARGUMENT, "infeasibility_tol must be positive");
         device = P.device;
         HIP_CHECK(hipSetDevice(device));
         streams.assign((size_t)std::min(8, std::max(1, P.batch_streams)), nullptr);
@@ -1325,6 +1410,7 @@ struct Batch {
         b.max_kkt_passes = P.max_kkt_passes;
         b.powm_maxiter = P.powm_maxiter;
         b.seed = P.powm_seed;
+        b.inf_tol = P.infeasibility_tol;
         return b;
     }
 
@@ -1615,8 +1701,12 @@ struct Batch {
                             else
                                 hipLaunchKernelGGL((k_batch_steps<float, false, WIDE>), grid, wblk, 0, s, L,
                                                    d_inst.p, d_state.p, bp);
-                            hipLaunchKernelGGL(k_batch_check<float>, grid, blk, 0, s, L, d_inst.p, d_state.p,
-                                               d_timeout.p, bp);
+                            if (P.detect_infeasibility)
+                                hipLaunchKernelGGL((k_batch_check<float, true>), grid, blk, 0, s, L, d_inst.p,
+                                                   d_state.p, d_timeout.p, bp);
+                            else
+                                hipLaunchKernelGGL(k_batch_check<float>, grid, blk, 0, s, L, d_inst.p, d_state.p,
+                                                   d_timeout.p, bp);
                         } else {
                             if (c < NLDS)
                                 hipLaunchKernelGGL((k_batch_steps<double, true>), grid, blk, lds_bytes[c], s, L,
@@ -1630,8 +1720,12 @@ struct Batch {
                             else
                                 hipLaunchKernelGGL((k_batch_steps<double, false, WIDE>), grid, wblk, 0, s, L,
                                                    d_inst.p, d_state.p, bp);
-                            hipLaunchKernelGGL(k_batch_check<double>, grid, blk, 0, s, L, d_inst.p, d_state.p,
-                                               d_timeout.p, bp);
+                            if (P.detect_infeasibility)
+                                hipLaunchKernelGGL((k_batch_check<double, true>), grid, blk, 0, s, L, d_inst.p,
+                                                   d_state.p, d_timeout.p, bp);
+                            else
+                                hipLaunchKernelGGL(k_batch_check<double>, grid, blk, 0, s, L, d_inst.p, d_state.p,
+                                                   d_timeout.p, bp);
                         }
                         HIP_CHECK(hipGetLastError());
                     }

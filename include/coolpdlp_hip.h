@@ -106,7 +106,7 @@ typedef struct cpdlp_params {
     int32_t long_row;               /* rows longer than this use a wavefront per row (64) */
     int32_t batch_streams;          /* HIP streams of a batch (cpdlp_batch_*), 1..8 (4) */
     int32_t detect_infeasibility;   /* 0; 1: test ray certificates at every check that is
-                                       not OPTIMAL, and stop on NaN errors (plain engine) */
+                                       not OPTIMAL, and stop on NaN errors */
     double infeasibility_tol;       /* 1e-8: a ray qualifies when its violation is at most
                                        this times its objective */
 } cpdlp_params;

@@ -3,7 +3,10 @@ counterpart in the reference): the engine's verdicts equal the oracle's (and so
 SciPy-HiGHS's classification) on the infeasible and unbounded LPs of
 tests/test_infeasibility.py, in FP64 and mixed precision, PDLP and PDHG, at about the
 same pass; feasible LPs keep their trajectory bit for bit; NaN stops the solve; the MOI
-front end maps the statuses; the batch engine refuses the option."""
+front end maps the statuses; the batch engine and both serving routes give the same
+verdicts."""
+import time
+
 import numpy as np
 import pytest
 
@@ -70,5 +73,36 @@ def test_nan_and_front_ends(cp):
         opt.add_constraint(moi.ScalarAffineFunction([(1.0, x), (1.0, y)]), moi.GreaterThan(3.0))
         opt.optimize()
         assert opt.termination_status == want
-    with pytest.raises(cp.ArgumentError):
-        cp.BatchSolver(cp.PDLP(detect_infeasibility=True), hbm_bytes=16 << 20)
+
+
+@pytest.mark.parametrize("prec", ["float64", "mixed"])
+def test_batch_and_serving_verdicts(cp, prec):
+    """The batch engine's verdicts equal the plain engine's on the same LPs, mixed in one
+    batch with feasible ones, which keep their bits (detection only reads the iterates)."""
+    from coolpdlp_hip import generators as G
+    from coolpdlp_hip.serving import Server
+    bad = [(name, milp, WANT[hs]) for name, milp, hs in cases()]
+    good = [G.column_lp(m, 2 * m, (4, 8), 40 + m) for m in (80, 200, 350)]
+    args = dict(termination_reltol=1e-6, max_kkt_passes=200_000, time_limit=60.0)
+    with cp.BatchSolver(cp.PDLP(prec, **args), hbm_bytes=64 << 20) as B:
+        ref = B.solve_all(good)
+    with cp.BatchSolver(cp.PDLP(prec, detect_infeasibility=True, **args), hbm_bytes=64 << 20) as B:
+        res = B.solve_all([milp for _, milp, _ in bad] + good)
+    for (name, milp, want), (_, st) in zip(bad, res):
+        assert st.termination_status.name == want, (name, prec, st.termination_status, st.kkt_passes)
+    for g, (sol, st), (sol0, st0) in zip(good, res[len(bad):], ref):
+        assert st.termination_status == st0.termination_status == cp.OPTIMAL
+        assert st.kkt_passes == st0.kkt_passes and np.array_equal(sol.x, sol0.x), g.name
+    # serving: an infeasible request on each route stops with its verdict
+    for limit in (8000, 0):
+        srv = Server(reltol=1e-6, precision=prec, batch_bytes=64 << 20, plain_bytes=1 << 30,
+                     batch_limit=limit, detect_infeasibility=True)
+        milp = bad[3][1]
+        g = G.GeneratedLP(milp.A.shape[0], milp.A.shape[1], milp.A.indptr.astype(np.int64),
+                          milp.A.indices.astype(np.int32), milp.A.data.copy(), milp.c, milp.lv,
+                          milp.uv, milp.lc, milp.uc)
+        srv.submit(g)
+        srv.drain(time.perf_counter() + 120)
+        srv.close()
+        r, = srv.done
+        assert r.status == bad[3][2] and r.route == ("batch" if limit else "plain"), (r.status, r.route)
