@@ -77,6 +77,11 @@ def make_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def polish(args) -> dict:
+    """The mixed precision's hand-over tolerance, when --polish-reltol sets it."""
+    return {} if args.polish_reltol is None else {"polish_reltol": args.polish_reltol}
+
+
 def bench_batch(args):
     """4,096 uneven LPs in one batch (one workgroup per LP, mixed precision, reltol 1e-6
     per LP) against the same LPs solved one at a time on the plain GPU path (timed on a
@@ -91,7 +96,7 @@ def bench_batch(args):
     nnz = sum(g.nnz for g in lps)
     log(f"[bench] generated {len(lps)} LPs, nnz={nnz} in {time.perf_counter() - t:.1f}s")
     reltol = 1e-6
-    algo = cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit)
+    algo = cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit, **polish(args))
     arena = int(sum(cp.batch.footprint(g.m, g.n, g.nnz) for g in lps) * 1.05) + (64 << 20)
     B = cp.BatchSolver(algo, hbm_bytes=arena)
     t = time.perf_counter()
@@ -127,7 +132,8 @@ def bench_batch(args):
     for k in sample:
         g = lps[k]
         milp = cp.MILP.from_generated(g)
-        eng = cp.Engine(milp, cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit))
+        eng = cp.Engine(milp, cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit,
+                                      **polish(args)))
         torch.cuda.synchronize()
         t = time.perf_counter()
         _, st = eng.solve()
@@ -143,6 +149,7 @@ def bench_batch(args):
         "data": "synthetic (seeded generator with known optima)",
         "config": {"workload": "batch_4096_uneven_lps", "count": len(lps), "nnz_total": nnz,
                    "m_range": [50, 5000], "reltol": reltol, "precision": "mixed",
+                   "polish_reltol": algo.params.polish_reltol,
                    "checks_per_round": args.checks_per_round},
         "batch_seconds": elapsed, "rounds": rounds,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
@@ -181,7 +188,7 @@ def bench_mcf(args):
         return viol / (1 + float(np.max(np.abs(g.lc))))
 
     eng = cp.Engine(milp, cp.PDLP("mixed", termination_reltol=1e-6, time_limit=args.time_limit,
-                                  powm_tol=args.powm_tol, show_progress=1))
+                                  powm_tol=args.powm_tol, show_progress=1, **polish(args)))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sol, st = eng.solve()
@@ -211,7 +218,8 @@ def bench_mcf(args):
         "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 bulk / f64 checks+polish", "data": "synthetic (seeded min-cost flow)",
         "config": {"workload": "min_cost_flow_mixed", "nodes": g.m, "arcs": g.n, "nnz": g.nnz,
-                   "reltol": 1e-6, "reference_reltol": 1e-8, "powm_tol": args.powm_tol},
+                   "reltol": 1e-6, "reference_reltol": 1e-8, "powm_tol": args.powm_tol,
+                   "polish_reltol": args.polish_reltol if args.polish_reltol is not None else 1e-5},
         "status": st.termination_status.name, "kkt_passes": st.kkt_passes,
         "fp32_passes": st.fp32_passes, "relative_kkt": st.err.relative(),
         "objective": obj, "objective_fp64_ref": obj_ref,
@@ -379,6 +387,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-limit", type=float, default=600.0)
     ap.add_argument("--powm-tol", type=float, default=1e-4)
+    ap.add_argument("--polish-reltol", type=float, default=None,
+                    help="mixed precision: FP32 phase stops at max(this, reltol) (engine default 1e-5)")
     # profiling runs only: cap the solve at this many passes (same kernels, fewer of
     # them); the default solves to the tolerance
     ap.add_argument("--max-passes", type=int, default=10**7)
