@@ -4,8 +4,9 @@ dependency, Project.toml:20,36; not vendored), which the reference turns into a 
 column layout), written from the format's definition, for linear and mixed-integer
 programs:
 
-  NAME, OBJSENSE (MIN / MAX, on the same or the next line), ROWS (N = objective, the
-  first N row only; E / L / G), COLUMNS (with MARKER INTORG / INTEND blocks for integer
+  NAME, OBJSENSE (MIN / MAX, on the same or the next line), ROWS (the first N row is the
+  objective, later N rows are free rows whose entries, RHS and RANGES are ignored; E / L /
+  G; an entry on an undeclared row is an error), COLUMNS (with MARKER INTORG / INTEND blocks for integer
   variables), RHS (a value on the objective row is minus the objective constant), RANGES
   (E: [rhs, rhs+|R|] for R > 0, [rhs-|R|, rhs] for R < 0; L: [rhs-|R|, rhs];
   G: [rhs, rhs+|R|]), BOUNDS (UP, LO, FX, FR, MI, PL, BV, LI, UI; an UP bound below 0 on
@@ -65,6 +66,7 @@ def read_mps(source) -> QPSData:
             text = f.read()
     q = QPSData()
     row_index: dict = {}
+    free_rows: set = set()             # N rows after the objective: ignored
     row_type: list = []
     col_index: dict = {}
     c_list: list = []
@@ -104,7 +106,9 @@ def read_mps(source) -> QPSData:
             if t == "N":
                 if not q.objname:
                     q.objname = nm
-                continue   # further N rows are free rows: ignored
+                else:
+                    free_rows.add(nm)   # further N rows are free rows: ignored
+                continue
             if t not in ("E", "L", "G"):
                 raise ArgumentError(f"unknown row type {t!r}")
             if nm in row_index:
@@ -135,7 +139,8 @@ def read_mps(source) -> QPSData:
                     arows.append(row_index[rn])
                     acols.append(j)
                     avals.append(val)
-                # entries of other N rows are ignored
+                elif rn not in free_rows:
+                    raise ArgumentError(f"entry on unknown row {rn!r}")
         elif section in ("RHS", "RANGES"):
             # (set name,) row, value (, row, value): an odd field count carries a set name
             pairs = f[1:] if len(f) % 2 == 1 else f
@@ -146,11 +151,11 @@ def read_mps(source) -> QPSData:
                         q.c0 = -val
                     elif rn in row_index:
                         rhs[row_index[rn]] = val
-                    else:
+                    elif rn not in free_rows:
                         raise ArgumentError(f"RHS on unknown row {rn!r}")
                 elif rn in row_index:
                     ranges[row_index[rn]] = val
-                else:
+                elif rn not in free_rows:
                     raise ArgumentError(f"RANGES on unknown row {rn!r}")
         elif section == "BOUNDS":
             # type (, set name), column (, value); FR / MI / PL carry no value, BV may

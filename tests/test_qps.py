@@ -110,7 +110,7 @@ RHS
  rhs e_pos 3 e_neg 3
  rhs e_zero 3 l_row 10
  rhs g_row -1 l_plain 8
- rhs obj -3.5
+ rhs obj -3.5 spare 4
 RANGES
  rng e_pos 2 e_neg -2
  rng l_row -4 g_row 6
@@ -211,6 +211,9 @@ def test_refusals():
     unknown = "NAME u\nROWS\n N o\n L r\nCOLUMNS\n x r 1\nBOUNDS\n UP b y 1\nENDATA\n"
     with pytest.raises(ArgumentError):
         read_mps(io.StringIO(unknown))
+    unknown_row = "NAME u\nROWS\n N o\n L r\nCOLUMNS\n x s 1\nENDATA\n"
+    with pytest.raises(ArgumentError):
+        read_mps(io.StringIO(unknown_row))
     bad_row = "NAME u\nROWS\n N o\n Q r\nENDATA\n"
     with pytest.raises(ArgumentError):
         read_mps(io.StringIO(bad_row))
