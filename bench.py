@@ -387,6 +387,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-limit", type=float, default=600.0)
     ap.add_argument("--powm-tol", type=float, default=1e-4)
+    ap.add_argument("--detect-infeasibility", action="store_true",
+                    help="opt-in ray certificates at every check (timed with the solve)")
     ap.add_argument("--polish-reltol", type=float, default=None,
                     help="mixed precision: FP32 phase stops at max(this, reltol) (engine default 1e-5)")
     # profiling runs only: cap the solve at this many passes (same kernels, fewer of
@@ -444,7 +446,7 @@ def main():
     # makes eta*||A|| > 1; the benchmark converges the power method to 1e-4 instead
     algo = cp.PDLP("float64", None, args.format, termination_reltol=reltol, time_limit=args.time_limit,
                    max_kkt_passes=args.max_passes, device=0, powm_tol=args.powm_tol,
-                   show_progress=1)
+                   show_progress=1, detect_infeasibility=int(args.detect_infeasibility))
     t = time.perf_counter()
     eng = cp.Engine(milp, algo)
     log(f"[bench] uploaded to HBM in {time.perf_counter() - t:.1f}s")
@@ -507,7 +509,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded generator with a known optimum)",
         "config": desc,
-        "solve_seconds": elapsed / args.steps,
+        "solve_seconds": elapsed / args.steps, "loop_seconds": st.loop_seconds,
+        "detect_infeasibility": bool(args.detect_infeasibility),
         "termination_status": st.termination_status.name, "kkt_passes": st.kkt_passes,
         "relative_kkt": st.err.relative(), "restarts": st.restarts,
         "objective": obj, "objective_closed_form": g.objective,
