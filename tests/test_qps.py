@@ -9,6 +9,7 @@ import os
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import oracle as O
 from coolpdlp_hip import MILP, generators as G, is_feasible, nbcons, nbvar, nbvar_int, objective_value
@@ -255,3 +256,36 @@ def test_suite_oracle_vs_highs(algo):
         else:
             assert is_feasible(r.x, milp, cons_tol=1e-1 * row_scale(milp), int_tol=INF, verbose=False), name
             assert abs(objective_value(r.x, milp) - obj) <= 1e-1 * abs(obj), name
+
+
+def test_round_trip_property():
+    """write_mps -> read_mps returns the same LP for random shapes, sparsity, bound kinds
+    (free, lower-only, upper-only, boxed, fixed) and row kinds (=, <=, >=, ranged with
+    integer-valued bounds, so that lc + (uc - lc) is exact)."""
+    from hypothesis import given, settings, strategies as st
+
+    @settings(max_examples=60, deadline=None)
+    @given(st.integers(1, 12), st.integers(1, 12), st.integers(0, 2**31 - 1))
+    def check(m, n, seed):
+        rng = np.random.default_rng(seed)
+        A = sp.random(m, n, density=rng.uniform(0.1, 0.8), format="csc", random_state=rng,
+                      data_rvs=lambda k: rng.standard_normal(k) * 10.0 ** rng.integers(-3, 4, k))
+        kinds_v = rng.integers(0, 5, n)
+        lv = np.where(kinds_v == 0, -INF, np.where(kinds_v == 2, -INF, rng.uniform(-5, 5, n)))
+        uv = np.where(kinds_v <= 1, INF, lv + rng.uniform(0, 5, n))
+        uv = np.where(kinds_v == 2, rng.uniform(-5, 5, n), uv)
+        uv = np.where(kinds_v == 4, lv, uv)
+        kinds_r = rng.integers(0, 4, m)
+        b = rng.integers(-20, 20, m).astype(float)
+        lc = np.where(kinds_r == 1, -INF, b)
+        uc = np.where(kinds_r == 2, INF, np.where(kinds_r == 3, b + rng.integers(1, 9, m), b))
+        c = rng.standard_normal(n) * (rng.random(n) < 0.7)
+        buf = io.StringIO()
+        write_mps(buf, c=c, lv=lv, uv=uv, A=A, lc=lc, uc=uc)
+        buf.seek(0)
+        back = MILP.from_qps(read_mps(buf))
+        for name, want in (("c", c), ("lv", lv), ("uv", uv), ("lc", lc), ("uc", uc)):
+            assert np.array_equal(getattr(back, name), want), name
+        assert back.A.shape == (m, n) and (back.A != A).nnz == 0
+
+    check()
