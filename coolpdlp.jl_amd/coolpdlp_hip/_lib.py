@@ -56,6 +56,17 @@ class CParams(C.Structure):
     ]
 
 
+SERVER_MAX_LANES = 4
+
+
+class CServerConfig(C.Structure):
+    _fields_ = [
+        ("batch_bytes", C.c_int64), ("plain_bytes", C.c_int64), ("batch_limit", C.c_int32),
+        ("checks_per_round", C.c_int32), ("n_lanes", C.c_int32),
+        ("lane_workers", C.c_int32 * SERVER_MAX_LANES), ("lane_max_nnz", C.c_double * SERVER_MAX_LANES),
+    ]
+
+
 class CKKT(C.Structure):
     _fields_ = [(f, C.c_double) for f in
                 ("primal", "primal_scale", "dual", "dual_scale", "gap", "gap_scale")]
@@ -98,6 +109,14 @@ _SIGS = {
     "cpdlp_batch_create": (C.c_int, [_P(CParams), C.c_int64, _P(C.c_void_p)]),
     "cpdlp_batch_add": (C.c_int, [C.c_void_p, _P(CLP), _P(C.c_int64)]),
     "cpdlp_batch_add_start": (C.c_int, [C.c_void_p, _P(CLP), C.c_void_p, C.c_void_p, _P(C.c_int64)]),
+    "cpdlp_server_default_config": (C.c_int, [_P(CServerConfig)]),
+    "cpdlp_server_create": (C.c_int, [_P(CParams), _P(CServerConfig), _P(C.c_void_p)]),
+    "cpdlp_server_submit": (C.c_int, [C.c_void_p, _P(CLP), C.c_void_p, C.c_void_p, _P(C.c_int64)]),
+    "cpdlp_server_pump": (C.c_int, [C.c_void_p, _P(C.c_int64)]),
+    "cpdlp_server_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _P(C.c_int64)]),
+    "cpdlp_server_get": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, _P(CStats),
+                                   _P(C.c_double), _P(C.c_int32)]),
+    "cpdlp_server_destroy": (None, [C.c_void_p]),
     "cpdlp_batch_footprint": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64]),
     "cpdlp_batch_run": (C.c_int, [C.c_void_p, C.c_int32, _P(C.c_int64)]),
     "cpdlp_batch_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _P(C.c_int64)]),

@@ -210,6 +210,117 @@ class Server:
             self.pump()
 
 
+class NativeServer:
+    """The same continuous batching as `Server`, run by the library itself
+    (cpdlp_server_*): requests are copied at submit; small ones (m + n <= batch_limit)
+    go to the batch engine, advanced by `pump`; larger ones to plain engines on the
+    library's worker threads, in lanes of (max nonzeros, workers), shortest job first,
+    under an HBM budget.  This is the interface a C or Julia caller uses."""
+
+    def __init__(self, algo=None, *, batch_bytes: int = 16 << 30, plain_bytes: int = 64 << 30,
+                 batch_limit: int = 8192, checks_per_round: int = 20,
+                 lanes=((1_000_000, 2), (float("inf"), 1))):
+        import ctypes as C
+        from . import _lib
+        self._C, self._lib = C, _lib
+        self.algo = algo if algo is not None else PDLP("mixed")
+        cfg = _lib.CServerConfig()
+        _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(cfg)))
+        cfg.batch_bytes, cfg.plain_bytes = int(batch_bytes), int(plain_bytes)
+        cfg.batch_limit, cfg.checks_per_round = int(batch_limit), int(checks_per_round)
+        if not 1 <= len(lanes) <= _lib.SERVER_MAX_LANES:
+            raise _lib.ArgumentError(f"1 to {_lib.SERVER_MAX_LANES} lanes")
+        cfg.n_lanes = len(lanes)
+        for k, (mx, w) in enumerate(lanes):
+            cfg.lane_max_nnz[k], cfg.lane_workers[k] = float(mx), int(w)
+        h = C.c_void_p()
+        _lib.check(_lib.lib.cpdlp_server_create(C.byref(self.algo.params), C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.lib.cpdlp_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, lp, sol=None) -> int:
+        """Queue an LP (a MILP or a generated LP); `sol` is an optional unscaled start."""
+        C, _lib = self._C, self._lib
+        if hasattr(lp, "A"):
+            A = lp.A
+            m, n = A.shape
+            colptr, rowval, nzval = A.indptr, A.indices, A.data
+        else:
+            m, n = lp.m, lp.n
+            colptr, rowval, nzval = lp.colptr, lp.rowval, lp.nzval
+        arrs = [np.ascontiguousarray(colptr, dtype=np.int64), np.ascontiguousarray(rowval, dtype=np.int64),
+                np.ascontiguousarray(nzval, dtype=np.float64)]
+        vecs = [np.ascontiguousarray(v, dtype=np.float64) for v in (lp.c, lp.lv, lp.uv, lp.lc, lp.uc)]
+        c_lp = _lib.CLP()
+        c_lp.m, c_lp.n, c_lp.index_base, c_lp.index_bits = m, n, 0, 64
+        c_lp.colptr, c_lp.rowval, c_lp.nzval = [a.ctypes.data_as(C.c_void_p) for a in arrs]
+        c_lp.c, c_lp.lv, c_lp.uv, c_lp.lc, c_lp.uc = [v.ctypes.data_as(C.c_void_p) for v in vecs]
+        x0 = y0 = None
+        if sol is not None:
+            x0 = np.ascontiguousarray(sol.x, dtype=np.float64)
+            y0 = np.ascontiguousarray(sol.y, dtype=np.float64)
+            if x0.shape != (n,) or y0.shape != (m,):
+                raise _lib.DimensionMismatch("start point does not match the LP's size")
+        i = C.c_int64()
+        _lib.check(_lib.lib.cpdlp_server_submit(
+            self.h, C.byref(c_lp), None if x0 is None else x0.ctypes.data_as(C.c_void_p),
+            None if y0 is None else y0.ctypes.data_as(C.c_void_p), C.byref(i)))
+        self.shapes[i.value] = (m, n)
+        return i.value
+
+    def pump(self) -> int:
+        """One scheduling round; returns how many requests are not finished."""
+        k = self._C.c_int64()
+        self._lib.check(self._lib.lib.cpdlp_server_pump(self.h, self._C.byref(k)))
+        return k.value
+
+    def poll(self) -> list:
+        C = self._C
+        ids = np.zeros(max(len(self.shapes), 1), np.int64)
+        cnt = C.c_int64()
+        self._lib.check(self._lib.lib.cpdlp_server_poll(self.h, ids.ctypes.data_as(C.c_void_p), len(ids),
+                                                        C.byref(cnt)))
+        return ids[: cnt.value].tolist()
+
+    def get(self, i: int):
+        """(PrimalDualSolution, ConvergenceStats, seconds from submit to finish, route:
+        -1 batch or the plain lane); frees the request."""
+        from .algorithms import ConvergenceStats, KKTErrors, TerminationStatus
+        from .problems import PrimalDualSolution
+        C, _lib = self._C, self._lib
+        if i not in self.shapes:
+            raise _lib.ArgumentError("unknown request id")
+        m, n = self.shapes.pop(i)
+        x, y = np.zeros(max(n, 1)), np.zeros(max(m, 1))
+        s, secs, rt = _lib.CStats(), C.c_double(), C.c_int32()
+        _lib.check(_lib.lib.cpdlp_server_get(self.h, int(i), x.ctypes.data_as(C.c_void_p),
+                                             y.ctypes.data_as(C.c_void_p), C.byref(s), C.byref(secs),
+                                             C.byref(rt)))
+        st = ConvergenceStats(
+            err=KKTErrors.from_c(s.err), time_elapsed=s.time_elapsed, kkt_passes=s.kkt_passes,
+            termination_status=TerminationStatus(s.termination_status), eta=s.eta, omega=s.omega,
+            powm_iters=s.powm_iters, restarts=s.restarts, fp32_passes=s.fp32_passes)
+        return PrimalDualSolution(x[:n], y[:m]), st, secs.value, rt.value
+
+    def drain(self, deadline: float):
+        """Pump until every request has finished or the deadline (perf_counter) passes."""
+        while self.pump() > 0 and time.perf_counter() < deadline:
+            time.sleep(0.0005)
+
+
 def route(m: int, n: int, nnz: int, batch_limit: int, lanes, overflow_queue: int = 0) -> int:
     """Where a request goes: -1 for the batch engine, else the index of its plain lane
     (the first whose max_nnz admits it, the last lane otherwise).  With overflow_queue > 0

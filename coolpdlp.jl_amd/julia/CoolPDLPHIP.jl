@@ -330,4 +330,96 @@ function CoolPDLP.solve(milps::Vector{<:MILP}, algo::Algorithm{A, T, Ti, M, HIPE
     end
 end
 
+# ---- 5. served requests (cpdlp_server_*; no counterpart in the reference) --------------
+mutable struct CServerConfig
+    batch_bytes::Int64
+    plain_bytes::Int64
+    batch_limit::Int32
+    checks_per_round::Int32
+    n_lanes::Int32
+    lane_workers::NTuple{4, Int32}
+    lane_max_nnz::NTuple{4, Float64}
+    CServerConfig() = new()
+end
+
+"""
+    HIPServer(algo; batch_limit = 8192, checks_per_round = 20)
+
+The library's continuous-batching server: `submit!` queues an LP (optionally with a start
+point), `pump!` runs one scheduling round, `poll!` lists finished requests and `take!`
+hands one over as `(PrimalDualSolution, ConvergenceStats, seconds)`.
+"""
+mutable struct HIPServer
+    ptr::Ptr{Cvoid}
+    shapes::Dict{Int64, Tuple{Int, Int}}
+end
+
+function HIPServer(algo::Algorithm{A, T, Ti, M, HIPEngine}; batch_limit::Integer = 8192,
+        checks_per_round::Integer = 20) where {A, T, Ti, M}
+    p = cparams(algo, algo.conversion.backend)
+    cfg = CServerConfig()
+    check(ccall((:cpdlp_server_default_config, libcpdlp), Cint, (Ref{CServerConfig},), cfg))
+    cfg.batch_limit = batch_limit
+    cfg.checks_per_round = checks_per_round
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:cpdlp_server_create, libcpdlp), Cint, (Ref{CParams}, Ref{CServerConfig}, Ptr{Ptr{Cvoid}}),
+        p, cfg, h))
+    s = HIPServer(h[], Dict{Int64, Tuple{Int, Int}}())
+    finalizer(s) do s
+        s.ptr == C_NULL || ccall((:cpdlp_server_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), s.ptr)
+        s.ptr = C_NULL
+    end
+    return s
+end
+
+function submit!(s::HIPServer, milp::MILP, sol::Union{Nothing, PrimalDualSolution} = nothing)
+    A_ = milp.A::SparseMatrixCSC
+    c, lv, uv, lc, uc = (Vector{Float64}(v) for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))
+    lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
+        pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
+    x0 = sol === nothing ? Float64[] : Vector{Float64}(Array(sol.x))
+    y0 = sol === nothing ? Float64[] : Vector{Float64}(Array(sol.y))
+    sol === nothing || (length(x0) == size(A_, 2) && length(y0) == size(A_, 1)) ||
+        throw(DimensionMismatch("start point does not match the LP's size"))
+    id = Ref{Int64}(0)
+    GC.@preserve A_ c lv uv lc uc x0 y0 begin   # copied by the library during the call
+        check(ccall((:cpdlp_server_submit, libcpdlp), Cint,
+            (Ptr{Cvoid}, Ref{CLP}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}), s.ptr, lp,
+            sol === nothing ? C_NULL : pointer(x0), sol === nothing ? C_NULL : pointer(y0), id))
+    end
+    s.shapes[id[]] = (size(A_, 1), size(A_, 2))
+    return id[]
+end
+
+function pump!(s::HIPServer)
+    pending = Ref{Int64}(0)
+    check(ccall((:cpdlp_server_pump, libcpdlp), Cint, (Ptr{Cvoid}, Ref{Int64}), s.ptr, pending))
+    return pending[]
+end
+
+function poll!(s::HIPServer)
+    ids = zeros(Int64, max(length(s.shapes), 1))
+    cnt = Ref{Int64}(0)
+    check(ccall((:cpdlp_server_poll, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Int64}, Int64, Ref{Int64}),
+        s.ptr, ids, length(ids), cnt))
+    return ids[1:cnt[]]
+end
+
+function Base.take!(s::HIPServer, id::Integer)
+    m, n = pop!(s.shapes, id)
+    x, y = zeros(n), zeros(m)
+    st = Ref{CStats}()
+    secs = Ref{Float64}(0.0)
+    route = Ref{Int32}(0)
+    check(ccall((:cpdlp_server_get, libcpdlp), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ref{CStats}, Ref{Float64}, Ref{Int32}),
+        s.ptr, id, x, y, st, secs, route))
+    e = st[].err
+    err = KKTErrors(; primal = e.primal, primal_scale = e.primal_scale, dual = e.dual,
+        dual_scale = e.dual_scale, gap = e.gap, gap_scale = e.gap_scale)
+    stats = ConvergenceStats(Float64; err, time_elapsed = st[].time_elapsed,
+        kkt_passes = st[].kkt_passes, termination_status = STATUS[st[].termination_status + 1])
+    return PrimalDualSolution(x, y), stats, secs[]
+end
+
 end # module

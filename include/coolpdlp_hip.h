@@ -290,6 +290,46 @@ int cpdlp_matrix_get_ell(cpdlp_matrix* A, int32_t* colval, double* nzval);
 
 void cpdlp_matrix_destroy(cpdlp_matrix* A);
 
+/* ---- serving: continuous batching of solve requests (no reference counterpart) ----
+ * The served form of solve(milp, sol, algo) (src/algorithms/common.jl:186-209).
+ * Requests are copied at submit.  Those with m + n <= batch_limit go to a batch engine
+ * (as cpdlp_batch_*: admitted as arena room frees up, advanced by cpdlp_server_pump);
+ * the others to plain engines (as cpdlp_solve, CSR format, one HIP stream each) run by
+ * worker threads in lanes by nonzero count (a request takes the first lane whose
+ * lane_max_nnz admits it, else the last), shortest job first within a lane, under the
+ * plain_bytes HBM budget.  Submit, poll and get may be called from any thread; pump
+ * from one thread at a time. */
+#define CPDLP_SERVER_MAX_LANES 4
+typedef struct cpdlp_server_config {
+    int64_t batch_bytes;            /* batch route's HBM arena (16 GiB) */
+    int64_t plain_bytes;            /* plain route's HBM budget (64 GiB) */
+    int32_t batch_limit;            /* m + n limit of the batch route, <= 65536; 0: none (8192) */
+    int32_t checks_per_round;       /* checks per pump for the smallest instances (20) */
+    int32_t n_lanes;                /* plain lanes, 1..CPDLP_SERVER_MAX_LANES (2) */
+    int32_t lane_workers[CPDLP_SERVER_MAX_LANES];   /* worker threads per lane (2, 1) */
+    double lane_max_nnz[CPDLP_SERVER_MAX_LANES];    /* nonzeros a lane admits (1e6, inf) */
+} cpdlp_server_config;
+typedef struct cpdlp_server cpdlp_server;
+
+int cpdlp_server_default_config(cpdlp_server_config* cfg);
+/* params as for cpdlp_create (the format is fixed to CSR on the plain route). */
+int cpdlp_server_create(const cpdlp_params* params, const cpdlp_server_config* cfg, cpdlp_server** out);
+/* Queue one LP; x0 / y0 (NULL: zero) are an unscaled start as in cpdlp_solve. */
+int cpdlp_server_submit(cpdlp_server* s, const cpdlp_lp* lp, const double* x0, const double* y0,
+                        int64_t* id);
+/* One scheduling round: admit, advance the batch route by checks_per_round checks, harvest.
+ * *pending = requests not yet finished. */
+int cpdlp_server_pump(cpdlp_server* s, int64_t* pending);
+/* Ids of finished requests (each reported once). */
+int cpdlp_server_poll(cpdlp_server* s, int64_t* ids, int64_t cap, int64_t* count);
+/* A finished request's unscaled x (n), y (m) and stats; *seconds = submit to finish;
+ * *route = -1 (batch) or the plain lane.  Frees the request.  Returns the request's own
+ * error code when its solve failed (the message in cpdlp_last_error). */
+int cpdlp_server_get(cpdlp_server* s, int64_t id, double* x, double* y, cpdlp_stats* st,
+                     double* seconds, int32_t* route);
+/* Waits for running plain solves; requests not yet admitted are dropped. */
+void cpdlp_server_destroy(cpdlp_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,7 @@ def test_struct_layout_matches_c(tmp_path):
     from coolpdlp_hip import _lib
     prog = tmp_path / "layout.c"
     structs = {"cpdlp_lp": _lib.CLP, "cpdlp_params": _lib.CParams, "cpdlp_kkt": _lib.CKKT,
-               "cpdlp_stats": _lib.CStats}
+               "cpdlp_stats": _lib.CStats, "cpdlp_server_config": _lib.CServerConfig}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -108,3 +108,24 @@ def test_kkt_struct_helpers():
     e = cp.KKTErrors(1.0, 2.0, 3.0, 4.0, 5.0, 10.0)
     assert e.relative() == 0.75
     assert e.absolute(2.0) == np.sqrt(4 + 9 / 4 + 25)
+
+
+def test_server_config_defaults_and_validation():
+    """cpdlp_server_default_config, and create-time validation that fails before any
+    device work (batch route off, so no arena is allocated)."""
+    from coolpdlp_hip import _lib
+    cfg = _lib.CServerConfig()
+    _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(cfg)))
+    assert (cfg.batch_bytes, cfg.plain_bytes) == (16 << 30, 64 << 30)
+    assert (cfg.batch_limit, cfg.checks_per_round, cfg.n_lanes) == (8192, 20, 2)
+    assert list(cfg.lane_workers)[:2] == [2, 1] and cfg.lane_max_nnz[0] == 1e6
+    assert cfg.lane_max_nnz[1] == float("inf")
+    import coolpdlp_hip as cp
+    for field, value in [("batch_limit", 70000), ("n_lanes", 0), ("n_lanes", 5), ("checks_per_round", 0)]:
+        bad = _lib.CServerConfig()
+        _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(bad)))
+        bad.batch_limit = 0
+        setattr(bad, field, value)
+        h = C.c_void_p()
+        with pytest.raises(cp.ArgumentError):
+            _lib.check(_lib.lib.cpdlp_server_create(C.byref(cp.PDLP().params), C.byref(bad), C.byref(h)))

@@ -1,0 +1,102 @@
+"""GPU tests of the library's own server (cpdlp_server_*, through coolpdlp_hip.serving.
+NativeServer): requests submitted while others run give the same bits as when solved
+alone -- batch-routed ones as the batch engine alone, plain-routed ones as a CSR-format
+plain solve alone; a start point is honoured on both routes; the opt-in infeasibility
+detection reaches both routes; errors are reported per request and per call."""
+import time
+
+import numpy as np
+import pytest
+
+from coolpdlp_hip import generators as G
+
+from test_infeasibility import WANT, cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cp():
+    import coolpdlp_hip
+    return coolpdlp_hip
+
+
+def _drain(srv, want, deadline_s=180):
+    """Pump until `want` requests have finished; returns {id: (sol, st, seconds, route)}."""
+    out = {}
+    t_end = time.perf_counter() + deadline_s
+    while len(out) < want and time.perf_counter() < t_end:
+        srv.pump()
+        for i in srv.poll():
+            out[i] = srv.get(i)
+        time.sleep(0.0005)
+    assert len(out) == want, f"{len(out)} of {want} finished"
+    return out
+
+
+def test_native_server_alone_equals_in_mix(cp):
+    from coolpdlp_hip.serving import NativeServer
+    small = [G.request_lp(m, 2 * m, 700 + m) for m in (60, 150, 420, 900, 1500, 2600)]
+    big = [G.request_lp(3000, 6000, 78), G.request_lp(5000, 10000, 79)]
+    algo = cp.PDLP("mixed", termination_reltol=1e-6)
+    with NativeServer(algo, batch_bytes=256 << 20, plain_bytes=4 << 30, batch_limit=8000,
+                      checks_per_round=2) as srv:
+        ids = {srv.submit(small[0]): small[0], srv.submit(big[0]): big[0]}
+        for g in small[1:] + big[1:]:
+            for _ in range(3):
+                srv.pump()
+            ids[srv.submit(g)] = g          # arrives while the others are on the GPU
+        res = _drain(srv, len(ids))
+    for i, g in ids.items():
+        sol, st, secs, route = res[i]
+        assert st.termination_status == cp.OPTIMAL, (g.name, st)
+        assert secs > 0
+        assert abs(float(sol.x @ g.c) - g.objective) <= 2e-5 * (1 + abs(g.objective))
+        if g.m + g.n <= 8000:
+            assert route == -1
+            with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
+                (alone, st_a), = B.solve_all([g])
+        else:
+            assert route == 0                # <= 1e6 nonzeros: the first lane
+            alone, st_a = cp.solve(cp.MILP.from_generated(g), cp.PDLP("mixed", None, "csr",
+                                                                      termination_reltol=1e-6))
+        assert np.array_equal(alone.x, sol.x) and np.array_equal(alone.y, sol.y), g.name
+        assert st_a.kkt_passes == st.kkt_passes
+
+
+@pytest.mark.parametrize("batch_limit", [8000, 0])
+def test_native_server_warm_start_and_detection(cp, batch_limit):
+    from coolpdlp_hip.serving import NativeServer
+    g = G.request_lp(400, 800, 91)
+    algo = cp.PDLP("mixed", termination_reltol=1e-6, detect_infeasibility=True)
+    with NativeServer(algo, batch_bytes=64 << 20, plain_bytes=1 << 30, batch_limit=batch_limit) as srv:
+        first = srv.submit(g)
+        (sol, st, _, route), = _drain(srv, 1).values()
+        assert st.termination_status == cp.OPTIMAL and route == (-1 if batch_limit else 0)
+        srv.submit(g, sol)
+        (_, st2, _, _), = _drain(srv, 1).values()
+        assert st2.termination_status == cp.OPTIMAL and st2.kkt_passes <= st.kkt_passes // 2
+        bad = {srv.submit(milp): (name, WANT[hs]) for name, milp, hs in cases()}
+        res = _drain(srv, len(bad))
+        for i, (name, want) in bad.items():
+            assert res[i][1].termination_status.name == want, (name, res[i][1].termination_status)
+        with pytest.raises(cp.ArgumentError):
+            srv.get(first)                   # already handed over and freed
+
+
+def test_native_server_reports_request_errors(cp):
+    """Errors are per call: a get of an id the library does not know and a start point of
+    the wrong size are refused, and the server keeps serving."""
+    from coolpdlp_hip.serving import NativeServer
+    with NativeServer(cp.PDLP("mixed", termination_reltol=1e-6, max_kkt_passes=2000),
+                      batch_bytes=16 << 20, plain_bytes=1 << 30) as srv:
+        g = G.request_lp(100, 200, 5)
+        i = srv.submit(g)
+        srv.shapes[10**9] = (1, 1)
+        with pytest.raises(cp.ArgumentError):
+            srv.get(10**9)
+        (_, st, _, _), = _drain(srv, 1).values()
+        assert st.termination_status == cp.OPTIMAL
+        with pytest.raises(cp.DimensionMismatch):
+            srv.submit(g, cp.PrimalDualSolution(np.zeros(g.n + 1), np.zeros(g.m)))
+        assert i not in srv.shapes
