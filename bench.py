@@ -15,6 +15,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|pla
 """
 import argparse
 import json
+from dataclasses import dataclass
 import os
 import sys
 import time
@@ -243,7 +244,7 @@ def bench_serve(args):
     import numpy as np
     import coolpdlp_hip as cp
     from coolpdlp_hip import generators as G
-    from coolpdlp_hip.serving import Server, fcfs_latencies, poisson_arrivals
+    from coolpdlp_hip.serving import NativeServer, Server, fcfs_latencies, poisson_arrivals
     rng = np.random.default_rng(17)
     reltol = 1e-6
     bounds = {"small": (50, 2_000), "medium": (2_000, 50_000), "large": (50_000, 500_000)}
@@ -272,25 +273,50 @@ def bench_serve(args):
         u = rng.random(len(arr))
         cls_of = np.where(u < 0.70, "small", np.where(u < 0.95, "medium", "large"))
         picks = [(c, int(rng.integers(len(pool[c])))) for c in cls_of]
-        srv = Server(reltol=reltol, batch_bytes=args.serve_batch_gib << 30,
-                     checks_per_round=args.serve_checks, time_limit=args.time_limit,
-                     lanes=((1_000_000, args.serve_medium_workers), (float("inf"), 1)),
-                     overflow_queue=args.serve_overflow, batch_limit=args.serve_batch_limit)
+        lanes = ((1_000_000, args.serve_medium_workers), (float("inf"), 1))
+        if args.serve_native:   # the library's own server (cpdlp_server_*)
+            srv = NativeServer(cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit,
+                                       batch_streams=1),   # + 3 plain workers: 4 hardware queues
+                               batch_bytes=args.serve_batch_gib << 30, checks_per_round=args.serve_checks,
+                               lanes=lanes, batch_limit=args.serve_batch_limit)
+            srv.done, srv.rounds, srv.round_seconds, meta = [], 0, [], {}
+            srv.start()           # the library pumps on its own thread
+        else:
+            srv = Server(reltol=reltol, batch_bytes=args.serve_batch_gib << 30,
+                         checks_per_round=args.serve_checks, time_limit=args.time_limit,
+                         lanes=lanes, overflow_queue=args.serve_overflow, batch_limit=args.serve_batch_limit)
         t0 = time.perf_counter()
         k = 0
+        pending = 0
         deadline = t0 + args.serve_duration + args.serve_drain
         last_log = t0
         while time.perf_counter() < deadline:
             now = time.perf_counter() - t0
             while k < len(arr) and arr[k] <= now:
-                srv.submit(pool[picks[k][0]][picks[k][1]], t0 + arr[k])
+                g = pool[picks[k][0]][picks[k][1]]
+                if args.serve_native:
+                    meta[srv.submit(g)] = (k, t0 + arr[k], time.perf_counter(), g)
+                else:
+                    srv.submit(g, t0 + arr[k])
                 k += 1
-            if k >= len(arr) and srv.pending() == 0:
-                break
-            srv.pump()
+            if args.serve_native:
+                if k >= len(arr) and pending == 0 and k == len(srv.done):
+                    break
+                pending = srv.pump()  # with the driver running: the pending count
+                time.sleep(0.001)
+                for i in srv.poll():
+                    rid, t_arr, t_sub, g = meta.pop(i)
+                    sol, st, secs, _ = srv.get(i)
+                    srv.done.append(NativeDone(rid, t_arr, t_sub + secs, st.termination_status.name,
+                                               float(sol.x @ g.c)))
+            else:
+                if k >= len(arr) and srv.pending() == 0:
+                    break
+                srv.pump()
+                pending = srv.pending()
             if time.perf_counter() - last_log > 5:
                 last_log = time.perf_counter()
-                log(f"[bench] qps={qps} t={now:.1f}s submitted={k} done={len(srv.done)} pending={srv.pending()}")
+                log(f"[bench] qps={qps} t={now:.1f}s submitted={k} done={len(srv.done)} pending={pending}")
         t_end = time.perf_counter()
         srv.close()
         done = srv.done
@@ -300,9 +326,9 @@ def bench_serve(args):
         cls_done = {r.rid: picks[r.rid][0] for r in done}
         rs = np.array(srv.round_seconds) if srv.round_seconds else np.zeros(1)
         waits = {c: float(np.median([r.t_start - r.t_arrival for r in done if cls_done[r.rid] == c] or [0.0]))
-                 for c in ("small", "medium", "large")}
+                 for c in ("small", "medium", "large")} if not args.serve_native else None
         runs = {c: float(np.median([r.t_done - r.t_start for r in done if cls_done[r.rid] == c] or [0.0]))
-                for c in ("small", "medium", "large")}
+                for c in ("small", "medium", "large")} if not args.serve_native else None
         per_class = {}
         for c in ("small", "medium", "large"):
             ls = np.array([r.t_done - r.t_arrival for r in done if cls_done[r.rid] == c])
@@ -335,10 +361,21 @@ def bench_serve(args):
         "config": {"workload": "serving_poisson_mix", "qps": args.qps, "duration_s": args.serve_duration,
                    "medium_workers": args.serve_medium_workers, "overflow_queue": args.serve_overflow,
                    "batch_limit": args.serve_batch_limit,
+                   "server": "library (cpdlp_server_*)" if args.serve_native else "python scheduler",
                    "mix": "70/25/5 small/medium/large", "reltol": reltol,
                    "pool": {k: len(v) for k, v in pool.items()}},
         "points": out_points,
     }
+
+
+@dataclass
+class NativeDone:
+    """A request finished by the library's server, in the fields the serving report reads."""
+    rid: int
+    t_arrival: float
+    t_done: float
+    status: str
+    objective: float
 
 
 def cpu_baseline(g, gpu_passes, reltol, budget_passes):
@@ -410,6 +447,8 @@ def main():
     ap.add_argument("--serve-medium-workers", type=int, default=2)
     ap.add_argument("--serve-overflow", type=int, default=0)
     ap.add_argument("--serve-batch-limit", type=int, default=8192)
+    ap.add_argument("--serve-native", action="store_true",
+                    help="serve with the library's own server (cpdlp_server_*) instead of the Python scheduler")
     ap.add_argument("--pool-small", type=int, default=48)
     ap.add_argument("--pool-medium", type=int, default=16)
     ap.add_argument("--pool-large", type=int, default=3)

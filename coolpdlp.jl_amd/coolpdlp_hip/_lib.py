@@ -112,7 +112,9 @@ _SIGS = {
     "cpdlp_server_default_config": (C.c_int, [_P(CServerConfig)]),
     "cpdlp_server_create": (C.c_int, [_P(CParams), _P(CServerConfig), _P(C.c_void_p)]),
     "cpdlp_server_submit": (C.c_int, [C.c_void_p, _P(CLP), C.c_void_p, C.c_void_p, _P(C.c_int64)]),
+    "cpdlp_server_submit_borrowed": (C.c_int, [C.c_void_p, _P(CLP), C.c_void_p, C.c_void_p, _P(C.c_int64)]),
     "cpdlp_server_pump": (C.c_int, [C.c_void_p, _P(C.c_int64)]),
+    "cpdlp_server_start": (C.c_int, [C.c_void_p]),
     "cpdlp_server_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _P(C.c_int64)]),
     "cpdlp_server_get": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, _P(CStats),
                                    _P(C.c_double), _P(C.c_int32)]),

@@ -223,7 +223,9 @@ class NativeServer:
         import ctypes as C
         from . import _lib
         self._C, self._lib = C, _lib
-        self.algo = algo if algo is not None else PDLP("mixed")
+        # one batch stream beside the plain workers' streams keeps the total within the
+        # device's 4 hardware queues (as Server does)
+        self.algo = algo if algo is not None else PDLP("mixed", batch_streams=1)
         cfg = _lib.CServerConfig()
         _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(cfg)))
         cfg.batch_bytes, cfg.plain_bytes = int(batch_bytes), int(plain_bytes)
@@ -237,11 +239,13 @@ class NativeServer:
         _lib.check(_lib.lib.cpdlp_server_create(C.byref(self.algo.params), C.byref(cfg), C.byref(h)))
         self.h = h
         self.shapes = {}
+        self._held = {}          # arrays lent to the library, by request id
 
     def close(self):
         if getattr(self, "h", None):
-            self._lib.lib.cpdlp_server_destroy(self.h)
+            self._lib.lib.cpdlp_server_destroy(self.h)   # joins the workers first
             self.h = None
+            self._held.clear()
 
     __del__ = close
 
@@ -251,8 +255,11 @@ class NativeServer:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, lp, sol=None) -> int:
-        """Queue an LP (a MILP or a generated LP); `sol` is an optional unscaled start."""
+    def submit(self, lp, sol=None, borrow: bool = True) -> int:
+        """Queue an LP (a MILP or a generated LP); `sol` is an optional unscaled start.
+        With `borrow` (the default) the library reads the arrays in place
+        (cpdlp_server_submit_borrowed): they are held here until `get` and must not be
+        modified meanwhile.  Otherwise the library copies them at submit."""
         C, _lib = self._C, self._lib
         if hasattr(lp, "A"):
             A = lp.A
@@ -261,11 +268,13 @@ class NativeServer:
         else:
             m, n = lp.m, lp.n
             colptr, rowval, nzval = lp.colptr, lp.rowval, lp.nzval
-        arrs = [np.ascontiguousarray(colptr, dtype=np.int64), np.ascontiguousarray(rowval, dtype=np.int64),
+        bits = 32 if int(colptr[-1]) < 2**31 else 64   # int32 row indices pass without a copy
+        it = np.int32 if bits == 32 else np.int64
+        arrs = [np.ascontiguousarray(colptr, dtype=it), np.ascontiguousarray(rowval, dtype=it),
                 np.ascontiguousarray(nzval, dtype=np.float64)]
         vecs = [np.ascontiguousarray(v, dtype=np.float64) for v in (lp.c, lp.lv, lp.uv, lp.lc, lp.uc)]
         c_lp = _lib.CLP()
-        c_lp.m, c_lp.n, c_lp.index_base, c_lp.index_bits = m, n, 0, 64
+        c_lp.m, c_lp.n, c_lp.index_base, c_lp.index_bits = m, n, 0, bits
         c_lp.colptr, c_lp.rowval, c_lp.nzval = [a.ctypes.data_as(C.c_void_p) for a in arrs]
         c_lp.c, c_lp.lv, c_lp.uv, c_lp.lc, c_lp.uc = [v.ctypes.data_as(C.c_void_p) for v in vecs]
         x0 = y0 = None
@@ -275,11 +284,18 @@ class NativeServer:
             if x0.shape != (n,) or y0.shape != (m,):
                 raise _lib.DimensionMismatch("start point does not match the LP's size")
         i = C.c_int64()
-        _lib.check(_lib.lib.cpdlp_server_submit(
-            self.h, C.byref(c_lp), None if x0 is None else x0.ctypes.data_as(C.c_void_p),
-            None if y0 is None else y0.ctypes.data_as(C.c_void_p), C.byref(i)))
+        fn = _lib.lib.cpdlp_server_submit_borrowed if borrow else _lib.lib.cpdlp_server_submit
+        _lib.check(fn(self.h, C.byref(c_lp), None if x0 is None else x0.ctypes.data_as(C.c_void_p),
+                      None if y0 is None else y0.ctypes.data_as(C.c_void_p), C.byref(i)))
         self.shapes[i.value] = (m, n)
+        if borrow:
+            self._held[i.value] = (arrs, vecs, x0, y0)
         return i.value
+
+    def start(self):
+        """Let the library pump on its own thread (cpdlp_server_start); `pump` then only
+        reports the pending count."""
+        self._lib.check(self._lib.lib.cpdlp_server_start(self.h))
 
     def pump(self) -> int:
         """One scheduling round; returns how many requests are not finished."""
@@ -306,9 +322,12 @@ class NativeServer:
         m, n = self.shapes.pop(i)
         x, y = np.zeros(max(n, 1)), np.zeros(max(m, 1))
         s, secs, rt = _lib.CStats(), C.c_double(), C.c_int32()
-        _lib.check(_lib.lib.cpdlp_server_get(self.h, int(i), x.ctypes.data_as(C.c_void_p),
-                                             y.ctypes.data_as(C.c_void_p), C.byref(s), C.byref(secs),
-                                             C.byref(rt)))
+        try:
+            _lib.check(_lib.lib.cpdlp_server_get(self.h, int(i), x.ctypes.data_as(C.c_void_p),
+                                                 y.ctypes.data_as(C.c_void_p), C.byref(s), C.byref(secs),
+                                                 C.byref(rt)))
+        finally:
+            self._held.pop(i, None)   # handed over (or failed): the library is done with them
         st = ConvergenceStats(
             err=KKTErrors.from_c(s.err), time_elapsed=s.time_elapsed, kkt_passes=s.kkt_passes,
             termination_status=TerminationStatus(s.termination_status), eta=s.eta, omega=s.omega,

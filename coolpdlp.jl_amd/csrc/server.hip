@@ -2,9 +2,9 @@
 //
 // No counterpart in the reference, which solves one LP per `solve` call
 // (src/algorithms/common.jl:186-209); this is the served form of that same solve.
-// Requests are copied at submit.  Small ones (m + n <= batch_limit) go to a batch engine
-// (cpdlp_batch_*: one workgroup per LP, iterates and restart state resident in its HBM
-// arena between rounds) and are admitted as arena room frees up; larger ones go to plain
+// Requests are copied at submit (or borrowed: cpdlp_server_submit_borrowed).  Small
+// ones (m + n <= batch_limit) go to a batch engine (cpdlp_batch_*: one workgroup per LP,
+// iterates and restart state resident in its HBM arena between rounds) and are admitted as arena room frees up; larger ones go to plain
 // engines (cpdlp_create / cpdlp_solve, one HIP stream each) run by worker threads, in
 // lanes by nonzero count with their own shortest-job-first queues, under an HBM budget.
 // The caller drives the batch with cpdlp_server_pump; plain workers run on their own.
@@ -37,6 +37,9 @@ struct Request {
     std::vector<int32_t> colptr, rowval;
     std::vector<double> nzval, c, lv, uv, lc, uc, x0, y0;
     bool warm = false;
+    bool borrowed = false;         // the caller's arrays, valid until the request is handed over
+    cpdlp_lp blp{};
+    const double *bx0 = nullptr, *by0 = nullptr;
     int lane = -1;                 // -1: batch route
     double t_submit = 0.0, t_done = 0.0;
     // result
@@ -46,6 +49,7 @@ struct Request {
     cpdlp_stats stats{};
 
     cpdlp_lp view() const {
+        if (borrowed) return blp;
         cpdlp_lp lp{};
         lp.m = m;
         lp.n = n;
@@ -61,6 +65,8 @@ struct Request {
         lp.uc = uc.data();
         return lp;
     }
+    const double* start_x() const { return borrowed ? bx0 : warm ? x0.data() : nullptr; }
+    const double* start_y() const { return borrowed ? by0 : warm ? y0.data() : nullptr; }
     void drop_input() {
         std::vector<int32_t>().swap(colptr);
         std::vector<int32_t>().swap(rowval);
@@ -93,6 +99,11 @@ struct Server {
     std::vector<std::unique_ptr<Lane>> lanes;   // Lane holds threads: not movable
     int64_t plain_used = 0;
     std::deque<int64_t> done;
+    // the optional driver thread (cpdlp_server_start) that pumps on its own
+    std::thread driver;
+    bool driving = false;
+    int32_t driver_code = CPDLP_OK;
+    std::string driver_error;
 
     Server(const cpdlp_params& p, const cpdlp_server_config& c) : P(p), cfg(c) {
         if (cfg.batch_limit < 0 || cfg.batch_limit > 65536)
@@ -123,6 +134,7 @@ struct Server {
             stop = true;
         }
         cv.notify_all();
+        if (driver.joinable()) driver.join();
         for (auto& ln : lanes)
             for (auto& t : ln->threads) t.join();
         if (batch) cpdlp_batch_destroy(batch);
@@ -149,7 +161,7 @@ struct Server {
             try {
                 rc = cpdlp_create(&lp, &p, &h);
                 if (rc == CPDLP_OK)
-                    rc = cpdlp_solve(h, r->warm ? r->x0.data() : nullptr, r->warm ? r->y0.data() : nullptr);
+                    rc = cpdlp_solve(h, r->start_x(), r->start_y());
                 if (rc == CPDLP_OK) {
                     r->x.assign((size_t)r->n, 0.0);
                     r->y.assign((size_t)r->m, 0.0);
@@ -179,8 +191,32 @@ struct Server {
         return (int)lanes.size() - 1;
     }
 
-    int64_t submit(const cpdlp_lp& lp, const double* x0, const double* y0) {
+    int64_t submit(const cpdlp_lp& lp, const double* x0, const double* y0, bool borrow) {
         auto r = std::make_unique<Request>();
+        if (borrow) {
+            // nothing is copied: the engines validate the arrays when they take the request
+            if (lp.m < 0 || lp.n < 0) throw Error(CPDLP_ERR_DIMENSION, "negative matrix size");
+            if (lp.index_base != 0 && lp.index_base != 1) throw Error(CPDLP_ERR_ARGUMENT, "index_base must be 0 or 1");
+            if (lp.index_bits != 32 && lp.index_bits != 64) throw Error(CPDLP_ERR_ARGUMENT, "index_bits must be 32 or 64");
+            if (!lp.colptr) throw Error(CPDLP_ERR_ARGUMENT, "colptr is NULL");
+            r->borrowed = true;
+            r->blp = lp;
+            r->bx0 = x0;
+            r->by0 = y0;
+            r->m = lp.m;
+            r->n = lp.n;
+            r->nnz = (lp.index_bits == 32 ? (int64_t)((const int32_t*)lp.colptr)[lp.n]
+                                          : ((const int64_t*)lp.colptr)[lp.n]) - lp.index_base;
+            r->lane = route(*r);
+            r->t_submit = snow();
+            std::lock_guard<std::mutex> g(mu);
+            r->id = next_id++;
+            Request* p = r.get();
+            reqs[p->id] = std::move(r);
+            if (p->lane < 0) batch_waiting.push_back(p);
+            else lanes[p->lane]->waiting.push_back(p);
+            return p->id;
+        }
         read_csc(lp.m, lp.n, lp.index_base, lp.index_bits, lp.colptr, lp.rowval, r->colptr, r->rowval);
         r->m = lp.m;
         r->n = lp.n;
@@ -265,8 +301,7 @@ struct Server {
             }
             const cpdlp_lp lp = r->view();
             int64_t bid = 0;
-            const int rc = cpdlp_batch_add_start(batch, &lp, r->warm ? r->x0.data() : nullptr,
-                                                 r->warm ? r->y0.data() : nullptr, &bid);
+            const int rc = cpdlp_batch_add_start(batch, &lp, r->start_x(), r->start_y(), &bid);
             to_batch.pop_front();
             if (rc == CPDLP_ERR_MEMORY) {
                 left.push_back(r);
@@ -323,7 +358,60 @@ struct Server {
         return pending_locked();
     }
 
+    // pumps until the server is destroyed; idles 200 us when the batch route has nothing
+    // to run.  A failure stops the driver and is reported by every later call.
+    void drive() {
+        for (;;) {
+            {
+                std::lock_guard<std::mutex> g(mu);
+                if (stop) return;
+            }
+            try {
+                pump();
+            } catch (const Error& e) {
+                std::lock_guard<std::mutex> g(mu);
+                driver_code = e.code;
+                driver_error = e.what();
+                return;
+            } catch (const std::exception& e) {
+                std::lock_guard<std::mutex> g(mu);
+                driver_code = CPDLP_ERR_ARGUMENT;
+                driver_error = e.what();
+                return;
+            }
+            bool idle;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                idle = in_batch.empty() && batch_waiting.empty();
+            }
+            if (idle) std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+    }
+
+    void start() {
+        std::lock_guard<std::mutex> g(mu);
+        if (driving) return;
+        driving = true;
+        driver = std::thread([this] { drive(); });
+    }
+
+    void check_driver() {
+        std::lock_guard<std::mutex> g(mu);
+        if (driver_code != CPDLP_OK) throw Error(driver_code, "server driver stopped: " + driver_error);
+    }
+
+    // cpdlp_server_pump: one round, or with the driver running only the pending count
+    int64_t pump_call() {
+        check_driver();
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (driving) return pending_locked();
+        }
+        return pump();
+    }
+
     int64_t poll(int64_t* ids, int64_t cap) {
+        check_driver();
         std::lock_guard<std::mutex> g(mu);
         int64_t k = 0;
         while (!done.empty() && k < cap) {
@@ -410,14 +498,23 @@ int cpdlp_server_submit(cpdlp_server* s, const cpdlp_lp* lp, const double* x0, c
     return guarded([&] {
         SERVER_OR_THROW(s);
         if (!lp || !id) throw Error(CPDLP_ERR_ARGUMENT, "NULL argument");
-        *id = s->s->submit(*lp, x0, y0);
+        *id = s->s->submit(*lp, x0, y0, false);
+    });
+}
+
+int cpdlp_server_submit_borrowed(cpdlp_server* s, const cpdlp_lp* lp, const double* x0,
+                                 const double* y0, int64_t* id) {
+    return guarded([&] {
+        SERVER_OR_THROW(s);
+        if (!lp || !id) throw Error(CPDLP_ERR_ARGUMENT, "NULL argument");
+        *id = s->s->submit(*lp, x0, y0, true);
     });
 }
 
 int cpdlp_server_pump(cpdlp_server* s, int64_t* pending) {
     return guarded([&] {
         SERVER_OR_THROW(s);
-        const int64_t k = s->s->pump();
+        const int64_t k = s->s->pump_call();
         if (pending) *pending = k;
     });
 }
@@ -435,6 +532,13 @@ int cpdlp_server_get(cpdlp_server* s, int64_t id, double* x, double* y, cpdlp_st
     return guarded([&] {
         SERVER_OR_THROW(s);
         s->s->get(id, x, y, st, seconds, route);
+    });
+}
+
+int cpdlp_server_start(cpdlp_server* s) {
+    return guarded([&] {
+        SERVER_OR_THROW(s);
+        s->s->start();
     });
 }
 

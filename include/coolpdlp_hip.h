@@ -292,13 +292,14 @@ void cpdlp_matrix_destroy(cpdlp_matrix* A);
 
 /* ---- serving: continuous batching of solve requests (no reference counterpart) ----
  * The served form of solve(milp, sol, algo) (src/algorithms/common.jl:186-209).
- * Requests are copied at submit.  Those with m + n <= batch_limit go to a batch engine
- * (as cpdlp_batch_*: admitted as arena room frees up, advanced by cpdlp_server_pump);
+ * Requests are copied at submit (or borrowed).  Those with m + n <= batch_limit go to a
+ * batch engine (as cpdlp_batch_*: admitted as arena room frees up, advanced by
+ * cpdlp_server_pump);
  * the others to plain engines (as cpdlp_solve, CSR format, one HIP stream each) run by
  * worker threads in lanes by nonzero count (a request takes the first lane whose
  * lane_max_nnz admits it, else the last), shortest job first within a lane, under the
  * plain_bytes HBM budget.  Submit, poll and get may be called from any thread; pump
- * from one thread at a time. */
+ * from one thread at a time, or by the library itself after cpdlp_server_start. */
 #define CPDLP_SERVER_MAX_LANES 4
 typedef struct cpdlp_server_config {
     int64_t batch_bytes;            /* batch route's HBM arena (16 GiB) */
@@ -312,14 +313,27 @@ typedef struct cpdlp_server_config {
 typedef struct cpdlp_server cpdlp_server;
 
 int cpdlp_server_default_config(cpdlp_server_config* cfg);
-/* params as for cpdlp_create (the format is fixed to CSR on the plain route). */
+/* params as for cpdlp_create (the format is fixed to CSR on the plain route).  Keep
+ * params->batch_streams plus the lanes' workers within the device's hardware queues (4 by
+ * default): streams beyond them share queues, and batch rounds then wait behind plain
+ * solves' kernels. */
 int cpdlp_server_create(const cpdlp_params* params, const cpdlp_server_config* cfg, cpdlp_server** out);
 /* Queue one LP; x0 / y0 (NULL: zero) are an unscaled start as in cpdlp_solve. */
 int cpdlp_server_submit(cpdlp_server* s, const cpdlp_lp* lp, const double* x0, const double* y0,
                         int64_t* id);
+/* cpdlp_server_submit without the copy: every array the LP and the start point name must
+ * stay valid and unchanged until cpdlp_server_get has returned for this id.  The arrays
+ * are validated when an engine takes the request; an invalid one is reported by
+ * cpdlp_server_get.  For callers that submit large LPs at a high rate. */
+int cpdlp_server_submit_borrowed(cpdlp_server* s, const cpdlp_lp* lp, const double* x0,
+                                 const double* y0, int64_t* id);
 /* One scheduling round: admit, advance the batch route by checks_per_round checks, harvest.
  * *pending = requests not yet finished. */
 int cpdlp_server_pump(cpdlp_server* s, int64_t* pending);
+/* Run the pump loop on a library thread from now on (idempotent): callers then only
+ * submit, poll and get, and cpdlp_server_pump only reports the pending count.  A failure
+ * of the loop is returned by every later pump and poll. */
+int cpdlp_server_start(cpdlp_server* s);
 /* Ids of finished requests (each reported once). */
 int cpdlp_server_poll(cpdlp_server* s, int64_t* ids, int64_t cap, int64_t* count);
 /* A finished request's unscaled x (n), y (m) and stats; *seconds = submit to finish;

@@ -34,18 +34,23 @@ def _drain(srv, want, deadline_s=180):
     return out
 
 
-def test_native_server_alone_equals_in_mix(cp):
+@pytest.mark.parametrize("borrow,driver", [(True, False), (False, False), (True, True)])
+def test_native_server_alone_equals_in_mix(cp, borrow, driver):
+    """Copied (cpdlp_server_submit) and borrowed (cpdlp_server_submit_borrowed) requests,
+    pumped by the caller or by the library's own thread (cpdlp_server_start)."""
     from coolpdlp_hip.serving import NativeServer
     small = [G.request_lp(m, 2 * m, 700 + m) for m in (60, 150, 420, 900, 1500, 2600)]
     big = [G.request_lp(3000, 6000, 78), G.request_lp(5000, 10000, 79)]
     algo = cp.PDLP("mixed", termination_reltol=1e-6)
     with NativeServer(algo, batch_bytes=256 << 20, plain_bytes=4 << 30, batch_limit=8000,
                       checks_per_round=2) as srv:
-        ids = {srv.submit(small[0]): small[0], srv.submit(big[0]): big[0]}
+        if driver:
+            srv.start()
+        ids = {srv.submit(small[0], borrow=borrow): small[0], srv.submit(big[0], borrow=borrow): big[0]}
         for g in small[1:] + big[1:]:
             for _ in range(3):
                 srv.pump()
-            ids[srv.submit(g)] = g          # arrives while the others are on the GPU
+            ids[srv.submit(g, borrow=borrow)] = g   # arrives while the others are on the GPU
         res = _drain(srv, len(ids))
     for i, g in ids.items():
         sol, st, secs, route = res[i]
