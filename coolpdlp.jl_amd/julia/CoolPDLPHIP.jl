@@ -346,8 +346,9 @@ end
     HIPServer(algo; batch_limit = 8192, checks_per_round = 20)
 
 The library's continuous-batching server: `submit!` queues an LP (optionally with a start
-point), `pump!` runs one scheduling round, `poll!` lists finished requests and `take!`
-hands one over as `(PrimalDualSolution, ConvergenceStats, seconds)`.
+point; the library copies it), `pump!` runs one scheduling round (or `start!` lets the
+library pump on its own thread), `poll!` lists finished requests and `take!` hands one
+over as `(PrimalDualSolution, ConvergenceStats, seconds)`.
 """
 mutable struct HIPServer
     ptr::Ptr{Cvoid}
@@ -390,6 +391,9 @@ function submit!(s::HIPServer, milp::MILP, sol::Union{Nothing, PrimalDualSolutio
     s.shapes[id[]] = (size(A_, 1), size(A_, 2))
     return id[]
 end
+
+"The library pumps on its own thread from now on (cpdlp_server_start)."
+start!(s::HIPServer) = (check(ccall((:cpdlp_server_start, libcpdlp), Cint, (Ptr{Cvoid},), s.ptr)); s)
 
 function pump!(s::HIPServer)
     pending = Ref{Int64}(0)
