@@ -179,8 +179,8 @@ inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
     L.rowval = std::move(rv);
     csc_to_csr(m, n, L.colptr, L.rowval, L.csr_rowptr, L.csr_col, L.perm);
     int32_t nlA = 0, nlAt = 0;
-    // First-touch order of the short rows, in two rounds.  Round 1: A's short rows by the
-    // first internal row of A' (column of A) that holds them; round 2: A's short columns
+    // First-touch order of the short rows, in two passes.  Pass 1: A's short rows by the
+    // first internal row of A' (column of A) that holds them; pass 2: A's short columns
     // by the first (new) row of A that holds them, then A's rows again.  Each step kernel
     // sweeps its rows in order (each XCD a contiguous range of tiles), so the first
     // gather of most entries of y and x comes in sequence and shares a line with its
