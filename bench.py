@@ -97,7 +97,8 @@ def bench_batch(args):
     nnz = sum(g.nnz for g in lps)
     log(f"[bench] generated {len(lps)} LPs, nnz={nnz} in {time.perf_counter() - t:.1f}s")
     reltol = 1e-6
-    algo = cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit, **polish(args))
+    algo = cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit,
+                   detect_infeasibility=int(args.detect_infeasibility), **polish(args))
     arena = int(sum(cp.batch.footprint(g.m, g.n, g.nnz) for g in lps) * 1.05) + (64 << 20)
     B = cp.BatchSolver(algo, hbm_bytes=arena)
     t = time.perf_counter()
@@ -151,6 +152,7 @@ def bench_batch(args):
         "config": {"workload": "batch_4096_uneven_lps", "count": len(lps), "nnz_total": nnz,
                    "m_range": [50, 5000], "reltol": reltol, "precision": "mixed",
                    "polish_reltol": algo.params.polish_reltol,
+                   "detect_infeasibility": bool(args.detect_infeasibility),
                    "checks_per_round": args.checks_per_round},
         "batch_seconds": elapsed, "rounds": rounds,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
