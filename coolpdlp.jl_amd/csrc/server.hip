@@ -98,6 +98,7 @@ struct Server {
     std::unordered_map<int64_t, Request*> in_batch;   // batch id -> request
     std::vector<std::unique_ptr<Lane>> lanes;   // Lane holds threads: not movable
     int64_t plain_used = 0;
+    int64_t admitting = 0;         // taken off batch_waiting by a pump, not yet placed
     std::deque<int64_t> done;
     // the optional driver thread (cpdlp_server_start) that pumps on its own
     std::thread driver;
@@ -277,7 +278,8 @@ struct Server {
     }
 
     int64_t pending_locked() const {
-        int64_t k = (int64_t)batch_waiting.size() + (int64_t)in_batch.size();
+        // admitting may count a request twice for a moment (never zero times)
+        int64_t k = (int64_t)batch_waiting.size() + (int64_t)in_batch.size() + admitting;
         for (auto& ln : lanes) k += (int64_t)ln->waiting.size() + ln->running;
         return k;
     }
@@ -289,6 +291,7 @@ struct Server {
             std::lock_guard<std::mutex> g(mu);
             admit_plain();
             to_batch.swap(batch_waiting);
+            admitting = (int64_t)to_batch.size();
         }
         // batch admission as arena room allows; the rest waits, in order
         std::deque<Request*> left;
@@ -322,6 +325,7 @@ struct Server {
         {
             std::lock_guard<std::mutex> g(mu);
             for (auto it = left.rbegin(); it != left.rend(); ++it) batch_waiting.push_front(*it);
+            admitting = 0;
             any = !in_batch.empty();
         }
         if (any) {
