@@ -4,12 +4,20 @@ Default workload: the single large LP -- m = 1,000,000 rows (half =, half <=),
 n = 2,000,000 columns, ~20M nonzeros with power-law row lengths (Zipf 1.5, capped at
 5,000, rescaled to 20M), N(0,1) values, 0 <= x <= 10 on 30% of the columns; plain
 FP64 PDLP with Ruiz + Chambolle-Pock scaling and adaptive restarts, reltol 1e-6.
-One step = one full solve (scaling, step size, the PDLP loop to termination) with
-the LP already resident in HBM.  `value` is LPs solved per second (1 / solve time).
+
+One step = one chunk of the PDLP loop: check_every (100) PDHG iterations and their
+termination/restart check, on the LP already resident in HBM.  The warmup and timed
+steps are the first chunks of ONE full solve, which then runs on to termination:
+`value` is LPs solved per second = 1 / that solve's time (scaling, step size and the
+loop; the LP resident in HBM), `ms_per_step` the mean time of the timed chunks.  A
+wall-clock guard (--budget seconds from process start) never starts a slice of the
+loop it cannot finish; a solve stopped by it is reported as such, with value null.
 
 Also reported: the roofline of the PDHG iteration (the two fused step kernels, timed
-live with HIP events on the solver's stream), per-kernel times, and the CPU oracle
-(FP64, OpenMP) on a bounded sample of the same LP on this host.
+live with HIP events on the solver's stream over the timed steps), per-kernel times,
+the step size of the reference's power-method rule beside the converged one, and the
+CPU oracle (FP64, OpenMP) on a bounded sample of the same LP on this host, projected to
+the GPU solve's pass count.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|plain|batch|serve|mcf]
 """
@@ -20,6 +28,7 @@ import os
 import sys
 import time
 
+T_PROC0 = time.perf_counter()   # the wall-clock guard counts from here
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "coolpdlp.jl_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -380,35 +389,39 @@ class NativeDone:
     objective: float
 
 
-def cpu_baseline(g, gpu_passes, reltol, budget_passes):
-    """Oracle FP64 PDLP on this host: a bounded sample of `budget_passes` passes (plus
-    their checks), projected to the GPU solve's pass count."""
+def cpu_baseline(g, gpu_passes, reltol, budget_passes, powm_tol, seconds_left):
+    """Oracle FP64 PDLP on this host: the whole solve when it fits in about 30 s,
+    else a bounded sample of `budget_passes` passes (plus their checks) projected to
+    the GPU solve's pass count (setup + passes x the sample's time per pass)."""
     import oracle as O
     cores = len(os.sched_getaffinity(0))
     omp = int(os.environ.get("OMP_NUM_THREADS", cores))
     threads = max(1, min(cores, omp))
     O.set_threads(threads)
-    p = O.default_params(termination_reltol=reltol, max_kkt_passes=budget_passes, time_limit=1e9)
+    p = O.default_params(termination_reltol=reltol, max_kkt_passes=budget_passes, time_limit=1e9,
+                         powm_tol=powm_tol)
     t = time.perf_counter()
     r = O.solve(g, p, hist_cap=16)
     wall = time.perf_counter() - t
     loop = max(wall - r.stats.setup_seconds, 1e-9)
     per_pass = loop / max(r.stats.kkt_passes, 1)
     projected = r.stats.setup_seconds + per_pass * gpu_passes
-    if projected <= 30.0:   # small enough: time the whole CPU solve instead
-        p = O.default_params(termination_reltol=reltol, time_limit=1e9)
+    if projected <= min(30.0, seconds_left - wall):   # small enough: time the whole solve
+        p = O.default_params(termination_reltol=reltol, time_limit=1e9, powm_tol=powm_tol)
         t = time.perf_counter()
         r = O.solve(g, p, hist_cap=16)
         wall = time.perf_counter() - t
         return {
             "value": 1.0 / wall, "unit": "LPs/s", "cores": threads, "kind": "port",
+            "projected": False,
             "sample": (f"oracle FP64 PDLP, the whole solve: {r.status} after {r.stats.kkt_passes} "
                        f"passes in {wall:.2f}s"),
             "status_at_sample": r.status,
         }
     return {
         "value": 1.0 / projected, "unit": "LPs/s", "cores": threads, "kind": "port",
-        "sample": (f"oracle FP64 PDLP, {r.stats.kkt_passes} passes incl. their checks "
+        "projected": True,
+        "sample": (f"PROJECTED: oracle FP64 PDLP, {r.stats.kkt_passes} passes incl. their checks "
                    f"({wall:.1f}s wall, setup {r.stats.setup_seconds:.1f}s, "
                    f"{1e3 * per_pass:.1f} ms/pass) projected to the GPU solve's "
                    f"{gpu_passes} passes: {projected:.0f}s per LP"),
@@ -416,15 +429,188 @@ def cpu_baseline(g, gpu_passes, reltol, budget_passes):
     }
 
 
+def budget_left(args) -> float:
+    """Seconds left of the wall-clock budget (counted from process start)."""
+    return args.budget - (time.perf_counter() - T_PROC0)
+
+
+def bench_single(args):
+    """One LP (large: the power-law LP; plain: the m=2,000 LP).  Large: a step is one
+    chunk of the loop inside one timed full solve.  Plain: a step is one full solve
+    (the whole solve takes ~9 chunks)."""
+    import torch
+    import coolpdlp_hip as cp
+    t = time.perf_counter()
+    g, reltol, desc = make_workload(args.workload)
+    log(f"[bench] generated {desc['workload']} m={g.m} n={g.n} nnz={g.nnz} in "
+        f"{time.perf_counter() - t:.1f}s")
+    milp = cp.MILP.from_generated(g)
+    # powm_tol: the reference's power-method rule (tol = eps*n^3) stops after one
+    # Rayleigh quotient of a random vector at n = 2e6, which underestimates ||A|| and
+    # makes eta*||A|| > 1; the benchmark converges the power method to 1e-4 instead and
+    # reports the reference rule's eta beside it
+    algo = cp.PDLP("float64", None, args.format, termination_reltol=reltol, time_limit=args.time_limit,
+                   max_kkt_passes=args.max_passes, device=0, powm_tol=args.powm_tol,
+                   show_progress=0, detect_infeasibility=int(args.detect_infeasibility))
+    t = time.perf_counter()
+    eng = cp.Engine(milp, algo)
+    torch.cuda.synchronize()
+    log(f"[bench] uploaded to HBM in {time.perf_counter() - t:.1f}s")
+    chunked = args.workload == "large"
+    guard = None
+    if chunked:
+        t_solve0 = time.perf_counter()
+        eng.initialize()
+        status = cp.STILL_RUNNING
+        for w in range(args.warmup):
+            status = eng.run_checks(1)
+        s0 = eng.stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        timed = 0
+        for k in range(args.steps):
+            if status != cp.STILL_RUNNING:
+                break
+            status = eng.run_checks(1)
+            timed += 1
+        torch.cuda.synchronize()
+        steps_elapsed = time.perf_counter() - t0
+        s1 = eng.stats()
+        per_check = steps_elapsed / max(timed, 1)
+        log(f"[bench] {timed} timed steps: {1e3 * per_check:.2f} ms per chunk of "
+            f"{algo.params.check_every} iterations")
+        # the rest of the same solve, in slices the guard can account for
+        last_log = time.perf_counter()
+        while status == cp.STILL_RUNNING:
+            left = budget_left(args) - args.reserve
+            slice_checks = max(1, min(args.slice_checks, int(left / max(per_check, 1e-6) / 1.25)))
+            if left <= 0 or slice_checks * per_check * 1.25 > left:
+                guard = f"stopped by the wall-clock guard after {eng.stats().kkt_passes} passes"
+                log(f"[bench] {guard}")
+                break
+            status = eng.run_checks(slice_checks)
+            if time.perf_counter() - last_log > 20:
+                last_log = time.perf_counter()
+                st = eng.stats()
+                log(f"[bench] solve t={time.perf_counter() - t_solve0:.1f}s passes={st.kkt_passes} "
+                    f"rel={st.err.relative():.3e}")
+        torch.cuda.synchronize()
+        solve_seconds = time.perf_counter() - t_solve0
+        st = eng.stats()
+        steps_done, warm_done = timed, args.warmup
+        ms_per_step = 1e3 * per_check
+        # roofline over the timed steps: HIP events on the solver's stream around each chunk
+        d_ms = s1.step_kernel_ms - s0.step_kernel_ms
+        d_launch = s1.step_kernel_launches - s0.step_kernel_launches
+    else:
+        for w in range(args.warmup):
+            _, st = eng.solve()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            _, st = eng.solve()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        solve_seconds = elapsed / max(args.steps, 1)
+        steps_done, warm_done = args.steps, args.warmup
+        ms_per_step = 1e3 * solve_seconds
+        d_ms, d_launch = st.step_kernel_ms, st.step_kernel_launches
+    log(f"[bench] solve: {st.termination_status.name} passes={st.kkt_passes} "
+        f"{solve_seconds:.2f}s rel={st.err.relative():.3e}")
+    sol = eng.solution()
+    obj = cp.objective_value(sol.x, milp)
+
+    bp, bd = iteration_bytes(g.m, g.n, g.nnz)
+    t_iter_ms = d_ms / max(d_launch // 2, 1)
+    achieved = (bp + bd) / (t_iter_ms * 1e-3) / 1e9
+    t_iter_all = st.step_kernel_ms / max(st.step_kernel_launches // 2, 1)
+    ms_p, ms_d = eng.profile_steps(20)
+    kernels = {
+        "primal_step": {"ms": ms_p, "bytes": bp, "GBs": bp / (ms_p * 1e-3) / 1e9,
+                        "frac": bp / (ms_p * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "dual_step": {"ms": ms_d, "bytes": bd, "GBs": bd / (ms_d * 1e-3) / 1e9,
+                      "frac": bd / (ms_d * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    traffic, traffic_src = measured_traffic(desc["workload"], FORMATS[st.format_A], FORMATS[st.format_At])
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": "pdhg_iteration (primal_step + dual_step)",
+                "timed_over": "the timed steps" if chunked else "the last solve",
+                "bytes_per_iteration": bp + bd, "ms_per_iteration": t_iter_ms,
+                "ms_per_iteration_whole_solve": t_iter_all, "kernels": kernels}
+    eng.close()
+
+    if args.calibrate_copy:
+        a = torch.empty(64 * 2**20, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 5 x 512 MiB read + 512 MiB written")
+        del a, b
+
+    # the reference's own power-method rule (powm! with tol = eps*n^3, maxiter = n)
+    eta_ref = None
+    if args.workload == "large" and not args.no_eta_ref and budget_left(args) > args.reserve + 30:
+        ref_algo = cp.PDLP("float64", None, args.format, termination_reltol=reltol, device=0,
+                           powm_tol=float("nan"))
+        with cp.Engine(milp, ref_algo) as e2:
+            e2.prepare()
+            eta_r, _, it_r = e2.step_size()
+        eta_ref = {"eta": eta_r, "powm_iters": it_r, "eta_over_converged": eta_r / st.eta,
+                   "rule": "powm! tol = eps(Float64)*n^3, maxiter = n (linalg.jl:86-96)"}
+        log(f"[bench] reference-rule eta {eta_r:.6e} after {it_r} power iterations "
+            f"(converged eta {st.eta:.6e})")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(g, st.kkt_passes, reltol, args.cpu_passes, args.powm_tol,
+                           budget_left(args) - 15)
+        log(f"[bench] cpu baseline: {cpu['sample']}")
+
+    finished = st.termination_status != cp.STILL_RUNNING and guard is None
+    return {
+        "metric": "solve time to rel-KKT 1e-6; batched LPs/s + p50/p99 vs plain PDLP, 1 GPU",
+        "value": (1.0 / solve_seconds) if finished else None, "unit": "LPs/s", "n_gpus": 1,
+        "steps": steps_done, "warmup": warm_done, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator with a known optimum)",
+        "config": desc,
+        "step": ("one chunk of check_every PDHG iterations + its check, inside the timed solve"
+                 if chunked else "one full solve"),
+        "solve_seconds": solve_seconds, "loop_seconds": st.loop_seconds,
+        "wall_clock_guard": guard,
+        "detect_infeasibility": bool(args.detect_infeasibility),
+        "termination_status": st.termination_status.name, "kkt_passes": st.kkt_passes,
+        "relative_kkt": st.err.relative(), "restarts": st.restarts,
+        "objective": obj, "objective_closed_form": g.objective,
+        "objective_rel_err": abs(obj - g.objective) / (1 + abs(g.objective)),
+        "formats": {"A": FORMATS[st.format_A], "At": FORMATS[st.format_At]},
+        "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
+        "powm_tol": args.powm_tol, "eta_reference_rule": eta_ref,
+        "roofline": roofline, "cpu_baseline": cpu,
+        "process_seconds": time.perf_counter() - T_PROC0,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="large")
     ap.add_argument("--cpu-passes", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-limit", type=float, default=600.0)
+    ap.add_argument("--budget", type=float, default=560.0,
+                    help="wall-clock seconds from process start the whole run must fit in")
+    ap.add_argument("--reserve", type=float, default=75.0,
+                    help="seconds of the budget kept for the report after the solve")
+    ap.add_argument("--slice-checks", type=int, default=100,
+                    help="checks per loop call after the timed steps (guard granularity)")
+    ap.add_argument("--no-eta-ref", action="store_true",
+                    help="skip the reference power-method rule's step size")
     ap.add_argument("--powm-tol", type=float, default=1e-4)
     ap.add_argument("--detect-infeasibility", action="store_true",
                     help="opt-in ray certificates at every check (timed with the solve)")
@@ -477,91 +663,7 @@ def main():
         print(json.dumps(bench_serve(args)), flush=True)
         return
 
-    t = time.perf_counter()
-    g, reltol, desc = make_workload(args.workload)
-    log(f"[bench] generated {desc['workload']} m={g.m} n={g.n} nnz={g.nnz} in "
-        f"{time.perf_counter() - t:.1f}s")
-    milp = cp.MILP.from_generated(g)
-    # powm_tol: the reference's power-method rule (tol = eps*n^3) stops after one
-    # Rayleigh quotient of a random vector at n = 2e6, which underestimates ||A|| and
-    # makes eta*||A|| > 1; the benchmark converges the power method to 1e-4 instead
-    algo = cp.PDLP("float64", None, args.format, termination_reltol=reltol, time_limit=args.time_limit,
-                   max_kkt_passes=args.max_passes, device=0, powm_tol=args.powm_tol,
-                   show_progress=1, detect_infeasibility=int(args.detect_infeasibility))
-    t = time.perf_counter()
-    eng = cp.Engine(milp, algo)
-    log(f"[bench] uploaded to HBM in {time.perf_counter() - t:.1f}s")
-
-    for w in range(args.warmup):
-        _, st = eng.solve()
-        log(f"[bench] warmup {w}: {st.termination_status.name} passes={st.kkt_passes} "
-            f"t={st.time_elapsed:.2f}s")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stats = []
-    for k in range(args.steps):
-        _, st = eng.solve()
-        stats.append(st)
-        log(f"[bench] step {k}: {st.termination_status.name} passes={st.kkt_passes} "
-            f"t={st.time_elapsed:.2f}s rel={st.err.relative():.3e}")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    st = stats[-1]
-    sol = eng.solution()
-    obj = cp.objective_value(sol.x, milp)
-
-    # roofline of the PDHG iteration, live (events around every chunk of steps)
-    bp, bd = iteration_bytes(g.m, g.n, g.nnz)
-    iters = st.step_kernel_launches // 2
-    t_iter_ms = st.step_kernel_ms / max(iters, 1)
-    achieved = (bp + bd) / (t_iter_ms * 1e-3) / 1e9
-    ms_p, ms_d = eng.profile_steps(20)
-    kernels = {
-        "primal_step": {"ms": ms_p, "bytes": bp, "GBs": bp / (ms_p * 1e-3) / 1e9},
-        "dual_step": {"ms": ms_d, "bytes": bd, "GBs": bd / (ms_d * 1e-3) / 1e9},
-    }
-    fmt = FORMATS
-    traffic, traffic_src = measured_traffic(desc["workload"], fmt[st.format_A], fmt[st.format_At])
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                "kernel": "pdhg_iteration (primal_step + dual_step)",
-                "bytes_per_iteration": bp + bd, "ms_per_iteration": t_iter_ms,
-                "kernels": kernels}
-
-    if args.calibrate_copy:
-        a = torch.empty(64 * 2**20, dtype=torch.float64, device="cuda:0").fill_(1.0)
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("[bench] calibration copy: 5 x 512 MiB read + 512 MiB written")
-
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(g, st.kkt_passes, reltol, args.cpu_passes)
-        log(f"[bench] cpu baseline: {cpu['sample']}")
-
-    value = args.steps / elapsed
-    out = {
-        "metric": "solve time to rel-KKT 1e-6; batched LPs/s + p50/p99 vs plain PDLP, 1 GPU",
-        "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded generator with a known optimum)",
-        "config": desc,
-        "solve_seconds": elapsed / args.steps, "loop_seconds": st.loop_seconds,
-        "detect_infeasibility": bool(args.detect_infeasibility),
-        "termination_status": st.termination_status.name, "kkt_passes": st.kkt_passes,
-        "relative_kkt": st.err.relative(), "restarts": st.restarts,
-        "objective": obj, "objective_closed_form": g.objective,
-        "objective_rel_err": abs(obj - g.objective) / (1 + abs(g.objective)),
-        "formats": {"A": FORMATS[st.format_A], "At": FORMATS[st.format_At]},
-        "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
-        "powm_tol": args.powm_tol,
-        "roofline": roofline, "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(bench_single(args)), flush=True)
 
 
 if __name__ == "__main__":

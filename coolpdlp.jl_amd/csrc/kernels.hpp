@@ -1,7 +1,7 @@
 // kernels.hpp -- device kernels and epilogues of the PDLP/PDHG loop.
 //
 // Step epilogues follow the reference's broadcasts operation by operation
-// (src/algorithms/pdlp.jl:110-117, pdhg.jl:286-293) and the averaging axpby!
+// (src/algorithms/pdlp.jl:110-117, pdhg.jl:91-97) and the averaging axpby!
 // (pdlp.jl:126-136), fused into the row loop of the product that feeds them.  KKT
 // epilogues follow kkt_errors! (src/components/errors.jl:65-109); they produce raw
 // sums, from which the host forms the six KKTErrors fields.

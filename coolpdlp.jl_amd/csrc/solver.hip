@@ -1010,7 +1010,7 @@ void Handle::run_chunk(IterState<V>& S, bool pdlp) {
     S.cur = g.end_parity;
 }
 
-// one PDLP/PDHG solve phase in precision V (pdlp.jl:69-94, pdhg.jl:252-270).
+// one PDLP/PDHG solve phase in precision V (pdlp.jl:69-94, pdhg.jl:56-74).
 // Returns when the phase terminates; phase_reltol replaces termination_reltol (mixed
 // precision's FP32 phase), final = false marks a non-final phase (no duplicate entry).
 template <class V>
@@ -1353,7 +1353,7 @@ void Handle::solve(const double* x0, const double* y0) {
     finish_stats();
 }
 
-// initialize(milp, sol, algo) (common.jl:176; pdlp.jl:45-67, pdhg.jl:40-56): everything
+// initialize(milp, sol, algo) (common.jl:176; pdlp.jl:45-67, pdhg.jl:40-54): everything
 // of solve before the loop; the loop then runs in pieces through run_checks.
 void Handle::initialize(const double* x0, const double* y0) {
     if (P.precision == CPDLP_PREC_MIXED)

@@ -152,7 +152,7 @@ int cpdlp_create(const cpdlp_lp* lp, const cpdlp_params* params, cpdlp_handle** 
 int cpdlp_solve(cpdlp_handle* h, const double* x0, const double* y0);
 
 /* initialize(milp, sol, algo), src/algorithms/common.jl:176 (pdlp.jl:45-67,
- * pdhg.jl:40-56): what cpdlp_solve does before its loop (preprocess, step size,
+ * pdhg.jl:40-54): what cpdlp_solve does before its loop (preprocess, step size,
  * primal weight, the start point on the device), the loop then run in pieces by
  * cpdlp_run_checks.  FP64 and FP32 precision; the mixed precision's two phases run
  * through cpdlp_solve (CPDLP_ERR_ARGUMENT here). */

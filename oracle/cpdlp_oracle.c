@@ -540,11 +540,45 @@ static int32_t term_status(const orc_stats* st, const orc_params* p) {
     return 3;
 }
 
+/* restart_check!, pdlp.jl:149-165 (candidate: the iterate if strictly better than the
+ * average, else the average; the same choice one step earlier, F10), then should_restart,
+ * restart.jl:40-57 */
+int32_t orc_restart_decision(const orc_kkt* e_sol, const orc_kkt* e_avg, const orc_kkt* e_last,
+                             const orc_kkt* e_avg_last, const orc_kkt* e_restart, double omega,
+                             int64_t inner, int64_t total, const orc_params* p, int32_t* from_avg) {
+    int32_t fa = !(absolute(e_sol, omega) < absolute(e_avg, omega));
+    const orc_kkt* cand = fa ? e_avg : e_sol;
+    const orc_kkt* cand_last = absolute(e_last, omega) < absolute(e_avg_last, omega) ? e_last : e_avg_last;
+    double ac = absolute(cand, omega), ar = absolute(e_restart, omega), acl = absolute(cand_last, omega);
+    int sufficient = ac <= p->sufficient_decay * ar;
+    int necessary = ac <= p->necessary_decay * ar;
+    int no_local = ac > acl;
+    int long_inner = (double)inner >= p->artificial_decay * (double)total;
+    if (from_avg) *from_avg = fa;
+    return (sufficient || (necessary && no_local) || long_inner) ? 1 : 0;
+}
+
+/* primal_weight_update!, step_size.jl:58-75 */
+double orc_primal_weight_update(double omega, double dx, double dy, double damping, double zero_tol) {
+    if (dx > zero_tol && dy > zero_tol)
+        return exp(damping * log(dy / dx) + (1.0 - damping) * log(omega));
+    return omega;
+}
+
 int orc_solve(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval,
               const double* nzval, const double* c_in, const double* lv_in, const double* uv_in,
               const double* lc_in, const double* uc_in, const double* x0, const double* y0,
               const orc_params* p, double* x_out, double* y_out, orc_stats* st,
               orc_hist* hist, int64_t hist_cap) {
+    return orc_solve_traced(m, n, colptr, rowval, nzval, c_in, lv_in, uv_in, lc_in, uc_in, x0, y0, p,
+                            x_out, y_out, st, hist, hist_cap, NULL);
+}
+
+int orc_solve_traced(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval,
+                     const double* nzval, const double* c_in, const double* lv_in, const double* uv_in,
+                     const double* lc_in, const double* uc_in, const double* x0, const double* y0,
+                     const orc_params* p, double* x_out, double* y_out, orc_stats* st,
+                     orc_hist* hist, int64_t hist_cap, orc_trace* tr) {
     double t_start = now_s();                         /* common.jl:191 */
     int64_t nnz = colptr[n];
     size_t nz1 = (size_t)(nnz > 0 ? nnz : 1), m1 = (size_t)(m > 0 ? m : 1), n1 = (size_t)(n > 0 ? n : 1);
@@ -619,7 +653,7 @@ int orc_solve(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval
         goto finish;
     }
 
-    for (;;) {                                         /* pdlp.jl:75-91 / pdhg.jl:258-267 */
+    for (;;) {                                         /* pdlp.jl:75-91 / pdhg.jl:62-73 */
         int terminated = 0;
         for (;;) {
             for (int32_t s = 0; s < p->check_every; ++s) {
@@ -663,31 +697,46 @@ int orc_solve(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval
                 int32_t inf = infeasibility_status(&L, p, &st->err, &sol, &rst, &sol_last);
                 if (inf != 3) { st->termination_status = inf; goto finish; }
             }
+            int64_t ti = tr ? tr->count : 0;
+            int trace_this = tr && ti < tr->cap;
+            if (tr) tr->count += 1;
+            if (trace_this) {
+                const orc_kkt nan_err = {NAN, NAN, NAN, NAN, NAN, NAN};
+                tr->passes[ti] = st->kkt_passes;
+                tr->err[5 * ti] = st->err;
+                for (int q = 1; q < 5; ++q) tr->err[5 * ti + q] = nan_err;
+                tr->from_avg[ti] = 0;
+                tr->restarted[ti] = 0;
+                tr->omega[ti] = tr->omega_after[ti] = omega;
+                tr->inner[ti] = inner;
+                tr->total[ti] = total;
+            }
             if (st->termination_status != 3) { terminated = 1; break; }
             if (!pdlp) continue;
             /* restart_check!, pdlp.jl:138-170 */
-            orc_kkt e_s = st->err, e_a, e_l, e_al, e_r, cand, cand_last;
+            orc_kkt e_s = st->err, e_a, e_l, e_al, e_r;
             kkt_errors(&L, avg.x, avg.y, &e_a);
-            int from_avg = !(absolute(&e_s, omega) < absolute(&e_a, omega));
-            cand = from_avg ? e_a : e_s;
             kkt_errors(&L, sol_last.x, sol_last.y, &e_l);
             kkt_errors(&L, avg_last.x, avg_last.y, &e_al);
-            cand_last = absolute(&e_l, omega) < absolute(&e_al, omega) ? e_l : e_al;
             kkt_errors(&L, rst.x, rst.y, &e_r);
-            /* should_restart, restart.jl:40-57 */
-            double ac = absolute(&cand, omega), ar = absolute(&e_r, omega), acl = absolute(&cand_last, omega);
-            int sufficient = ac <= p->sufficient_decay * ar;
-            int necessary = ac <= p->necessary_decay * ar;
-            int no_local = ac > acl;
-            int long_inner = (double)inner >= p->artificial_decay * (double)total;
-            if (sufficient || (necessary && no_local) || long_inner) {
+            int32_t from_avg = 0;
+            int32_t restart = orc_restart_decision(&e_s, &e_a, &e_l, &e_al, &e_r, omega, inner, total, p,
+                                                   &from_avg);
+            if (trace_this) {
+                tr->err[5 * ti + 1] = e_a;
+                tr->err[5 * ti + 2] = e_l;
+                tr->err[5 * ti + 3] = e_al;
+                tr->err[5 * ti + 4] = e_r;
+                tr->from_avg[ti] = from_avg;
+                tr->restarted[ti] = restart;
+            }
+            if (restart) {
                 /* restart!, pdlp.jl:172-196 */
                 pd sc = from_avg ? avg : sol;
                 eta_sum = 0.0;
                 double dx = dist2(n, sc.x, rst.x), dy = dist2(m, sc.y, rst.y);
-                double th = p->primal_weight_damping;     /* step_size.jl:58-75 */
-                if (dx > p->zero_tol && dy > p->zero_tol)
-                    omega = exp(th * log(dy / dx) + (1.0 - th) * log(omega));
+                omega = orc_primal_weight_update(omega, dx, dy, p->primal_weight_damping, p->zero_tol);
+                if (trace_this) tr->omega_after[ti] = omega;
                 if (from_avg) {
                     memcpy(sol.x, avg.x, (size_t)n * sizeof(double));
                     memcpy(sol.y, avg.y, (size_t)m * sizeof(double));
@@ -713,6 +762,8 @@ finish:
     /* get_solution: unprecondition, preconditioning.jl:53-59 */
     for (int64_t j = 0; j < n; ++j) x_out[j] = D2[j] * sol.x[j];
     for (int64_t i = 0; i < m; ++i) y_out[i] = D1[i] * sol.y[i];
+    if (tr && tr->x_avg) for (int64_t j = 0; j < n; ++j) tr->x_avg[j] = D2[j] * avg.x[j];
+    if (tr && tr->y_avg) for (int64_t i = 0; i < m; ++i) tr->y_avg[i] = D1[i] * avg.y[i];
     st->eta = eta;
     st->omega = omega;
     free(buf); free(L.sx); free(L.sr); free(L.sy);

@@ -119,8 +119,45 @@ double orc_spectral_norm(int64_t m, int64_t n,
                          const int64_t* at_rowptr, const int32_t* at_col, const double* at_val,
                          double tol, int64_t maxiter, uint64_t seed, int64_t* iters_out);
 
+/* restart_check!'s candidate choice (pdlp.jl:149-165) and should_restart
+ * (src/components/restart.jl:40-57) on the five KKT errors of a check: the iterate,
+ * the average, both one PDHG step earlier, and the restart point.  Returns 1 to
+ * restart; *from_avg = 1 when the candidate is the average (ties go to the average). */
+int32_t orc_restart_decision(const orc_kkt* e_sol, const orc_kkt* e_avg, const orc_kkt* e_last,
+                             const orc_kkt* e_avg_last, const orc_kkt* e_restart, double omega,
+                             int64_t inner, int64_t total, const orc_params* p, int32_t* from_avg);
+
+/* primal_weight_update!, src/components/step_size.jl:58-75, given the distances
+ * dx = ||x_cand - x_restart||, dy = ||y_cand - y_restart||. */
+double orc_primal_weight_update(double omega, double dx, double dy, double damping, double zero_tol);
+
+/* Per-check record of a solve (for tests): arrays of `cap` entries, filled for the
+ * first min(count, cap) checks.  err holds 5 entries per check: the iterate, the
+ * average, the iterate and the average one step earlier, and the restart point (PDHG:
+ * the iterate only, the rest NaN).  x_avg / y_avg (may be NULL) receive the final
+ * average, unscaled. */
+typedef struct {
+    int64_t cap;
+    int64_t count;
+    int64_t* passes;       /* kkt_passes at the check */
+    orc_kkt* err;          /* 5 * cap */
+    int32_t* from_avg;     /* the candidate was the average */
+    int32_t* restarted;    /* the check restarted */
+    double* omega;         /* primal weight in force at the check */
+    double* omega_after;   /* primal weight after the check (updated on a restart) */
+    int64_t* inner;        /* inner / total iteration counters at the check */
+    int64_t* total;
+    double* x_avg;
+    double* y_avg;
+} orc_trace;
+
 /* Full solve: solve(milp, sol, algo), common.jl:186-201.  Input is the UNSCALED LP
  * as CSC; x0/y0 may be NULL (zero start).  Outputs unscaled x (n), y (m). */
+int orc_solve_traced(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval,
+                     const double* nzval, const double* c, const double* lv, const double* uv,
+                     const double* lc, const double* uc, const double* x0, const double* y0,
+                     const orc_params* p, double* x_out, double* y_out, orc_stats* stats,
+                     orc_hist* hist, int64_t hist_cap, orc_trace* trace);
 int orc_solve(int64_t m, int64_t n, const int64_t* colptr, const int32_t* rowval,
               const double* nzval, const double* c, const double* lv, const double* uv,
               const double* lc, const double* uc, const double* x0, const double* y0,

@@ -71,6 +71,15 @@ class Stats(C.Structure):
                 ("restarts", C.c_int64), ("n_history", C.c_int64), ("setup_seconds", C.c_double)]
 
 
+class Trace(C.Structure):
+    _fields_ = [("cap", C.c_int64), ("count", C.c_int64), ("passes", C.POINTER(C.c_int64)),
+                ("err", C.POINTER(KKT)), ("from_avg", C.POINTER(C.c_int32)),
+                ("restarted", C.POINTER(C.c_int32)), ("omega", C.POINTER(C.c_double)),
+                ("omega_after", C.POINTER(C.c_double)), ("inner", C.POINTER(C.c_int64)),
+                ("total", C.POINTER(C.c_int64)), ("x_avg", C.POINTER(C.c_double)),
+                ("y_avg", C.POINTER(C.c_double))]
+
+
 _lib = None
 
 
@@ -107,6 +116,13 @@ def lib():
         L.orc_ray_test.argtypes = [C.c_int64, C.c_int64, i64, i32, d, i64, i32, d,
                                    d, d, d, d, d, d, d, d, d, C.c_double]
         L.orc_ray_test.restype = C.c_int32
+        L.orc_restart_decision.argtypes = [P(KKT)] * 5 + [C.c_double, C.c_int64, C.c_int64,
+                                                         P(Params), i32]
+        L.orc_restart_decision.restype = C.c_int32
+        L.orc_primal_weight_update.argtypes = [C.c_double] * 5
+        L.orc_primal_weight_update.restype = C.c_double
+        L.orc_solve_traced.argtypes = L.orc_solve.argtypes + [P(Trace)]
+        L.orc_solve_traced.restype = C.c_int
         _lib = L
     return _lib
 
@@ -260,3 +276,78 @@ def solve(lp, params: Params | None = None, x0=None, y0=None, hist_cap=100_000, 
                     _p(xo, C.c_double), _p(yo, C.c_double), C.byref(st), hist, hist_cap)
     h = [(hist[i].kkt_passes, hist[i].err.as_dict()) for i in range(min(st.n_history, hist_cap))]
     return OracleResult(xo[:n], yo[:m], st, h)
+
+
+def kkt(**fields) -> KKT:
+    """A KKTErrors value; unset fields are 0 (scales 1)."""
+    e = KKT()
+    for f, _ in KKT._fields_:
+        setattr(e, f, fields.get(f, 1.0 if f.endswith("_scale") else 0.0))
+    return e
+
+
+def restart_decision(e_sol, e_avg, e_last, e_avg_last, e_restart, omega, inner, total,
+                     params: Params | None = None):
+    """(restart?, candidate is the average?) of restart_check! + should_restart
+    (pdlp.jl:149-169, restart.jl:40-57) on five KKT errors."""
+    params = params or default_params()
+    fa = C.c_int32()
+    r = lib().orc_restart_decision(C.byref(e_sol), C.byref(e_avg), C.byref(e_last),
+                                   C.byref(e_avg_last), C.byref(e_restart), float(omega),
+                                   int(inner), int(total), C.byref(params), C.byref(fa))
+    return bool(r), bool(fa.value)
+
+
+def primal_weight_update(omega, dx, dy, damping=0.5, zero_tol=1e-8) -> float:
+    """primal_weight_update!, step_size.jl:58-75, on the distances dx, dy."""
+    return lib().orc_primal_weight_update(float(omega), float(dx), float(dy), float(damping),
+                                          float(zero_tol))
+
+
+@dataclass
+class TraceResult:
+    result: OracleResult
+    passes: np.ndarray       # kkt_passes at each check
+    err: list                # per check: dict name -> KKT (sol, avg, last, avg_last, restart)
+    from_avg: np.ndarray
+    restarted: np.ndarray
+    omega: np.ndarray
+    omega_after: np.ndarray
+    inner: np.ndarray
+    total: np.ndarray
+    x_avg: np.ndarray        # final average, unscaled
+    y_avg: np.ndarray
+
+
+def solve_traced(lp, params: Params, cap=4096, x0=None, y0=None) -> TraceResult:
+    """solve() with a per-check record of the restart check's inputs and decisions."""
+    m, n = lp.m, lp.n
+    colptr, rowval, nzval = _i64(lp.colptr), _i32(lp.rowval), _f64(lp.nzval)
+    vs = [_f64(v) for v in (lp.c, lp.lv, lp.uv, lp.lc, lp.uc)]
+    xo, yo = np.zeros(max(n, 1)), np.zeros(max(m, 1))
+    st = Stats()
+    hist = (Hist * 16)()
+    arr = {"passes": np.zeros(cap, np.int64), "from_avg": np.zeros(cap, np.int32),
+           "restarted": np.zeros(cap, np.int32), "omega": np.zeros(cap), "omega_after": np.zeros(cap),
+           "inner": np.zeros(cap, np.int64), "total": np.zeros(cap, np.int64)}
+    errs = (KKT * (5 * cap))()
+    xa, ya = np.zeros(max(n, 1)), np.zeros(max(m, 1))
+    tr = Trace()
+    tr.cap, tr.count = cap, 0
+    tr.err = errs
+    ctype = {np.dtype(np.int64): C.c_int64, np.dtype(np.int32): C.c_int32, np.dtype(np.float64): C.c_double}
+    for k, a in arr.items():
+        setattr(tr, k, _p(a, ctype[a.dtype]))
+    tr.x_avg, tr.y_avg = _p(xa, C.c_double), _p(ya, C.c_double)
+    x0p = _p(_f64(x0), C.c_double) if x0 is not None else None
+    y0p = _p(_f64(y0), C.c_double) if y0 is not None else None
+    lib().orc_solve_traced(m, n, _p(colptr, C.c_int64), _p(rowval, C.c_int32), _p(nzval, C.c_double),
+                           *[_p(v, C.c_double) for v in vs], x0p, y0p, C.byref(params),
+                           _p(xo, C.c_double), _p(yo, C.c_double), C.byref(st), hist, 16, C.byref(tr))
+    k = min(tr.count, cap)
+    names = ("sol", "avg", "last", "avg_last", "restart")
+    err = [{nm: errs[5 * i + q] for q, nm in enumerate(names)} for i in range(k)]
+    res = OracleResult(xo[:n], yo[:m], st, [])
+    return TraceResult(res, arr["passes"][:k], err, arr["from_avg"][:k].astype(bool),
+                       arr["restarted"][:k].astype(bool), arr["omega"][:k], arr["omega_after"][:k],
+                       arr["inner"][:k], arr["total"][:k], xa[:n], ya[:m])
