@@ -15,8 +15,9 @@
  * on those scalars is tolerance-based.)
  *
  * Pinning status of each part (what checks it) is listed in the oracle tests:
- * tests/test_oracle.py, and tests/test_infeasibility.py for the opt-in ray test (which
- * has no counterpart in the reference).  Parity unpinned: orc_powm_start (the reference draws
+ * tests/test_oracle.py, tests/test_oracle_restart.py (restart decision, averaging, primal
+ * weight update, the restart itself), and tests/test_infeasibility.py for the opt-in ray
+ * test (which has no counterpart in the reference).  Parity unpinned: orc_powm_start (the reference draws
  * StableRNG(0) randn, linalg.jl:92, not reproducible here), and the last bits of
  * the BLAS-backed reductions.
  */

@@ -15,6 +15,15 @@ Parity pinning of each oracle function (see tests/test_oracle_*.py):
   solve                 -- closed-form optima c'x0, scipy HiGHS, the reference's
                            known answers (205.0, test/moi.jl:44-52; OPTIMAL on
                            test/components/termination.jl:2-9)
+  restart_decision      -- hand-worked known answers at every boundary of should_restart
+                           and the candidate choice (tests/test_oracle_restart.py)
+  primal_weight_update  -- its power form (dy/dx)^t w^(1-t), t in {0, 0.3, 0.5, 1}, and
+                           the zero_tol guards; at the first restart of a solve against
+                           distances of independently computed iterates
+  averaging / restart   -- the recorded check errors against iterates computed one pass
+                           at a time with restarts disabled (mean of the iterates since
+                           the start, the iterate one step earlier, F10) and against a
+                           warm-started solve from the restart candidate
   powm_start            -- parity unpinned (the reference draws StableRNG(0) randn)
 """
 from __future__ import annotations
