@@ -26,6 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from ._lib import ArgumentError, DimensionMismatch
 from .algorithms import Engine, PDLP
 from .batch import BatchSolver, footprint
 from .problems import MILP
@@ -176,6 +177,13 @@ class Server:
                 bid = self.batch.add(r.lp, r.sol)
             except MemoryError:
                 return   # no room until a running request finishes
+            except (ArgumentError, DimensionMismatch) as e:   # never admissible: reported, not retried
+                self.waiting.popleft()
+                r.status = f"ERROR: {e}"
+                r.t_done = time.perf_counter()
+                with self.lock:
+                    self.done.append(r)
+                continue
             self.waiting.popleft()
             r.t_start = time.perf_counter()
             self.in_batch[bid] = r
@@ -319,15 +327,20 @@ class NativeServer:
         C, _lib = self._C, self._lib
         if i not in self.shapes:
             raise _lib.ArgumentError("unknown request id")
-        m, n = self.shapes.pop(i)
+        m, n = self.shapes[i]
         x, y = np.zeros(max(n, 1)), np.zeros(max(m, 1))
         s, secs, rt = _lib.CStats(), C.c_double(), C.c_int32()
-        try:
-            _lib.check(_lib.lib.cpdlp_server_get(self.h, int(i), x.ctypes.data_as(C.c_void_p),
-                                                 y.ctypes.data_as(C.c_void_p), C.byref(s), C.byref(secs),
-                                                 C.byref(rt)))
-        finally:
-            self._held.pop(i, None)   # handed over (or failed): the library is done with them
+        rc = _lib.lib.cpdlp_server_get(self.h, int(i), x.ctypes.data_as(C.c_void_p),
+                                       y.ctypes.data_as(C.c_void_p), C.byref(s), C.byref(secs), C.byref(rt))
+        running = (rc == _lib.ERR_STATE
+                   and _lib.lib.cpdlp_last_error().decode(errors="replace") == "request still running")
+        if not running:
+            # the request is finished (solved or failed) and freed by the library, which
+            # no longer reads a borrowed request's arrays; while it still runs (ERR_STATE)
+            # they stay held and the id stays known
+            self.shapes.pop(i, None)
+            self._held.pop(i, None)
+        _lib.check(rc)
         st = ConvergenceStats(
             err=KKTErrors.from_c(s.err), time_elapsed=s.time_elapsed, kkt_passes=s.kkt_passes,
             termination_status=TerminationStatus(s.termination_status), eta=s.eta, omega=s.omega,

@@ -1425,19 +1425,34 @@ struct Batch {
             if (n > 0 && !v) throw Error(CPDLP_ERR_ARGUMENT, "NULL variable vector");
         for (const void* v : {(const void*)lp.lc, (const void*)lp.uc})
             if (m > 0 && !v) throw Error(CPDLP_ERR_ARGUMENT, "NULL constraint vector");
-        if (free_slots.empty()) throw Error(CPDLP_ERR_MEMORY, "no free batch slot");
         const size_t bytes = footprint(m, n, nnz);
-        size_t off = 0;
-        if (!arena.alloc(bytes, off)) throw Error(CPDLP_ERR_MEMORY, "batch arena full");
+        if (bytes > arena.size)   // can never fit: not a transient shortage
+            throw Error(CPDLP_ERR_ARGUMENT, "LP needs " + std::to_string(bytes) + " bytes, more than the whole batch arena (" +
+                                                std::to_string(arena.size) + ")");
+        if (free_slots.empty()) throw Error(CPDLP_ERR_MEMORY, "no free batch slot");
+        // host-side preparation before any arena bytes are taken
         csc_to_csr(m, n, cp, rv, rp, col, perm);
-        BInst I = carve(arena_mem.p + off, m, n, nnz);
-        I.La = lanes_for(nnz, m);
-        I.Lat = lanes_for(nnz, n);
         std::vector<TileMeta> ta, tat;
         build_tiles(rp.data(), 0, (int32_t)m, ta);
         build_tiles(cp.data(), 0, (int32_t)n, tat);
         if ((int64_t)ta.size() > max_tiles(m, nnz) || (int64_t)tat.size() > max_tiles(n, nnz))
             throw Error(CPDLP_ERR_STATE, "internal: tile count above its bound");
+        std::vector<int32_t> ecol(nnz);
+        for (int64_t j = 0; j < n; ++j)
+            for (int32_t k = cp[j]; k < cp[j + 1]; ++k) ecol[k] = (int32_t)j;
+        size_t off = 0;
+        if (!arena.alloc(bytes, off)) throw Error(CPDLP_ERR_MEMORY, "batch arena full");
+        struct RegionGuard {   // returns the region to the arena unless the add completes
+            Arena& a;
+            size_t off, len;
+            bool armed = true;
+            ~RegionGuard() {
+                if (armed) a.release(off, len);
+            }
+        } region{arena, off, bytes};
+        BInst I = carve(arena_mem.p + off, m, n, nnz);
+        I.La = lanes_for(nnz, m);
+        I.Lat = lanes_for(nnz, n);
         I.n_a_tiles = (int32_t)ta.size();
         I.n_at_tiles = (int32_t)tat.size();
         const hipStream_t s = streams[0];
@@ -1449,9 +1464,6 @@ struct Batch {
         up(I.at_rp, cp.data(), (n + 1) * 4);
         up(I.at_col, rv.data(), nnz * 4);
         up(I.perm, perm.data(), nnz * 4);
-        std::vector<int32_t> ecol(nnz);
-        for (int64_t j = 0; j < n; ++j)
-            for (int32_t k = cp[j]; k < cp[j + 1]; ++k) ecol[k] = (int32_t)j;
         up(I.ecol, ecol.data(), nnz * 4);
         up((void*)I.a_tiles, ta.data(), ta.size() * sizeof(TileMeta));
         up((void*)I.at_tiles, tat.data(), tat.size() * sizeof(TileMeta));
@@ -1469,9 +1481,7 @@ struct Batch {
             up(I.xr64, x0 ? x0 : zeros.data(), n * 8);
             up(I.yr64, y0 ? y0 : zeros.data(), m * 8);
         }
-        const int32_t slot = free_slots.back();
-        free_slots.pop_back();
-        hi_slot = std::max(hi_slot, slot + 1);
+        const int32_t slot = free_slots.back();   // taken below, once the copies have completed
         h_inst[slot] = I;
         BState st{};
         st.status = CPDLP_STILL_RUNNING;
@@ -1482,6 +1492,9 @@ struct Batch {
         HIP_CHECK(hipMemcpyAsync(d_state.p + slot, &h_state[slot], sizeof(BState), hipMemcpyHostToDevice, s));
         HIP_CHECK(hipMemcpyAsync(d_timeout.p + slot, &h_timeout[slot], 4, hipMemcpyHostToDevice, s));
         HIP_CHECK(hipStreamSynchronize(s));   // host vectors above go out of scope
+        free_slots.pop_back();
+        hi_slot = std::max(hi_slot, slot + 1);
+        region.armed = false;
         BHost h;
         h.id = next_id++;
         h.slot = slot;

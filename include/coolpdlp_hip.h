@@ -232,7 +232,9 @@ typedef struct cpdlp_batch cpdlp_batch;
 int cpdlp_batch_create(const cpdlp_params* params, int64_t hbm_bytes, cpdlp_batch** out);
 
 /* Admit one LP (host arrays, copied); *id identifies it.  Returns CPDLP_ERR_MEMORY
- * (and admits nothing) when the arena lacks room: the caller retries after a release. */
+ * (and admits nothing) when the arena lacks room: the caller retries after a release.
+ * Returns CPDLP_ERR_ARGUMENT when the LP's footprint exceeds the whole arena (it can
+ * never be admitted).  A failed add leaves the arena as it was. */
 int cpdlp_batch_add(cpdlp_batch* b, const cpdlp_lp* lp, int64_t* id);
 
 /* cpdlp_batch_add with a start point: x0 (n) / y0 (m) are the unscaled start, as the
@@ -338,7 +340,9 @@ int cpdlp_server_start(cpdlp_server* s);
 int cpdlp_server_poll(cpdlp_server* s, int64_t* ids, int64_t cap, int64_t* count);
 /* A finished request's unscaled x (n), y (m) and stats; *seconds = submit to finish;
  * *route = -1 (batch) or the plain lane.  Frees the request.  Returns the request's own
- * error code when its solve failed (the message in cpdlp_last_error). */
+ * error code when its solve failed (the message in cpdlp_last_error).  A request that
+ * has not finished gives CPDLP_ERR_STATE with the message "request still running" and
+ * stays queued (a borrowed request's arrays are still read). */
 int cpdlp_server_get(cpdlp_server* s, int64_t id, double* x, double* y, cpdlp_stats* st,
                      double* seconds, int32_t* route);
 /* Waits for running plain solves; requests not yet admitted are dropped. */

@@ -208,3 +208,73 @@ def test_wide_workgroups_bit_identical(cp, monkeypatch):
         assert t1.termination_status == t2.termination_status == cp.OPTIMAL
         assert t1.kkt_passes == t2.kkt_passes and t1.fp32_passes == t2.fp32_passes
         assert np.array_equal(s1.x, s2.x) and np.array_equal(s1.y, s2.y)
+
+
+def _upper_range():
+    """c3's upper range: m up to 5,000, n = m x U(1.5, 3), 6-12 nonzeros per column."""
+    shapes = [(5000, 14000, 91), (5000, 9000, 92), (3100, 7500, 93), (2400, 4000, 94), (700, 1500, 95),
+              (60, 150, 96)]
+    return [G.column_lp(m, n, (6, 12), sd, name=f"u{m}_{n}") for m, n, sd in shapes]
+
+
+def _solve_batch(cp, lps, precision, env=None):
+    import os
+    old = {k: os.environ.get(k) for k in (env or {})}
+    try:
+        os.environ.update(env or {})
+        with cp.BatchSolver(cp.PDLP(precision, termination_reltol=1e-6), hbm_bytes=512 << 20) as B:
+            return B.solve_all(lps, checks_per_round=10)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("precision", ["float64", "mixed"])
+def test_batch_upper_range_gathers_from_global_memory(cp, precision):
+    """Instances up to m = 5,000 (the largest FP64 ones exceed the LDS budget) solved
+    with the default LDS classes and with every instance gathering from global memory
+    (CPDLP_BATCH_LDS_BYTES=0: the narrow and the wide k_batch_steps<V, false, ...>).  The
+    two runs give the same bits; the results match the closed form c'x0 and the oracle's
+    FP64 solve."""
+    lps = _upper_range()
+    res_lds = _solve_batch(cp, lps, precision)
+    res_glb = _solve_batch(cp, lps, precision, {"CPDLP_BATCH_LDS_BYTES": "0"})
+    for g, (sa, ta), (sb, tb) in zip(lps, res_lds, res_glb):
+        assert ta.termination_status == tb.termination_status == cp.OPTIMAL, (g.name, ta, tb)
+        assert ta.kkt_passes == tb.kkt_passes, g.name
+        assert np.array_equal(sa.x, sb.x) and np.array_equal(sa.y, sb.y), g.name
+        obj = float(sb.x @ g.c)
+        assert abs(obj - g.objective) <= 20e-6 * (1 + abs(g.objective)), (g.name, obj, g.objective)
+    for g, (sol, st) in list(zip(lps, res_glb))[:2]:
+        ref = O.solve(g, O.default_params(termination_reltol=1e-6))
+        assert ref.status == "OPTIMAL"
+        assert abs(float(sol.x @ g.c) - float(ref.x @ g.c)) <= 20e-6 * (1 + abs(g.objective)), g.name
+
+
+def test_batch_add_beyond_the_whole_arena(cp):
+    """An LP larger than the whole arena is refused with CPDLP_ERR_ARGUMENT (it can never
+    fit); one that does not fit now gives MemoryError; neither changes the arena."""
+    from coolpdlp_hip.batch import footprint
+    small = G.column_lp(60, 150, 6, seed=41)
+    big = G.column_lp(1500, 3500, 8, seed=42)
+    fp = footprint(small.m, small.n, small.nnz)
+    with cp.BatchSolver(cp.PDLP("float64", termination_reltol=1e-6), hbm_bytes=2 * fp) as B:
+        with pytest.raises(cp.ArgumentError, match="whole batch arena"):
+            B.add(big)
+        assert B.memory()[0] == 0
+        i1, i2 = B.add(small), B.add(small)
+        used = B.memory()[0]
+        with pytest.raises(MemoryError):
+            B.add(small)
+        assert B.memory()[0] == used
+        while B.run(10):
+            pass
+        for i in (i1, i2):
+            sol, st = B.get(i)
+            assert st.termination_status == cp.OPTIMAL
+            assert abs(float(sol.x @ small.c) - small.objective) <= 20e-6 * (1 + abs(small.objective))
+            B.release(i)
+        assert B.memory()[0] == 0

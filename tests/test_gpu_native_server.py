@@ -105,3 +105,57 @@ def test_native_server_reports_request_errors(cp):
         with pytest.raises(cp.DimensionMismatch):
             srv.submit(g, cp.PrimalDualSolution(np.zeros(g.n + 1), np.zeros(g.m)))
         assert i not in srv.shapes
+
+
+def test_get_on_a_running_borrowed_request_keeps_its_arrays(cp):
+    """cpdlp_server_get on a request that has not finished fails with "request still
+    running" and leaves the request queued: the borrowed arrays stay held (the library
+    still reads them at admission) and the id stays valid for a later get."""
+    import gc
+    from coolpdlp_hip.serving import NativeServer
+    g = G.request_lp(3000, 6000, 81)          # plain route (m + n > batch_limit)
+    algo = cp.PDLP("mixed", termination_reltol=1e-6)
+    with NativeServer(algo, batch_bytes=64 << 20, plain_bytes=4 << 30, batch_limit=2000) as srv:
+        i = srv.submit(G.request_lp(3000, 6000, 81), borrow=True)   # no other reference to the LP
+        with pytest.raises(Exception, match="request still running"):
+            srv.get(i)                      # not even admitted: nothing pumped yet
+        assert i in srv._held and i in srv.shapes
+        gc.collect()
+        res = _drain(srv, 1)
+    sol, st, _, route = res[i]
+    assert route >= 0 and st.termination_status == cp.OPTIMAL
+    assert abs(float(sol.x @ g.c) - g.objective) <= 2e-5 * (1 + abs(g.objective))
+    assert not srv._held and not srv.shapes
+
+
+def test_request_larger_than_the_batch_arena_fails_alone(cp):
+    """A batch-route request whose footprint exceeds the whole arena can never be
+    admitted: it is reported as failed (CPDLP_ERR_ARGUMENT) instead of blocking the
+    requests behind it, which complete."""
+    from coolpdlp_hip.batch import footprint
+    from coolpdlp_hip.serving import NativeServer
+    big = G.request_lp(1800, 3600, 82)
+    small = [G.request_lp(m, 2 * m, 83 + m) for m in (60, 120)]
+    arena = footprint(small[1].m, small[1].n, small[1].nnz) * 3
+    assert footprint(big.m, big.n, big.nnz) > arena
+    algo = cp.PDLP("mixed", termination_reltol=1e-6)
+    with NativeServer(algo, batch_bytes=arena, plain_bytes=1 << 30, batch_limit=8000,
+                      checks_per_round=5) as srv:
+        ib = srv.submit(big)
+        ids = [srv.submit(g) for g in small]
+        done, err = {}, None
+        t_end = time.perf_counter() + 120
+        while len(done) + (err is not None) < 3 and time.perf_counter() < t_end:
+            srv.pump()
+            for i in srv.poll():
+                try:
+                    done[i] = srv.get(i)
+                except cp.ArgumentError as e:
+                    assert i == ib
+                    err = e
+            time.sleep(0.0005)
+    assert err is not None and "arena" in str(err)
+    for i, g in zip(ids, small):
+        sol, st, _, route = done[i]
+        assert route == -1 and st.termination_status == cp.OPTIMAL
+        assert abs(float(sol.x @ g.c) - g.objective) <= 2e-5 * (1 + abs(g.objective))

@@ -166,6 +166,32 @@ def test_solve_matches_oracle(cp, algo, use_graph, fmt):
     assert len(st.error_history) == ref.stats.n_history
 
 
+@pytest.mark.parametrize("fmt", ["csr", "auto"])
+def test_small_dense_lp_at_its_stated_size(cp, fmt):
+    """The reference's CPU-runnable case at its stated size: m = 2,000 (1,000 =, 1,000 <=),
+    n = 5,000, 8 nonzeros per column, 0 <= x, FP64 PDLP to relative KKT 1e-4 with the
+    reference's power-method rule, against the oracle on the same seeded LP: same status,
+    step size, pass count within two checks, early checks to reduction-order rounding,
+    objective against the oracle and the closed form c'x0."""
+    g = G.small_dense_lp()
+    assert (g.m, g.n, g.nnz) == (2_000, 5_000, 40_000)
+    milp = cp.MILP.from_generated(g)
+    sol, st = cp.solve(milp, cp.PDLP("float64", None, fmt, termination_reltol=1e-4))
+    ref = O.solve(g, O.default_params(termination_reltol=1e-4))
+    assert st.termination_status.name == ref.status == "OPTIMAL"
+    assert abs(st.eta - ref.stats.eta) <= 1e-9 * ref.stats.eta
+    assert abs(st.powm_iters - ref.stats.powm_iters) <= 2
+    assert abs(st.kkt_passes - ref.stats.kkt_passes) <= 200
+    assert st.err.relative() <= 1e-4
+    for (p1, e1), (p2, e2) in list(zip(st.error_history, ref.history))[:3]:
+        assert p1 == p2
+        for f in ("primal", "dual", "gap"):
+            assert abs(getattr(e1, f) - e2[f]) <= 1e-8 * (1 + abs(e2[f])), (p1, f)
+    obj, obj_ref = cp.objective_value(sol.x, milp), float(ref.x @ g.c)
+    assert abs(obj - obj_ref) <= 1e-3 * (1 + abs(obj_ref))
+    assert abs(obj - g.objective) <= 1e-3 * (1 + abs(g.objective))
+
+
 def test_trajectory_matches_oracle_exactly_early(cp):
     """Short rows + FP64: the first checks' KKT errors agree to reduction-order
     rounding (sums differ only in order)."""
@@ -263,6 +289,26 @@ def test_min_cost_flow_mixed_vs_highs(cp, powm_tol):
     obj = float(sol.x @ g.c)
     assert abs(obj - h.fun) <= 1e-5 * (1 + abs(h.fun)), (obj, h.fun)
     assert np.max(np.abs(A @ sol.x - g.lc)) <= 1e-5 * (1 + np.max(np.abs(g.lc)))
+
+
+def test_min_cost_flow_mixed_vs_oracle_fp64_at_1e8(cp):
+    """The min-cost-flow workload's stated check at reduced size (5,000 nodes, 40,000
+    arcs): the mixed-precision solve at relative KKT 1e-6 against the oracle's FP64 solve
+    of the same LP at 1e-8.  Stated errors: objective within 1e-5 relative of the oracle's,
+    flow conservation |Ax - b| within 1e-5 of (1 + max |b|), bounds within 1e-9."""
+    g = G.min_cost_flow_lp(nodes=5000, out_degree=8, seed=6)
+    A = g.csc()
+    ref = O.solve(g, O.default_params(termination_reltol=1e-8, powm_tol=1e-6, max_kkt_passes=2_000_000,
+                                      time_limit=600))
+    assert ref.status == "OPTIMAL"
+    milp = cp.MILP.from_generated(g)
+    sol, st = cp.solve(milp, cp.PDLP("mixed", termination_reltol=1e-6, powm_tol=1e-6,
+                                     max_kkt_passes=2_000_000))
+    assert st.termination_status == cp.OPTIMAL and st.fp32_passes > 0
+    obj, obj_ref = float(sol.x @ g.c), float(ref.x @ g.c)
+    assert abs(obj - obj_ref) <= 1e-5 * (1 + abs(obj_ref)), (obj, obj_ref)
+    assert np.max(np.abs(A @ sol.x - g.lc)) <= 1e-5 * (1 + np.max(np.abs(g.lc)))
+    assert np.all(sol.x >= g.lv - 1e-9) and np.all(sol.x <= g.uv + 1e-9)
 
 
 @pytest.mark.parametrize("fmt,code", [("xslice", 3), ("xslice_all", 4)])
