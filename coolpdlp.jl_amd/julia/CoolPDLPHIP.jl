@@ -3,25 +3,38 @@
 
 Julia binding of `libcoolpdlp_hip.so` (the MI355X PDLP/PDHG engine) for CoolPDLP.jl.
 
-It plugs into the two extension points CoolPDLP already has, so `solve`,
-`MOI.optimize!` and every option name stay as they are:
+It plugs into the extension points CoolPDLP already has, so `solve`, `MOI.optimize!`
+and every option name stay as they are:
 
-1. the matrix plug-in (`src/utils/mat_csr.jl:8-92`): `HIPSparseMatrixCSR(A)` builds the
-   matrix in HBM and implements the 5-argument `mul!` on device vectors;
-2. algorithm dispatch (`src/algorithms/common.jl:176,216`): `solve(milp, sol, algo)` with
+1. the backend type parameter `B <: Backend` of `Algorithm{A,T,Ti,M,B}`
+   (`src/algorithms/common.jl:8-14`, `src/components/conversion.jl:14-15`):
+   `HIPEngine <: KernelAbstractions.GPU`.  `solve(milp, sol, algo)` with
    `backend = HIPEngine()` runs the whole PDLP/PDHG loop (scaling, step size, restarts,
-   termination) on the GPU through `cpdlp_solve` and returns the reference's
-   `(PrimalDualSolution, ConvergenceStats)`.
+   termination) in the library through `cpdlp_solve` and returns the reference's
+   `(PrimalDualSolution, ConvergenceStats)`.  The float type `T` selects the engine's
+   precision (`Float64`, `Float32`; `HIPEngine(mixed = true)` with `Float64` for FP32 bulk
+   with FP64 checks and polish), the matrix type `M` its sparse format
+   (`GPUSparseMatrixCSR` or `HIPSparseMatrixCSR` -> CSR, `GPUSparseMatrixELL` -> ELL,
+   anything else -> autotuned), the index width of the input's `colptr`/`rowval` the
+   C ABI's `index_bits`.  Through MOI: `JuMP.set_attribute(model, "backend", HIPEngine())`.
+2. the matrix plug-in (`src/utils/mat_csr.jl:8-92`): `HIPSparseMatrixCSR(A)` builds the
+   matrix in HBM and implements `get_backend`, `Adapt.adapt_structure` and the
+   5-argument `mul!` on AMDGPU device vectors, so the reference's own loop can run on
+   AMDGPU's `ROCBackend()` with the library's SpMV.
 
 This file is not loaded by anything in this repository (there is no Julia in the
 image); it is the binding a maintainer would add, written against
-`include/coolpdlp_hip.h`.
+`include/coolpdlp_hip.h` (tests/test_julia_binding.py checks its structs and ccall
+signatures against the header).
 """
 module CoolPDLPHIP
 
 using CoolPDLP
 using CoolPDLP: Algorithm, ConvergenceStats, KKTErrors, MILP, PrimalDualSolution,
-    OPTIMAL, TIME_LIMIT, ITERATION_LIMIT, STILL_RUNNING
+    GPUSparseMatrixCSR, GPUSparseMatrixELL, OPTIMAL, TIME_LIMIT, ITERATION_LIMIT, STILL_RUNNING
+using Adapt: Adapt
+using AMDGPU: AMDGPU, ROCBackend, ROCVector
+using KernelAbstractions: KernelAbstractions
 using LinearAlgebra
 using SparseArrays
 
@@ -113,54 +126,98 @@ function check(code::Cint)
 end
 
 # ---- 1. matrix plug-in ----------------------------------------------------------
+# the C ABI's index width of a CSC matrix's colptr / rowval (int32 or int64)
+function index_bits(A::SparseMatrixCSC)
+    Ti = eltype(A.colptr)
+    eltype(A.rowval) === Ti || throw(ArgumentError("colptr and rowval must have the same integer type"))
+    Ti in (Int32, Int64) || throw(ArgumentError("index type $Ti: the library reads Int32 or Int64 indices"))
+    return Int32(8 * sizeof(Ti))
+end
+
 """
-    HIPSparseMatrixCSR(A::SparseMatrixCSC{Float64})
+    HIPSparseMatrixCSR(A::SparseMatrixCSC{Float64, <:Union{Int32, Int64}}; format = 1, device = 0)
 
 `GPUSparseMatrixCSR` equivalent stored in HBM by libcoolpdlp_hip
 (`cpdlp_matrix_create`, replaces `GPUSparseMatrixCSR(A)`, `src/utils/mat_csr.jl:58-61`).
+`set_matrix_type` (`src/problems/modify.jl:80-105`) builds one for `A` and one for `At`.
+Values are Float64 (the library's matrix plug-in computes in FP64).
 """
 mutable struct HIPSparseMatrixCSR <: AbstractSparseMatrix{Float64, Int32}
     handle::Ptr{Cvoid}
     m::Int
     n::Int
     nnz::Int
+    device::Int
     function HIPSparseMatrixCSR(A::SparseMatrixCSC{Float64}; format::Integer = 1, device::Integer = 0)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:cpdlp_matrix_create, libcpdlp), Cint,
             (Int64, Int64, Int32, Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
-            size(A, 1), size(A, 2), 1, 64, A.colptr, A.rowval, A.nzval, format, 0, device, h))
-        M = new(h[], size(A, 1), size(A, 2), nnz(A))
+            size(A, 1), size(A, 2), 1, index_bits(A), A.colptr, A.rowval, A.nzval, format, 0, device, h))
+        M = new(h[], size(A, 1), size(A, 2), nnz(A), device)
         finalizer(x -> ccall((:cpdlp_matrix_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), x.handle), M)
         return M
     end
 end
 
+HIPSparseMatrixCSR(A::SparseMatrixCSC{T}; kwargs...) where {T} =
+    throw(ArgumentError("HIPSparseMatrixCSR stores Float64 values (got $T); for Float32 use backend = HIPEngine()"))
+
 Base.size(A::HIPSparseMatrixCSR) = (A.m, A.n)
 SparseArrays.nnz(A::HIPSparseMatrixCSR) = A.nnz
 
-# mul!(c, A, b, α, β), src/utils/mat_csr.jl:81-92 -> cpdlp_matrix_mul on device pointers
-# (here: any array type whose pointer is a HIP device pointer, e.g. an AMDGPU.ROCArray)
-function LinearAlgebra.mul!(c::AbstractVector{Float64}, A::HIPSparseMatrixCSR,
-        b::AbstractVector{Float64}, α::Number, β::Number)
-    check(ccall((:cpdlp_matrix_mul, libcpdlp), Cint,
-        (Ptr{Cvoid}, Float64, Ptr{Float64}, Float64, Ptr{Float64}, Ptr{Cvoid}),
-        A.handle, Float64(α), pointer(b), Float64(β), pointer(c), C_NULL))
+# src/utils/mat_csr.jl:44-46: the backend of the vectors it multiplies (AMDGPU's arrays)
+KernelAbstractions.get_backend(::HIPSparseMatrixCSR) = ROCBackend()
+
+# src/utils/mat_csr.jl:48-56: the matrix already lives in HBM, owned by the library
+Adapt.adapt_structure(to, A::HIPSparseMatrixCSR) = A
+
+# mul!(c, A, b, α, β), src/utils/mat_csr.jl:81-92 -> cpdlp_matrix_mul on the device
+# pointers of AMDGPU vectors, on AMDGPU's current stream (ordered with its broadcasts)
+function LinearAlgebra.mul!(c::ROCVector{Float64}, A::HIPSparseMatrixCSR,
+        b::ROCVector{Float64}, α::Number, β::Number)
+    length(b) == A.n && length(c) == A.m || throw(DimensionMismatch("mul!: sizes of A, b and c"))
+    GC.@preserve b c begin
+        check(ccall((:cpdlp_matrix_mul, libcpdlp), Cint,
+            (Ptr{Cvoid}, Float64, Ptr{Float64}, Float64, Ptr{Float64}, Ptr{Cvoid}),
+            A.handle, Float64(α), reinterpret(Ptr{Float64}, pointer(b)), Float64(β),
+            reinterpret(Ptr{Float64}, pointer(c)), AMDGPU.stream().stream))
+    end
     return c
 end
 
 # ---- 2. the whole loop ------------------------------------------------------------
 """
-    HIPEngine(; device = 0, precision = :fp64, format = :auto)
+    HIPEngine(; device = 0, mixed = false)
 
-Backend selector: `PDLP(...; backend = HIPEngine())` makes `solve` run on libcoolpdlp_hip.
+Backend of `Algorithm{A,T,Ti,M,B}` (`B <: Backend`, `src/algorithms/common.jl:8-14`):
+`PDLP(T, Ti, M; backend = HIPEngine())` makes `solve` run on libcoolpdlp_hip.  The
+precision follows `T`; `mixed = true` (with `T = Float64`) runs FP32 matrix, iterates and
+products with FP64 checks and an FP64 polish.  The whole solve runs in the library, so
+nothing is allocated on this backend through KernelAbstractions.
 """
-Base.@kwdef struct HIPEngine
+Base.@kwdef struct HIPEngine <: KernelAbstractions.GPU
     device::Int = 0
-    precision::Symbol = :fp64
-    format::Symbol = :auto
+    mixed::Bool = false
 end
 
-function cparams(algo::Algorithm{A}, eng::HIPEngine) where {A}
+KernelAbstractions.allocate(::HIPEngine, ::Type, dims...) = throw(ArgumentError(
+    "HIPEngine runs the whole solve in libcoolpdlp_hip; for device arrays use AMDGPU's ROCBackend()"))
+
+# T of Algorithm{A,T,...} -> cpdlp_params.precision (common.jl:55-57)
+precision_code(::Type{Float64}, eng::HIPEngine) = Int32(eng.mixed ? 2 : 0)
+precision_code(::Type{Float32}, eng::HIPEngine) =
+    eng.mixed ? throw(ArgumentError("mixed precision runs with T = Float64")) : Int32(1)
+precision_code(::Type{T}, ::HIPEngine) where {T} =
+    throw(ArgumentError("float type $T: the library computes in Float64 or Float32"))
+
+# M of Algorithm{A,T,Ti,M,...} -> cpdlp_params.format
+format_code(::Type{<:GPUSparseMatrixELL}) = Int32(2)
+format_code(::Type{<:GPUSparseMatrixCSR}) = Int32(1)
+format_code(::Type{HIPSparseMatrixCSR}) = Int32(1)
+format_code(::Type) = Int32(0)   # SparseMatrixCSC and others: autotuned
+
+function cparams(algo::Algorithm{A, T, Ti, M, HIPEngine}) where {A, T, Ti, M}
+    eng = algo.conversion.backend
     p = CParams()
     check(ccall((:cpdlp_default_params, libcpdlp), Cint, (Ref{CParams}, Int32), p, A === :PDLP ? 0 : 1))
     (; preconditioning, step_size, restart, generic, termination) = algo
@@ -178,27 +235,64 @@ function cparams(algo::Algorithm{A}, eng::HIPEngine) where {A}
     p.termination_reltol = termination.termination_reltol
     p.max_kkt_passes = termination.max_kkt_passes
     p.time_limit = termination.time_limit
-    p.precision = Dict(:fp64 => 0, :fp32 => 1, :mixed => 2)[eng.precision]
-    p.format = Dict(:auto => 0, :csr => 1, :ell => 2, :xslice => 3, :xslice_all => 4)[eng.format]
+    p.precision = precision_code(T, eng)
+    p.format = format_code(M)
     p.device = eng.device
     return p
 end
 
 const STATUS = (OPTIMAL, TIME_LIMIT, ITERATION_LIMIT, STILL_RUNNING)
 
+"""
+    HostLP(milp)
+
+The C ABI's view of an unscaled CPU `MILP` (as `solve` receives it, `common.jl:186-190`):
+CSC arrays with their own index width and Float64 copies of the values where needed.
+Keep it alive (`GC.@preserve`) while the library reads it.
+"""
+struct HostLP
+    A::SparseMatrixCSC
+    nz::Vector{Float64}
+    vecs::NTuple{5, Vector{Float64}}
+    lp::CLP
+end
+
+function HostLP(milp::MILP)
+    A_ = milp.A
+    A_ isa SparseMatrixCSC || throw(ArgumentError("solve with HIPEngine takes a CPU MILP (SparseMatrixCSC)"))
+    (all(isone, milp.D1.diag) && all(isone, milp.D2.diag)) ||
+        throw(ArgumentError("solve with HIPEngine takes an unpreconditioned MILP (D1 = D2 = I)"))
+    nz = Vector{Float64}(A_.nzval)
+    vecs = map(v -> Vector{Float64}(v), (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))
+    lp = CLP(size(A_, 1), size(A_, 2), 1, index_bits(A_), pointer(A_.colptr), pointer(A_.rowval),
+        pointer(nz), pointer(vecs[1]), pointer(vecs[2]), pointer(vecs[3]), pointer(vecs[4]), pointer(vecs[5]))
+    return HostLP(A_, nz, vecs, lp)
+end
+
+kkt_errors(::Type{T}, e::CKKT) where {T} = KKTErrors{T}(; primal = T(e.primal), primal_scale = T(e.primal_scale),
+    dual = T(e.dual), dual_scale = T(e.dual_scale), gap = T(e.gap), gap_scale = T(e.gap_scale))
+
+status_of(code::Integer) = code in 0:3 ? STATUS[code + 1] :
+    error("libcoolpdlp_hip: termination status $code has no TerminationStatus value")
+
+function convergence_stats(::Type{T}, s::CStats, history) where {T}
+    return ConvergenceStats(T; err = kkt_errors(T, s.err), time_elapsed = s.time_elapsed,
+        kkt_passes = s.kkt_passes, termination_status = status_of(s.termination_status),
+        error_history = history)
+end
+
 # solve(milp, sol, algo), src/algorithms/common.jl:186-201, for backend = HIPEngine()
 function CoolPDLP.solve(milp::MILP, sol::PrimalDualSolution, algo::Algorithm{A, T, Ti, M, HIPEngine}) where {A, T, Ti, M}
-    A_ = milp.A::SparseMatrixCSC
-    c, lv, uv, lc, uc = (Vector{Float64}(v) for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))
-    lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
-        pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
-    p = cparams(algo, algo.conversion.backend)
+    H = HostLP(milp)
+    length(sol.x) == H.lp.n && length(sol.y) == H.lp.m ||
+        throw(DimensionMismatch("starting point does not match the LP's size"))
+    p = cparams(algo)
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve A_ c lv uv lc uc begin
-        check(ccall((:cpdlp_create, libcpdlp), Cint, (Ref{CLP}, Ref{CParams}, Ptr{Ptr{Cvoid}}), lp, p, h))
+    GC.@preserve H begin
+        check(ccall((:cpdlp_create, libcpdlp), Cint, (Ref{CLP}, Ref{CParams}, Ptr{Ptr{Cvoid}}), H.lp, p, h))
     end
     try
-        x0, y0 = Vector{Float64}(sol.x), Vector{Float64}(sol.y)
+        x0, y0 = Vector{Float64}(Array(sol.x)), Vector{Float64}(Array(sol.y))
         check(ccall((:cpdlp_solve, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), h[], x0, y0))
         x, y = similar(x0), similar(y0)
         check(ccall((:cpdlp_get_solution, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), h[], x, y))
@@ -211,12 +305,8 @@ function CoolPDLP.solve(milp::MILP, sol::PrimalDualSolution, algo::Algorithm{A, 
         errs = Vector{CKKT}(undef, count[])
         check(ccall((:cpdlp_get_history, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ptr{CKKT}, Int64, Ref{Int64}),
             h[], passes, errs, count[], count))
-        tokkt(e) = KKTErrors(; primal = e.primal, primal_scale = e.primal_scale, dual = e.dual,
-            dual_scale = e.dual_scale, gap = e.gap, gap_scale = e.gap_scale)
-        stats = ConvergenceStats(Float64; err = tokkt(s[].err), time_elapsed = s[].time_elapsed,
-            kkt_passes = s[].kkt_passes, termination_status = STATUS[s[].termination_status + 1],
-            error_history = [(passes[i], tokkt(errs[i])) for i in eachindex(passes)])
-        return PrimalDualSolution(x, y), stats
+        history = Tuple{Int, KKTErrors{T}}[(Int(passes[i]), kkt_errors(T, errs[i])) for i in eachindex(passes)]
+        return PrimalDualSolution(Vector{T}(x), Vector{T}(y)), convergence_stats(T, s[], history)
     finally
         ccall((:cpdlp_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), h[])
     end
@@ -228,27 +318,27 @@ end
 
 Handle of a solve run chunk by chunk: `initialize` (common.jl:176) -> `cpdlp_initialize`,
 `solve!` (pdlp.jl:69-94) -> `cpdlp_run_checks` one chunk of `check_every` steps at a time,
-yielding in between as the reference does (pdlp.jl:77).  FP64 / FP32 only.
+yielding in between as the reference does (pdlp.jl:77).  T = Float64 or Float32 (the
+mixed precision's two phases run through `solve`).
 """
-mutable struct HIPState
+mutable struct HIPState{T}
     handle::Ptr{Cvoid}
     m::Int
     n::Int
 end
 
 function hip_initialize(milp::MILP, sol::PrimalDualSolution, algo::Algorithm{A, T, Ti, M, HIPEngine}) where {A, T, Ti, M}
-    A_ = milp.A::SparseMatrixCSC
-    c, lv, uv, lc, uc = (Vector{Float64}(v) for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))
-    lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
-        pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
-    p = cparams(algo, algo.conversion.backend)
+    H = HostLP(milp)
+    p = cparams(algo)
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve A_ c lv uv lc uc begin
-        check(ccall((:cpdlp_create, libcpdlp), Cint, (Ref{CLP}, Ref{CParams}, Ptr{Ptr{Cvoid}}), lp, p, h))
+    GC.@preserve H begin
+        check(ccall((:cpdlp_create, libcpdlp), Cint, (Ref{CLP}, Ref{CParams}, Ptr{Ptr{Cvoid}}), H.lp, p, h))
     end
-    state = HIPState(h[], size(A_, 1), size(A_, 2))
+    state = HIPState{T}(h[], H.lp.m, H.lp.n)
     finalizer(s -> ccall((:cpdlp_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), s.handle), state)
-    x0, y0 = Vector{Float64}(sol.x), Vector{Float64}(sol.y)
+    x0, y0 = Vector{Float64}(Array(sol.x)), Vector{Float64}(Array(sol.y))
+    length(x0) == state.n && length(y0) == state.m ||
+        throw(DimensionMismatch("starting point does not match the LP's size"))
     check(ccall((:cpdlp_initialize, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
         state.handle, x0, y0))
     return state
@@ -261,7 +351,17 @@ function hip_solve!(state::HIPState)
             state.handle, 1, status))
         yield()
     end
-    return STATUS[status[] + 1]
+    return status_of(status[])
+end
+
+"The solution and stats of a chunked solve, as `solve` returns them."
+function hip_result(state::HIPState{T}) where {T}
+    x, y = zeros(state.n), zeros(state.m)
+    check(ccall((:cpdlp_get_solution, libcpdlp), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+        state.handle, x, y))
+    s = Ref{CStats}()
+    check(ccall((:cpdlp_get_stats, libcpdlp), Cint, (Ptr{Cvoid}, Ref{CStats}), state.handle, s))
+    return PrimalDualSolution(Vector{T}(x), Vector{T}(y)), convergence_stats(T, s[], Tuple{Int, KKTErrors{T}}[])
 end
 
 # ---- 4. many LPs at once (cpdlp_batch_*; no counterpart in the reference) ----------
@@ -277,31 +377,25 @@ function CoolPDLP.solve(milps::Vector{<:MILP}, algo::Algorithm{A, T, Ti, M, HIPE
         hbm_bytes::Integer = 0, checks_per_round::Integer = 20, sols = nothing) where {A, T, Ti, M}
     starts = sols === nothing ? fill(nothing, length(milps)) : sols
     length(starts) == length(milps) || throw(DimensionMismatch("one start point (or nothing) per LP"))
-    p = cparams(algo, algo.conversion.backend)
-    lps = map(milps) do milp
-        A_ = milp.A::SparseMatrixCSC
-        (A_, (Vector{Float64}(v) for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))...)
-    end
+    p = cparams(algo)
+    lps = map(HostLP, milps)
     if hbm_bytes == 0
-        hbm_bytes = sum(lps) do (A_, _...)
-            ccall((:cpdlp_batch_footprint, libcpdlp), Int64, (Int64, Int64, Int64),
-                size(A_, 1), size(A_, 2), nnz(A_))
+        hbm_bytes = sum(lps) do H
+            ccall((:cpdlp_batch_footprint, libcpdlp), Int64, (Int64, Int64, Int64), H.lp.m, H.lp.n, nnz(H.A))
         end + (64 << 20)
     end
     b = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:cpdlp_batch_create, libcpdlp), Cint, (Ref{CParams}, Int64, Ptr{Ptr{Cvoid}}), p, hbm_bytes, b))
     try
-        ids = map(zip(lps, starts)) do ((A_, c, lv, uv, lc, uc), sol)
-            lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
-                pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
+        ids = map(zip(lps, starts)) do (H, sol)
             id = Ref{Int64}(0)
             x0 = sol === nothing ? Float64[] : Vector{Float64}(Array(sol.x))
             y0 = sol === nothing ? Float64[] : Vector{Float64}(Array(sol.y))
-            sol === nothing || (length(x0) == size(A_, 2) && length(y0) == size(A_, 1)) ||
+            sol === nothing || (length(x0) == H.lp.n && length(y0) == H.lp.m) ||
                 throw(DimensionMismatch("start point does not match the LP's size"))
-            GC.@preserve A_ c lv uv lc uc x0 y0 begin
+            GC.@preserve H x0 y0 begin
                 check(ccall((:cpdlp_batch_add_start, libcpdlp), Cint,
-                    (Ptr{Cvoid}, Ref{CLP}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}), b[], lp,
+                    (Ptr{Cvoid}, Ref{CLP}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}), b[], H.lp,
                     sol === nothing ? C_NULL : pointer(x0), sol === nothing ? C_NULL : pointer(y0), id))
             end
             id[]
@@ -312,18 +406,13 @@ function CoolPDLP.solve(milps::Vector{<:MILP}, algo::Algorithm{A, T, Ti, M, HIPE
                 b[], checks_per_round, running))
             yield()
         end
-        return map(zip(ids, lps)) do (id, (A_, _...))
-            x, y = zeros(size(A_, 2)), zeros(size(A_, 1))
+        return map(zip(ids, lps)) do (id, H)
+            x, y = zeros(H.lp.n), zeros(H.lp.m)
             s = Ref{CStats}()
             check(ccall((:cpdlp_batch_get, libcpdlp), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ref{CStats}),
                 b[], id, x, y, s))
             check(ccall((:cpdlp_batch_release, libcpdlp), Cint, (Ptr{Cvoid}, Int64), b[], id))
-            e = s[].err
-            err = KKTErrors(; primal = e.primal, primal_scale = e.primal_scale, dual = e.dual,
-                dual_scale = e.dual_scale, gap = e.gap, gap_scale = e.gap_scale)
-            stats = ConvergenceStats(Float64; err, time_elapsed = s[].time_elapsed,
-                kkt_passes = s[].kkt_passes, termination_status = STATUS[s[].termination_status + 1])
-            (PrimalDualSolution(x, y), stats)
+            (PrimalDualSolution(Vector{T}(x), Vector{T}(y)), convergence_stats(T, s[], Tuple{Int, KKTErrors{T}}[]))
         end
     finally
         ccall((:cpdlp_batch_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), b[])
@@ -350,14 +439,14 @@ point; the library copies it), `pump!` runs one scheduling round (or `start!` le
 library pump on its own thread), `poll!` lists finished requests and `take!` hands one
 over as `(PrimalDualSolution, ConvergenceStats, seconds)`.
 """
-mutable struct HIPServer
+mutable struct HIPServer{T}
     ptr::Ptr{Cvoid}
     shapes::Dict{Int64, Tuple{Int, Int}}
 end
 
 function HIPServer(algo::Algorithm{A, T, Ti, M, HIPEngine}; batch_limit::Integer = 8192,
         checks_per_round::Integer = 20) where {A, T, Ti, M}
-    p = cparams(algo, algo.conversion.backend)
+    p = cparams(algo)
     cfg = CServerConfig()
     check(ccall((:cpdlp_server_default_config, libcpdlp), Cint, (Ref{CServerConfig},), cfg))
     cfg.batch_limit = batch_limit
@@ -365,7 +454,7 @@ function HIPServer(algo::Algorithm{A, T, Ti, M, HIPEngine}; batch_limit::Integer
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:cpdlp_server_create, libcpdlp), Cint, (Ref{CParams}, Ref{CServerConfig}, Ptr{Ptr{Cvoid}}),
         p, cfg, h))
-    s = HIPServer(h[], Dict{Int64, Tuple{Int, Int}}())
+    s = HIPServer{T}(h[], Dict{Int64, Tuple{Int, Int}}())
     finalizer(s) do s
         s.ptr == C_NULL || ccall((:cpdlp_server_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), s.ptr)
         s.ptr = C_NULL
@@ -374,21 +463,18 @@ function HIPServer(algo::Algorithm{A, T, Ti, M, HIPEngine}; batch_limit::Integer
 end
 
 function submit!(s::HIPServer, milp::MILP, sol::Union{Nothing, PrimalDualSolution} = nothing)
-    A_ = milp.A::SparseMatrixCSC
-    c, lv, uv, lc, uc = (Vector{Float64}(v) for v in (milp.c, milp.lv, milp.uv, milp.lc, milp.uc))
-    lp = CLP(size(A_, 1), size(A_, 2), 1, 64, pointer(A_.colptr), pointer(A_.rowval),
-        pointer(A_.nzval), pointer(c), pointer(lv), pointer(uv), pointer(lc), pointer(uc))
+    H = HostLP(milp)
     x0 = sol === nothing ? Float64[] : Vector{Float64}(Array(sol.x))
     y0 = sol === nothing ? Float64[] : Vector{Float64}(Array(sol.y))
-    sol === nothing || (length(x0) == size(A_, 2) && length(y0) == size(A_, 1)) ||
+    sol === nothing || (length(x0) == H.lp.n && length(y0) == H.lp.m) ||
         throw(DimensionMismatch("start point does not match the LP's size"))
     id = Ref{Int64}(0)
-    GC.@preserve A_ c lv uv lc uc x0 y0 begin   # copied by the library during the call
+    GC.@preserve H x0 y0 begin   # copied by the library during the call
         check(ccall((:cpdlp_server_submit, libcpdlp), Cint,
-            (Ptr{Cvoid}, Ref{CLP}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}), s.ptr, lp,
+            (Ptr{Cvoid}, Ref{CLP}, Ptr{Float64}, Ptr{Float64}, Ref{Int64}), s.ptr, H.lp,
             sol === nothing ? C_NULL : pointer(x0), sol === nothing ? C_NULL : pointer(y0), id))
     end
-    s.shapes[id[]] = (size(A_, 1), size(A_, 2))
+    s.shapes[id[]] = (Int(H.lp.m), Int(H.lp.n))
     return id[]
 end
 
@@ -409,21 +495,22 @@ function poll!(s::HIPServer)
     return ids[1:cnt[]]
 end
 
-function Base.take!(s::HIPServer, id::Integer)
-    m, n = pop!(s.shapes, id)
+function Base.take!(s::HIPServer{T}, id::Integer) where {T}
+    haskey(s.shapes, id) || throw(ArgumentError("unknown request id $id"))
+    m, n = s.shapes[id]
     x, y = zeros(n), zeros(m)
     st = Ref{CStats}()
     secs = Ref{Float64}(0.0)
     route = Ref{Int32}(0)
-    check(ccall((:cpdlp_server_get, libcpdlp), Cint,
+    rc = ccall((:cpdlp_server_get, libcpdlp), Cint,
         (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ref{CStats}, Ref{Float64}, Ref{Int32}),
-        s.ptr, id, x, y, st, secs, route))
-    e = st[].err
-    err = KKTErrors(; primal = e.primal, primal_scale = e.primal_scale, dual = e.dual,
-        dual_scale = e.dual_scale, gap = e.gap, gap_scale = e.gap_scale)
-    stats = ConvergenceStats(Float64; err, time_elapsed = st[].time_elapsed,
-        kkt_passes = st[].kkt_passes, termination_status = STATUS[st[].termination_status + 1])
-    return PrimalDualSolution(x, y), stats, secs[]
+        s.ptr, id, x, y, st, secs, route)
+    # a request that has not finished stays queued under its id (CPDLP_ERR_STATE)
+    still_running = rc == 5 && unsafe_string(ccall((:cpdlp_last_error, libcpdlp), Cstring, ())) == "request still running"
+    still_running || delete!(s.shapes, id)
+    check(rc)
+    stats = convergence_stats(T, st[], Tuple{Int, KKTErrors{T}}[])
+    return PrimalDualSolution(Vector{T}(x), Vector{T}(y)), stats, secs[]
 end
 
 end # module

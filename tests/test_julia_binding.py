@@ -1,0 +1,186 @@
+"""The Julia binding (coolpdlp.jl_amd/julia/CoolPDLPHIP.jl) against the C ABI
+(include/coolpdlp_hip.h).  There is no Julia in this image, so the binding is checked as
+text: every C struct it mirrors has the header's fields in the header's order with the
+matching Julia types, every ccall names an exported function with the header's argument
+count and argument kinds, and the plug-in points the reference requires are present
+(Algorithm{A,T,Ti,M,B <: Backend}, src/algorithms/common.jl:8-14; get_backend and
+Adapt.adapt_structure of the matrix type, src/utils/mat_csr.jl:44-56).  CPU only.
+"""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "coolpdlp_hip.h")
+BINDING = os.path.join(ROOT, "coolpdlp.jl_amd", "julia", "CoolPDLPHIP.jl")
+
+STRUCTS = {"CLP": "cpdlp_lp", "CParams": "cpdlp_params", "CKKT": "cpdlp_kkt", "CStats": "cpdlp_stats",
+           "CServerConfig": "cpdlp_server_config"}
+SCALARS = {"int64_t": "Int64", "int32_t": "Int32", "uint64_t": "UInt64", "double": "Float64"}
+
+
+def _header():
+    text = open(HEADER).read()
+    defines = {k: int(v) for k, v in re.findall(r"#define\s+(\w+)\s+(\d+)", text)}
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return text, defines
+
+
+def _c_structs():
+    text, defines = _header()
+    out = {}
+    for name, body, alias in re.findall(r"typedef struct (\w+) \{(.*?)\} (\w+);", text, flags=re.S):
+        assert name == alias
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            m = re.match(r"((?:const\s+)?\w+\s*\**)\s*(.*)", decl)
+            ctype, names = m.group(1).replace(" ", ""), m.group(2)
+            for nm in names.split(","):
+                nm = nm.strip()
+                arr = re.match(r"(\w+)\[(\w+)\]", nm)
+                if arr:
+                    fields.append((arr.group(1), ctype, defines.get(arr.group(2), arr.group(2))))
+                else:
+                    fields.append((nm.lstrip("*"), ctype + "*" * nm.count("*"), None))
+        out[name] = fields
+    return out
+
+
+def _julia_type(ctype, arr):
+    base = ctype.replace("const", "")
+    if base.endswith("*"):
+        inner = base[:-1]
+        jt = "Ptr{Cvoid}" if inner == "void" else f"Ptr{{{SCALARS[inner]}}}"
+    elif base in SCALARS:
+        jt = SCALARS[base]
+    else:
+        jt = {v: k for k, v in STRUCTS.items()}[base]
+    return f"NTuple{{{arr}, {jt}}}" if arr is not None else jt
+
+
+def _jl_structs():
+    text = open(BINDING).read()
+    out = {}
+    for name, body in re.findall(r"^(?:mutable )?struct (\w+)\n(.*?)^end", text, flags=re.S | re.M):
+        if name not in STRUCTS:
+            continue
+        fields = []
+        for line in body.splitlines():
+            line = line.split("#")[0].strip()
+            m = re.match(r"(\w+)::(.+)$", line)
+            if m:
+                fields.append((m.group(1), m.group(2).strip()))
+        out[name] = fields
+    return out
+
+
+def test_structs_match_the_header():
+    cs, js = _c_structs(), _jl_structs()
+    assert set(js) == set(STRUCTS), set(STRUCTS) - set(js)
+    for jl, c in STRUCTS.items():
+        want = [(nm, _julia_type(ct, arr)) for nm, ct, arr in cs[c]]
+        assert js[jl] == want, (jl, js[jl], want)
+
+
+def _c_prototypes():
+    text, _ = _header()
+    protos = {}
+    for ret, name, args in re.findall(r"^\s*(int64_t|int|void|const char\*)\s+(cpdlp_\w+)\s*\(([^;]*?)\)\s*;",
+                                      text, flags=re.S | re.M):
+        args = " ".join(args.split())
+        params = [] if args in ("", "void") else [a.strip() for a in args.split(",")]
+        protos[name] = (ret, params)
+    return protos
+
+
+def _split_top(s):
+    """Split a Julia tuple body at top-level commas."""
+    out, depth, cur = [], 0, ""
+    for ch in s:
+        if ch in "({":
+            depth += 1
+        elif ch in ")}":
+            depth -= 1
+        if ch == "," and depth == 0:
+            out.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        out.append(cur.strip())
+    return out
+
+
+def _jl_ccalls():
+    text = open(BINDING).read()
+    calls = []
+    for m in re.finditer(r"ccall\(\(:(cpdlp_\w+), libcpdlp\),\s*(\w+),\s*\(", text):
+        i, depth = m.end(), 1
+        while depth:
+            depth += {"(": 1, ")": -1}.get(text[i], 0)
+            i += 1
+        calls.append((m.group(1), m.group(2), _split_top(text[m.end():i - 1])))
+    return calls
+
+
+RET = {"int": "Cint", "void": "Cvoid", "int64_t": "Int64", "const char*": "Cstring"}
+
+
+def _accepts(cparam, jt):
+    """Whether the Julia ccall argument type jt can carry the C parameter cparam."""
+    ctype = re.sub(r"\s*\w+$", "", cparam).replace("const", "").replace(" ", "")
+    if not ctype.endswith("*"):
+        return jt == SCALARS[ctype]
+    inner = ctype[:-1]
+    if inner.endswith("*"):                                # cpdlp_handle** out
+        return jt == "Ptr{Ptr{Cvoid}}"
+    if inner in ("void", "cpdlp_handle", "cpdlp_matrix", "cpdlp_batch", "cpdlp_server"):
+        return jt == "Ptr{Cvoid}"
+    elem = SCALARS.get(inner) or {v: k for k, v in STRUCTS.items()}[inner]
+    return jt in (f"Ptr{{{elem}}}", f"Ref{{{elem}}}")
+
+
+def test_ccalls_match_the_prototypes():
+    protos = _c_prototypes()
+    calls = _jl_ccalls()
+    assert len(calls) >= 30
+    seen = set()
+    for name, ret, args in calls:
+        assert name in protos, f"{name} is not declared in the header"
+        cret, params = protos[name]
+        assert ret == RET[cret], (name, ret, cret)
+        assert len(args) == len(params), (name, args, params)
+        for a, p in zip(args, params):
+            assert _accepts(p, a), (name, a, p)
+        seen.add(name)
+    # the binding reaches the solve, the chunked loop, the matrix plug-in, batch and server
+    for fn in ("cpdlp_create", "cpdlp_solve", "cpdlp_initialize", "cpdlp_run_checks", "cpdlp_matrix_create",
+               "cpdlp_matrix_mul", "cpdlp_batch_add_start", "cpdlp_batch_run", "cpdlp_server_submit",
+               "cpdlp_server_get"):
+        assert fn in seen, fn
+
+
+def test_reference_plugin_points_are_implemented():
+    text = open(BINDING).read()
+    # B <: Backend in Algorithm{A,T,Ti,M,B} and ConversionParameters (common.jl:8-14, conversion.jl:14-15)
+    assert re.search(r"struct HIPEngine <: KernelAbstractions\.GPU", text)
+    # the matrix type's backend and adaptation (mat_csr.jl:44-56; used by linalg.jl:67,91)
+    assert re.search(r"KernelAbstractions\.get_backend\(::HIPSparseMatrixCSR\)", text)
+    assert re.search(r"Adapt\.adapt_structure\(to, A::HIPSparseMatrixCSR\)", text)
+    assert re.search(r"LinearAlgebra\.mul!\(c::\w+\{Float64\}, A::HIPSparseMatrixCSR", text)
+    # T selects the precision (common.jl:55-57), the input's index type the index width
+    assert "p.precision = precision_code(T, eng)" in text
+    assert "p.format = format_code(M)" in text
+    assert not re.search(r"CLP\([^)]*,\s*1,\s*64,", text), "hard-coded 64-bit indices"
+    assert "index_bits(A_)" in text and "8 * sizeof(Ti)" in text
+    # results come back in the algorithm's float type
+    assert "PrimalDualSolution(Vector{T}(x), Vector{T}(y))" in text
+
+
+@pytest.mark.parametrize("jl,c", sorted(STRUCTS.items()))
+def test_struct_field_counts(jl, c):
+    assert len(_jl_structs()[jl]) == len(_c_structs()[c])
