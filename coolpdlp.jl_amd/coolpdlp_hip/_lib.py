@@ -96,6 +96,7 @@ _SIGS = {
     "cpdlp_get_history": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                     _P(C.c_int64)]),
     "cpdlp_kkt_errors": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _P(CKKT)]),
+    "cpdlp_kkt_errors_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _P(CKKT)]),
     "cpdlp_prepare": (C.c_int, [C.c_void_p]),
     "cpdlp_get_scaling": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),

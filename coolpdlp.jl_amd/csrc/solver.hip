@@ -820,13 +820,21 @@ void Handle::prepare() {
 template <class V, int K>
 void Handle::kkt_launch(const V* const* X, const V* const* Y, const V* xr, const V* yr, bool dist,
                         double* host_sums) {
+    const int gA = A.view<V>().partial_blocks(), gAt = At.view<V>().partial_blocks();
+    constexpr int NQR = EpiKktRows<V, K>::NQ, NQC = EpiKktCols<V, K>::NQ;
+    ensure_partials((size_t)std::max(gA * NQR, gAt * NQC) + 16);
+    kkt_launch_into<V, K>(X, Y, xr, yr, dist, host_sums, d_partials.p, d_partials2.p, d_sums.p);
+}
+
+template <class V, int K>
+void Handle::kkt_launch_into(const V* const* X, const V* const* Y, const V* xr, const V* yr, bool dist,
+                             double* host_sums, double* partials, double* partials2, double* sums_dev) {
     const hipStream_t s = stream;
     const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
     const int gA = vA.partial_blocks(), gAt = vAt.partial_blocks();
     constexpr int NQR = EpiKktRows<V, K>::NQ, NQC = EpiKktCols<V, K>::NQ;
-    ensure_partials((size_t)std::max(gA * NQR, gAt * NQC) + 16);
     EpiKktRows<V, K> er;
-    er.partials = d_partials.p;
+    er.partials = partials;
     for (int k = 0; k < K; ++k) er.Y[k] = Y[k];
     er.y_rst = yr ? yr : Y[0];
     er.lc = lc_s.p;
@@ -837,7 +845,7 @@ void Handle::kkt_launch(const V* const* X, const V* const* Y, const V* xr, const
     for (int k = 0; k < K; ++k) xp.p[k] = X[k];
     launch_spmv<V, double, K>(vA, xp, er, s);
     EpiKktCols<V, K> ec;
-    ec.partials = d_partials2.p;
+    ec.partials = partials2;
     for (int k = 0; k < K; ++k) ec.X[k] = X[k];
     ec.x_rst = xr ? xr : X[0];
     ec.c = c_s.p;
@@ -849,17 +857,17 @@ void Handle::kkt_launch(const V* const* X, const V* const* Y, const V* xr, const
     for (int k = 0; k < K; ++k) yp.p[k] = Y[k];
     launch_spmv<V, double, K>(vAt, yp, ec, s);
     if (gA > 0)
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gA, NQR, d_partials.p,
-                           d_sums.p, (PowState*)nullptr, POW_NONE);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gA, NQR, partials,
+                           sums_dev, (PowState*)nullptr, POW_NONE);
     else
-        hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQR, d_sums.p, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQR, sums_dev, 0.0);
     if (gAt > 0)
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gAt, NQC, d_partials2.p,
-                           d_sums.p + NQR, (PowState*)nullptr, POW_NONE);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gAt, NQC, partials2,
+                           sums_dev + NQR, (PowState*)nullptr, POW_NONE);
     else
-        hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQC, d_sums.p + NQR, 0.0);
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQC, sums_dev + NQR, 0.0);
     HIP_CHECK(hipGetLastError());
-    d_sums.download(host_sums, NQR + NQC, s);
+    if (NQR + NQC) HIP_CHECK(hipMemcpyAsync(host_sums, sums_dev, (NQR + NQC) * sizeof(double), hipMemcpyDeviceToHost, s));
     sync();
 }
 
@@ -1516,6 +1524,46 @@ cpdlp_kkt Handle::kkt_host(const double* x, const double* y) {
     return kkt_from_sums(sums, 1, 0);
 }
 
+// layout of kkt_device's scratch, in doubles, each part 32-double (256-byte) aligned:
+// x (n, internal order, scaled), y (m), partials of A's rows, partials of A''s rows, sums
+static size_t round32(size_t k) { return (k + 31) & ~(size_t)31; }
+
+size_t Handle::kkt_scratch_doubles() const {
+    const int gA = A.view<double>().partial_blocks(), gAt = At.view<double>().partial_blocks();
+    constexpr int NQR = EpiKktRows<double, 1>::NQ, NQC = EpiKktCols<double, 1>::NQ;
+    return round32((size_t)n) + round32((size_t)m) + round32((size_t)gA * NQR + 16) +
+           round32((size_t)gAt * NQC + 16) + round32(NQR + NQC);
+}
+
+// kkt_errors! (errors.jl:65-109) of an unscaled device point, as kkt_host, with every
+// intermediate in the caller-given device scratch: no device memory is allocated.
+// Requires the preprocessing (cpdlp_prepare or a solve), which allocates.
+cpdlp_kkt Handle::kkt_device(const double* x_dev, const double* y_dev, double* scratch, size_t scratch_len) {
+    HIP_CHECK(hipSetDevice(device));
+    // the preprocessing also fixes the formats, on which the partials' size depends
+    if (!prepared) throw Error(CPDLP_ERR_STATE, "call cpdlp_prepare (or a solve) first: preprocessing allocates");
+    const size_t need = kkt_scratch_doubles();
+    if (need > scratch_len)
+        throw Error(CPDLP_ERR_ARGUMENT, "LP needs " + std::to_string(need) + " doubles of scratch; the "
+                                        "allocation-free KKT evaluation has " + std::to_string(scratch_len));
+    const int gA = A.view<double>().partial_blocks(), gAt = At.view<double>().partial_blocks();
+    constexpr int NQR = EpiKktRows<double, 1>::NQ, NQC = EpiKktCols<double, 1>::NQ;
+    double* xi = scratch;
+    double* yi = xi + round32((size_t)n);
+    double* pa = yi + round32((size_t)m);
+    double* pat = pa + round32((size_t)gA * NQR + 16);
+    double* sums_dev = pat + round32((size_t)gAt * NQC + 16);
+    hipLaunchKernelGGL(k_to_internal<double>, dim3(grid_n()), dim3(BLOCK), 0, stream, (int32_t)n,
+                       d_col_of_int.p, x_dev, d_D2o.p, xi);
+    hipLaunchKernelGGL(k_to_internal<double>, dim3(grid_m()), dim3(BLOCK), 0, stream, (int32_t)m,
+                       d_row_of_int.p, y_dev, d_D1o.p, yi);
+    const double* X[1] = {xi};
+    const double* Y[1] = {yi};
+    double sums[16];
+    kkt_launch_into<double, 1>(X, Y, nullptr, nullptr, false, sums, pa, pat, sums_dev);
+    return kkt_from_sums(sums, 1, 0);
+}
+
 }  // namespace cpdlp
 
 // ================================================================ C ABI
@@ -1632,6 +1680,40 @@ int cpdlp_kkt_errors(cpdlp_handle* h, const double* x, const double* y, cpdlp_kk
         *out = h->h->kkt_host(x, y);
     });
 }
+// ---- allocation-free KKT evaluation (cpdlp_kkt_errors_device_noalloc) ----------------
+// Its intermediates live in this file-scope device array: each GPU holds its own copy,
+// 32 MiB, from the library's load until the process ends.  Host code never uses the
+// array's name as an address: the per-device address comes from hipGetSymbolAddress,
+// taken once per device with that device current.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)1 << 22;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+static std::mutex scratch_mu;
+static std::vector<double*> scratch_ptr;   // per device ordinal; nullptr until looked up
+
+static double* scratch_on(int device) {   // the calling thread's current device is `device`
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if ((int)scratch_ptr.size() <= device) scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !p)
+            throw Error(CPDLP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+        scratch_ptr[device] = (double*)p;
+    }
+    return scratch_ptr[device];
+}
+
+int cpdlp_kkt_errors_device_noalloc(cpdlp_handle* h, const double* x_dev, const double* y_dev, cpdlp_kkt* out) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        Handle& H = *h->h;
+        if ((!x_dev && H.n) || (!y_dev && H.m) || !out) throw Error(CPDLP_ERR_ARGUMENT, "NULL argument");
+        HIP_CHECK(hipSetDevice(H.device));
+        *out = H.kkt_device(x_dev, y_dev, scratch_on(H.device), NOALLOC_SCRATCH_DOUBLES);
+    });
+}
+
 int cpdlp_get_scaling(cpdlp_handle* h, double* D1, double* D2, double* a_csc, double* at_csc) {
     return guarded([&] {
         HANDLE_OR_THROW(h);

@@ -115,6 +115,15 @@ struct Handle {
     template <class V, int K>
     void kkt_launch(const V* const* X, const V* const* Y, const V* xr, const V* yr, bool dist,
                     double* host_sums);
+    // kkt_launch on caller-given device buffers: partials (rows of A), partials2 (rows of
+    // A'), sums_dev (the reduced sums); allocates nothing
+    template <class V, int K>
+    void kkt_launch_into(const V* const* X, const V* const* Y, const V* xr, const V* yr, bool dist,
+                         double* host_sums, double* partials, double* partials2, double* sums_dev);
+    // doubles of device scratch kkt_device needs (x and y in internal order, block
+    // partials of both products, the sums)
+    size_t kkt_scratch_doubles() const;
+    cpdlp_kkt kkt_device(const double* x_dev, const double* y_dev, double* scratch, size_t scratch_len);
     cpdlp_kkt kkt_from_sums(const double* sums, int K, int k) const;
     template <class V>
     int32_t ray_check(IterState<V>& S);

@@ -186,6 +186,19 @@ int cpdlp_get_history(cpdlp_handle* h, int64_t* kkt_passes, cpdlp_kkt* errs, int
  * original space as in the reference.  Requires cpdlp_solve or cpdlp_prepare. */
 int cpdlp_kkt_errors(cpdlp_handle* h, const double* x, const double* y, cpdlp_kkt* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * cpdlp_kkt_errors for an UNSCALED point given as DEVICE pointers x_dev (n) and y_dev (m),
+ * original order, computed on the handle's stream with every intermediate in a fixed
+ * per-device scratch array of 32 MiB (4,194,304 doubles) inside the library: no device
+ * memory is allocated.  The LP must fit: n + m plus the block partials of both products
+ * (a few per 256 rows) within that scratch, else CPDLP_ERR_ARGUMENT; the preprocessing
+ * must have run (cpdlp_prepare or a solve, which allocate), else CPDLP_ERR_STATE.  Same
+ * errors as cpdlp_kkt_errors on the same point.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (from different threads or handles) race. */
+int cpdlp_kkt_errors_device_noalloc(cpdlp_handle* h, const double* x_dev, const double* y_dev,
+                                    cpdlp_kkt* out);
+
 /* preprocess(...) + initialize's step sizes without running the loop:
  * pdlp_preconditioner / precondition (preconditioning.jl:146-155, :61-91),
  * fixed_stepsize (step_size.jl:22-27), primal_weight_init (:29-40). */
