@@ -19,7 +19,7 @@ the step size of the reference's power-method rule beside the converged one, and
 CPU oracle (FP64, OpenMP) on a bounded sample of the same LP on this host, projected to
 the GPU solve's pass count.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|plain|batch|serve|mcf]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|plain|batch|serve|mcf|gather]
 """
 import argparse
 import json
@@ -54,6 +54,20 @@ def measured_traffic(workload: str, fmt_a: str, fmt_at: str):
     if rec.get("workload") != workload or rec.get("A") != fmt_a or rec.get("At") != fmt_at:
         return None, None
     return int(rec["traffic_bytes_per_iteration"]), os.path.relpath(path, ROOT)
+
+
+def gather_ceiling():
+    """The L2-resident gather rate of the CSR tile kernel, as recorded in
+    profiles/gather_ceiling_csr_tile_kernel.md (bench.py --workload gather); None when
+    that record is absent."""
+    path = os.path.join(ROOT, "profiles", "gather_ceiling_csr_tile_kernel.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("record:"):
+            rec = dict(kv.partition("=")[::2] for kv in line[len("record:"):].split())
+            return float(rec["gathers_per_s"]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def log(*a):
@@ -257,6 +271,61 @@ def bench_mcf(args):
         "fp64_same_tol_status": f64st.termination_status.name,
         "speedup_vs_fp64_same_tol": el_f64 / el,
     }
+
+
+def bench_gather(args):
+    """The gather-rate ceiling of the CSR tile kernel the steps run: the library's own
+    product (cpdlp_matrix_mul, CSR format, thread-per-row sums) of a 2,000,000-row matrix
+    with 10 uniform random distinct columns per row (20M FP64 gathers), against a
+    gathered vector of 16 entries (one 128-byte line: every gather hits the L1) and of
+    1, 4, 8 and 16 MB.  Reports the time per product (HIP events on the launching
+    stream), gathers per second and algorithmic GB/s."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+    import coolpdlp_hip as cp
+    rows, per_row = args.gather_rows, 10
+    rng = np.random.default_rng(5)
+    points = []
+    for n in (16, 131_072, 524_288, 1_048_576, 2_097_152):
+        t = time.perf_counter()
+        # distinct columns per row: a random offset plus sorted distinct steps, modulo n
+        steps = np.sort(rng.choice(max(n // per_row, 1), size=(rows, per_row)), axis=1) if n > 16 else None
+        if n > 16:
+            cols = (rng.integers(0, n, size=(rows, 1)) + steps * per_row + np.arange(per_row)) % n
+        else:
+            cols = np.argsort(rng.random((rows, 16)), axis=1)[:, :per_row]
+        cols = np.sort(cols, axis=1).astype(np.int32)
+        indptr = np.arange(0, rows * per_row + 1, per_row, dtype=np.int64)
+        A = sp.csr_matrix((rng.standard_normal(rows * per_row), cols.ravel(), indptr), shape=(rows, n)).tocsc()
+        M = cp.GPUSparseMatrixCSR(A)
+        b = torch.rand(n, dtype=torch.float64, device="cuda:0")
+        c = torch.empty(rows, dtype=torch.float64, device="cuda:0")
+        for _ in range(3):
+            M.mul(c, b)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = args.gather_reps
+        e0.record()
+        for _ in range(reps):
+            M.mul(c, b)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nnz = rows * per_row
+        alg = 12 * nnz + 4 * (rows + 1) + 8 * rows + 8 * n
+        pt = {"vector_entries": n, "vector_mb": 8 * n / 2**20, "ms": ms, "gathers_per_s": nnz / (ms * 1e-3),
+              "algorithmic_GBs": alg / (ms * 1e-3) / 1e9}
+        log(f"[bench] gather n={n}: {ms * 1e3:.1f} us, {pt['gathers_per_s'] / 1e9:.1f} G gathers/s "
+            f"(setup {time.perf_counter() - t:.1f}s)")
+        points.append(pt)
+        M.close()
+        del b, c
+    return {"metric": "gather-rate ceiling of the CSR tile kernel (not the headline metric)",
+            "value": points[-1]["gathers_per_s"], "unit": "gathers/s", "n_gpus": 1,
+            "steps": args.gather_reps, "warmup": 3, "ms_per_step": points[-1]["ms"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "gather_ceiling_csr_tile", "rows": rows, "nnz_per_row": per_row},
+            "points": points}
 
 
 def O_threads() -> int:
@@ -558,6 +627,14 @@ def bench_single(args):
                 "timed_over": "the timed steps" if chunked else "the last solve",
                 "bytes_per_iteration": bp + bd, "ms_per_iteration": t_iter_ms,
                 "ms_per_iteration_whole_solve": t_iter_all, "kernels": kernels}
+    # the bound these kernels meet first: L2 request latency of the random gathers (one per
+    # nonzero and product), against the tile kernel's rate with an L2-resident vector
+    ceil_rate, ceil_src = gather_ceiling()
+    gathers = 2 * g.nnz
+    g_rate = gathers / (t_iter_ms * 1e-3)
+    gather_roofline = {"bound": "l2_gather_latency", "achieved": g_rate, "peak": ceil_rate,
+                       "unit": "gathers/s", "frac": (g_rate / ceil_rate) if ceil_rate else None,
+                       "gathers_per_iteration": gathers, "peak_source": ceil_src}
     eng.close()
 
     if args.calibrate_copy:
@@ -589,7 +666,8 @@ def bench_single(args):
                            budget_left(args) - 15)
         log(f"[bench] cpu baseline: {cpu['sample']}")
 
-    finished = st.termination_status != cp.STILL_RUNNING and guard is None
+    # a solve capped by --max-passes (profiling runs) or stopped by the guard has no solve time
+    finished = st.termination_status == cp.OPTIMAL and guard is None
     return {
         "metric": "solve time to rel-KKT 1e-6; batched LPs/s + p50/p99 vs plain PDLP, 1 GPU",
         "value": (1.0 / solve_seconds) if finished else None, "unit": "LPs/s", "n_gpus": 1,
@@ -609,7 +687,7 @@ def bench_single(args):
         "formats": {"A": FORMATS[st.format_A], "At": FORMATS[st.format_At]},
         "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
         "powm_tol": args.powm_tol, "eta_reference_rule": eta_ref,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "gather_roofline": gather_roofline, "cpu_baseline": cpu,
         "process_seconds": time.perf_counter() - T_PROC0,
     }
 
@@ -661,6 +739,8 @@ def main():
     ap.add_argument("--pool-medium", type=int, default=16)
     ap.add_argument("--pool-large", type=int, default=3)
     ap.add_argument("--mcf-nodes", type=int, default=500_000)
+    ap.add_argument("--gather-rows", type=int, default=2_000_000)
+    ap.add_argument("--gather-reps", type=int, default=20)
     ap.add_argument("--mcf-oracle-limit", type=float, default=900.0,
                     help="time limit (s) of the oracle's FP64 reference solve of the min-cost flow")
     args = ap.parse_args()
@@ -683,6 +763,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
+        return
+    if args.workload == "gather":
+        print(json.dumps(bench_gather(args)), flush=True)
         return
 
     print(json.dumps(bench_single(args)), flush=True)

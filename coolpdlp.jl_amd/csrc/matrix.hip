@@ -159,7 +159,9 @@ int cpdlp_matrix_mul(cpdlp_matrix* A, double alpha, const double* b_dev, double 
         if (!A) throw Error(CPDLP_ERR_ARGUMENT, "NULL matrix");
         if ((!b_dev && A->n) || (!c_dev && A->m)) throw Error(CPDLP_ERR_ARGUMENT, "NULL vector");
         HIP_CHECK(hipSetDevice(A->device));
-        A->launch(alpha, b_dev, beta, c_dev, stream ? (hipStream_t)stream : A->stream);
+        // NULL is HIP's null stream (a caller's default stream), not the matrix's own: the
+        // product must be ordered with the caller's work on b and c
+        A->launch(alpha, b_dev, beta, c_dev, (hipStream_t)stream);
     });
 }
 

@@ -337,9 +337,10 @@ const double* DevSide::sl_vals<double>() const { return sl_val64.p; }
 template <>
 const float* DevSide::sl_vals<float>() const { return sl_val32.p; }
 
-// NS = 8 column slices of rows [0, nrows), one per XCD of the MI355X
-void DevSide::build_slices(const HostSide& S, int32_t nrows, hipStream_t s) {
-    constexpr int32_t NS = 8;
+// NS column slices of rows [0, nrows); NS divides the 8 XCDs of the MI355X, and slice s
+// is summed on the XCDs x with x % NS == s (workgroups go to the XCDs round-robin)
+void DevSide::build_slices(const HostSide& S, int32_t nrows, int32_t NS, hipStream_t s) {
+    if (NS != 2 && NS != 4 && NS != 8) throw Error(CPDLP_ERR_ARGUMENT, "slice count must be 2, 4 or 8");
     HostSlices H;
     cpdlp::build_slices(S, NS, nrows, H);
     sl_ns = NS;
@@ -525,13 +526,24 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
     // of every row for a large side whose gathered vector outgrows one XCD's 4 MiB L2
     // (the tuner then times both variants), else of the long rows when it has any.
     const size_t elem = P.precision == CPDLP_PREC_FP64 ? 8 : 4;
+    // slices per side: 8 (one per XCD) unless CPDLP_SLICES="<A>,<A'>" (2, 4 or 8) says
+    // otherwise, for measurements
+    int32_t ns_of[2] = {8, 8};
+    if (const char* e = std::getenv("CPDLP_SLICES")) {
+        const std::string spec(e);
+        const size_t comma = spec.find(',');
+        ns_of[0] = (int32_t)std::strtol(spec.substr(0, comma).c_str(), nullptr, 10);
+        ns_of[1] = comma == std::string::npos ? ns_of[0] : (int32_t)std::strtol(spec.substr(comma + 1).c_str(), nullptr, 10);
+    }
+    int side_k = 0;
     for (auto side : {std::make_pair(&A, &L.A), std::make_pair(&At, &L.At)}) {
         const HostSide& H = *side.second;
+        const int32_t ns = ns_of[side_k++];
         const bool big = (size_t)H.cols * elem > ((size_t)4 << 20) && H.nnz >= ((int64_t)4 << 20);
         if (P.format == CPDLP_FMT_XSLICE_ALL || (P.format == CPDLP_FMT_AUTO && big))
-            side.first->build_slices(H, H.rows, s);
+            side.first->build_slices(H, H.rows, ns, s);
         else if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && H.n_long > 0))
-            side.first->build_slices(H, H.n_long, s);
+            side.first->build_slices(H, H.n_long, ns, s);
     }
     HIP_CHECK(hipStreamSynchronize(s));
     // free the big host-side copies that are no longer needed

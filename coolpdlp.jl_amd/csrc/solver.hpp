@@ -41,7 +41,7 @@ struct DevSide {
     template <class V>
     MatView<V> view(bool allow_slices = true) const;
     void upload_structure(const HostSide& S, hipStream_t s);
-    void build_slices(const HostSide& S, int32_t nrows, hipStream_t s);
+    void build_slices(const HostSide& S, int32_t nrows, int32_t ns, hipStream_t s);
     bool slices_all() const { return sl_ns > 0 && sl_built == rows; }
     void build_ell(const HostSide& S, hipStream_t s, double max_fill);
     void fill_values(const double* src_csc, bool want32, hipStream_t s);

@@ -288,8 +288,9 @@ int cpdlp_matrix_create(int64_t m, int64_t n, int32_t index_base, int32_t index_
                         int32_t format, int32_t transpose, int32_t device, cpdlp_matrix** out);
 
 /* LinearAlgebra.mul!(c, A, b, alpha, beta), mat_csr.jl:81-92 / mat_ell.jl:87-98:
- * c = alpha*A*b + beta*c on DEVICE pointers, asynchronously on `stream`
- * (a hipStream_t; NULL = the matrix's own stream).  beta == 0 does not read c. */
+ * c = alpha*A*b + beta*c on DEVICE pointers, asynchronously on `stream` (a hipStream_t;
+ * NULL = HIP's null stream), ordered with the caller's other work on that stream.
+ * beta == 0 does not read c. */
 int cpdlp_matrix_mul(cpdlp_matrix* A, double alpha, const double* b_dev, double beta,
                      double* c_dev, void* stream);
 

@@ -141,6 +141,25 @@ def test_termination_not_skipped(cp):
     assert abs(sol.x.sum() - 1.0) < 1e-3
 
 
+def test_mul_on_torch_default_stream_is_ordered(cp):
+    """mul! on device tensors runs on the caller's stream (torch's current one, the null
+    stream by default): b written by torch just before and c read by torch just after
+    need no synchronisation of their own."""
+    import torch
+    rng = np.random.default_rng(7)
+    m, n = 300_000, 200_000
+    A = sp.random(m, n, density=10 / n, format="csc", random_state=rng)
+    M = cp.GPUSparseMatrixCSR(A)
+    for seed in range(3):
+        torch.manual_seed(seed)
+        b = torch.rand(n, dtype=torch.float64, device="cuda:0")
+        c = torch.full((m,), float("nan"), dtype=torch.float64, device="cuda:0")
+        M.mul(c, b)
+        out = c.cpu().numpy()
+        np.testing.assert_allclose(out, A @ b.cpu().numpy(), rtol=1e-12, atol=1e-13)
+    M.close()
+
+
 @pytest.mark.parametrize("algo,use_graph,fmt", [
     ("PDLP", 0, "csr"), ("PDLP", 1, "csr"), ("PDLP", 1, "ell"), ("PDLP", 1, "xslice"),
     ("PDLP", 1, "xslice_all"), ("PDLP", 1, "auto"), ("PDHG", 0, "csr"), ("PDHG", 1, "auto")])
