@@ -334,13 +334,45 @@ def O_threads() -> int:
     return max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
 
 
+def _class_stats(recs, picks, classes=("small", "medium", "large")):
+    """Per request class: submitted, completed, OPTIMAL, p50/p99 latency of the OPTIMAL ones."""
+    import numpy as np
+    out = {}
+    for c in classes:
+        mine = [r for r in recs if picks[r.rid][0] == c]
+        lat = np.array([r.t_done - r.t_arrival for r in mine if r.status == "OPTIMAL"])
+        out[c] = {"submitted": int(sum(p[0] == c for p in picks)), "completed": len(mine),
+                  "optimal": int(len(lat)),
+                  "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+                  "p99_s": float(np.percentile(lat, 99)) if len(lat) else None}
+    return out
+
+
+def _served(recs, t0_rel=0.0):
+    """OPTIMAL results per second of the stream (until the last OPTIMAL completion), and
+    the p50/p99 latency of those results."""
+    import numpy as np
+    opt = [r for r in recs if r.status == "OPTIMAL"]
+    span = max((r.t_done for r in opt), default=0.0) - t0_rel
+    lat = np.array([r.t_done - r.t_arrival for r in opt])
+    return {"completed": len(recs), "optimal": len(opt),
+            "served_lps_per_s": len(opt) / span if span > 0 else 0.0,
+            "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_s": float(np.percentile(lat, 99)) if len(lat) else None}
+
+
 def bench_serve(args):
     """Serving loop: Poisson arrivals at each QPS of the sweep for `serve_duration`
-    seconds, request mix 70% small (m in [50, 2k]), 25% medium ([2k, 50k]), 5% large
-    ([50k, 500k]), n = 2m, 8 nonzeros per column.  Requests are drawn from a seeded pool
-    of distinct LPs per class.  Latency = finish - arrival to relative KKT 1e-6.  The
-    first-come-first-served baseline is the one-solve-at-a-time queue over the same
-    arrivals with each request's measured plain-engine solve time."""
+    seconds (then up to `serve_drain` seconds more to finish), request mix 70% small
+    (m log-uniform in [50, 2k]), 25% medium ([2k, 50k]), 5% large ([50k, 500k]), n = 2m,
+    8 nonzeros per column.  Requests are drawn from a seeded pool of distinct LPs per
+    class.  Latency = finish - arrival, to relative KKT 1e-6; only OPTIMAL results count
+    as served.  Every request, on every route and in the baseline, runs the power method
+    to powm_tol (the reference's eps*n^3 rule stops after one Rayleigh quotient for
+    n >~ 3e5 and the large requests then diverge).  The baseline is a first-come-first-
+    served server that solves one request at a time on a plain engine, run on the same
+    arrivals (--serve-fcfs run), or modelled from each pool LP's measured solve time
+    (--serve-fcfs replay)."""
     import numpy as np
     import coolpdlp_hip as cp
     from coolpdlp_hip import generators as G
@@ -355,18 +387,23 @@ def bench_serve(args):
         ms = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=sizes[cls]))).astype(int)
         pool[cls] = [G.request_lp(int(m), 2 * int(m), 1000 + 97 * k + len(pool)) for k, m in enumerate(ms)]
     log(f"[bench] request pool {({k: len(v) for k, v in pool.items()})} in {time.perf_counter() - t:.1f}s")
-    # plain-engine service time of every pool LP (the FCFS baseline's server)
-    service = {}
-    for cls, lps in pool.items():
-        for k, g in enumerate(lps):
-            milp = cp.MILP.from_generated(g)
-            eng = cp.Engine(milp, cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit))
-            t = time.perf_counter()
-            sol, st = eng.solve()
-            service[(cls, k)] = time.perf_counter() - t
+    kw = dict(termination_reltol=reltol, time_limit=args.time_limit, powm_tol=args.powm_tol)
+
+    def plain_solve(g):
+        """The baseline server's work for one request: a plain engine, alone."""
+        eng = cp.Engine(cp.MILP.from_generated(g), cp.PDLP("mixed", None, "csr", **kw))
+        try:
+            return eng.solve()
+        finally:
             eng.close()
-            log(f"[bench] plain {cls}[{k}] m={g.m} {st.termination_status.name} passes={st.kkt_passes} "
-                f"{service[(cls, k)]:.3f}s")
+
+    service = {}
+    if args.serve_fcfs == "replay":
+        for cls, lps in pool.items():
+            for k, g in enumerate(lps):
+                t = time.perf_counter()
+                plain_solve(g)
+                service[(cls, k)] = time.perf_counter() - t
     out_points = []
     for qps in args.qps:
         arr = poisson_arrivals(qps, args.serve_duration, seed=int(qps))
@@ -374,20 +411,20 @@ def bench_serve(args):
         cls_of = np.where(u < 0.70, "small", np.where(u < 0.95, "medium", "large"))
         picks = [(c, int(rng.integers(len(pool[c])))) for c in cls_of]
         lanes = ((1_000_000, args.serve_medium_workers), (float("inf"), 1))
+        # ---- the server (continuous batching)
         if args.serve_native:   # the library's own server (cpdlp_server_*)
-            srv = NativeServer(cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit,
-                                       batch_streams=1),   # + 3 plain workers: 4 hardware queues
+            srv = NativeServer(cp.PDLP("mixed", batch_streams=1, **kw),   # + 3 plain workers: 4 queues
                                batch_bytes=args.serve_batch_gib << 30, checks_per_round=args.serve_checks,
                                lanes=lanes, batch_limit=args.serve_batch_limit)
             srv.done, srv.rounds, srv.round_seconds, meta = [], 0, [], {}
             srv.start()           # the library pumps on its own thread
         else:
-            srv = Server(reltol=reltol, batch_bytes=args.serve_batch_gib << 30,
+            srv = Server(reltol=reltol, batch_bytes=args.serve_batch_gib << 30, powm_tol=args.powm_tol,
                          checks_per_round=args.serve_checks, time_limit=args.time_limit,
                          lanes=lanes, overflow_queue=args.serve_overflow, batch_limit=args.serve_batch_limit)
+        max_obj_err = 0.0
         t0 = time.perf_counter()
         k = 0
-        pending = 0
         deadline = t0 + args.serve_duration + args.serve_drain
         last_log = t0
         while time.perf_counter() < deadline:
@@ -395,75 +432,95 @@ def bench_serve(args):
             while k < len(arr) and arr[k] <= now:
                 g = pool[picks[k][0]][picks[k][1]]
                 if args.serve_native:
-                    meta[srv.submit(g)] = (k, t0 + arr[k], time.perf_counter(), g)
+                    meta[srv.submit(g)] = (k, arr[k], time.perf_counter() - t0, g)
                 else:
                     srv.submit(g, t0 + arr[k])
                 k += 1
             if args.serve_native:
-                if k >= len(arr) and pending == 0 and k == len(srv.done):
+                if k >= len(arr) and not meta:
                     break
-                pending = srv.pump()  # with the driver running: the pending count
                 time.sleep(0.001)
                 for i in srv.poll():
                     rid, t_arr, t_sub, g = meta.pop(i)
-                    sol, st, secs, _ = srv.get(i)
-                    srv.done.append(NativeDone(rid, t_arr, t_sub + secs, st.termination_status.name,
-                                               float(sol.x @ g.c)))
+                    try:
+                        sol, st, secs, _ = srv.get(i)
+                        status, obj = st.termination_status.name, float(sol.x @ g.c)
+                    except Exception as e:   # noqa: BLE001 -- a failed request is reported
+                        status, obj, secs = f"ERROR: {e}", float("nan"), 0.0
+                    srv.done.append(NativeDone(rid, t_arr, t_sub + secs, status, obj))
+                    if status == "OPTIMAL":
+                        max_obj_err = max(max_obj_err, abs(obj - g.objective) / (1 + abs(g.objective)))
             else:
                 if k >= len(arr) and srv.pending() == 0:
                     break
                 srv.pump()
-                pending = srv.pending()
-            if time.perf_counter() - last_log > 5:
+            if time.perf_counter() - last_log > 10:
                 last_log = time.perf_counter()
-                log(f"[bench] qps={qps} t={now:.1f}s submitted={k} done={len(srv.done)} pending={pending}")
-        t_end = time.perf_counter()
+                log(f"[bench] qps={qps} t={now:.1f}s submitted={k} done={len(srv.done)}")
         srv.close()
-        done = srv.done
-        lat = np.array([r.t_done - r.t_arrival for r in done])
-        ok = sum(r.status == "OPTIMAL" for r in done)
-        fc = fcfs_latencies(arr, np.array([service[p] for p in picks]))
-        cls_done = {r.rid: picks[r.rid][0] for r in done}
-        rs = np.array(srv.round_seconds) if srv.round_seconds else np.zeros(1)
-        waits = {c: float(np.median([r.t_start - r.t_arrival for r in done if cls_done[r.rid] == c] or [0.0]))
-                 for c in ("small", "medium", "large")} if not args.serve_native else None
-        runs = {c: float(np.median([r.t_done - r.t_start for r in done if cls_done[r.rid] == c] or [0.0]))
-                for c in ("small", "medium", "large")} if not args.serve_native else None
-        per_class = {}
-        for c in ("small", "medium", "large"):
-            ls = np.array([r.t_done - r.t_arrival for r in done if cls_done[r.rid] == c])
-            fs = np.array([fc[k] for k in range(len(arr)) if picks[k][0] == c])
-            per_class[c] = {"submitted": int(sum(p[0] == c for p in picks)), "completed": int(len(ls)),
-                            "p50_s": float(np.percentile(ls, 50)) if len(ls) else None,
-                            "p99_s": float(np.percentile(ls, 99)) if len(ls) else None,
-                            "fcfs_p50_s": float(np.percentile(fs, 50)) if len(fs) else None,
-                            "fcfs_p99_s": float(np.percentile(fs, 99)) if len(fs) else None}
-        span = max((r.t_done for r in done), default=t0) - t0
-        pt = {"qps": qps, "submitted": int(len(arr)), "completed": len(done), "optimal": int(ok),
-              "served_lps_per_s": len(done) / max(span, 1e-9),
-              "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
-              "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
-              "fcfs_lps_per_s": len(arr) / float((arr + fc).max()) if len(arr) else None,
-              "fcfs_p50_s": float(np.percentile(fc, 50)) if len(fc) else None,
-              "fcfs_p99_s": float(np.percentile(fc, 99)) if len(fc) else None,
-              "wall_s": t_end - t0, "per_class": per_class,
-              "batch_rounds": srv.rounds, "round_s_median": float(np.median(rs)), "round_s_max": float(rs.max()),
-              "median_wait_to_start_s": waits, "median_start_to_done_s": runs}
-        log(f"[bench] qps={qps}: {pt}")
+        if args.serve_native:
+            done = srv.done
+        else:   # the Python scheduler keeps absolute times
+            done = [NativeDone(r.rid, r.t_arrival - t0, r.t_done - t0, r.status, r.objective) for r in srv.done]
+            for r in srv.done:
+                if r.status == "OPTIMAL":
+                    g = r.lp
+                    max_obj_err = max(max_obj_err, abs(r.objective - g.objective) / (1 + abs(g.objective)))
+        server = _served(done)
+        server["per_class"] = _class_stats(done, picks)
+        server["max_objective_rel_err"] = max_obj_err
+        log(f"[bench] qps={qps} server: {server}")
+        # ---- the baseline: one request at a time, first come first served
+        if args.serve_fcfs == "run":
+            fdone = []
+            t0 = time.perf_counter()
+            j = 0
+            while j < len(arr):
+                now = time.perf_counter() - t0
+                if now > args.serve_duration + args.serve_drain:
+                    break
+                if arr[j] > now:
+                    time.sleep(min(arr[j] - now, 0.01))
+                    continue
+                g = pool[picks[j][0]][picks[j][1]]
+                _, st = plain_solve(g)
+                fdone.append(NativeDone(j, arr[j], time.perf_counter() - t0, st.termination_status.name, 0.0))
+                j += 1
+            fcfs = _served(fdone)
+            fcfs["per_class"] = _class_stats(fdone, picks)
+            fcfs["mode"] = "run"
+        elif args.serve_fcfs == "replay":
+            fl = fcfs_latencies(arr, np.array([service[p] for p in picks]))
+            fdone = [NativeDone(j, arr[j], arr[j] + fl[j], "OPTIMAL", 0.0) for j in range(len(arr))
+                     if arr[j] + fl[j] <= args.serve_duration + args.serve_drain]
+            fcfs = _served(fdone)
+            fcfs["per_class"] = _class_stats(fdone, picks)
+            fcfs["mode"] = "replay of measured solve times"
+        else:
+            fcfs = None
+        if fcfs is not None:
+            log(f"[bench] qps={qps} fcfs: {fcfs}")
+        pt = {"qps": qps, "offered": int(len(arr)), "server": server, "fcfs": fcfs,
+              "served_over_offered": server["served_lps_per_s"] / qps,
+              "speedup_over_fcfs": (server["served_lps_per_s"] / fcfs["served_lps_per_s"])
+              if fcfs and fcfs["served_lps_per_s"] > 0 else None}
         out_points.append(pt)
-    best = max(out_points, key=lambda p: p["served_lps_per_s"])
+    best = max(out_points, key=lambda p: p["server"]["served_lps_per_s"])
+    tracking = [p["qps"] for p in out_points if p["served_over_offered"] >= 0.9]
     return {
         "metric": "solve time to rel-KKT 1e-6; batched LPs/s + p50/p99 vs plain PDLP, 1 GPU",
-        "value": best["served_lps_per_s"], "unit": "LPs/s", "n_gpus": 1, "steps": len(out_points),
-        "warmup": 0, "ms_per_step": 1e3 * args.serve_duration, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 bulk / f64 checks+polish",
+        "value": best["server"]["served_lps_per_s"], "unit": "LPs/s (OPTIMAL results)", "n_gpus": 1,
+        "steps": len(out_points), "warmup": 0, "ms_per_step": 1e3 * args.serve_duration,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 bulk / f64 checks+polish",
         "data": "synthetic (seeded request pool with known optima)",
         "config": {"workload": "serving_poisson_mix", "qps": args.qps, "duration_s": args.serve_duration,
-                   "medium_workers": args.serve_medium_workers, "overflow_queue": args.serve_overflow,
-                   "batch_limit": args.serve_batch_limit,
+                   "drain_s": args.serve_drain, "medium_workers": args.serve_medium_workers,
+                   "overflow_queue": args.serve_overflow, "batch_limit": args.serve_batch_limit,
                    "server": "library (cpdlp_server_*)" if args.serve_native else "python scheduler",
+                   "fcfs": args.serve_fcfs, "powm_tol": args.powm_tol,
                    "mix": "70/25/5 small/medium/large", "reltol": reltol,
                    "pool": {k: len(v) for k, v in pool.items()}},
+        "knee_qps": max(tracking) if tracking else None,
         "points": out_points,
     }
 
@@ -725,16 +782,18 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--checks-per-round", type=int, default=20)
     ap.add_argument("--plain-sample", type=int, default=16)
-    ap.add_argument("--qps", type=float, nargs="+", default=[100.0, 1000.0])
-    ap.add_argument("--serve-duration", type=float, default=20.0)
-    ap.add_argument("--serve-drain", type=float, default=60.0)
+    ap.add_argument("--qps", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 3000.0, 5000.0])
+    ap.add_argument("--serve-duration", type=float, default=60.0)
+    ap.add_argument("--serve-drain", type=float, default=30.0)
+    ap.add_argument("--serve-fcfs", default="run", choices=["run", "replay", "none"],
+                    help="the one-at-a-time baseline: run it on the same arrivals, model it, or skip it")
     ap.add_argument("--serve-checks", type=int, default=20)
     ap.add_argument("--serve-batch-gib", type=int, default=16)
     ap.add_argument("--serve-medium-workers", type=int, default=2)
     ap.add_argument("--serve-overflow", type=int, default=0)
     ap.add_argument("--serve-batch-limit", type=int, default=8192)
-    ap.add_argument("--serve-native", action="store_true",
-                    help="serve with the library's own server (cpdlp_server_*) instead of the Python scheduler")
+    ap.add_argument("--serve-python", dest="serve_native", action="store_false",
+                    help="serve with the Python scheduler instead of the library's own server (cpdlp_server_*)")
     ap.add_argument("--pool-small", type=int, default=48)
     ap.add_argument("--pool-medium", type=int, default=16)
     ap.add_argument("--pool-large", type=int, default=3)

@@ -86,11 +86,15 @@ class Server:
     # opt-in: infeasible / unbounded / NaN requests stop at the check that proves it
     # (PRIMAL_INFEASIBLE, DUAL_INFEASIBLE, NUMERICAL_ERROR) instead of at a limit
     detect_infeasibility: bool = False
+    # power-method tolerance of every request (NaN: the reference's eps(T)*n^3 rule, which
+    # stops after one Rayleigh quotient for n >~ 3e5 and then diverges)
+    powm_tol: float = float("nan")
     done: list = field(default_factory=list)
 
     def __post_init__(self):
         algo = PDLP(self.precision, termination_reltol=self.reltol, time_limit=self.time_limit,
-                    batch_streams=self.batch_streams, detect_infeasibility=int(self.detect_infeasibility))
+                    batch_streams=self.batch_streams, detect_infeasibility=int(self.detect_infeasibility),
+                    powm_tol=self.powm_tol)
         self.batch = BatchSolver(algo, hbm_bytes=self.batch_bytes)
         self.waiting: deque = deque()
         self.in_batch: dict = {}
@@ -135,7 +139,8 @@ class Server:
             milp = MILP.from_generated(r.lp)
             # fixed CSR format: a request's result must not depend on autotuner timing
             algo = PDLP(self.precision, None, "csr", termination_reltol=self.reltol,
-                        time_limit=self.time_limit, detect_infeasibility=int(self.detect_infeasibility))
+                        time_limit=self.time_limit, detect_infeasibility=int(self.detect_infeasibility),
+                        powm_tol=self.powm_tol)
             with Engine(milp, algo) as eng:
                 sol, st = eng.solve(r.sol)
             r.status = st.termination_status.name
