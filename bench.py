@@ -108,8 +108,9 @@ def polish(args) -> dict:
 
 def bench_batch(args):
     """4,096 uneven LPs in one batch (one workgroup per LP, mixed precision, reltol 1e-6
-    per LP) against the same LPs solved one at a time on the plain GPU path (timed on a
-    stratified sample and scaled to all LPs)."""
+    per LP) against the same LPs solved one at a time on the plain GPU path (all of them
+    by default; --plain-sample N times a stratified sample and scales it, marked
+    projected)."""
     import numpy as np
     import torch
     import coolpdlp_hip as cp
@@ -150,10 +151,13 @@ def bench_batch(args):
                         for g, (sol, st) in zip(lps, res)])
     passes = np.array([st.kkt_passes for _, st in res])
     powm = np.array([st.powm_iters for _, st in res])
-    # plain one-at-a-time GPU path on a stratified sample (sorted by size, evenly spaced)
+    # plain one-at-a-time GPU path: every LP (the default), or a stratified sample
+    # (sorted by size, evenly spaced) whose mean is scaled to all LPs
     order = np.argsort([g.nnz for g in lps])
-    sample = order[np.linspace(0, len(lps) - 1, args.plain_sample).astype(int)]
-    plain_t = []
+    n_plain = len(lps) if args.plain_sample <= 0 else min(args.plain_sample, len(lps))
+    sample = order[np.linspace(0, len(lps) - 1, n_plain).astype(int)] if n_plain < len(lps) else order
+    plain_t, plain_status, plain_err = [], [], []
+    t_plain0 = time.perf_counter()
     for k in sample:
         g = lps[k]
         milp = cp.MILP.from_generated(g)
@@ -161,10 +165,15 @@ def bench_batch(args):
                                       **polish(args)))
         torch.cuda.synchronize()
         t = time.perf_counter()
-        _, st = eng.solve()
+        sol_p, st = eng.solve()
         plain_t.append(time.perf_counter() - t)
         eng.close()
-    plain_total = float(np.mean(plain_t)) * len(lps)
+        plain_status.append(st.termination_status.name)
+        plain_err.append(abs(float(sol_p.x @ g.c) - g.objective) / (1 + abs(g.objective)))
+        if len(plain_t) % 512 == 0:
+            log(f"[bench] plain path: {len(plain_t)}/{len(sample)} LPs in {time.perf_counter() - t_plain0:.1f}s")
+    measured = len(sample) == len(lps)
+    plain_total = float(np.sum(plain_t)) if measured else float(np.mean(plain_t)) * len(lps)
     value = len(lps) / elapsed
     return {
         "metric": "solve time to rel-KKT 1e-6; batched LPs/s + p50/p99 vs plain PDLP, 1 GPU",
@@ -176,7 +185,8 @@ def bench_batch(args):
                    "m_range": [50, 5000], "reltol": reltol, "precision": "mixed",
                    "polish_reltol": algo.params.polish_reltol,
                    "detect_infeasibility": bool(args.detect_infeasibility),
-                   "checks_per_round": args.checks_per_round},
+                   "checks_per_round": args.checks_per_round,
+                   "timed_region": "batch (LPs admitted beforehand)"},
         "batch_seconds": elapsed, "rounds": rounds,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
         "optimal": int(sum(s == "OPTIMAL" for s in status)), "count": len(lps),
@@ -186,9 +196,18 @@ def bench_batch(args):
         "max_objective_rel_err": float(rel_obj.max()), "median_kkt_passes": float(np.median(passes)),
         "mean_kkt_passes": float(np.mean(passes)), "fp32_pass_share": float(np.sum([st.fp32_passes for _, st in res]) / max(1, passes.sum())),
         "powm_iters_median": float(np.median(powm)), "powm_iters_max": int(powm.max()),
-        "plain_gpu": {"sample": int(len(sample)), "mean_seconds_per_lp": float(np.mean(plain_t)),
-                      "projected_total_seconds": plain_total, "lps_per_s": len(lps) / plain_total},
+        "plain_gpu": {"lps_solved": int(len(sample)), "measured_over_all_lps": measured,
+                      "mean_seconds_per_lp": float(np.mean(plain_t)),
+                      ("total_seconds" if measured else "projected_total_seconds"): plain_total,
+                      "lps_per_s": len(lps) / plain_total,
+                      "optimal": int(sum(x == "OPTIMAL" for x in plain_status)),
+                      "max_objective_rel_err": float(max(plain_err)),
+                      "wall_seconds_incl_engine_creation": time.perf_counter() - t_plain0},
         "speedup_vs_plain": plain_total / elapsed,
+        # the tail: the last instances to finish (one workgroup each) set the batch's time
+        "last_finished": [{"m": lps[i].m, "n": lps[i].n, "nnz": lps[i].nnz, "kkt_passes": int(passes[i]),
+                           "fp32_passes": int(res[i][1].fp32_passes), "finished_s": float(lat[i])}
+                          for i in np.argsort(lat)[-8:]],
     }
 
 
@@ -781,7 +800,8 @@ def main():
                     help="A,At formats for an auto run (sets CPDLP_FORMATS; for counter runs)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--checks-per-round", type=int, default=20)
-    ap.add_argument("--plain-sample", type=int, default=16)
+    ap.add_argument("--plain-sample", type=int, default=0,
+                    help="batch: LPs solved one at a time for the comparison (0: all of them)")
     ap.add_argument("--qps", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 3000.0, 5000.0])
     ap.add_argument("--serve-duration", type=float, default=60.0)
     ap.add_argument("--serve-drain", type=float, default=30.0)

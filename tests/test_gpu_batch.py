@@ -278,3 +278,4 @@ def test_batch_add_beyond_the_whole_arena(cp):
             assert abs(float(sol.x @ small.c) - small.objective) <= 20e-6 * (1 + abs(small.objective))
             B.release(i)
         assert B.memory()[0] == 0
+
