@@ -459,6 +459,7 @@ static int64_t wrap_cube(int64_t n) {
 }
 
 Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
+    no_rst_cache = std::getenv("CPDLP_KKT_FULL_CHECK") != nullptr;
     if (P.check_every <= 0) throw Error(CPDLP_ERR_ARGUMENT, "check_every must be positive");
     if (P.precision < 0 || P.precision > 2) throw Error(CPDLP_ERR_ARGUMENT, "unknown precision");
     if (P.format < CPDLP_FMT_AUTO || P.format > CPDLP_FMT_XSLICE_ALL)
@@ -1080,7 +1081,12 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_
         // check: KKT of sol (and, for PDLP, avg, sol_last, avg_last, restart)
         const int c = S.cur, l = 1 - S.cur;
         int K = 1;
-        if (pdlp) {
+        if (pdlp && rst_err_valid && !no_rst_cache) {   // the restart point's errors are known
+            const V* X[4] = {S.x[c].p, S.xavg.p, S.x[l].p, S.xavgl.p};
+            const V* Y[4] = {S.y[c].p, S.yavg.p, S.y[l].p, S.yavgl.p};
+            kkt_launch<V, 4>(X, Y, S.xrst.p, S.yrst.p, true, sums);
+            K = 4;
+        } else if (pdlp) {
             const V* X[5] = {S.x[c].p, S.xavg.p, S.x[l].p, S.xavgl.p, S.xrst.p};
             const V* Y[5] = {S.y[c].p, S.yavg.p, S.y[l].p, S.yavgl.p, S.yrst.p};
             kkt_launch<V, 5>(X, Y, S.xrst.p, S.yrst.p, true, sums);
@@ -1147,7 +1153,11 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_
         // restart_check!, pdlp.jl:138-170 and should_restart, restart.jl:40-57
         const cpdlp_kkt e_s = stats.err, e_a = kkt_from_sums(sums, K, 1);
         const cpdlp_kkt e_l = kkt_from_sums(sums, K, 2), e_al = kkt_from_sums(sums, K, 3);
-        const cpdlp_kkt e_r = kkt_from_sums(sums, K, 4);
+        // the restart point's errors: computed when K == 5, else those kept (the same
+        // vector, evaluated by the same per-vector sums, so the same bits)
+        const cpdlp_kkt e_r = K == 5 ? kkt_from_sums(sums, K, 4) : rst_err;
+        rst_err = e_r;
+        rst_err_valid = true;
         const bool from_avg = !(absolute(e_s, omega) < absolute(e_a, omega));
         const cpdlp_kkt cand = from_avg ? e_a : e_s;
         const cpdlp_kkt cand_last = absolute(e_l, omega) < absolute(e_al, omega) ? e_l : e_al;
@@ -1167,6 +1177,7 @@ void Handle::loop(IterState<V>& S, double phase_reltol, bool final_phase, int64_
             const double th = P.primal_weight_damping;
             if (dx > P.zero_tol && dy > P.zero_tol)
                 omega = std::exp(th * std::log(dy / dx) + (1.0 - th) * std::log(omega));
+            rst_err = cand;   // the restart point is now the candidate
             hipLaunchKernelGGL(k_restart<V>, dim3(grid_n()), dim3(BLOCK), 0, stream, (int64_t)n,
                                (int32_t)from_avg, S.x[c].p, S.xavg.p, S.xrst.p);
             hipLaunchKernelGGL(k_restart<V>, dim3(grid_m()), dim3(BLOCK), 0, stream, (int64_t)m,
@@ -1196,6 +1207,7 @@ void Handle::progress(bool final_line) {
 template <class V>
 void Handle::init_state(IterState<V>& S, const double* x0_dev, const double* y0_dev) {
     const hipStream_t s = stream;
+    rst_err_valid = false;
     S.alloc(n, m);
     const int gn = grid_n(), gm = grid_m();
     const int32_t nn = (int32_t)n, mm = (int32_t)m;
@@ -1356,6 +1368,7 @@ void Handle::solve(const double* x0, const double* y0) {
         const bool reached = stats.termination_status == CPDLP_OPTIMAL && r32 > P.termination_reltol;
         if (polish && (reached || phase_switch)) {
             st64.alloc(n, m);
+            rst_err_valid = false;   // the restart point in FP64 has FP64 errors
             if (blowup) {   // restart from the best FP32 iterate with its primal weight
                 omega = best_omega;
                 upcast(st32, st64, n, m, stream);

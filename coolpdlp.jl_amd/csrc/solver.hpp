@@ -95,6 +95,12 @@ struct Handle {
     double best_rel = INFINITY, best_omega = 1.0, last_progress = 0.0;
     int stall = 0;
     bool phase_switch = false, blowup = false;
+    // KKT errors of the restart point (PDLP): it changes only at a restart, where it becomes
+    // the candidate, so the check evaluates 4 candidates and reuses these; invalid after
+    // the start and after a change of precision
+    cpdlp_kkt rst_err{};
+    bool rst_err_valid = false;
+    bool no_rst_cache = false;   // CPDLP_KKT_FULL_CHECK set: evaluate all 5 every check
     IterState<double> st64;
     IterState<float> st32;
     cpdlp_stats stats{};

@@ -420,3 +420,31 @@ def test_full_size_xslice_tracks_csr(cp, large_lp, fmt, code):
     for (p1, e1), (p2, e2) in zip(s_csr.error_history, s_xs.error_history):
         assert p1 == p2
         assert abs(e1.relative() - e2.relative()) <= 1e-8 * e1.relative(), (p1, e1, e2)
+
+
+@pytest.mark.parametrize("precision,fmt,which", [("float64", "csr", "column"), ("mixed", "csr", "column"),
+                                                 ("float64", "xslice", "power_law")])
+def test_restart_error_cache_keeps_the_bits(cp, precision, fmt, which):
+    """The check reuses the restart point's KKT errors between restarts (4 candidates
+    evaluated instead of 5): the whole error history, restarts, pass count and solution
+    are the same bits as evaluating all 5 at every check (CPDLP_KKT_FULL_CHECK)."""
+    import os
+    g = (G.column_lp(400, 1000, 6, seed=23, box_frac=0.3) if which == "column"
+         else G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4))
+    milp = cp.MILP.from_generated(g)
+    kw = dict(termination_reltol=1e-7, powm_tol=1e-6, max_kkt_passes=20_000)
+    runs = []
+    for full in (False, True):
+        if full:
+            os.environ["CPDLP_KKT_FULL_CHECK"] = "1"
+        try:
+            runs.append(cp.solve(milp, cp.PDLP(precision, None, fmt, **kw)))
+        finally:
+            os.environ.pop("CPDLP_KKT_FULL_CHECK", None)
+    (s1, t1), (s2, t2) = runs
+    assert t1.restarts >= 2 and t1.restarts == t2.restarts
+    assert t1.kkt_passes == t2.kkt_passes and t1.termination_status == t2.termination_status
+    assert len(t1.error_history) == len(t2.error_history)
+    for (p1, e1), (p2, e2) in zip(t1.error_history, t2.error_history):
+        assert p1 == p2 and e1 == e2, (p1, e1, e2)
+    assert np.array_equal(s1.x, s2.x) and np.array_equal(s1.y, s2.y)
