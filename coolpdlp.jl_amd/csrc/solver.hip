@@ -835,7 +835,8 @@ void Handle::kkt_launch(const V* const* X, const V* const* Y, const V* xr, const
                         double* host_sums) {
     const int gA = A.view<V>().partial_blocks(), gAt = At.view<V>().partial_blocks();
     constexpr int NQR = EpiKktRows<V, K>::NQ, NQC = EpiKktCols<V, K>::NQ;
-    ensure_partials((size_t)std::max(gA * NQR, gAt * NQC) + 16);
+    // (the 4-candidate rows side may run as two passes of 2: 2 * 6 partials per block)
+    ensure_partials((size_t)std::max(std::max(gA * NQR, 2 * gA * EpiKktRows<V, 2>::NQ), gAt * NQC) + 16);
     kkt_launch_into<V, K>(X, Y, xr, yr, dist, host_sums, d_partials.p, d_partials2.p, d_sums.p);
 }
 
@@ -846,17 +847,42 @@ void Handle::kkt_launch_into(const V* const* X, const V* const* Y, const V* xr, 
     const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
     const int gA = vA.partial_blocks(), gAt = vAt.partial_blocks();
     constexpr int NQR = EpiKktRows<V, K>::NQ, NQC = EpiKktCols<V, K>::NQ;
-    EpiKktRows<V, K> er;
-    er.partials = partials;
-    for (int k = 0; k < K; ++k) er.Y[k] = Y[k];
-    er.y_rst = yr ? yr : Y[0];
-    er.lc = lc_s.p;
-    er.uc = uc_s.p;
-    er.invD1 = invD1_s.p;
-    er.dist = dist;
-    XPack<V, K> xp;
-    for (int k = 0; k < K; ++k) xp.p[k] = X[k];
-    launch_spmv<V, double, K>(vA, xp, er, s);
+    // The rows side gathers the K candidates' x through the column slices: K * n / ns
+    // elements per XCD.  With 4 candidates beyond an XCD's 4 MiB L2 it runs as two passes
+    // of 2 (each vector's sums are the same numbers in either form).
+    bool split = false;
+    if constexpr (K == 4)
+        split = vA.format == CPDLP_FMT_XSLICE && vA.ns > 0 &&
+                (size_t)n * sizeof(V) * K / (size_t)vA.ns > ((size_t)4 << 20);
+    if (split) {
+        if constexpr (K == 4) {
+            constexpr int NQ2 = EpiKktRows<V, 2>::NQ;
+            for (int half = 0; half < 2; ++half) {
+                EpiKktRows<V, 2> er;
+                er.partials = partials + (size_t)half * gA * NQ2;
+                er.Y[0] = Y[2 * half];
+                er.Y[1] = Y[2 * half + 1];
+                er.y_rst = yr ? yr : Y[0];
+                er.lc = lc_s.p;
+                er.uc = uc_s.p;
+                er.invD1 = invD1_s.p;
+                er.dist = half == 0 ? dist : false;
+                launch_spmv<V, double, 2>(vA, XPack<V, 2>{{X[2 * half], X[2 * half + 1]}}, er, s);
+            }
+        }
+    } else {
+        EpiKktRows<V, K> er;
+        er.partials = partials;
+        for (int k = 0; k < K; ++k) er.Y[k] = Y[k];
+        er.y_rst = yr ? yr : Y[0];
+        er.lc = lc_s.p;
+        er.uc = uc_s.p;
+        er.invD1 = invD1_s.p;
+        er.dist = dist;
+        XPack<V, K> xp;
+        for (int k = 0; k < K; ++k) xp.p[k] = X[k];
+        launch_spmv<V, double, K>(vA, xp, er, s);
+    }
     EpiKktCols<V, K> ec;
     ec.partials = partials2;
     for (int k = 0; k < K; ++k) ec.X[k] = X[k];
@@ -869,7 +895,13 @@ void Handle::kkt_launch_into(const V* const* X, const V* const* Y, const V* xr, 
     XPack<V, K> yp;
     for (int k = 0; k < K; ++k) yp.p[k] = Y[k];
     launch_spmv<V, double, K>(vAt, yp, ec, s);
-    if (gA > 0)
+    if (split) {   // two passes' sums, reassembled below into the K = 4 layout
+        constexpr int NQ2 = EpiKktRows<V, 2>::NQ;
+        for (int half = 0; half < 2; ++half)
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gA, NQ2,
+                               partials + (size_t)half * gA * NQ2, sums_dev + NQR + NQC + half * NQ2,
+                               (PowState*)nullptr, POW_NONE);
+    } else if (gA > 0)
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(BLOCK), 0, s, gA, NQR, partials,
                            sums_dev, (PowState*)nullptr, POW_NONE);
     else
@@ -880,8 +912,26 @@ void Handle::kkt_launch_into(const V* const* X, const V* const* Y, const V* xr, 
     else
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, s, (int64_t)NQC, sums_dev + NQR, 0.0);
     HIP_CHECK(hipGetLastError());
-    if (NQR + NQC) HIP_CHECK(hipMemcpyAsync(host_sums, sums_dev, (NQR + NQC) * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (!split) {
+        if (NQR + NQC) HIP_CHECK(hipMemcpyAsync(host_sums, sums_dev, (NQR + NQC) * sizeof(double), hipMemcpyDeviceToHost, s));
+        sync();
+        return;
+    }
+    // rows layout of K = 4: the 2 sums of each candidate (first pass: 0, 1; second: 2, 3),
+    // then the 2 distances (first pass)
+    constexpr int NQ2 = EpiKktRows<V, 2>::NQ;
+    double tmp[64];
+    HIP_CHECK(hipMemcpyAsync(tmp, sums_dev, (NQR + NQC + 2 * NQ2) * sizeof(double), hipMemcpyDeviceToHost, s));
     sync();
+    const double* a = tmp + NQR + NQC;
+    const double* b = a + NQ2;
+    for (int q = 0; q < 4; ++q) {
+        host_sums[q] = a[q];
+        host_sums[4 + q] = b[q];
+    }
+    host_sums[8] = a[4];
+    host_sums[9] = a[5];
+    for (int q = 0; q < NQC; ++q) host_sums[NQR + q] = tmp[NQR + q];
 }
 
 // errors.jl:77-98 from the raw sums of candidate k

@@ -423,16 +423,20 @@ def test_full_size_xslice_tracks_csr(cp, large_lp, fmt, code):
 
 
 @pytest.mark.parametrize("precision,fmt,which", [("float64", "csr", "column"), ("mixed", "csr", "column"),
-                                                 ("float64", "xslice", "power_law")])
+                                                 ("float64", "xslice", "power_law"),
+                                                 ("float64", "xslice", "wide")])
 def test_restart_error_cache_keeps_the_bits(cp, precision, fmt, which):
     """The check reuses the restart point's KKT errors between restarts (4 candidates
-    evaluated instead of 5): the whole error history, restarts, pass count and solution
-    are the same bits as evaluating all 5 at every check (CPDLP_KKT_FULL_CHECK)."""
+    evaluated instead of 5), and with n beyond 1M sliced columns runs its rows side as two
+    passes of 2 candidates ("wide": 4 x 8 B x n / 8 slices > 4 MiB per XCD): the whole error
+    history, restarts, pass count and solution are the same bits as evaluating all 5 in
+    one pass at every check (CPDLP_KKT_FULL_CHECK)."""
     import os
     g = (G.column_lp(400, 1000, 6, seed=23, box_frac=0.3) if which == "column"
-         else G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4))
+         else G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4) if which == "power_law"
+         else G.power_law_lp(20_000, 1_100_000, 3_000_000, cap=40_000, seed=6))
     milp = cp.MILP.from_generated(g)
-    kw = dict(termination_reltol=1e-7, powm_tol=1e-6, max_kkt_passes=20_000)
+    kw = dict(termination_reltol=1e-7, powm_tol=1e-6, max_kkt_passes=20_000 if which != "wide" else 4_000)
     runs = []
     for full in (False, True):
         if full:
