@@ -632,6 +632,10 @@ __global__ __launch_bounds__(NSUB * BB) void k_batch_setup(const int32_t* list, 
         I.y64[0][i] = I.ya64[i] = I.yr64[i] = ys;
         I.y64[1][i] = I.yal64[i] = 0.0;
     }
+    // publish: every thread's writes are in this XCD's L2 after the barrier; thread 0 writes
+    // the state, writes the L2 back (agent-scope fence) and only then sets the flag, so the
+    // host may take the instance up while the rest of the launch still runs
+    __syncthreads();
     if (tid == 0) {
         S->eta = eta;
         S->omega = omega;
@@ -647,7 +651,8 @@ __global__ __launch_bounds__(NSUB * BB) void k_batch_setup(const int32_t* list, 
         S->phase = P.precision == CPDLP_PREC_FP64 ? 1 : 0;
         S->cur = 0;
         S->stall = 0;
-        S->setup = 1;
+        __threadfence();
+        __hip_atomic_store(&S->setup, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1582,6 +1587,21 @@ struct Batch {
             pending_setup.clear();
             pending_ids.clear();
         }
+        // instances of setup launches still running become ready one by one: each
+        // publishes its state and then its setup flag (k_batch_setup), so a slow power
+        // method holds back only its own instance
+        if (!setups.empty() && hi_slot > 0) {
+            d_state.download(h_state.data(), hi_slot, streams[0]);
+            HIP_CHECK(hipStreamSynchronize(streams[0]));
+            for (const SetupChunk& c : setups)
+                for (size_t q = 0; q < c.ids.size(); ++q) {
+                    auto it = inst.find(c.ids[q]);
+                    if (it != inst.end() && !it->second.ready && h_state[c.slots[q]].setup == 1) {
+                        it->second.ready = true;
+                        newly_ready = true;
+                    }
+                }
+        }
         bool any_ready = false;
         for (auto& kv : inst)
             if (kv.second.ready && !kv.second.finished) {
@@ -1589,7 +1609,8 @@ struct Batch {
                 break;
             }
         if (!any_ready && !setups.empty()) retire_done(true);
-        if (newly_ready && hi_slot > 0) {   // their set-up states (phase, eta, ...) to the host
+        if (newly_ready && hi_slot > 0) {   // their set-up states (phase, eta, ...) to the host,
+                                            // read after their flags: whole
             d_state.download(h_state.data(), hi_slot, streams[0]);
             HIP_CHECK(hipStreamSynchronize(streams[0]));
         }
