@@ -432,9 +432,10 @@ def bench_serve(args):
         lanes = ((1_000_000, args.serve_medium_workers), (float("inf"), 1))
         # ---- the server (continuous batching)
         if args.serve_native:   # the library's own server (cpdlp_server_*)
-            srv = NativeServer(cp.PDLP("mixed", batch_streams=1, **kw),   # + 3 plain workers: 4 queues
+            srv = NativeServer(cp.PDLP("mixed", batch_streams=1, **kw),   # per batch route
                                batch_bytes=args.serve_batch_gib << 30, checks_per_round=args.serve_checks,
-                               lanes=lanes, batch_limit=args.serve_batch_limit)
+                               lanes=lanes, batch_limit=args.serve_batch_limit,
+                               batch2_limit=args.serve_batch2_limit, batch2_bytes=int(args.serve_batch2_gib * 2**30))
             srv.done, srv.rounds, srv.round_seconds, meta = [], 0, [], {}
             srv.start()           # the library pumps on its own thread
         else:
@@ -535,6 +536,8 @@ def bench_serve(args):
         "config": {"workload": "serving_poisson_mix", "qps": args.qps, "duration_s": args.serve_duration,
                    "drain_s": args.serve_drain, "medium_workers": args.serve_medium_workers,
                    "overflow_queue": args.serve_overflow, "batch_limit": args.serve_batch_limit,
+                   "batch2_limit": args.serve_batch2_limit if args.serve_native else 0,
+                   "batch2_gib": args.serve_batch2_gib,
                    "server": "library (cpdlp_server_*)" if args.serve_native else "python scheduler",
                    "fcfs": args.serve_fcfs, "powm_tol": args.powm_tol,
                    "mix": "70/25/5 small/medium/large", "reltol": reltol,
@@ -761,6 +764,7 @@ def bench_single(args):
         "objective": obj, "objective_closed_form": g.objective,
         "objective_rel_err": abs(obj - g.objective) / (1 + abs(g.objective)),
         "formats": {"A": FORMATS[st.format_A], "At": FORMATS[st.format_At]},
+        "row_order": os.environ.get("CPDLP_ORDER", "first_touch"),
         "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
         "powm_tol": args.powm_tol, "eta_reference_rule": eta_ref,
         "roofline": roofline, "gather_roofline": gather_roofline, "cpu_baseline": cpu,
@@ -809,7 +813,10 @@ def main():
                     help="the one-at-a-time baseline: run it on the same arrivals, model it, or skip it")
     ap.add_argument("--serve-checks", type=int, default=20)
     ap.add_argument("--serve-batch-gib", type=int, default=16)
-    ap.add_argument("--serve-medium-workers", type=int, default=2)
+    ap.add_argument("--serve-medium-workers", type=int, default=1)
+    ap.add_argument("--serve-batch2-limit", type=int, default=65536,
+                    help="m + n limit of the library server's second batch route (0: off)")
+    ap.add_argument("--serve-batch2-gib", type=float, default=2.0)
     ap.add_argument("--serve-overflow", type=int, default=0)
     ap.add_argument("--serve-batch-limit", type=int, default=8192)
     ap.add_argument("--serve-python", dest="serve_native", action="store_false",

@@ -64,6 +64,7 @@ class CServerConfig(C.Structure):
         ("batch_bytes", C.c_int64), ("plain_bytes", C.c_int64), ("batch_limit", C.c_int32),
         ("checks_per_round", C.c_int32), ("n_lanes", C.c_int32),
         ("lane_workers", C.c_int32 * SERVER_MAX_LANES), ("lane_max_nnz", C.c_double * SERVER_MAX_LANES),
+        ("batch2_bytes", C.c_int64), ("batch2_limit", C.c_int32), ("batch2_checks", C.c_int32),
     ]
 
 

@@ -226,23 +226,29 @@ class Server:
 class NativeServer:
     """The same continuous batching as `Server`, run by the library itself
     (cpdlp_server_*): requests are copied at submit; small ones (m + n <= batch_limit)
-    go to the batch engine, advanced by `pump`; larger ones to plain engines on the
-    library's worker threads, in lanes of (max nonzeros, workers), shortest job first,
-    under an HBM budget.  This is the interface a C or Julia caller uses."""
+    go to the batch engine, medium ones (m + n <= batch2_limit) to a second batch engine
+    with rounds of its own, both advanced by `pump` (or the library's threads after
+    `start`); larger ones to plain engines on the library's worker threads, in lanes of
+    (max nonzeros, workers), shortest job first, under an HBM budget.  batch2_limit <=
+    batch_limit turns the second batch route off.  This is the interface a C or Julia
+    caller uses."""
 
     def __init__(self, algo=None, *, batch_bytes: int = 16 << 30, plain_bytes: int = 64 << 30,
                  batch_limit: int = 8192, checks_per_round: int = 20,
-                 lanes=((1_000_000, 2), (float("inf"), 1))):
+                 lanes=((1_000_000, 1), (float("inf"), 1)),
+                 batch2_limit: int = 65536, batch2_bytes: int = 2 << 30, batch2_checks: int = 0):
         import ctypes as C
         from . import _lib
         self._C, self._lib = C, _lib
-        # one batch stream beside the plain workers' streams keeps the total within the
-        # device's 4 hardware queues (as Server does)
+        # one stream per batch route beside the plain workers' streams keeps the total
+        # within the device's 4 hardware queues
         self.algo = algo if algo is not None else PDLP("mixed", batch_streams=1)
         cfg = _lib.CServerConfig()
         _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(cfg)))
         cfg.batch_bytes, cfg.plain_bytes = int(batch_bytes), int(plain_bytes)
         cfg.batch_limit, cfg.checks_per_round = int(batch_limit), int(checks_per_round)
+        cfg.batch2_limit, cfg.batch2_bytes = int(batch2_limit), int(batch2_bytes)
+        cfg.batch2_checks = int(batch2_checks)
         if not 1 <= len(lanes) <= _lib.SERVER_MAX_LANES:
             raise _lib.ArgumentError(f"1 to {_lib.SERVER_MAX_LANES} lanes")
         cfg.n_lanes = len(lanes)

@@ -10,7 +10,7 @@
 //   * the internal layouts used by the kernels: rows of A (resp. A') longer than
 //     `long_row` are moved to the front by a stable partition, and column indices are
 //     relabelled through the other side's partition; the short rows follow in
-//     first-touch order (build_layout).  Entries inside each row keep their original
+//     first-touch order (build_layout; density or the caller's order on request).  Entries inside each row keep their original
 //     order, so the sequential row sums are unchanged.
 #pragma once
 #include <algorithm>
@@ -169,8 +169,15 @@ inline void build_side(HostSide& S, int32_t rows, int32_t cols, const std::vecto
     build_tiles(S.rowptr.data(), S.n_long, S.rows, S.tiles);
 }
 
+// Order of the short rows and columns: first touch (the default, below), density
+// (increasing stored-entry count, stable: the order sort_rows_columns gives the whole MILP,
+// src/components/permutation.jl:16-46) or the caller's own order.  Long rows and columns
+// come first, longest first, in every order.
+enum class RowOrder { first_touch = 0, density = 1, given = 2 };
+
 inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
-                               std::vector<int32_t>&& rv, int32_t long_row) {
+                               std::vector<int32_t>&& rv, int32_t long_row,
+                               RowOrder order = RowOrder::first_touch) {
     HostLayout L;
     L.m = m;
     L.n = n;
@@ -193,13 +200,25 @@ inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
                 first[idx[k]] = std::min(first[idx[k]], order_of_outer[o]);
         return first;
     };
-    partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt);
-    std::vector<int32_t> key = first_touch(n, L.colptr, L.rowval, L.int_of_col, m);
-    partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, &key);
-    key = first_touch(m, L.csr_rowptr, L.csr_col, L.int_of_row, n);
-    partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt, &key);
-    key = first_touch(n, L.colptr, L.rowval, L.int_of_col, m);
-    partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, &key);
+    if (order == RowOrder::first_touch) {
+        partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt);
+        std::vector<int32_t> key = first_touch(n, L.colptr, L.rowval, L.int_of_col, m);
+        partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, &key);
+        key = first_touch(m, L.csr_rowptr, L.csr_col, L.int_of_row, n);
+        partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt, &key);
+        key = first_touch(n, L.colptr, L.rowval, L.int_of_col, m);
+        partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, &key);
+    } else {
+        const bool density = order == RowOrder::density;
+        auto lengths = [](const std::vector<int32_t>& ptr) {
+            std::vector<int32_t> len(ptr.size() - 1);
+            for (size_t i = 0; i + 1 < ptr.size(); ++i) len[i] = ptr[i + 1] - ptr[i];
+            return len;
+        };
+        const std::vector<int32_t> row_len = lengths(L.csr_rowptr), col_len = lengths(L.colptr);
+        partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, density ? &row_len : nullptr);
+        partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt, density ? &col_len : nullptr);
+    }
     build_side(L.A, (int32_t)m, (int32_t)n, L.csr_rowptr, L.csr_col, &L.perm, L.row_of_int,
                &L.int_of_col, nlA);
     build_side(L.At, (int32_t)n, (int32_t)m, L.colptr, L.rowval, nullptr, L.col_of_int,

@@ -4,10 +4,13 @@
 // (src/algorithms/common.jl:186-209); this is the served form of that same solve.
 // Requests are copied at submit (or borrowed: cpdlp_server_submit_borrowed).  Small
 // ones (m + n <= batch_limit) go to a batch engine (cpdlp_batch_*: one workgroup per LP,
-// iterates and restart state resident in its HBM arena between rounds) and are admitted as arena room frees up; larger ones go to plain
-// engines (cpdlp_create / cpdlp_solve, one HIP stream each) run by worker threads, in
-// lanes by nonzero count with their own shortest-job-first queues, under an HBM budget.
-// The caller drives the batch with cpdlp_server_pump; plain workers run on their own.
+// iterates and restart state resident in its HBM arena between rounds), medium ones
+// (m + n <= batch2_limit) to a second batch engine with rounds of its own, so that their
+// longer rounds never hold up the small ones; both admit as arena room frees up.  Larger
+// ones go to plain engines (cpdlp_create / cpdlp_solve, one HIP stream each) run by worker
+// threads, in lanes by nonzero count with their own shortest-job-first queues, under an
+// HBM budget.  The caller drives the batch routes with cpdlp_server_pump, or the library
+// does on one thread per route after cpdlp_server_start; plain workers run on their own.
 // Finished requests are reported by cpdlp_server_poll and handed over (and freed) by
 // cpdlp_server_get.
 #include <algorithm>
@@ -83,25 +86,33 @@ struct Lane {
     std::vector<std::thread> threads;
 };
 
+// one batch engine and its queues
+struct BatchRoute {
+    cpdlp_batch* b = nullptr;
+    int32_t limit = 0;             // m + n upper bound of its requests
+    int32_t checks = 20;           // checks per round for its smallest instances
+    int32_t code = -1;             // the route cpdlp_server_get reports: -1 or -2
+    std::deque<Request*> waiting;
+    std::unordered_map<int64_t, Request*> in_batch;   // batch id -> request
+    int64_t admitting = 0;         // taken off waiting by a pump, not yet placed
+};
+
 }  // namespace
 
 struct Server {
     cpdlp_params P;
     cpdlp_server_config cfg;
-    cpdlp_batch* batch = nullptr;
+    BatchRoute routes[2];          // small (-1) and medium (-2) requests
     std::mutex mu;
     std::condition_variable cv;
     bool stop = false;
     int64_t next_id = 1;
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
-    std::deque<Request*> batch_waiting;
-    std::unordered_map<int64_t, Request*> in_batch;   // batch id -> request
     std::vector<std::unique_ptr<Lane>> lanes;   // Lane holds threads: not movable
     int64_t plain_used = 0;
-    int64_t admitting = 0;         // taken off batch_waiting by a pump, not yet placed
     std::deque<int64_t> done;
-    // the optional driver thread (cpdlp_server_start) that pumps on its own
-    std::thread driver;
+    // the optional driver threads (cpdlp_server_start), one per batch route
+    std::thread drivers[2];
     bool driving = false;
     int32_t driver_code = CPDLP_OK;
     std::string driver_error;
@@ -116,9 +127,24 @@ struct Server {
         for (int k = 0; k < cfg.n_lanes; ++k)
             if (cfg.lane_workers[k] < 1 || cfg.lane_workers[k] > 16)
                 throw Error(CPDLP_ERR_ARGUMENT, "lane_workers must be in [1, 16]");
-        if (cfg.batch_limit > 0) {
-            const int rc = cpdlp_batch_create(&P, cfg.batch_bytes, &batch);
-            if (rc != CPDLP_OK) throw Error(rc, std::string("batch route: ") + cpdlp_last_error());
+        if (cfg.batch2_limit < 0 || cfg.batch2_limit > 65536)
+            throw Error(CPDLP_ERR_ARGUMENT, "batch2_limit must be in [0, 65536]");
+        if (cfg.batch2_checks < 0) throw Error(CPDLP_ERR_ARGUMENT, "batch2_checks must not be negative");
+        const int32_t limits[2] = {cfg.batch_limit, cfg.batch2_limit > cfg.batch_limit ? cfg.batch2_limit : 0};
+        const int64_t bytes[2] = {cfg.batch_bytes, cfg.batch2_bytes};
+        for (int k = 0; k < 2; ++k) {
+            BatchRoute& br = routes[k];
+            br.limit = limits[k];
+            br.checks = k == 1 && cfg.batch2_checks > 0 ? cfg.batch2_checks : cfg.checks_per_round;
+            br.code = -1 - k;
+            if (br.limit <= 0) continue;
+            const int rc = cpdlp_batch_create(&P, bytes[k], &br.b);
+            if (rc != CPDLP_OK) {   // no destructor runs for a constructor that throws
+                const std::string msg = std::string("batch route: ") + cpdlp_last_error();
+                for (BatchRoute& made : routes)
+                    if (made.b) cpdlp_batch_destroy(made.b);
+                throw Error(rc, msg);
+            }
         }
         for (int k = 0; k < cfg.n_lanes; ++k) {
             lanes.push_back(std::make_unique<Lane>());
@@ -135,10 +161,12 @@ struct Server {
             stop = true;
         }
         cv.notify_all();
-        if (driver.joinable()) driver.join();
+        for (auto& d : drivers)
+            if (d.joinable()) d.join();
         for (auto& ln : lanes)
             for (auto& t : ln->threads) t.join();
-        if (batch) cpdlp_batch_destroy(batch);
+        for (auto& br : routes)
+            if (br.b) cpdlp_batch_destroy(br.b);
     }
 
     // one plain solve on a worker thread; fixed CSR format, so that a request's result
@@ -186,7 +214,8 @@ struct Server {
     }
 
     int route(const Request& r) const {
-        if (batch && r.m + r.n <= cfg.batch_limit) return -1;
+        for (const BatchRoute& br : routes)
+            if (br.b && r.m + r.n <= br.limit) return br.code;
         for (int k = 0; k < (int)lanes.size(); ++k)
             if ((double)r.nnz <= lanes[k]->max_nnz) return k;
         return (int)lanes.size() - 1;
@@ -214,8 +243,7 @@ struct Server {
             r->id = next_id++;
             Request* p = r.get();
             reqs[p->id] = std::move(r);
-            if (p->lane < 0) batch_waiting.push_back(p);
-            else lanes[p->lane]->waiting.push_back(p);
+            enqueue_locked(p);
             return p->id;
         }
         read_csc(lp.m, lp.n, lp.index_base, lp.index_bits, lp.colptr, lp.rowval, r->colptr, r->rowval);
@@ -248,9 +276,13 @@ struct Server {
         r->id = next_id++;
         Request* p = r.get();
         reqs[p->id] = std::move(r);
-        if (p->lane < 0) batch_waiting.push_back(p);
-        else lanes[p->lane]->waiting.push_back(p);
+        enqueue_locked(p);
         return p->id;
+    }
+
+    void enqueue_locked(Request* p) {
+        if (p->lane < 0) routes[-1 - p->lane].waiting.push_back(p);
+        else lanes[p->lane]->waiting.push_back(p);
     }
 
     // plain admission (caller holds mu): shortest job first within a lane; a job waits
@@ -279,19 +311,20 @@ struct Server {
 
     int64_t pending_locked() const {
         // admitting may count a request twice for a moment (never zero times)
-        int64_t k = (int64_t)batch_waiting.size() + (int64_t)in_batch.size() + admitting;
+        int64_t k = 0;
+        for (const BatchRoute& br : routes) k += (int64_t)br.waiting.size() + (int64_t)br.in_batch.size() + br.admitting;
         for (auto& ln : lanes) k += (int64_t)ln->waiting.size() + ln->running;
         return k;
     }
 
-    // one scheduling round: admit, advance the batch by checks_per_round checks, harvest
-    int64_t pump() {
+    // one round of a batch route: admit, advance by its checks per round, harvest
+    void pump_route(BatchRoute& br) {
+        if (!br.b) return;
         std::deque<Request*> to_batch;
         {
             std::lock_guard<std::mutex> g(mu);
-            admit_plain();
-            to_batch.swap(batch_waiting);
-            admitting = (int64_t)to_batch.size();
+            to_batch.swap(br.waiting);
+            br.admitting = (int64_t)to_batch.size();
         }
         // batch admission as arena room allows; the rest waits, in order
         std::deque<Request*> left;
@@ -304,7 +337,7 @@ struct Server {
             }
             const cpdlp_lp lp = r->view();
             int64_t bid = 0;
-            const int rc = cpdlp_batch_add_start(batch, &lp, r->start_x(), r->start_y(), &bid);
+            const int rc = cpdlp_batch_add_start(br.b, &lp, r->start_x(), r->start_y(), &bid);
             to_batch.pop_front();
             if (rc == CPDLP_ERR_MEMORY) {
                 left.push_back(r);
@@ -318,60 +351,71 @@ struct Server {
             } else {
                 r->drop_input();
                 std::lock_guard<std::mutex> g(mu);
-                in_batch[bid] = r;
+                br.in_batch[bid] = r;
             }
         }
         bool any;
         {
             std::lock_guard<std::mutex> g(mu);
-            for (auto it = left.rbegin(); it != left.rend(); ++it) batch_waiting.push_front(*it);
-            admitting = 0;
-            any = !in_batch.empty();
+            for (auto it = left.rbegin(); it != left.rend(); ++it) br.waiting.push_front(*it);
+            br.admitting = 0;
+            any = !br.in_batch.empty();
         }
-        if (any) {
-            int64_t running = 0;
-            int rc = cpdlp_batch_run(batch, cfg.checks_per_round, &running);
+        if (!any) return;
+        int64_t running = 0;
+        int rc = cpdlp_batch_run(br.b, br.checks, &running);
+        if (rc != CPDLP_OK) throw Error(rc, std::string("batch route: ") + cpdlp_last_error());
+        std::vector<int64_t> ids(64);
+        for (;;) {
+            int64_t cnt = 0;
+            rc = cpdlp_batch_poll(br.b, ids.data(), (int64_t)ids.size(), &cnt);
             if (rc != CPDLP_OK) throw Error(rc, std::string("batch route: ") + cpdlp_last_error());
-            std::vector<int64_t> ids(64);
-            for (;;) {
-                int64_t cnt = 0;
-                rc = cpdlp_batch_poll(batch, ids.data(), (int64_t)ids.size(), &cnt);
-                if (rc != CPDLP_OK) throw Error(rc, std::string("batch route: ") + cpdlp_last_error());
-                if (cnt == 0) break;
-                const double now = snow();
-                for (int64_t k = 0; k < cnt; ++k) {
-                    Request* r;
-                    {
-                        std::lock_guard<std::mutex> g(mu);
-                        r = in_batch.at(ids[k]);
-                    }
-                    r->x.assign((size_t)r->n, 0.0);
-                    r->y.assign((size_t)r->m, 0.0);
-                    rc = cpdlp_batch_get(batch, ids[k], r->x.data(), r->y.data(), &r->stats);
-                    if (rc == CPDLP_OK) rc = cpdlp_batch_release(batch, ids[k]);
-                    if (rc != CPDLP_OK) throw Error(rc, std::string("batch route: ") + cpdlp_last_error());
+            if (cnt == 0) break;
+            const double now = snow();
+            for (int64_t k = 0; k < cnt; ++k) {
+                Request* r;
+                {
                     std::lock_guard<std::mutex> g(mu);
-                    in_batch.erase(ids[k]);
-                    r->t_done = now;
-                    done.push_back(r->id);
+                    r = br.in_batch.at(ids[k]);
                 }
+                r->x.assign((size_t)r->n, 0.0);
+                r->y.assign((size_t)r->m, 0.0);
+                rc = cpdlp_batch_get(br.b, ids[k], r->x.data(), r->y.data(), &r->stats);
+                if (rc == CPDLP_OK) rc = cpdlp_batch_release(br.b, ids[k]);
+                if (rc != CPDLP_OK) throw Error(rc, std::string("batch route: ") + cpdlp_last_error());
+                std::lock_guard<std::mutex> g(mu);
+                br.in_batch.erase(ids[k]);
+                r->t_done = now;
+                done.push_back(r->id);
             }
         }
+    }
+
+    // one scheduling round of every route (the caller's cpdlp_server_pump)
+    int64_t pump() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            admit_plain();
+        }
+        for (BatchRoute& br : routes) pump_route(br);
         std::lock_guard<std::mutex> g(mu);
         admit_plain();   // plain jobs that finished during the round free budget
         return pending_locked();
     }
 
-    // pumps until the server is destroyed; idles 200 us when the batch route has nothing
-    // to run.  A failure stops the driver and is reported by every later call.
-    void drive() {
+    // pumps batch route k (route 0 also admits plain jobs) until the server is destroyed;
+    // idles 200 us when the route has nothing to run.  A failure stops every driver and
+    // is reported by every later call.
+    void drive(int k) {
+        BatchRoute& br = routes[k];
         for (;;) {
             {
                 std::lock_guard<std::mutex> g(mu);
-                if (stop) return;
+                if (stop || driver_code != CPDLP_OK) return;
+                if (k == 0) admit_plain();
             }
             try {
-                pump();
+                pump_route(br);
             } catch (const Error& e) {
                 std::lock_guard<std::mutex> g(mu);
                 driver_code = e.code;
@@ -386,7 +430,7 @@ struct Server {
             bool idle;
             {
                 std::lock_guard<std::mutex> g(mu);
-                idle = in_batch.empty() && batch_waiting.empty();
+                idle = br.in_batch.empty() && br.waiting.empty();
             }
             if (idle) std::this_thread::sleep_for(std::chrono::microseconds(200));
         }
@@ -396,7 +440,8 @@ struct Server {
         std::lock_guard<std::mutex> g(mu);
         if (driving) return;
         driving = true;
-        driver = std::thread([this] { drive(); });
+        drivers[0] = std::thread([this] { drive(0); });   // also admits the plain lanes
+        if (routes[1].b) drivers[1] = std::thread([this] { drive(1); });
     }
 
     void check_driver() {
@@ -471,9 +516,12 @@ int cpdlp_server_default_config(cpdlp_server_config* c) {
         c->checks_per_round = 20;
         c->n_lanes = 2;
         c->lane_max_nnz[0] = 1e6;
-        c->lane_workers[0] = 2;
+        c->lane_workers[0] = 1;   // 2 batch streams + 2 plain workers: 4 hardware queues
         c->lane_max_nnz[1] = INFINITY;
         c->lane_workers[1] = 1;
+        c->batch2_bytes = (int64_t)2 << 30;
+        c->batch2_limit = 65536;
+        c->batch2_checks = 0;
     });
 }
 

@@ -488,7 +488,16 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreate(&ev0));
     HIP_CHECK(hipEventCreate(&ev1));
-    L = build_layout(m, n, std::move(cp), std::move(rv), P.long_row);
+    // row and column order: first touch unless CPDLP_ORDER names "density" or "given",
+    // for measurements
+    RowOrder order = RowOrder::first_touch;
+    if (const char* e = std::getenv("CPDLP_ORDER")) {
+        const std::string v(e);
+        if (v == "density") order = RowOrder::density;
+        else if (v == "given") order = RowOrder::given;
+        else if (v != "first_touch") throw Error(CPDLP_ERR_ARGUMENT, "CPDLP_ORDER must be first_touch, density or given");
+    }
+    L = build_layout(m, n, std::move(cp), std::move(rv), P.long_row, order);
     const hipStream_t s = stream;
     auto up64 = [&](DevBuf<double>& d, const double* h, int64_t len) {
         d.alloc(std::max<int64_t>(len, 1));

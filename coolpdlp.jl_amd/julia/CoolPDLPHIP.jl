@@ -428,16 +428,21 @@ mutable struct CServerConfig
     n_lanes::Int32
     lane_workers::NTuple{4, Int32}
     lane_max_nnz::NTuple{4, Float64}
+    batch2_bytes::Int64
+    batch2_limit::Int32
+    batch2_checks::Int32
     CServerConfig() = new()
 end
 
 """
-    HIPServer(algo; batch_limit = 8192, checks_per_round = 20)
+    HIPServer(algo; batch_limit = 8192, batch2_limit = 65536, checks_per_round = 20)
 
 The library's continuous-batching server: `submit!` queues an LP (optionally with a start
 point; the library copies it), `pump!` runs one scheduling round (or `start!` lets the
-library pump on its own thread), `poll!` lists finished requests and `take!` hands one
-over as `(PrimalDualSolution, ConvergenceStats, seconds)`.
+library pump on its own threads), `poll!` lists finished requests and `take!` hands one
+over as `(PrimalDualSolution, ConvergenceStats, seconds)`.  LPs with m + n <= batch_limit
+go to one batch engine, those with m + n <= batch2_limit to a second one with rounds of
+its own, larger ones to plain engines.
 """
 mutable struct HIPServer{T}
     ptr::Ptr{Cvoid}
@@ -445,11 +450,12 @@ mutable struct HIPServer{T}
 end
 
 function HIPServer(algo::Algorithm{A, T, Ti, M, HIPEngine}; batch_limit::Integer = 8192,
-        checks_per_round::Integer = 20) where {A, T, Ti, M}
+        batch2_limit::Integer = 65536, checks_per_round::Integer = 20) where {A, T, Ti, M}
     p = cparams(algo)
     cfg = CServerConfig()
     check(ccall((:cpdlp_server_default_config, libcpdlp), Cint, (Ref{CServerConfig},), cfg))
     cfg.batch_limit = batch_limit
+    cfg.batch2_limit = batch2_limit
     cfg.checks_per_round = checks_per_round
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:cpdlp_server_create, libcpdlp), Cint, (Ref{CParams}, Ref{CServerConfig}, Ptr{Ptr{Cvoid}}),

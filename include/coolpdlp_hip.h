@@ -310,7 +310,9 @@ void cpdlp_matrix_destroy(cpdlp_matrix* A);
  * The served form of solve(milp, sol, algo) (src/algorithms/common.jl:186-209).
  * Requests are copied at submit (or borrowed).  Those with m + n <= batch_limit go to a
  * batch engine (as cpdlp_batch_*: admitted as arena room frees up, advanced by
- * cpdlp_server_pump);
+ * cpdlp_server_pump), those with m + n <= batch2_limit to a second batch engine whose
+ * rounds run apart from the first one's (on a thread of their own after
+ * cpdlp_server_start), so that long rounds of medium LPs never hold up small ones;
  * the others to plain engines (as cpdlp_solve, CSR format, one HIP stream each) run by
  * worker threads in lanes by nonzero count (a request takes the first lane whose
  * lane_max_nnz admits it, else the last), shortest job first within a lane, under the
@@ -323,16 +325,19 @@ typedef struct cpdlp_server_config {
     int32_t batch_limit;            /* m + n limit of the batch route, <= 65536; 0: none (8192) */
     int32_t checks_per_round;       /* checks per pump for the smallest instances (20) */
     int32_t n_lanes;                /* plain lanes, 1..CPDLP_SERVER_MAX_LANES (2) */
-    int32_t lane_workers[CPDLP_SERVER_MAX_LANES];   /* worker threads per lane (2, 1) */
+    int32_t lane_workers[CPDLP_SERVER_MAX_LANES];   /* worker threads per lane (1, 1) */
     double lane_max_nnz[CPDLP_SERVER_MAX_LANES];    /* nonzeros a lane admits (1e6, inf) */
+    int64_t batch2_bytes;           /* second batch route's HBM arena (2 GiB) */
+    int32_t batch2_limit;           /* its m + n limit, <= 65536; none when <= batch_limit (65536) */
+    int32_t batch2_checks;          /* its checks per round; 0: checks_per_round (0) */
 } cpdlp_server_config;
 typedef struct cpdlp_server cpdlp_server;
 
 int cpdlp_server_default_config(cpdlp_server_config* cfg);
 /* params as for cpdlp_create (the format is fixed to CSR on the plain route).  Keep
- * params->batch_streams plus the lanes' workers within the device's hardware queues (4 by
- * default): streams beyond them share queues, and batch rounds then wait behind plain
- * solves' kernels. */
+ * params->batch_streams (per batch route) plus the lanes' workers within the device's
+ * hardware queues (4 by default): streams beyond them share queues, and batch rounds then
+ * wait behind plain solves' kernels. */
 int cpdlp_server_create(const cpdlp_params* params, const cpdlp_server_config* cfg, cpdlp_server** out);
 /* Queue one LP; x0 / y0 (NULL: zero) are an unscaled start as in cpdlp_solve. */
 int cpdlp_server_submit(cpdlp_server* s, const cpdlp_lp* lp, const double* x0, const double* y0,
@@ -343,7 +348,7 @@ int cpdlp_server_submit(cpdlp_server* s, const cpdlp_lp* lp, const double* x0, c
  * cpdlp_server_get.  For callers that submit large LPs at a high rate. */
 int cpdlp_server_submit_borrowed(cpdlp_server* s, const cpdlp_lp* lp, const double* x0,
                                  const double* y0, int64_t* id);
-/* One scheduling round: admit, advance the batch route by checks_per_round checks, harvest.
+/* One scheduling round: admit, advance each batch route by its checks per round, harvest.
  * *pending = requests not yet finished. */
 int cpdlp_server_pump(cpdlp_server* s, int64_t* pending);
 /* Run the pump loop on a library thread from now on (idempotent): callers then only
@@ -353,7 +358,7 @@ int cpdlp_server_start(cpdlp_server* s);
 /* Ids of finished requests (each reported once). */
 int cpdlp_server_poll(cpdlp_server* s, int64_t* ids, int64_t cap, int64_t* count);
 /* A finished request's unscaled x (n), y (m) and stats; *seconds = submit to finish;
- * *route = -1 (batch) or the plain lane.  Frees the request.  Returns the request's own
+ * *route = -1 (batch), -2 (second batch) or the plain lane.  Frees the request.  Returns the request's own
  * error code when its solve failed (the message in cpdlp_last_error).  A request that
  * has not finished gives CPDLP_ERR_STATE with the message "request still running" and
  * stays queued (a borrowed request's arrays are still read). */

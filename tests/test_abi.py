@@ -118,13 +118,15 @@ def test_server_config_defaults_and_validation():
     _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(cfg)))
     assert (cfg.batch_bytes, cfg.plain_bytes) == (16 << 30, 64 << 30)
     assert (cfg.batch_limit, cfg.checks_per_round, cfg.n_lanes) == (8192, 20, 2)
-    assert list(cfg.lane_workers)[:2] == [2, 1] and cfg.lane_max_nnz[0] == 1e6
+    assert list(cfg.lane_workers)[:2] == [1, 1] and cfg.lane_max_nnz[0] == 1e6
     assert cfg.lane_max_nnz[1] == float("inf")
+    assert (cfg.batch2_bytes, cfg.batch2_limit, cfg.batch2_checks) == (2 << 30, 65536, 0)
     import coolpdlp_hip as cp
-    for field, value in [("batch_limit", 70000), ("n_lanes", 0), ("n_lanes", 5), ("checks_per_round", 0)]:
+    for field, value in [("batch_limit", 70000), ("n_lanes", 0), ("n_lanes", 5), ("checks_per_round", 0),
+                         ("batch2_limit", 70000), ("batch2_limit", -1), ("batch2_checks", -1)]:
         bad = _lib.CServerConfig()
         _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(bad)))
-        bad.batch_limit = 0
+        bad.batch_limit = bad.batch2_limit = 0
         setattr(bad, field, value)
         h = C.c_void_p()
         with pytest.raises(cp.ArgumentError):

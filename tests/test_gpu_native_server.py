@@ -1,8 +1,9 @@
 """GPU tests of the library's own server (cpdlp_server_*, through coolpdlp_hip.serving.
 NativeServer): requests submitted while others run give the same bits as when solved
-alone -- batch-routed ones as the batch engine alone, plain-routed ones as a CSR-format
-plain solve alone; a start point is honoured on both routes; the opt-in infeasibility
-detection reaches both routes; errors are reported per request and per call."""
+alone -- batch-routed ones (either batch route) as the batch engine alone, plain-routed
+ones as a CSR-format plain solve alone; a start point is honoured on every route; the
+opt-in infeasibility detection reaches every route; errors are reported per request and
+per call."""
 import time
 
 import numpy as np
@@ -34,16 +35,19 @@ def _drain(srv, want, deadline_s=180):
     return out
 
 
-@pytest.mark.parametrize("borrow,driver", [(True, False), (False, False), (True, True)])
-def test_native_server_alone_equals_in_mix(cp, borrow, driver):
+@pytest.mark.parametrize("borrow,driver,batch2", [(True, False, 0), (False, False, 12000), (True, True, 0),
+                                                  (True, True, 12000)])
+def test_native_server_alone_equals_in_mix(cp, borrow, driver, batch2):
     """Copied (cpdlp_server_submit) and borrowed (cpdlp_server_submit_borrowed) requests,
-    pumped by the caller or by the library's own thread (cpdlp_server_start)."""
+    pumped by the caller or by the library's own threads (cpdlp_server_start), with the
+    second batch route off or taking 8000 < m + n <= 12000."""
     from coolpdlp_hip.serving import NativeServer
     small = [G.request_lp(m, 2 * m, 700 + m) for m in (60, 150, 420, 900, 1500, 2600)]
-    big = [G.request_lp(3000, 6000, 78), G.request_lp(5000, 10000, 79)]
+    big = [G.request_lp(3000, 6000, 78), G.request_lp(5000, 10000, 79), G.request_lp(3500, 7000, 80)]
     algo = cp.PDLP("mixed", termination_reltol=1e-6)
     with NativeServer(algo, batch_bytes=256 << 20, plain_bytes=4 << 30, batch_limit=8000,
-                      checks_per_round=2) as srv:
+                      checks_per_round=2, batch2_limit=batch2, batch2_bytes=256 << 20,
+                      batch2_checks=3) as srv:
         if driver:
             srv.start()
         ids = {srv.submit(small[0], borrow=borrow): small[0], srv.submit(big[0], borrow=borrow): big[0]}
@@ -57,8 +61,8 @@ def test_native_server_alone_equals_in_mix(cp, borrow, driver):
         assert st.termination_status == cp.OPTIMAL, (g.name, st)
         assert secs > 0
         assert abs(float(sol.x @ g.c) - g.objective) <= 2e-5 * (1 + abs(g.objective))
-        if g.m + g.n <= 8000:
-            assert route == -1
+        if g.m + g.n <= 8000 or g.m + g.n <= batch2:
+            assert route == (-1 if g.m + g.n <= 8000 else -2)
             with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
                 (alone, st_a), = B.solve_all([g])
         else:
@@ -69,15 +73,16 @@ def test_native_server_alone_equals_in_mix(cp, borrow, driver):
         assert st_a.kkt_passes == st.kkt_passes
 
 
-@pytest.mark.parametrize("batch_limit", [8000, 0])
-def test_native_server_warm_start_and_detection(cp, batch_limit):
+@pytest.mark.parametrize("batch_limit,batch2,want", [(8000, 0, -1), (0, 0, 0), (0, 8000, -2)])
+def test_native_server_warm_start_and_detection(cp, batch_limit, batch2, want):
     from coolpdlp_hip.serving import NativeServer
     g = G.request_lp(400, 800, 91)
     algo = cp.PDLP("mixed", termination_reltol=1e-6, detect_infeasibility=True)
-    with NativeServer(algo, batch_bytes=64 << 20, plain_bytes=1 << 30, batch_limit=batch_limit) as srv:
+    with NativeServer(algo, batch_bytes=64 << 20, plain_bytes=1 << 30, batch_limit=batch_limit,
+                      batch2_limit=batch2, batch2_bytes=64 << 20) as srv:
         first = srv.submit(g)
         (sol, st, _, route), = _drain(srv, 1).values()
-        assert st.termination_status == cp.OPTIMAL and route == (-1 if batch_limit else 0)
+        assert st.termination_status == cp.OPTIMAL and route == want
         srv.submit(g, sol)
         (_, st2, _, _), = _drain(srv, 1).values()
         assert st2.termination_status == cp.OPTIMAL and st2.kkt_passes <= st.kkt_passes // 2
@@ -94,7 +99,7 @@ def test_native_server_reports_request_errors(cp):
     the wrong size are refused, and the server keeps serving."""
     from coolpdlp_hip.serving import NativeServer
     with NativeServer(cp.PDLP("mixed", termination_reltol=1e-6, max_kkt_passes=2000),
-                      batch_bytes=16 << 20, plain_bytes=1 << 30) as srv:
+                      batch_bytes=16 << 20, plain_bytes=1 << 30, batch2_bytes=16 << 20) as srv:
         g = G.request_lp(100, 200, 5)
         i = srv.submit(g)
         srv.shapes[10**9] = (1, 1)
@@ -115,7 +120,8 @@ def test_get_on_a_running_borrowed_request_keeps_its_arrays(cp):
     from coolpdlp_hip.serving import NativeServer
     g = G.request_lp(3000, 6000, 81)          # plain route (m + n > batch_limit)
     algo = cp.PDLP("mixed", termination_reltol=1e-6)
-    with NativeServer(algo, batch_bytes=64 << 20, plain_bytes=4 << 30, batch_limit=2000) as srv:
+    with NativeServer(algo, batch_bytes=64 << 20, plain_bytes=4 << 30, batch_limit=2000,
+                      batch2_limit=0) as srv:
         i = srv.submit(G.request_lp(3000, 6000, 81), borrow=True)   # no other reference to the LP
         with pytest.raises(Exception, match="request still running"):
             srv.get(i)                      # not even admitted: nothing pumped yet
@@ -140,7 +146,7 @@ def test_request_larger_than_the_batch_arena_fails_alone(cp):
     assert footprint(big.m, big.n, big.nnz) > arena
     algo = cp.PDLP("mixed", termination_reltol=1e-6)
     with NativeServer(algo, batch_bytes=arena, plain_bytes=1 << 30, batch_limit=8000,
-                      checks_per_round=5) as srv:
+                      checks_per_round=5, batch2_limit=0) as srv:
         ib = srv.submit(big)
         ids = [srv.submit(g) for g in small]
         done, err = {}, None

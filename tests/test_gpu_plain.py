@@ -452,3 +452,36 @@ def test_restart_error_cache_keeps_the_bits(cp, precision, fmt, which):
     for (p1, e1), (p2, e2) in zip(t1.error_history, t2.error_history):
         assert p1 == p2 and e1 == e2, (p1, e1, e2)
     assert np.array_equal(s1.x, s2.x) and np.array_equal(s1.y, s2.y)
+
+
+@pytest.mark.parametrize("order", ["density", "given"])
+def test_row_orders_reach_the_same_optimum(cp, order):
+    """CPDLP_ORDER lays the short rows and columns out in density order (increasing
+    stored-entry count, the order sort_rows_columns gives, src/components/permutation.jl:
+    16-46) or in the caller's order instead of first touch: the same LP reaches the same
+    optimum (objective to its closed form and to the first-touch solve's) through every
+    format; an unknown order is refused."""
+    import os
+    g = G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4)
+    milp = cp.MILP.from_generated(g)
+    kw = dict(termination_reltol=1e-8, powm_tol=1e-6, max_kkt_passes=40_000)
+    runs = {}
+    for o in ("first_touch", order):
+        os.environ["CPDLP_ORDER"] = o
+        try:
+            for fmt in ("csr", "auto"):
+                runs[o, fmt] = cp.solve(milp, cp.PDLP("float64", None, fmt, **kw))
+        finally:
+            os.environ.pop("CPDLP_ORDER", None)
+    ref = float(runs["first_touch", "csr"][0].x @ g.c)
+    for (o, fmt), (sol, st) in runs.items():
+        assert st.termination_status == cp.OPTIMAL, (o, fmt, st)
+        obj = float(sol.x @ g.c)
+        assert abs(obj - g.objective) <= 1e-6 * (1 + abs(g.objective)), (o, fmt)
+        assert abs(obj - ref) <= 1e-6 * (1 + abs(ref)), (o, fmt)
+    os.environ["CPDLP_ORDER"] = "random"
+    try:
+        with pytest.raises(cp.ArgumentError, match="CPDLP_ORDER"):
+            cp.solve(milp, cp.PDLP("float64", **kw))
+    finally:
+        os.environ.pop("CPDLP_ORDER", None)
