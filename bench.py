@@ -124,7 +124,8 @@ def bench_batch(args):
     algo = cp.PDLP("mixed", termination_reltol=reltol, time_limit=args.time_limit,
                    detect_infeasibility=int(args.detect_infeasibility), **polish(args))
     arena = int(sum(cp.batch.footprint(g.m, g.n, g.nnz) for g in lps) * 1.05) + (64 << 20)
-    B = cp.BatchSolver(algo, hbm_bytes=arena)
+    B = cp.BatchSolver(algo, hbm_bytes=arena, tail=args.batch_tail, tail_min_passes=args.batch_tail_passes,
+                       tail_workers=args.batch_tail_workers)
     t = time.perf_counter()
     ids = [B.add(g) for g in lps]
     log(f"[bench] admitted to HBM ({arena / 2**30:.2f} GiB arena) in {time.perf_counter() - t:.1f}s")
@@ -186,7 +187,11 @@ def bench_batch(args):
                    "polish_reltol": algo.params.polish_reltol,
                    "detect_infeasibility": bool(args.detect_infeasibility),
                    "checks_per_round": args.checks_per_round,
+                   "tail_continuation": ({"below": args.batch_tail, "min_passes": args.batch_tail_passes,
+                                          "workers": args.batch_tail_workers}
+                                         if args.batch_tail > 0 else None),
                    "timed_region": "batch (LPs admitted beforehand)"},
+        "continued_on_plain_engines": int(sum(B.continued(i) for i in ids)),
         "batch_seconds": elapsed, "rounds": rounds,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
         "optimal": int(sum(s == "OPTIMAL" for s in status)), "count": len(lps),
@@ -804,6 +809,10 @@ def main():
                     help="A,At formats for an auto run (sets CPDLP_FORMATS; for counter runs)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--checks-per-round", type=int, default=20)
+    ap.add_argument("--batch-tail", type=int, default=0,
+                    help="continue the last N unfinished batch LPs on plain engines (0: off)")
+    ap.add_argument("--batch-tail-passes", type=int, default=2000)
+    ap.add_argument("--batch-tail-workers", type=int, default=2)
     ap.add_argument("--plain-sample", type=int, default=0,
                     help="batch: LPs solved one at a time for the comparison (0: all of them)")
     ap.add_argument("--qps", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 3000.0, 5000.0])
@@ -813,8 +822,8 @@ def main():
                     help="the one-at-a-time baseline: run it on the same arrivals, model it, or skip it")
     ap.add_argument("--serve-checks", type=int, default=20)
     ap.add_argument("--serve-batch-gib", type=int, default=16)
-    ap.add_argument("--serve-medium-workers", type=int, default=1)
-    ap.add_argument("--serve-batch2-limit", type=int, default=65536,
+    ap.add_argument("--serve-medium-workers", type=int, default=2)
+    ap.add_argument("--serve-batch2-limit", type=int, default=0,
                     help="m + n limit of the library server's second batch route (0: off)")
     ap.add_argument("--serve-batch2-gib", type=float, default=2.0)
     ap.add_argument("--serve-overflow", type=int, default=0)

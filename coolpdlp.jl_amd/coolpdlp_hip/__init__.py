@@ -7,8 +7,8 @@ All computation runs in libcoolpdlp_hip.so (hand-written HIP kernels for gfx950)
 from ._lib import ArgumentError, DimensionMismatch, HIPError
 from .algorithms import (DUAL_INFEASIBLE, ITERATION_LIMIT, NUMERICAL_ERROR, OPTIMAL,
                          PRIMAL_INFEASIBLE, STILL_RUNNING, TIME_LIMIT, Algorithm,
-                         ConvergenceStats, Engine, KKTErrors, PDHG, PDLP, TerminationStatus,
-                         absolute, relative, solve)
+                         ConvergenceStats, Engine, KKTErrors, LoopState, PDHG, PDLP,
+                         TerminationStatus, absolute, relative, solve)
 from .batch import BatchSolver
 from .matrices import GPUSparseMatrixCSR, GPUSparseMatrixELL
 from .problems import (MILP, PrimalDualSolution, is_feasible, nbcons, nbcons_eq, nbcons_ineq,
@@ -24,7 +24,7 @@ __all__ = [
     "Algorithm", "is_feasible", "objective_value", "TerminationStatus", "OPTIMAL",
     "TIME_LIMIT", "ITERATION_LIMIT", "STILL_RUNNING", "PRIMAL_INFEASIBLE", "DUAL_INFEASIBLE",
     "NUMERICAL_ERROR", "KKTErrors", "ConvergenceStats",
-    "Engine", "BatchSolver", "relative", "absolute", "DimensionMismatch", "ArgumentError", "HIPError",
+    "Engine", "BatchSolver", "LoopState", "relative", "absolute", "DimensionMismatch", "ArgumentError", "HIPError",
     "Optimizer", "moi", "QPSData", "read_mps", "write_mps",
 ]
 

@@ -83,6 +83,15 @@ class CStats(C.Structure):
                 ("step_kernel_ms", C.c_double), ("step_kernel_launches", C.c_int64)]
 
 
+STATE_VECTORS = 5
+
+
+class CState(C.Structure):
+    _fields_ = [("eta", C.c_double), ("omega", C.c_double), ("eta_sum", C.c_double),
+                ("kkt_passes", C.c_int64), ("inner", C.c_int64), ("total", C.c_int64),
+                ("restarts", C.c_int64), ("fp32_passes", C.c_int64)]
+
+
 # every symbol the header declares, with its signature
 _P = C.POINTER
 _SIGS = {
@@ -91,6 +100,8 @@ _SIGS = {
     "cpdlp_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cpdlp_initialize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cpdlp_run_checks": (C.c_int, [C.c_void_p, C.c_int64, _P(C.c_int32)]),
+    "cpdlp_get_state": (C.c_int, [C.c_void_p, _P(CState), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cpdlp_resume": (C.c_int, [C.c_void_p, _P(CState), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cpdlp_get_solution_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cpdlp_get_solution": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cpdlp_get_stats": (C.c_int, [C.c_void_p, _P(CStats)]),
@@ -126,6 +137,8 @@ _SIGS = {
     "cpdlp_batch_poll": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, _P(C.c_int64)]),
     "cpdlp_batch_get": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, _P(CStats)]),
     "cpdlp_batch_release": (C.c_int, [C.c_void_p, C.c_int64]),
+    "cpdlp_batch_take_state": (C.c_int, [C.c_void_p, C.c_int64, _P(CState), C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]),
     "cpdlp_batch_memory": (C.c_int, [C.c_void_p, _P(C.c_int64), _P(C.c_int64)]),
     "cpdlp_batch_destroy": (None, [C.c_void_p]),
     "cpdlp_matrix_create": (C.c_int, [C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,

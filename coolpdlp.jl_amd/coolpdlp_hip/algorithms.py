@@ -103,6 +103,45 @@ _FLOAT_TYPES = {"float64": _lib.PREC_FP64, "Float64": _lib.PREC_FP64, np.float64
                 "mixed": _lib.PREC_MIXED}
 
 
+@dataclass
+class LoopState:
+    """The PDLP loop's state between two checks (PDLPState, src/algorithms/pdlp.jl:19-43,
+    less its scratch and per-check statistics): the five primal vectors `x` (rows sol,
+    sol_last, sol_avg, sol_avg_last, sol_restart; n entries each) and dual vectors `y` (m
+    each), scaled, in the LP's original order; the scaling they belong to (`D2`, `D1`:
+    the unscaled point is x * D2, y * D1); and the scalars.  Engine.get_state and
+    BatchSolver.take_state produce it, Engine.resume continues from it."""
+    x: np.ndarray
+    y: np.ndarray
+    D2: np.ndarray
+    D1: np.ndarray
+    eta: float
+    omega: float
+    eta_sum: float
+    kkt_passes: int
+    inner: int
+    total: int
+    restarts: int
+    fp32_passes: int
+
+    @classmethod
+    def from_c(cls, s, x, y, D2, D1):
+        return cls(x, y, D2, D1, s.eta, s.omega, s.eta_sum, s.kkt_passes, s.inner, s.total,
+                   s.restarts, s.fp32_passes)
+
+    def to_c(self):
+        s = _lib.CState()
+        for f, _ in s._fields_:
+            setattr(s, f, getattr(self, f))
+        return s
+
+    @classmethod
+    def buffers(cls, m: int, n: int):
+        k = _lib.STATE_VECTORS
+        return (np.zeros((k, max(n, 1))), np.zeros((k, max(m, 1))), np.zeros(max(n, 1)),
+                np.zeros(max(m, 1)))
+
+
 class Algorithm:
     """Algorithm{A}(T, Ti, M; backend, kwargs...), src/algorithms/common.jl:53-116.
 
@@ -222,6 +261,27 @@ class Engine:
         st = C.c_int32()
         check(lib.cpdlp_run_checks(self.h, int(max_checks), C.byref(st)))
         return TerminationStatus(st.value)
+
+    def get_state(self) -> LoopState:
+        """The loop's state after the last check (cpdlp_get_state): a checkpoint that
+        `resume` continues, here or on another engine of the same LP."""
+        x, y, D2, D1 = LoopState.buffers(self.m, self.n)
+        s = _lib.CState()
+        check(lib.cpdlp_get_state(self.h, C.byref(s), _ptr(x), _ptr(y), _ptr(D2), _ptr(D1)))
+        return LoopState.from_c(s, x[:, :self.n], y[:, :self.m], D2[:self.n], D1[:self.m])
+
+    def resume(self, state: LoopState):
+        """initialize, then continue from `state` (cpdlp_resume): run_checks carries on
+        that run.  A state taken from an engine with the same scaling resumes bit for bit."""
+        k = _lib.STATE_VECTORS
+        x = np.ascontiguousarray(state.x, dtype=np.float64)
+        y = np.ascontiguousarray(state.y, dtype=np.float64)
+        D2 = np.ascontiguousarray(state.D2, dtype=np.float64)
+        D1 = np.ascontiguousarray(state.D1, dtype=np.float64)
+        if x.shape != (k, self.n) or y.shape != (k, self.m) or D2.shape != (self.n,) or D1.shape != (self.m,):
+            raise _lib.DimensionMismatch("state does not match the LP's size")
+        s = state.to_c()
+        check(lib.cpdlp_resume(self.h, C.byref(s), _ptr(x), _ptr(y), _ptr(D2), _ptr(D1)))
 
     def solution_device(self, x_dev, y_dev):
         """Unscaled solution into torch device tensors (float64, cuda): no host copy."""

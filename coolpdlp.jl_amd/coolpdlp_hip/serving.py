@@ -230,13 +230,13 @@ class NativeServer:
     with rounds of its own, both advanced by `pump` (or the library's threads after
     `start`); larger ones to plain engines on the library's worker threads, in lanes of
     (max nonzeros, workers), shortest job first, under an HBM budget.  batch2_limit <=
-    batch_limit turns the second batch route off.  This is the interface a C or Julia
-    caller uses."""
+    batch_limit (the default) turns the second batch route off.  This is the interface a
+    C or Julia caller uses."""
 
     def __init__(self, algo=None, *, batch_bytes: int = 16 << 30, plain_bytes: int = 64 << 30,
                  batch_limit: int = 8192, checks_per_round: int = 20,
-                 lanes=((1_000_000, 1), (float("inf"), 1)),
-                 batch2_limit: int = 65536, batch2_bytes: int = 2 << 30, batch2_checks: int = 0):
+                 lanes=((1_000_000, 2), (float("inf"), 1)),
+                 batch2_limit: int = 0, batch2_bytes: int = 2 << 30, batch2_checks: int = 0):
         import ctypes as C
         from . import _lib
         self._C, self._lib = C, _lib

@@ -1134,6 +1134,33 @@ __global__ void k_batch_unscale(const BInst* insts, const BState* states, int32_
         y[i] = I.D1[i] * (f64 ? y64[i] : (double)y32[i]);
 }
 
+// an instance's loop state for cpdlp_batch_take_state: the five iterates of its current
+// precision (sol, sol_last, avg, avg_last, restart), scaled, widened to double, and its
+// scaling D2 / D1; one workgroup grid-strides over n and m
+__global__ void k_batch_state(const BInst* insts, const BState* states, int32_t slot, double* x, double* y,
+                              double* D2, double* D1) {
+    const BInst I = insts[slot];
+    const BState* S = states + slot;
+    const int c = S->cur;
+    const bool f64 = S->phase == 1;
+    const double* xd[5] = {c ? I.x64[1] : I.x64[0], c ? I.x64[0] : I.x64[1], I.xa64, I.xal64, I.xr64};
+    const double* yd[5] = {c ? I.y64[1] : I.y64[0], c ? I.y64[0] : I.y64[1], I.ya64, I.yal64, I.yr64};
+    const float* xf[5] = {c ? I.x32[1] : I.x32[0], c ? I.x32[0] : I.x32[1], I.xa32, I.xal32, I.xr32};
+    const float* yf[5] = {c ? I.y32[1] : I.y32[0], c ? I.y32[0] : I.y32[1], I.ya32, I.yal32, I.yr32};
+    const int64_t n = I.n, m = I.m;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) x[k * n + j] = f64 ? xd[k][j] : (double)xf[k][j];
+        D2[j] = I.D2[j];
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) y[k * m + i] = f64 ? yd[k][i] : (double)yf[k][i];
+        D1[i] = I.D1[i];
+    }
+}
+
 // ================================================================ host side
 
 static double bnow() {
@@ -1835,6 +1862,47 @@ struct Batch {
         free_slots.push_back(h.slot);
         inst.erase(id);
     }
+
+    // a running instance's loop state, then the instance leaves the batch (run() has
+    // completed its rounds, so no kernel reads its slot or arena region any more)
+    void take_state(int64_t id, cpdlp_state* st, double* x, double* y, double* D2, double* D1) {
+        HIP_CHECK(hipSetDevice(device));
+        BHost& h = get_host(id);
+        if (!h.ready) throw Error(CPDLP_ERR_STATE, "instance not set up yet: run the batch first");
+        if (h.finished) throw Error(CPDLP_ERR_STATE, "instance finished: take its result with cpdlp_batch_get");
+        if ((h.n && (!x || !D2)) || (h.m && (!y || !D1))) throw Error(CPDLP_ERR_ARGUMENT, "NULL state buffer");
+        const int64_t n = h.n, m = h.m;
+        DevBuf<double> xs(std::max<int64_t>(5 * n, 1)), ys(std::max<int64_t>(5 * m, 1));
+        DevBuf<double> d2(std::max<int64_t>(n, 1)), d1(std::max<int64_t>(m, 1));
+        hipLaunchKernelGGL(k_batch_state, dim3(64), dim3(256), 0, streams[0], d_inst.p, d_state.p, h.slot, xs.p,
+                           ys.p, d2.p, d1.p);
+        HIP_CHECK(hipGetLastError());
+        BState s{};
+        HIP_CHECK(hipMemcpyAsync(&s, d_state.p + h.slot, sizeof(BState), hipMemcpyDeviceToHost, streams[0]));
+        if (n) {
+            xs.download(x, 5 * n, streams[0]);
+            d2.download(D2, n, streams[0]);
+        }
+        if (m) {
+            ys.download(y, 5 * m, streams[0]);
+            d1.download(D1, m, streams[0]);
+        }
+        HIP_CHECK(hipStreamSynchronize(streams[0]));
+        if (s.status != CPDLP_STILL_RUNNING) throw Error(CPDLP_ERR_STATE, "instance finished: run the batch first");
+        if (st) {
+            st->eta = s.eta;
+            st->omega = s.omega;
+            st->eta_sum = s.eta_sum;
+            st->kkt_passes = s.kkt_passes;
+            st->inner = s.inner;
+            st->total = s.total;
+            st->restarts = s.restarts;
+            st->fp32_passes = s.phase == 1 ? s.fp32_passes : s.kkt_passes;
+        }
+        arena.release(h.off, h.bytes);
+        free_slots.push_back(h.slot);
+        inst.erase(id);
+    }
 };
 
 }  // namespace cpdlp
@@ -1903,6 +1971,14 @@ int cpdlp_batch_poll(cpdlp_batch* b, int64_t* ids, int64_t cap, int64_t* count) 
             ++k;
         }
         if (count) *count = k;
+    });
+}
+
+int cpdlp_batch_take_state(cpdlp_batch* b, int64_t id, cpdlp_state* st, double* x, double* y, double* D2,
+                           double* D1) {
+    return guarded([&] {
+        BATCH_OR_THROW(b);
+        b->b->take_state(id, st, x, y, D2, D1);
     });
 }
 

@@ -516,11 +516,11 @@ int cpdlp_server_default_config(cpdlp_server_config* c) {
         c->checks_per_round = 20;
         c->n_lanes = 2;
         c->lane_max_nnz[0] = 1e6;
-        c->lane_workers[0] = 1;   // 2 batch streams + 2 plain workers: 4 hardware queues
+        c->lane_workers[0] = 2;
         c->lane_max_nnz[1] = INFINITY;
         c->lane_workers[1] = 1;
         c->batch2_bytes = (int64_t)2 << 30;
-        c->batch2_limit = 65536;
+        c->batch2_limit = 0;   // off: measured, it did not pay on the 70/25/5 mix
         c->batch2_checks = 0;
     });
 }

@@ -11,9 +11,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "layout.hpp"
@@ -1466,6 +1468,128 @@ void Handle::initialize(const double* x0, const double* y0) {
     finish_stats();
 }
 
+// the five iterates of a state (sol, sol_last, avg, avg_last, restart: PDLPState,
+// pdlp.jl:19-43), scaled, in the original order: k_from_internal with D = 1 only permutes
+template <class V>
+void Handle::state_to(IterState<V>& S, double* xs, double* ys, const double* ones) {
+    const int c = S.cur, l = 1 - S.cur;
+    const V* X[CPDLP_STATE_VECTORS] = {S.x[c].p, S.x[l].p, S.xavg.p, S.xavgl.p, S.xrst.p};
+    const V* Y[CPDLP_STATE_VECTORS] = {S.y[c].p, S.y[l].p, S.yavg.p, S.yavgl.p, S.yrst.p};
+    for (int k = 0; k < CPDLP_STATE_VECTORS; ++k) {
+        if (n)
+            hipLaunchKernelGGL(k_from_internal<V>, dim3(grid_n()), dim3(BLOCK), 0, stream, (int32_t)n,
+                               d_int_of_col.p, ones, X[k], xs + k * n);
+        if (m)
+            hipLaunchKernelGGL(k_from_internal<V>, dim3(grid_m()), dim3(BLOCK), 0, stream, (int32_t)m,
+                               d_int_of_row.p, ones, Y[k], ys + k * m);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void Handle::get_state(cpdlp_state* st, double* x, double* y, double* D2, double* D1) {
+    HIP_CHECK(hipSetDevice(device));
+    if (!solved) throw Error(CPDLP_ERR_STATE, "no state: call cpdlp_initialize or cpdlp_solve first");
+    if ((n && (!x || !D2)) || (m && (!y || !D1))) throw Error(CPDLP_ERR_ARGUMENT, "NULL state buffer");
+    const int64_t nn = std::max<int64_t>(n, 1), mm = std::max<int64_t>(m, 1);
+    DevBuf<double> ones(std::max(nn, mm)), xs(CPDLP_STATE_VECTORS * nn), ys(CPDLP_STATE_VECTORS * mm);
+    hipLaunchKernelGGL(k_fill, dim3(std::max(1, ceil_div(std::max(nn, mm), BLOCK))), dim3(BLOCK), 0, stream,
+                       std::max(nn, mm), ones.p, 1.0);
+    if (active64) state_to(st64, xs.p, ys.p, ones.p);
+    else state_to(st32, xs.p, ys.p, ones.p);
+    if (n) {
+        xs.download(x, CPDLP_STATE_VECTORS * n, stream);
+        d_D2o.download(D2, n, stream);
+    }
+    if (m) {
+        ys.download(y, CPDLP_STATE_VECTORS * m, stream);
+        d_D1o.download(D1, m, stream);
+    }
+    sync();
+    if (st) {
+        st->eta = eta;
+        st->omega = omega;
+        st->eta_sum = eta_sum;
+        st->kkt_passes = stats.kkt_passes;
+        st->inner = inner;
+        st->total = total;
+        st->restarts = restarts;
+        st->fp32_passes = active64 ? fp32_passes : stats.kkt_passes;
+    }
+}
+
+// initialize, then the given state in place of the start point: its vectors relative to
+// this handle's scaling (as given when D2 / D1 equal this handle's bit for bit, else
+// through the unscaled point), its scalars and counters
+void Handle::resume(const cpdlp_state& st, const double* x, const double* y, const double* D2,
+                    const double* D1) {
+    if (P.precision == CPDLP_PREC_MIXED)
+        throw Error(CPDLP_ERR_ARGUMENT, "mixed precision runs its two phases through cpdlp_solve; use fp64 or fp32");
+    if ((n && (!x || !D2)) || (m && (!y || !D1))) throw Error(CPDLP_ERR_ARGUMENT, "NULL state buffer");
+    if (!(std::isfinite(st.eta) && st.eta > 0) || !(std::isfinite(st.omega) && st.omega > 0) ||
+        !(std::isfinite(st.eta_sum) && st.eta_sum >= 0))
+        throw Error(CPDLP_ERR_ARGUMENT, "state: eta and omega must be positive, eta_sum non-negative");
+    if (st.kkt_passes < 0 || st.inner < 0 || st.total < st.inner || st.restarts < 0 || st.fp32_passes < 0)
+        throw Error(CPDLP_ERR_ARGUMENT, "state: inconsistent counters");
+    if (begin(nullptr, nullptr)) {   // no constraints and c == 0: solved at once
+        finish_stats();
+        return;
+    }
+    std::vector<double> own2(n), own1(m);
+    if (n) d_D2o.download(own2.data(), n, stream);
+    if (m) d_D1o.download(own1.data(), m, stream);
+    sync();
+    const bool same2 = n == 0 || std::memcmp(own2.data(), D2, (size_t)n * 8) == 0;
+    const bool same1 = m == 0 || std::memcmp(own1.data(), D1, (size_t)m * 8) == 0;
+    const int64_t nn = std::max<int64_t>(n, 1), mm = std::max<int64_t>(m, 1);
+    std::vector<double> hx, hy;
+    if (!same2) {
+        hx.resize((size_t)CPDLP_STATE_VECTORS * n);
+        for (int k = 0; k < CPDLP_STATE_VECTORS; ++k)
+            for (int64_t j = 0; j < n; ++j) hx[k * n + j] = x[k * n + j] * D2[j] / own2[j];
+    }
+    if (!same1) {
+        hy.resize((size_t)CPDLP_STATE_VECTORS * m);
+        for (int k = 0; k < CPDLP_STATE_VECTORS; ++k)
+            for (int64_t i = 0; i < m; ++i) hy[k * m + i] = y[k * m + i] * D1[i] / own1[i];
+    }
+    DevBuf<double> ones(std::max(nn, mm)), xs(CPDLP_STATE_VECTORS * nn), ys(CPDLP_STATE_VECTORS * mm);
+    hipLaunchKernelGGL(k_fill, dim3(std::max(1, ceil_div(std::max(nn, mm), BLOCK))), dim3(BLOCK), 0, stream,
+                       std::max(nn, mm), ones.p, 1.0);
+    if (n) xs.upload(same2 ? x : hx.data(), CPDLP_STATE_VECTORS * n, stream);
+    if (m) ys.upload(same1 ? y : hy.data(), CPDLP_STATE_VECTORS * m, stream);
+    auto put = [&](auto& S) {
+        using V = std::remove_reference_t<decltype(*S.x[0].p)>;
+        S.alloc(n, m);
+        V* X[CPDLP_STATE_VECTORS] = {S.x[0].p, S.x[1].p, S.xavg.p, S.xavgl.p, S.xrst.p};
+        V* Y[CPDLP_STATE_VECTORS] = {S.y[0].p, S.y[1].p, S.yavg.p, S.yavgl.p, S.yrst.p};
+        for (int k = 0; k < CPDLP_STATE_VECTORS; ++k) {   // v / 1.0: the same bits
+            if (n)
+                hipLaunchKernelGGL(k_to_internal<V>, dim3(grid_n()), dim3(BLOCK), 0, stream, (int32_t)n,
+                                   d_col_of_int.p, xs.p + k * n, ones.p, X[k]);
+            if (m)
+                hipLaunchKernelGGL(k_to_internal<V>, dim3(grid_m()), dim3(BLOCK), 0, stream, (int32_t)m,
+                                   d_row_of_int.p, ys.p + k * m, ones.p, Y[k]);
+        }
+        S.cur = 0;
+        HIP_CHECK(hipGetLastError());
+    };
+    active64 = P.precision == CPDLP_PREC_FP64;
+    if (active64) put(st64);
+    else put(st32);
+    sync();   // the staging buffers are freed on return
+    eta = st.eta;
+    omega = st.omega;
+    eta_sum = st.eta_sum;
+    inner = st.inner;
+    total = st.total;
+    restarts = st.restarts;
+    fp32_passes = st.fp32_passes;
+    stats.kkt_passes = st.kkt_passes;
+    rst_err_valid = false;
+    stepping = true;
+    finish_stats();
+}
+
 // solve!'s loop (pdlp.jl:69-94, pdhg.jl:58-74) for at most max_checks chunks of
 // check_every steps; the reference yields once per chunk (pdlp.jl:77).
 int32_t Handle::run_checks(int64_t max_checks) {
@@ -1727,6 +1851,17 @@ int cpdlp_run_checks(cpdlp_handle* h, int64_t max_checks, int32_t* status) {
         HANDLE_OR_THROW(h);
         const int32_t st = h->h->run_checks(max_checks);
         if (status) *status = st;
+    });
+}
+int cpdlp_get_state(cpdlp_handle* h, cpdlp_state* st, double* x, double* y, double* D2, double* D1) {
+    return guarded([&] { HANDLE_OR_THROW(h); h->h->get_state(st, x, y, D2, D1); });
+}
+int cpdlp_resume(cpdlp_handle* h, const cpdlp_state* st, const double* x, const double* y, const double* D2,
+                 const double* D1) {
+    return guarded([&] {
+        HANDLE_OR_THROW(h);
+        if (!st) throw Error(CPDLP_ERR_ARGUMENT, "NULL state");
+        h->h->resume(*st, x, y, D2, D1);
     });
 }
 int cpdlp_get_solution_device(cpdlp_handle* h, double* x_dev, double* y_dev) {

@@ -147,6 +147,10 @@ struct Handle {
     void solve(const double* x0, const double* y0);
     void initialize(const double* x0, const double* y0);
     int32_t run_checks(int64_t max_checks);
+    template <class V>
+    void state_to(IterState<V>& S, double* xs, double* ys, const double* ones);
+    void get_state(cpdlp_state* st, double* x, double* y, double* D2, double* D1);
+    void resume(const cpdlp_state& st, const double* x, const double* y, const double* D2, const double* D1);
     void solution_to(double* xo, double* yo);
     void get_solution_device(double* x_dev, double* y_dev);
     void progress(bool final_line);

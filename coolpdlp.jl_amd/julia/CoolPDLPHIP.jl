@@ -354,6 +354,68 @@ function hip_solve!(state::HIPState)
     return status_of(status[])
 end
 
+# the loop's state between two checks (cpdlp_state): PDLPState, pdlp.jl:19-43, less its
+# scratch and per-check statistics
+struct CState
+    eta::Float64
+    omega::Float64
+    eta_sum::Float64
+    kkt_passes::Int64
+    inner::Int64
+    total::Int64
+    restarts::Int64
+    fp32_passes::Int64
+end
+
+"""
+    HIPCheckpoint
+
+A chunked solve's loop state (`hip_checkpoint`): the scalars, the five primal and dual
+vectors (sol, sol_last, sol_avg, sol_avg_last, sol_restart; scaled, original order, one
+column each) and the scaling they belong to.  `hip_resume` continues from it.
+"""
+struct HIPCheckpoint
+    st::CState
+    x::Matrix{Float64}
+    y::Matrix{Float64}
+    D2::Vector{Float64}
+    D1::Vector{Float64}
+end
+
+function hip_checkpoint(state::HIPState)
+    x, y = zeros(state.n, 5), zeros(state.m, 5)
+    D2, D1 = zeros(state.n), zeros(state.m)
+    st = Ref{CState}()
+    check(ccall((:cpdlp_get_state, libcpdlp), Cint,
+        (Ptr{Cvoid}, Ref{CState}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        state.handle, st, x, y, D2, D1))
+    return HIPCheckpoint(st[], x, y, D2, D1)
+end
+
+"""
+    hip_resume(milp, checkpoint, algo)
+
+A new chunked solve of `milp` that continues the run `checkpoint` was taken from
+(`cpdlp_resume`): with the same scaling the iterates are those of the run that never
+stopped.  Then `hip_solve!` and `hip_result` as after `hip_initialize`.
+"""
+function hip_resume(milp::MILP, cpt::HIPCheckpoint, algo::Algorithm{A, T, Ti, M, HIPEngine}) where {A, T, Ti, M}
+    H = HostLP(milp)
+    p = cparams(algo)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve H begin
+        check(ccall((:cpdlp_create, libcpdlp), Cint, (Ref{CLP}, Ref{CParams}, Ptr{Ptr{Cvoid}}), H.lp, p, h))
+    end
+    state = HIPState{T}(h[], H.lp.m, H.lp.n)
+    finalizer(s -> ccall((:cpdlp_destroy, libcpdlp), Cvoid, (Ptr{Cvoid},), s.handle), state)
+    size(cpt.x) == (state.n, 5) && size(cpt.y) == (state.m, 5) ||
+        throw(DimensionMismatch("checkpoint does not match the LP's size"))
+    check(ccall((:cpdlp_resume, libcpdlp), Cint,
+        (Ptr{Cvoid}, Ref{CState}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        state.handle, cpt.st, cpt.x, cpt.y, cpt.D2, cpt.D1))
+    return state
+end
+
 "The solution and stats of a chunked solve, as `solve` returns them."
 function hip_result(state::HIPState{T}) where {T}
     x, y = zeros(state.n), zeros(state.m)
@@ -435,7 +497,7 @@ mutable struct CServerConfig
 end
 
 """
-    HIPServer(algo; batch_limit = 8192, batch2_limit = 65536, checks_per_round = 20)
+    HIPServer(algo; batch_limit = 8192, batch2_limit = 0, checks_per_round = 20)
 
 The library's continuous-batching server: `submit!` queues an LP (optionally with a start
 point; the library copies it), `pump!` runs one scheduling round (or `start!` lets the
@@ -450,7 +512,7 @@ mutable struct HIPServer{T}
 end
 
 function HIPServer(algo::Algorithm{A, T, Ti, M, HIPEngine}; batch_limit::Integer = 8192,
-        batch2_limit::Integer = 65536, checks_per_round::Integer = 20) where {A, T, Ti, M}
+        batch2_limit::Integer = 0, checks_per_round::Integer = 20) where {A, T, Ti, M}
     p = cparams(algo)
     cfg = CServerConfig()
     check(ccall((:cpdlp_server_default_config, libcpdlp), Cint, (Ref{CServerConfig},), cfg))

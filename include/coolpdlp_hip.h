@@ -135,6 +135,25 @@ typedef struct cpdlp_stats {
     int64_t step_kernel_launches;
 } cpdlp_stats;
 
+/* The loop's state between two checks (no reference counterpart as a value; it is
+ * PDLPState, src/algorithms/pdlp.jl:19-43, less its scratch and per-check statistics, which
+ * the next check recomputes): for checkpoint and resume, and for continuing a batch
+ * instance on a plain engine.  The vectors travel beside it in caller buffers:
+ * CPDLP_STATE_VECTORS primal vectors of n entries (x) and as many dual vectors of m (y),
+ * in the order sol, sol_last, sol_avg, sol_avg_last, sol_restart, SCALED (the solver's own
+ * variables), in the LP's original order, with the diagonal scaling they belong to (D2,
+ * n; D1, m; the unscaled point is x .* D2, y .* D1). */
+#define CPDLP_STATE_VECTORS 5
+typedef struct cpdlp_state {
+    double eta;                     /* step size (StepSizes.η) */
+    double omega;                   /* primal weight (StepSizes.ω) */
+    double eta_sum;                 /* step sizes summed since the last restart (StepSizes.η_sum) */
+    int64_t kkt_passes;             /* ConvergenceStats.kkt_passes */
+    int64_t inner, total;           /* IterationCounter (iteration.jl:1-5) */
+    int64_t restarts;               /* IterationCounter.outer */
+    int64_t fp32_passes;            /* passes run in FP32 so far */
+} cpdlp_state;
+
 typedef struct cpdlp_handle cpdlp_handle;
 typedef struct cpdlp_matrix cpdlp_matrix;
 
@@ -164,6 +183,23 @@ int cpdlp_initialize(cpdlp_handle* h, const double* x0, const double* y0);
  * (CPDLP_STILL_RUNNING while the loop has not terminated).  Running the chunks one
  * call at a time gives the same iterates as one cpdlp_solve. */
 int cpdlp_run_checks(cpdlp_handle* h, int64_t max_checks, int32_t* status);
+
+/* The loop's state after the last check (cpdlp_run_checks, cpdlp_solve): *st, x
+ * (CPDLP_STATE_VECTORS * n), y (CPDLP_STATE_VECTORS * m), D2 (n) and D1 (m), host buffers
+ * (see cpdlp_state; FP32 iterates are widened to double).  No reference counterpart. */
+int cpdlp_get_state(cpdlp_handle* h, cpdlp_state* st, double* x, double* y, double* D2, double* D1);
+
+/* cpdlp_initialize, then the state of another run of the same LP in place of the start
+ * point: cpdlp_run_checks continues that run.  Vectors whose scaling (D2, D1) equals this
+ * handle's, bit for bit, are taken as they are, so a run stopped after some checks and
+ * resumed here gives the same iterates as one that never stopped (checkpoint and
+ * resume); others are carried over through the unscaled point (x .* D2 ./ D2_own), so a
+ * batch instance continues on this engine (cpdlp_batch_take_state).  The step size is the
+ * state's.  The time limit counts from the resume, and the error history restarts.
+ * FP64 and FP32 handles (CPDLP_ERR_ARGUMENT for mixed, as cpdlp_initialize).  No
+ * reference counterpart. */
+int cpdlp_resume(cpdlp_handle* h, const cpdlp_state* st, const double* x, const double* y,
+                 const double* D2, const double* D1);
 
 /* The solution kept on the device, as the reference's sol (common.jl:200): unscaled x
  * (n) and y (m), original order, into caller-owned DEVICE buffers (either may be NULL).
@@ -274,6 +310,14 @@ int cpdlp_batch_get(cpdlp_batch* b, int64_t id, double* x, double* y, cpdlp_stat
 /* Return an instance's HBM to the arena (after cpdlp_batch_get). */
 int cpdlp_batch_release(cpdlp_batch* b, int64_t id);
 
+/* Take a running instance out of the batch with its loop state (as cpdlp_get_state:
+ * *st, x (CPDLP_STATE_VECTORS * n), y (CPDLP_STATE_VECTORS * m), D2 (n), D1 (m), host
+ * buffers), for cpdlp_resume on a plain engine of the same LP: the instance's HBM returns
+ * to the arena and its id becomes unknown.  Only between rounds, for an instance that is
+ * set up and still running (else CPDLP_ERR_STATE).  No reference counterpart. */
+int cpdlp_batch_take_state(cpdlp_batch* b, int64_t id, cpdlp_state* st, double* x, double* y,
+                           double* D2, double* D1);
+
 /* Bytes of the arena in use and its size. */
 int cpdlp_batch_memory(cpdlp_batch* b, int64_t* used, int64_t* size);
 
@@ -325,10 +369,10 @@ typedef struct cpdlp_server_config {
     int32_t batch_limit;            /* m + n limit of the batch route, <= 65536; 0: none (8192) */
     int32_t checks_per_round;       /* checks per pump for the smallest instances (20) */
     int32_t n_lanes;                /* plain lanes, 1..CPDLP_SERVER_MAX_LANES (2) */
-    int32_t lane_workers[CPDLP_SERVER_MAX_LANES];   /* worker threads per lane (1, 1) */
+    int32_t lane_workers[CPDLP_SERVER_MAX_LANES];   /* worker threads per lane (2, 1) */
     double lane_max_nnz[CPDLP_SERVER_MAX_LANES];    /* nonzeros a lane admits (1e6, inf) */
     int64_t batch2_bytes;           /* second batch route's HBM arena (2 GiB) */
-    int32_t batch2_limit;           /* its m + n limit, <= 65536; none when <= batch_limit (65536) */
+    int32_t batch2_limit;           /* its m + n limit, <= 65536; none when <= batch_limit (0) */
     int32_t batch2_checks;          /* its checks per round; 0: checks_per_round (0) */
 } cpdlp_server_config;
 typedef struct cpdlp_server cpdlp_server;

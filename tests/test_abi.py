@@ -36,7 +36,8 @@ def test_struct_layout_matches_c(tmp_path):
     from coolpdlp_hip import _lib
     prog = tmp_path / "layout.c"
     structs = {"cpdlp_lp": _lib.CLP, "cpdlp_params": _lib.CParams, "cpdlp_kkt": _lib.CKKT,
-               "cpdlp_stats": _lib.CStats, "cpdlp_server_config": _lib.CServerConfig}
+               "cpdlp_stats": _lib.CStats, "cpdlp_server_config": _lib.CServerConfig,
+               "cpdlp_state": _lib.CState}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -118,9 +119,9 @@ def test_server_config_defaults_and_validation():
     _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(cfg)))
     assert (cfg.batch_bytes, cfg.plain_bytes) == (16 << 30, 64 << 30)
     assert (cfg.batch_limit, cfg.checks_per_round, cfg.n_lanes) == (8192, 20, 2)
-    assert list(cfg.lane_workers)[:2] == [1, 1] and cfg.lane_max_nnz[0] == 1e6
+    assert list(cfg.lane_workers)[:2] == [2, 1] and cfg.lane_max_nnz[0] == 1e6
     assert cfg.lane_max_nnz[1] == float("inf")
-    assert (cfg.batch2_bytes, cfg.batch2_limit, cfg.batch2_checks) == (2 << 30, 65536, 0)
+    assert (cfg.batch2_bytes, cfg.batch2_limit, cfg.batch2_checks) == (2 << 30, 0, 0)
     import coolpdlp_hip as cp
     for field, value in [("batch_limit", 70000), ("n_lanes", 0), ("n_lanes", 5), ("checks_per_round", 0),
                          ("batch2_limit", 70000), ("batch2_limit", -1), ("batch2_checks", -1)]:

@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "coolpdlp_hip.h")
 BINDING = os.path.join(ROOT, "coolpdlp.jl_amd", "julia", "CoolPDLPHIP.jl")
 
 STRUCTS = {"CLP": "cpdlp_lp", "CParams": "cpdlp_params", "CKKT": "cpdlp_kkt", "CStats": "cpdlp_stats",
-           "CServerConfig": "cpdlp_server_config"}
+           "CServerConfig": "cpdlp_server_config", "CState": "cpdlp_state"}
 SCALARS = {"int64_t": "Int64", "int32_t": "Int32", "uint64_t": "UInt64", "double": "Float64"}
 
 
