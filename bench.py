@@ -809,10 +809,10 @@ def main():
                     help="A,At formats for an auto run (sets CPDLP_FORMATS; for counter runs)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--checks-per-round", type=int, default=20)
-    ap.add_argument("--batch-tail", type=int, default=0,
+    ap.add_argument("--batch-tail", type=int, default=16,
                     help="continue the last N unfinished batch LPs on plain engines (0: off)")
     ap.add_argument("--batch-tail-passes", type=int, default=2000)
-    ap.add_argument("--batch-tail-workers", type=int, default=2)
+    ap.add_argument("--batch-tail-workers", type=int, default=8)
     ap.add_argument("--plain-sample", type=int, default=0,
                     help="batch: LPs solved one at a time for the comparison (0: all of them)")
     ap.add_argument("--qps", type=float, nargs="+", default=[100.0, 300.0, 1000.0, 3000.0, 5000.0])

@@ -35,12 +35,12 @@ class BatchSolver:
     unfinished, every one of them that has run at least `tail_min_passes` passes leaves
     the batch with its loop state (take_state) and continues on a plain FP64 engine (CSR,
     Engine.resume, many CUs per LP) on one of `tail_workers` threads; poll and get report
-    it like any other.  Its result then comes from a different engine than the batch's,
-    so it is not the batch-alone result bit for bit; its pass count includes the batch's
-    passes, and its time limit counts again from the hand-over."""
+    it like any other.  A continued LP's result comes from a different engine than the
+    batch's, so it is not the batch-alone result bit for bit; its pass count includes the
+    batch's passes, and its time limit counts again from the hand-over."""
 
     def __init__(self, algo: Algorithm | None = None, hbm_bytes: int = 1 << 30, *, tail: int = 0,
-                 tail_min_passes: int = 2000, tail_workers: int = 2):
+                 tail_min_passes: int = 2000, tail_workers: int = 8):
         self.algo = algo if algo is not None else PDLP("mixed")
         h = C.c_void_p()
         check(lib.cpdlp_batch_create(C.byref(self.algo.params), int(hbm_bytes), C.byref(h)))
@@ -110,7 +110,7 @@ class BatchSolver:
         running = r.value
         if self.tail > 0:
             if 0 < running + self._cont_running() <= self.tail:
-                running -= self._continue_stragglers()
+                running -= self._continue_stragglers(self.tail_min_passes)
             pending = [f for f in self._cont.values() if not f.done()]
             if running == 0 and pending:   # only continued LPs left: wait for one
                 cf.wait(pending, timeout=0.05, return_when=cf.FIRST_COMPLETED)
@@ -128,14 +128,14 @@ class BatchSolver:
             termination_status=TerminationStatus(s.termination_status), eta=s.eta, omega=s.omega,
             powm_iters=s.powm_iters, restarts=s.restarts, fp32_passes=s.fp32_passes)
 
-    def _continue_stragglers(self) -> int:
-        """Hand the unfinished LPs with enough passes to plain engines; returns how many."""
+    def _continue_stragglers(self, min_passes: int) -> int:
+        """Hand the unfinished LPs past min_passes passes to plain engines; returns how many."""
         k = 0
         for i in list(self.shapes):
             if i in self._cont:
                 continue
             st0 = self._stats_only(i)
-            if st0.termination_status != TerminationStatus.STILL_RUNNING or st0.kkt_passes < self.tail_min_passes:
+            if st0.termination_status != TerminationStatus.STILL_RUNNING or st0.kkt_passes < min_passes:
                 continue
             shape = self.shapes[i]
             state = self.take_state(i)
