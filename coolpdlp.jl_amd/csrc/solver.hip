@@ -154,6 +154,18 @@ __global__ void k_from_internal(int32_t n, const int32_t* int_of, const double* 
     if (j >= n) return;
     out[j] = D[j] * (double)v[int_of[j]];
 }
+// four vectors interleaved, out[4 i + k] = v_k[i]: one 4-wide store per entry
+template <class V>
+__global__ void k_interleave4(int64_t n, const V* a, const V* b, const V* c, const V* d, V* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    VecK<V, 4> t;
+    t.v[0] = a[i];
+    t.v[1] = b[i];
+    t.v[2] = c[i];
+    t.v[3] = d[i];
+    *(VecK<V, 4>*)(out + 4 * i) = t;
+}
 template <class S, class T>
 __global__ void k_convert(int64_t n, const S* in, T* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -462,6 +474,7 @@ static int64_t wrap_cube(int64_t n) {
 
 Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
     no_rst_cache = std::getenv("CPDLP_KKT_FULL_CHECK") != nullptr;
+    no_aos = std::getenv("CPDLP_KKT_SEPARATE") != nullptr;
     if (P.check_every <= 0) throw Error(CPDLP_ERR_ARGUMENT, "check_every must be positive");
     if (P.precision < 0 || P.precision > 2) throw Error(CPDLP_ERR_ARGUMENT, "unknown precision");
     if (P.format < CPDLP_FMT_AUTO || P.format > CPDLP_FMT_XSLICE_ALL)
@@ -858,14 +871,52 @@ void Handle::kkt_launch_into(const V* const* X, const V* const* Y, const V* xr, 
     const MatView<V> vA = A.view<V>(), vAt = At.view<V>();
     const int gA = vA.partial_blocks(), gAt = vAt.partial_blocks();
     constexpr int NQR = EpiKktRows<V, K>::NQ, NQC = EpiKktCols<V, K>::NQ;
-    // The rows side gathers the K candidates' x through the column slices: K * n / ns
+    // 4 candidates: both sides gather them interleaved (one request per entry instead of
+    // 4), the same products summed in the same order, so the same bits
+    bool aos = false;
+    if constexpr (K == 4)
+        aos = !no_aos && vA.format != CPDLP_FMT_ELL && vAt.format != CPDLP_FMT_ELL;
+    // Else the rows side gathers the K candidates' x through the column slices: K * n / ns
     // elements per XCD.  With 4 candidates beyond an XCD's 4 MiB L2 it runs as two passes
     // of 2 (each vector's sums are the same numbers in either form).
     bool split = false;
     if constexpr (K == 4)
-        split = vA.format == CPDLP_FMT_XSLICE && vA.ns > 0 &&
+        split = !aos && vA.format == CPDLP_FMT_XSLICE && vA.ns > 0 &&
                 (size_t)n * sizeof(V) * K / (size_t)vA.ns > ((size_t)4 << 20);
-    if (split) {
+    V* ax = nullptr;
+    V* ay = nullptr;
+    if constexpr (K == 4) {
+        if (aos) {
+            const size_t need_x = ((size_t)std::max<int64_t>(n, 1) * 4 * sizeof(V) + 7) / 8;
+            const size_t need_y = ((size_t)std::max<int64_t>(m, 1) * 4 * sizeof(V) + 7) / 8;
+            if (aos_x.n < need_x || aos_y.n < need_y) {
+                sync();   // a buffer in use by queued work is never freed under it
+                if (aos_x.n < need_x) aos_x.alloc(need_x);
+                if (aos_y.n < need_y) aos_y.alloc(need_y);
+            }
+            ax = (V*)aos_x.p;
+            ay = (V*)aos_y.p;
+            if (n)
+                hipLaunchKernelGGL(k_interleave4<V>, dim3(ceil_div(n, BLOCK)), dim3(BLOCK), 0, s, (int64_t)n, X[0],
+                                   X[1], X[2], X[3], ax);
+            if (m)
+                hipLaunchKernelGGL(k_interleave4<V>, dim3(ceil_div(m, BLOCK)), dim3(BLOCK), 0, s, (int64_t)m, Y[0],
+                                   Y[1], Y[2], Y[3], ay);
+        }
+    }
+    if (aos) {
+        if constexpr (K == 4) {
+            EpiKktRows<V, K> er;
+            er.partials = partials;
+            for (int k = 0; k < K; ++k) er.Y[k] = Y[k];
+            er.y_rst = yr ? yr : Y[0];
+            er.lc = lc_s.p;
+            er.uc = uc_s.p;
+            er.invD1 = invD1_s.p;
+            er.dist = dist;
+            launch_spmv<V, double, K>(vA, XPackAoS<V, K>{ax}, er, s);
+        }
+    } else if (split) {
         if constexpr (K == 4) {
             constexpr int NQ2 = EpiKktRows<V, 2>::NQ;
             for (int half = 0; half < 2; ++half) {
@@ -903,9 +954,13 @@ void Handle::kkt_launch_into(const V* const* X, const V* const* Y, const V* xr, 
     ec.uv = uv_s.p;
     ec.invD2 = invD2_s.p;
     ec.dist = dist;
-    XPack<V, K> yp;
-    for (int k = 0; k < K; ++k) yp.p[k] = Y[k];
-    launch_spmv<V, double, K>(vAt, yp, ec, s);
+    if (aos) {
+        if constexpr (K == 4) launch_spmv<V, double, K>(vAt, XPackAoS<V, K>{ay}, ec, s);
+    } else {
+        XPack<V, K> yp;
+        for (int k = 0; k < K; ++k) yp.p[k] = Y[k];
+        launch_spmv<V, double, K>(vAt, yp, ec, s);
+    }
     if (split) {   // two passes' sums, reassembled below into the K = 4 layout
         constexpr int NQ2 = EpiKktRows<V, 2>::NQ;
         for (int half = 0; half < 2; ++half)

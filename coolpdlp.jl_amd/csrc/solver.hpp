@@ -81,6 +81,9 @@ struct Handle {
     DevSide A, At;
     // scratch
     DevBuf<double> d_partials, d_partials2, d_sums;
+    // the 4-candidate check's vectors interleaved (x: 4 n, y: 4 m entries of the check's
+    // precision), so that a gather brings all four with one request
+    DevBuf<double> aos_x, aos_y;
     DevBuf<StepCoef> d_coef;
     GraphCache graphs[4];
     // solver scalars
@@ -101,6 +104,7 @@ struct Handle {
     cpdlp_kkt rst_err{};
     bool rst_err_valid = false;
     bool no_rst_cache = false;   // CPDLP_KKT_FULL_CHECK set: evaluate all 5 every check
+    bool no_aos = false;         // CPDLP_KKT_SEPARATE set: the 4 candidates as separate vectors
     IterState<double> st64;
     IterState<float> st32;
     cpdlp_stats stats{};

@@ -102,16 +102,34 @@ constexpr int SL_KMAX = 5;   // most vectors per product in the sliced format
 constexpr int SL_RPT = 8;    // short rows per thread in a sliced tile
 
 template <class V, int K>
-struct XPack {
+struct XPack {   // K separate vectors
+    static constexpr bool aos = false;
     const V* p[K];
 };
+
+// K vectors interleaved: vector kv's entry c at p[c * K + kv] (K a power of two, p aligned
+// to K * sizeof(V)), so one wide load brings an entry's K values: one L2 request per
+// gather instead of K (the K-candidate check)
+template <class V, int K>
+struct XPackAoS {
+    static constexpr bool aos = true;
+    const V* p;
+};
+template <class V, int K>
+struct alignas(K * sizeof(V)) VecK {
+    V v[K];
+};
+template <class V, int K>
+__device__ __forceinline__ VecK<V, K> load_k(const XPackAoS<V, K>& X, int c) {
+    return *(const VecK<V, K>*)(X.p + (size_t)c * K);
+}
 
 // Long row r (one wavefront): lane l takes k = l, l+64, ...; U entries are loaded
 // (clamped, so every load is unconditional and all are in flight together) before any
 // is accumulated; accumulation stays in ascending k per lane; then a butterfly.
-template <class V, class Acc, int K>
+template <class V, class Acc, int K, class XP>
 __device__ __forceinline__ void long_row_sums(const int32_t* rowptr, const int32_t* col,
-                                              const V* val, int r, const XPack<V, K>& X,
+                                              const V* val, int r, const XP& X,
                                               Acc (&acc)[K]) {
     const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
@@ -127,14 +145,25 @@ __device__ __forceinline__ void long_row_sums(const int32_t* rowptr, const int32
             c[u] = __builtin_nontemporal_load(col + k);
             v[u] = (Acc)__builtin_nontemporal_load(val + k);
         }
+        if constexpr (XP::aos) {
+            VecK<V, K> xk[U];
 #pragma unroll
-        for (int kv = 0; kv < K; ++kv) {
-            Acc xv[U];
+            for (int u = 0; u < U; ++u) xk[u] = load_k(X, c[u]);
 #pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[kv][c[u]];
+            for (int kv = 0; kv < K; ++kv)
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (kb + u * WAVE < k1) acc[kv] += v[u] * xv[u];
+                for (int u = 0; u < U; ++u)
+                    if (kb + u * WAVE < k1) acc[kv] += v[u] * (Acc)xk[u].v[kv];
+        } else {
+#pragma unroll
+            for (int kv = 0; kv < K; ++kv) {
+                Acc xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[kv][c[u]];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (kb + u * WAVE < k1) acc[kv] += v[u] * xv[u];
+            }
         }
     }
 #pragma unroll
@@ -146,9 +175,9 @@ __device__ __forceinline__ void long_row_sums(const int32_t* rowptr, const int32
 // gathers are in flight together; products go to LDS and each thread sums its row
 // sequentially from zero in ascending k.  Slots past the tile's nnz are written but
 // never summed.  All threads of the block must call this (it synchronises).
-template <class V, class Acc, int K>
+template <class V, class Acc, int K, class XP>
 __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* col,
-                                          const V* val, const TileMeta& tm, const XPack<V, K>& X,
+                                          const V* val, const TileMeta& tm, const XP& X,
                                           Acc* prod, Acc (&acc)[K]) {
     const int tid = threadIdx.x;
     const int base = tm.base, cnt = tm.cnt;
@@ -169,23 +198,37 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
         cc[j] = __builtin_nontemporal_load(col + base + e);
         vv[j] = __builtin_nontemporal_load(val + base + e);
     }
+    if constexpr (XP::aos) {   // an entry's K values in one load, then the K products in turn
+        VecK<V, K> xk[PER_T];
 #pragma unroll
-    for (int kv = 0; kv < K; ++kv) {
-        Acc xv[PER_T];
+        for (int j = 0; j < PER_T; ++j) xk[j] = load_k(X, cc[j]);
 #pragma unroll
-        for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
+        for (int kv = 0; kv < K; ++kv) {
 #pragma unroll
-        for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
-        __syncthreads();
-        acc[kv] = seq_sum(prod, lo, hi);
-        __syncthreads();
+            for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * (Acc)xk[j].v[kv];
+            __syncthreads();
+            acc[kv] = seq_sum(prod, lo, hi);
+            __syncthreads();
+        }
+    } else {
+#pragma unroll
+        for (int kv = 0; kv < K; ++kv) {
+            Acc xv[PER_T];
+#pragma unroll
+            for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
+#pragma unroll
+            for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
+            __syncthreads();
+            acc[kv] = seq_sum(prod, lo, hi);
+            __syncthreads();
+        }
     }
 }
 
 // (65 VGPRs, 7 waves per SIMD for the FP64 primal step.  Forcing 8 waves cut it to 49
 // VGPRs but fewer loads in flight: 145 us instead of 130 us on the large LP.)
-template <class V, class Acc, int K, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X, Epi epi) {
+template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>>
+__global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XP X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
     double q[Epi::NQ > 0 ? Epi::NQ : 1];
 #pragma unroll
@@ -197,7 +240,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X,
         if (r < A.n_long) {
             typename Epi::Pre pre{};
             if ((threadIdx.x & (WAVE - 1)) == 0) pre = epi.load(r);
-            long_row_sums<V, Acc, K>(A.rowptr, A.col, A.val, r, X, acc);
+            long_row_sums<V, Acc, K, XP>(A.rowptr, A.col, A.val, r, X, acc);
             if ((threadIdx.x & (WAVE - 1)) == 0) epi.row(r, acc, q, pre);
         }
     } else if ((int)blockIdx.x >= A.csr_short0()) {
@@ -209,7 +252,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XPack<V, K> X,
         const int r = tm.r0 + (int)threadIdx.x;
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);   // issued before the gathers
-        tile_sums<V, Acc, K>(A.rowptr, A.col, A.val, tm, X, prod, acc);
+        tile_sums<V, Acc, K, XP>(A.rowptr, A.col, A.val, tm, X, prod, acc);
         if (r < tm.r1) epi.row(r, acc, q, pre);
     }
     if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
@@ -234,8 +277,8 @@ __device__ __forceinline__ Acc group_sum(Acc v) {
 // finished by the epilogue exactly as in k_spmv_csr, block partials at index
 // b - ns * sl_max_tiles.  Deterministic; the sliced rows' order of summation differs
 // from a sequential sum (as the wavefront path's does).
-template <class V, class Acc, int K, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K> X, Epi epi) {
+template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>>
+__global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XP X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
     __shared__ Acc red[K][BLOCK / WAVE];
     if (epi.skip()) return;
@@ -250,7 +293,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K>
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);
         Acc acc[K];
-        tile_sums<V, Acc, K>(A.rowptr, A.col, A.val, tm, X, prod, acc);
+        tile_sums<V, Acc, K, XP>(A.rowptr, A.col, A.val, tm, X, prod, acc);
         if (r < tm.r1) epi.row(r, acc, q, pre);
         if constexpr (Epi::NQ > 0)
             block_sum<Epi::NQ>(q, epi.partials + (size_t)(blockIdx.x - nsb) * Epi::NQ);
@@ -278,14 +321,25 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K>
                 c[u] = __builtin_nontemporal_load(A.sl_col + k);
                 v[u] = (Acc)__builtin_nontemporal_load(A.sl_val + k);
             }
+            if constexpr (XP::aos) {
+                VecK<V, K> xk[U];
 #pragma unroll
-            for (int kv = 0; kv < K; ++kv) {
-                Acc xv[U];
+                for (int u = 0; u < U; ++u) xk[u] = load_k(X, c[u]);
 #pragma unroll
-                for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[kv][c[u]];
+                for (int kv = 0; kv < K; ++kv)
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (kb + u * BLOCK < k1) acc[kv] += v[u] * xv[u];
+                    for (int u = 0; u < U; ++u)
+                        if (kb + u * BLOCK < k1) acc[kv] += v[u] * (Acc)xk[u].v[kv];
+            } else {
+#pragma unroll
+                for (int kv = 0; kv < K; ++kv) {
+                    Acc xv[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) xv[u] = (Acc)X.p[kv][c[u]];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (kb + u * BLOCK < k1) acc[kv] += v[u] * xv[u];
+                }
             }
         }
 #pragma unroll
@@ -328,33 +382,57 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XPack<V, K>
     };
     int cc[PER_T];
     V vv[PER_T];
-    if (tm.cnt > 0) {   // uniform per block; loads and the first vector's gathers together
+    if constexpr (XP::aos) {   // an entry's K values in one load, then the K products in turn
+        VecK<V, K> xk[PER_T];
+        if (tm.cnt > 0) {   // uniform per block
 #pragma unroll
-        for (int j = 0; j < PER_T; ++j) {
-            const int e = min(tid + j * BLOCK, tm.cnt - 1);
-            cc[j] = __builtin_nontemporal_load(A.sl_col + tm.base + e);
-            vv[j] = __builtin_nontemporal_load(A.sl_val + tm.base + e);
+            for (int j = 0; j < PER_T; ++j) {
+                const int e = min(tid + j * BLOCK, tm.cnt - 1);
+                cc[j] = __builtin_nontemporal_load(A.sl_col + tm.base + e);
+                vv[j] = __builtin_nontemporal_load(A.sl_val + tm.base + e);
+            }
+#pragma unroll
+            for (int j = 0; j < PER_T; ++j) xk[j] = load_k(X, cc[j]);
         }
-        Acc xv[PER_T];
 #pragma unroll
-        for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[0][cc[j]];
+        for (int kv = 0; kv < K; ++kv) {   // prod is reused by the next vector
+            if (kv > 0) __syncthreads();
+            if (tm.cnt > 0) {
 #pragma unroll
-        for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
-    }
-    __syncthreads();
-    row_parts(0);
+                for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * (Acc)xk[j].v[kv];
+            }
+            __syncthreads();
+            row_parts(kv);
+        }
+    } else {
+        if (tm.cnt > 0) {   // uniform per block; loads and the first vector's gathers together
 #pragma unroll
-    for (int kv = 1; kv < K; ++kv) {   // prod is reused by the next vector
-        __syncthreads();
-        if (tm.cnt > 0) {
+            for (int j = 0; j < PER_T; ++j) {
+                const int e = min(tid + j * BLOCK, tm.cnt - 1);
+                cc[j] = __builtin_nontemporal_load(A.sl_col + tm.base + e);
+                vv[j] = __builtin_nontemporal_load(A.sl_val + tm.base + e);
+            }
             Acc xv[PER_T];
 #pragma unroll
-            for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
+            for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[0][cc[j]];
 #pragma unroll
             for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
         }
         __syncthreads();
-        row_parts(kv);
+        row_parts(0);
+#pragma unroll
+        for (int kv = 1; kv < K; ++kv) {   // prod is reused by the next vector
+            __syncthreads();
+            if (tm.cnt > 0) {
+                Acc xv[PER_T];
+#pragma unroll
+                for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
+#pragma unroll
+                for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
+            }
+            __syncthreads();
+            row_parts(kv);
+        }
     }
 }
 
@@ -425,27 +503,28 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_ell(MatView<V> A, XPack<V, K> X,
     if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
 }
 
-template <class V, class Acc, int K, class Epi>
-inline void launch_spmv(const MatView<V>& A, const XPack<V, K>& X, const Epi& epi, hipStream_t s) {
+template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>>
+inline void launch_spmv(const MatView<V>& A, const XP& X, const Epi& epi, hipStream_t s) {
     const int g = A.grid();
     if (g <= 0) return;
     if constexpr (K <= SL_KMAX) {
         if (A.format == CPDLP_FMT_XSLICE) {
-            hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
+            hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi, XP>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
             if (A.sl_rows > 0)
                 hipLaunchKernelGGL((k_xslice_combine<V, Acc, K, Epi>), dim3(ceil_div(A.sl_rows, BLOCK)),
                                    dim3(BLOCK), 0, s, A, epi);
             return;
         }
     }
-    if (A.format == CPDLP_FMT_ELL)
-        hipLaunchKernelGGL((k_spmv_ell<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
-    else {
+    if (A.format == CPDLP_FMT_ELL) {
+        if constexpr (XP::aos) throw Error(CPDLP_ERR_ARGUMENT, "interleaved vectors need a CSR or sliced format");
+        else hipLaunchKernelGGL((k_spmv_ell<V, Acc, K, Epi>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
+    } else {
         // the sliced format's long rows are also in the plain CSR, which serves K > 1
         // and the block-partial reductions
         MatView<V> v = A;
         v.format = CPDLP_FMT_CSR;
-        hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
+        hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi, XP>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
     }
 }
 

@@ -427,10 +427,12 @@ def test_full_size_xslice_tracks_csr(cp, large_lp, fmt, code):
                                                  ("float64", "xslice", "wide")])
 def test_restart_error_cache_keeps_the_bits(cp, precision, fmt, which):
     """The check reuses the restart point's KKT errors between restarts (4 candidates
-    evaluated instead of 5), and with n beyond 1M sliced columns runs its rows side as two
-    passes of 2 candidates ("wide": 4 x 8 B x n / 8 slices > 4 MiB per XCD): the whole error
-    history, restarts, pass count and solution are the same bits as evaluating all 5 in
-    one pass at every check (CPDLP_KKT_FULL_CHECK)."""
+    evaluated instead of 5) and gathers the 4 candidates interleaved (one request per
+    entry); with the candidates as separate vectors (CPDLP_KKT_SEPARATE) and n beyond 1M
+    sliced columns it runs its rows side as two passes of 2 candidates ("wide": 4 x 8 B x
+    n / 8 slices > 4 MiB per XCD).  In every form the whole error history, restarts, pass
+    count and solution are the same bits as evaluating all 5 as separate vectors in one
+    pass at every check (CPDLP_KKT_FULL_CHECK)."""
     import os
     g = (G.column_lp(400, 1000, 6, seed=23, box_frac=0.3) if which == "column"
          else G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4) if which == "power_law"
@@ -438,20 +440,22 @@ def test_restart_error_cache_keeps_the_bits(cp, precision, fmt, which):
     milp = cp.MILP.from_generated(g)
     kw = dict(termination_reltol=1e-7, powm_tol=1e-6, max_kkt_passes=20_000 if which != "wide" else 4_000)
     runs = []
-    for full in (False, True):
-        if full:
-            os.environ["CPDLP_KKT_FULL_CHECK"] = "1"
+    for env in (None, "CPDLP_KKT_SEPARATE", "CPDLP_KKT_FULL_CHECK"):
+        if env:
+            os.environ[env] = "1"
         try:
             runs.append(cp.solve(milp, cp.PDLP(precision, None, fmt, **kw)))
         finally:
-            os.environ.pop("CPDLP_KKT_FULL_CHECK", None)
-    (s1, t1), (s2, t2) = runs
-    assert t1.restarts >= 2 and t1.restarts == t2.restarts
-    assert t1.kkt_passes == t2.kkt_passes and t1.termination_status == t2.termination_status
-    assert len(t1.error_history) == len(t2.error_history)
-    for (p1, e1), (p2, e2) in zip(t1.error_history, t2.error_history):
-        assert p1 == p2 and e1 == e2, (p1, e1, e2)
-    assert np.array_equal(s1.x, s2.x) and np.array_equal(s1.y, s2.y)
+            if env:
+                os.environ.pop(env, None)
+    s2, t2 = runs[-1]
+    for s1, t1 in runs[:-1]:
+        assert t1.restarts >= 2 and t1.restarts == t2.restarts
+        assert t1.kkt_passes == t2.kkt_passes and t1.termination_status == t2.termination_status
+        assert len(t1.error_history) == len(t2.error_history)
+        for (p1, e1), (p2, e2) in zip(t1.error_history, t2.error_history):
+            assert p1 == p2 and e1 == e2, (p1, e1, e2)
+        assert np.array_equal(s1.x, s2.x) and np.array_equal(s1.y, s2.y)
 
 
 @pytest.mark.parametrize("order", ["density", "given"])
