@@ -135,6 +135,20 @@ class LoopState:
             setattr(s, f, getattr(self, f))
         return s
 
+    _SCALARS = ("eta", "omega", "eta_sum", "kkt_passes", "inner", "total", "restarts", "fp32_passes")
+
+    def save(self, path):
+        """Write the state to an .npz file (arrays and scalars only, no pickled objects)."""
+        np.savez(path, x=self.x, y=self.y, D2=self.D2, D1=self.D1,
+                 **{f: np.asarray(getattr(self, f)) for f in self._SCALARS})
+
+    @classmethod
+    def load(cls, path) -> "LoopState":
+        """Read a state written by `save`."""
+        with np.load(path, allow_pickle=False) as z:
+            kw = {f: z[f].item() for f in cls._SCALARS}
+            return cls(z["x"].copy(), z["y"].copy(), z["D2"].copy(), z["D1"].copy(), **kw)
+
     @classmethod
     def buffers(cls, m: int, n: int):
         k = _lib.STATE_VECTORS
