@@ -769,6 +769,7 @@ def bench_single(args):
         "objective": obj, "objective_closed_form": g.objective,
         "objective_rel_err": abs(obj - g.objective) / (1 + abs(g.objective)),
         "formats": {"A": FORMATS[st.format_A], "At": FORMATS[st.format_At]},
+        "line_grouped_tiles": {"A": bool(st.line_tiles & 1), "At": bool(st.line_tiles & 2)},
         "row_order": os.environ.get("CPDLP_ORDER", "first_touch"),
         "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
         "powm_tol": args.powm_tol, "eta_reference_rule": eta_ref,

@@ -76,7 +76,7 @@ class CKKT(C.Structure):
 class CStats(C.Structure):
     _fields_ = [("err", CKKT), ("time_elapsed", C.c_double), ("kkt_passes", C.c_int64),
                 ("termination_status", C.c_int32), ("format_A", C.c_int32),
-                ("format_At", C.c_int32), ("pad", C.c_int32), ("eta", C.c_double),
+                ("format_At", C.c_int32), ("line_tiles", C.c_int32), ("eta", C.c_double),
                 ("omega", C.c_double), ("powm_iters", C.c_int64), ("restarts", C.c_int64),
                 ("n_history", C.c_int64), ("fp32_passes", C.c_int64),
                 ("setup_seconds", C.c_double), ("loop_seconds", C.c_double),

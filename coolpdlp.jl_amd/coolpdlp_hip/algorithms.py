@@ -96,6 +96,7 @@ class ConvergenceStats:
     step_kernel_launches: int = 0
     format_A: int = 0
     format_At: int = 0
+    line_tiles: int = 0   # line-grouped short tiles in use: bit 0 A, bit 1 A'
 
 
 _FLOAT_TYPES = {"float64": _lib.PREC_FP64, "Float64": _lib.PREC_FP64, np.float64: _lib.PREC_FP64,
@@ -332,7 +333,7 @@ class Engine:
             fp32_passes=s.fp32_passes, setup_seconds=s.setup_seconds,
             loop_seconds=s.loop_seconds, step_kernel_ms=s.step_kernel_ms,
             step_kernel_launches=s.step_kernel_launches, format_A=s.format_A,
-            format_At=s.format_At)
+            format_At=s.format_At, line_tiles=s.line_tiles)
 
     def kkt_errors(self, sol: PrimalDualSolution) -> KKTErrors:
         """kkt_errors! on the preconditioned problem for an unscaled point, errors.jl:65-109"""

@@ -226,6 +226,62 @@ inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
     return L;
 }
 
+// Line-grouped copy of a side's short tiles.  The entries of each short tile are stored a
+// second time, sorted by column (stably), each packed with its position in the tile:
+// col[k] = (slot << LG_COL_BITS) | column.  A 64-lane load instruction of the tile then
+// holds neighbouring columns, so lanes whose gathers fall in one 128-byte line of the
+// vector share one L2 request.  The kernel writes each product to LDS at its slot, so
+// every row is summed from the same products in the same order as from the row-ordered
+// tile (bit-identical).  Only the short tiles' range [rowptr[n_long], nnz) is filled.
+// The estimate counts, per 64-entry window of every tile, the distinct lines of
+// `line_entries` vector entries the gathers touch, in row order and in column order.
+struct HostLineTiles {
+    std::vector<uint32_t> col;
+    std::vector<int32_t> vmap;
+    int64_t req_rows = 0, req_lines = 0;   // estimated gather requests, row / column order
+};
+
+inline bool line_tiles_packable(const HostSide& S) {
+    return S.cols <= (int64_t(1) << LG_COL_BITS) && TILE_NNZ <= (1 << (32 - LG_COL_BITS));
+}
+
+// estimate only (T.col/T.vmap stay empty) when build == false
+inline void line_group_tiles(const HostSide& S, int32_t line_entries, bool build, HostLineTiles& T) {
+    T.req_rows = T.req_lines = 0;
+    if (build) {
+        T.col.assign((size_t)std::max<int64_t>(S.nnz, 1), 0u);
+        T.vmap.assign((size_t)std::max<int64_t>(S.nnz, 1), 0);
+    }
+    std::vector<int32_t> idx, win;
+    auto lines_in_windows = [&](const int32_t* colv, int32_t cnt) {
+        int64_t req = 0;
+        for (int32_t w0 = 0; w0 < cnt; w0 += WAVE) {
+            win.assign(colv + w0, colv + std::min(cnt, w0 + WAVE));
+            for (int32_t& c : win) c /= line_entries;
+            std::sort(win.begin(), win.end());
+            req += std::unique(win.begin(), win.end()) - win.begin();
+        }
+        return req;
+    };
+    std::vector<int32_t> sorted_cols;
+    for (const TileMeta& t : S.tiles) {
+        if (t.cnt <= 0) continue;
+        const int32_t* c = S.col.data() + t.base;
+        T.req_rows += lines_in_windows(c, t.cnt);
+        idx.resize(t.cnt);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return c[a] < c[b]; });
+        sorted_cols.resize(t.cnt);
+        for (int32_t e = 0; e < t.cnt; ++e) sorted_cols[e] = c[idx[e]];
+        T.req_lines += lines_in_windows(sorted_cols.data(), t.cnt);
+        if (build)
+            for (int32_t e = 0; e < t.cnt; ++e) {
+                T.col[t.base + e] = ((uint32_t)idx[e] << LG_COL_BITS) | (uint32_t)c[idx[e]];
+                T.vmap[t.base + e] = S.vmap[t.base + idx[e]];
+            }
+    }
+}
+
 // ELL (column-major rows x width, -1 padding) of an internal side, or width -1 if the
 // padding would exceed `max_fill` times nnz.
 inline int32_t build_ell(const HostSide& S, double max_fill, std::vector<int32_t>& ecol,

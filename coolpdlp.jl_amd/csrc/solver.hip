@@ -317,6 +317,10 @@ MatView<V> DevSide::view(bool allow_slices) const {
     v.n_long_blocks = ceil_div(n_long, LONG_ROWS_PER_BLOCK);
     v.tiles = tiles.p;
     v.n_tiles = n_tiles;
+    if (lg_built && lg_on) {
+        v.lg_col = lg_col.p;
+        v.lg_val = lg_vals<V>();
+    }
     v.format = format;
     v.ell_width = ell_w;
     v.ell_col = ell_col.p;
@@ -350,6 +354,18 @@ template <>
 const double* DevSide::sl_vals<double>() const { return sl_val64.p; }
 template <>
 const float* DevSide::sl_vals<float>() const { return sl_val32.p; }
+template <>
+const double* DevSide::lg_vals<double>() const { return lg_val64.p; }
+template <>
+const float* DevSide::lg_vals<float>() const { return lg_val32.p; }
+
+void DevSide::build_line_tiles(const HostLineTiles& T, hipStream_t s) {
+    lg_col.upload(T.col, s);
+    lg_vmap.upload(T.vmap, s);
+    lg_built = true;
+    lg_on = true;
+    HIP_CHECK(hipStreamSynchronize(s));
+}
 
 // NS column slices of rows [0, nrows); NS divides the 8 XCDs of the MI355X, and slice s
 // is summed on the XCDs x with x % NS == s (workgroups go to the XCDs round-robin)
@@ -410,6 +426,16 @@ void DevSide::fill_values(const double* src_csc, bool want32, hipStream_t s) {
             if (sl_val32.n < (size_t)sl_nnz) sl_val32.alloc(sl_nnz);
             hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(sl_nnz, BLOCK)), dim3(BLOCK), 0,
                                s, sl_nnz, sl_vmap.p, src_csc, sl_val32.p);
+        }
+    }
+    if (lg_built && nnz) {
+        if (lg_val64.n < (size_t)nnz) lg_val64.alloc(nnz);
+        hipLaunchKernelGGL(k_gather_vals<double>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
+                           nnz, lg_vmap.p, src_csc, lg_val64.p);
+        if (want32) {
+            if (lg_val32.n < (size_t)nnz) lg_val32.alloc(nnz);
+            hipLaunchKernelGGL(k_gather_vals<float>, dim3(ceil_div(nnz, BLOCK)), dim3(BLOCK), 0, s,
+                               nnz, lg_vmap.p, src_csc, lg_val32.p);
         }
     }
     if (ell_w > 0) {
@@ -569,6 +595,24 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
             side.first->build_slices(H, H.rows, ns, s);
         else if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && H.n_long > 0))
             side.first->build_slices(H, H.n_long, ns, s);
+    }
+    // Line-grouped short tiles: built for a side whose short tiles (>= 1M entries) would
+    // need >= 10% fewer gather requests in column order, counted over 128-byte lines of
+    // the loop's element size; the format tuner then keeps them only if they time
+    // faster.  CPDLP_LINE_TILES=0 never builds them, =1 always does (and keeps them).
+    lg_mode = -1;
+    if (const char* e = std::getenv("CPDLP_LINE_TILES")) lg_mode = std::strtol(e, nullptr, 10) != 0 ? 1 : 0;
+    if (lg_mode != 0 && P.format != CPDLP_FMT_ELL && P.format != CPDLP_FMT_XSLICE_ALL) {
+        for (auto side : {std::make_pair(&A, &L.A), std::make_pair(&At, &L.At)}) {
+            const HostSide& H = *side.second;
+            const int64_t short_nnz = H.nnz - H.rowptr[H.n_long];
+            if (H.tiles.empty() || !line_tiles_packable(H)) continue;
+            if (lg_mode < 0 && short_nnz < ((int64_t)1 << 20)) continue;
+            HostLineTiles T;
+            line_group_tiles(H, (int32_t)(128 / elem), true, T);
+            if (lg_mode < 0 && (double)T.req_lines > 0.9 * (double)T.req_rows) continue;
+            side.first->build_line_tiles(T, s);
+        }
     }
     HIP_CHECK(hipStreamSynchronize(s));
     // free the big host-side copies that are no longer needed
@@ -825,6 +869,35 @@ void Handle::choose_formats() {
             }
         }
         S->format = best;
+    }
+    // line-grouped short tiles (AUTO, no override): kept where they time >= 3% faster
+    // with the chosen format
+    for (DevSide* S : {&A, &At}) {
+        if (!S->lg_built || lg_mode == 1 || P.format != CPDLP_FMT_AUTO || std::getenv("CPDLP_FORMATS")) continue;
+        if (S->format != CPDLP_FMT_CSR && S->format != CPDLP_FMT_XSLICE) {
+            S->lg_on = false;
+            continue;
+        }
+        DevBuf<double> xin(std::max<int64_t>(S->cols, 1)), out(std::max<int64_t>(S->rows, 1));
+        hipLaunchKernelGGL(k_fill, dim3(std::max(1, ceil_div(S->cols, BLOCK))), dim3(BLOCK), 0,
+                           stream, (int64_t)S->cols, xin.p, 1.0);
+        float ms_of[2] = {0.f, 0.f};
+        for (int on = 1; on >= 0; --on) {
+            S->lg_on = on != 0;
+            const MatView<double> v = S->view<double>();
+            EpiStore<double> e;
+            e.out = out.p;
+            e.st = nullptr;
+            for (int rep = 0; rep < 3; ++rep)
+                launch_spmv<double, double, 1>(v, XPack<double, 1>{{xin.p}}, e, stream);
+            HIP_CHECK(hipEventRecord(ev0, stream));
+            for (int rep = 0; rep < 10; ++rep)
+                launch_spmv<double, double, 1>(v, XPack<double, 1>{{xin.p}}, e, stream);
+            HIP_CHECK(hipEventRecord(ev1, stream));
+            HIP_CHECK(hipEventSynchronize(ev1));
+            HIP_CHECK(hipEventElapsedTime(&ms_of[on], ev0, ev1));
+        }
+        S->lg_on = ms_of[1] < 0.97f * ms_of[0];
     }
 }
 
@@ -1438,6 +1511,7 @@ void Handle::finish_stats() {
     stats.step_kernel_launches = step_launches;
     stats.format_A = A.format;
     stats.format_At = At.format;
+    stats.line_tiles = (A.lg_built && A.lg_on ? 1 : 0) | (At.lg_built && At.lg_on ? 2 : 0);
     solved = true;
 }
 

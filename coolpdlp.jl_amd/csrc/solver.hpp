@@ -30,6 +30,13 @@ struct DevSide {
     DevBuf<TileMeta> sl_tiles;
     DevBuf<double> sl_val64, sl_part;
     DevBuf<float> sl_val32;
+    // line-grouped copy of the short tiles (built when it saves gather requests; used
+    // while lg_on)
+    bool lg_built = false, lg_on = false;
+    DevBuf<uint32_t> lg_col;
+    DevBuf<int32_t> lg_vmap;
+    DevBuf<double> lg_val64;
+    DevBuf<float> lg_val32;
 
     template <class V>
     const V* vals() const;
@@ -37,6 +44,8 @@ struct DevSide {
     const V* ell_vals() const;
     template <class V>
     const V* sl_vals() const;
+    template <class V>
+    const V* lg_vals() const;
     // allow_slices = false: the plain CSR copy instead of the XCD slices
     template <class V>
     MatView<V> view(bool allow_slices = true) const;
@@ -44,6 +53,7 @@ struct DevSide {
     void build_slices(const HostSide& S, int32_t nrows, int32_t ns, hipStream_t s);
     bool slices_all() const { return sl_ns > 0 && sl_built == rows; }
     void build_ell(const HostSide& S, hipStream_t s, double max_fill);
+    void build_line_tiles(const HostLineTiles& T, hipStream_t s);
     void fill_values(const double* src_csc, bool want32, hipStream_t s);
 };
 
@@ -105,6 +115,7 @@ struct Handle {
     bool rst_err_valid = false;
     bool no_rst_cache = false;   // CPDLP_KKT_FULL_CHECK set: evaluate all 5 every check
     bool no_aos = false;         // CPDLP_KKT_SEPARATE set: the 4 candidates as separate vectors
+    int lg_mode = -1;            // CPDLP_LINE_TILES: -1 by estimate and timing, 0 never, 1 always
     IterState<double> st64;
     IterState<float> st32;
     cpdlp_stats stats{};

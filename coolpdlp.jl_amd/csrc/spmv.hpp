@@ -31,6 +31,10 @@ namespace cpdlp {
 
 constexpr int TILE_NNZ = 2048;
 constexpr int PER_T = TILE_NNZ / BLOCK;   // nonzeros held per thread in a short tile
+// line-grouped short tiles: an entry packs its slot in the tile (11 bits) above its
+// column (21 bits)
+constexpr int LG_COL_BITS = 21;
+constexpr uint32_t LG_COL_MASK = (1u << LG_COL_BITS) - 1u;
 constexpr int LONG_ROWS_PER_BLOCK = BLOCK / WAVE;
 
 // one short tile: rows [r0, r1), nonzeros [base, base + cnt); read with one 16-byte
@@ -49,6 +53,10 @@ struct MatView {
     int32_t n_long_blocks = 0;    // ceil(n_long / 4)
     const TileMeta* tiles = nullptr;     // short tiles
     int32_t n_tiles = 0;
+    // line-grouped copy of the short tiles' entries (column-sorted within each tile,
+    // packed slot | column, and their values), or null: the tiles read col / val
+    const uint32_t* lg_col = nullptr;
+    const V* lg_val = nullptr;
     // column-major ELL (format 2): rows x width, col -1 = padding
     int32_t format = CPDLP_FMT_CSR;
     int32_t ell_width = 0;
@@ -174,10 +182,14 @@ __device__ __forceinline__ void long_row_sums(const int32_t* rowptr, const int32
 // (index clamped to the tile), so all PER_T (col, val) pairs and then all PER_T
 // gathers are in flight together; products go to LDS and each thread sums its row
 // sequentially from zero in ascending k.  Slots past the tile's nnz are written but
-// never summed.  All threads of the block must call this (it synchronises).
-template <class V, class Acc, int K, class XP>
+// never summed.  LG: the tile's line-grouped copy (lgc, lgv): entries in column order,
+// each product written to its own slot, so the sums are the same.  (A separate
+// instantiation: the row-ordered kernel keeps its registers and paired LDS stores.)
+// All threads of the block must call this (it synchronises).
+template <class V, class Acc, int K, class XP, bool LG>
 __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* col,
-                                          const V* val, const TileMeta& tm, const XP& X,
+                                          const V* val, const uint32_t* lgc, const V* lgv,
+                                          const TileMeta& tm, const XP& X,
                                           Acc* prod, Acc (&acc)[K]) {
     const int tid = threadIdx.x;
     const int base = tm.base, cnt = tm.cnt;
@@ -191,12 +203,22 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
     for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
     if (cnt <= 0) return;   // uniform per block
     int cc[PER_T];
+    int slot[PER_T];   // LDS slot of product j
     V vv[PER_T];
 #pragma unroll
     for (int j = 0; j < PER_T; ++j) {
-        const int e = min(tid + j * BLOCK, cnt - 1);
-        cc[j] = __builtin_nontemporal_load(col + base + e);
-        vv[j] = __builtin_nontemporal_load(val + base + e);
+        const int i = tid + j * BLOCK;
+        const int e = min(i, cnt - 1);
+        if constexpr (LG) {
+            const uint32_t p = __builtin_nontemporal_load(lgc + base + e);
+            cc[j] = (int)(p & LG_COL_MASK);
+            slot[j] = i < cnt ? (int)(p >> LG_COL_BITS) : i;
+            vv[j] = __builtin_nontemporal_load(lgv + base + e);
+        } else {
+            cc[j] = __builtin_nontemporal_load(col + base + e);
+            slot[j] = i;
+            vv[j] = __builtin_nontemporal_load(val + base + e);
+        }
     }
     if constexpr (XP::aos) {   // an entry's K values in one load, then the K products in turn
         VecK<V, K> xk[PER_T];
@@ -205,7 +227,7 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
 #pragma unroll
         for (int kv = 0; kv < K; ++kv) {
 #pragma unroll
-            for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * (Acc)xk[j].v[kv];
+            for (int j = 0; j < PER_T; ++j) prod[slot[j]] = (Acc)vv[j] * (Acc)xk[j].v[kv];
             __syncthreads();
             acc[kv] = seq_sum(prod, lo, hi);
             __syncthreads();
@@ -217,7 +239,7 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
 #pragma unroll
             for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
 #pragma unroll
-            for (int j = 0; j < PER_T; ++j) prod[tid + j * BLOCK] = (Acc)vv[j] * xv[j];
+            for (int j = 0; j < PER_T; ++j) prod[slot[j]] = (Acc)vv[j] * xv[j];
             __syncthreads();
             acc[kv] = seq_sum(prod, lo, hi);
             __syncthreads();
@@ -227,7 +249,7 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
 
 // (65 VGPRs, 7 waves per SIMD for the FP64 primal step.  Forcing 8 waves cut it to 49
 // VGPRs but fewer loads in flight: 145 us instead of 130 us on the large LP.)
-template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>>
+template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>, bool LG = false>
 __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XP X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
     double q[Epi::NQ > 0 ? Epi::NQ : 1];
@@ -252,7 +274,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XP X, Epi epi)
         const int r = tm.r0 + (int)threadIdx.x;
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);   // issued before the gathers
-        tile_sums<V, Acc, K, XP>(A.rowptr, A.col, A.val, tm, X, prod, acc);
+        tile_sums<V, Acc, K, XP, LG>(A.rowptr, A.col, A.val, A.lg_col, A.lg_val, tm, X, prod, acc);
         if (r < tm.r1) epi.row(r, acc, q, pre);
     }
     if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
@@ -277,7 +299,7 @@ __device__ __forceinline__ Acc group_sum(Acc v) {
 // finished by the epilogue exactly as in k_spmv_csr, block partials at index
 // b - ns * sl_max_tiles.  Deterministic; the sliced rows' order of summation differs
 // from a sequential sum (as the wavefront path's does).
-template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>>
+template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>, bool LG = false>
 __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XP X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
     __shared__ Acc red[K][BLOCK / WAVE];
@@ -293,7 +315,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XP X, Epi e
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);
         Acc acc[K];
-        tile_sums<V, Acc, K, XP>(A.rowptr, A.col, A.val, tm, X, prod, acc);
+        tile_sums<V, Acc, K, XP, LG>(A.rowptr, A.col, A.val, A.lg_col, A.lg_val, tm, X, prod, acc);
         if (r < tm.r1) epi.row(r, acc, q, pre);
         if constexpr (Epi::NQ > 0)
             block_sum<Epi::NQ>(q, epi.partials + (size_t)(blockIdx.x - nsb) * Epi::NQ);
@@ -509,7 +531,10 @@ inline void launch_spmv(const MatView<V>& A, const XP& X, const Epi& epi, hipStr
     if (g <= 0) return;
     if constexpr (K <= SL_KMAX) {
         if (A.format == CPDLP_FMT_XSLICE) {
-            hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi, XP>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
+            if (A.lg_col)
+                hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi, XP, true>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
+            else
+                hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi, XP>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
             if (A.sl_rows > 0)
                 hipLaunchKernelGGL((k_xslice_combine<V, Acc, K, Epi>), dim3(ceil_div(A.sl_rows, BLOCK)),
                                    dim3(BLOCK), 0, s, A, epi);
@@ -524,7 +549,10 @@ inline void launch_spmv(const MatView<V>& A, const XP& X, const Epi& epi, hipStr
         // and the block-partial reductions
         MatView<V> v = A;
         v.format = CPDLP_FMT_CSR;
-        hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi, XP>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
+        if (v.lg_col)
+            hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi, XP, true>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
+        else
+            hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi, XP>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
     }
 }
 

@@ -103,7 +103,7 @@ struct CStats
     termination_status::Int32
     format_A::Int32
     format_At::Int32
-    pad::Int32
+    line_tiles::Int32
     eta::Float64
     omega::Float64
     powm_iters::Int64

@@ -123,7 +123,7 @@ typedef struct cpdlp_stats {
     int64_t kkt_passes;
     int32_t termination_status;     /* CPDLP_OPTIMAL ... */
     int32_t format_A, format_At;    /* chosen formats */
-    int32_t pad;
+    int32_t line_tiles;             /* line-grouped short tiles in use: bit 0 A, bit 1 A' */
     double eta, omega;
     int64_t powm_iters;
     int64_t restarts;

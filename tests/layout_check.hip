@@ -1,5 +1,5 @@
 // Host-only checks of the device layouts built by csrc/layout.hpp (no GPU needed):
-// partition_rows, build_tiles, build_layout and build_slices on random matrices with
+// partition_rows, build_tiles, build_layout, build_slices and line_group_tiles on random matrices with
 // long and short rows.  Prints "ok" and exits 0, or prints the first violated invariant
 // and exits 1.  Built and run by tests/test_layout.py.
 #include <climits>
@@ -201,6 +201,55 @@ static void check_build_layout(std::mt19937& rng) {
     }
 }
 
+// line-grouped short tiles: each tile's copy holds every slot of the tile once, sorted by
+// column (stable: equal columns in slot order), with the column and value map of the entry
+// at that slot; the request estimate counts distinct lines per 64-entry window
+static void check_line_tiles(HostSide S, int line) {
+    S.tiles.clear();
+    build_tiles(S.rowptr.data(), S.n_long, S.rows, S.tiles);
+    HostLineTiles T;
+    line_group_tiles(S, line, true, T);
+    if ((long)T.col.size() != std::max<long>(S.nnz, 1)) fail("line tiles size", (long)T.col.size(), S.nnz);
+    long windows = 0, rows_req = 0;
+    for (const TileMeta& tm : S.tiles) {
+        std::vector<int> seen(tm.cnt, 0);
+        for (int e = 0; e < tm.cnt; ++e) {
+            const uint32_t p = T.col[tm.base + e];
+            const int slot = (int)(p >> LG_COL_BITS), c = (int)(p & LG_COL_MASK);
+            if (slot < 0 || slot >= tm.cnt || seen[slot]++) fail("line tile slots not a permutation", tm.base, e);
+            if (c != S.col[tm.base + slot] || T.vmap[tm.base + e] != S.vmap[tm.base + slot]) fail("line tile entry", tm.base, e);
+            if (e > 0) {
+                const uint32_t q = T.col[tm.base + e - 1];
+                const int cq = (int)(q & LG_COL_MASK), sq = (int)(q >> LG_COL_BITS);
+                if (cq > c || (cq == c && sq > slot)) fail("line tile not sorted stably", tm.base, e);
+            }
+        }
+        // row-order estimate recounted here: distinct lines of each 64-entry window
+        for (int w0 = 0; w0 < tm.cnt; w0 += 64, ++windows) {
+            std::vector<int> ls;
+            for (int e = w0; e < std::min(tm.cnt, w0 + 64); ++e) ls.push_back(S.col[tm.base + e] / line);
+            std::sort(ls.begin(), ls.end());
+            rows_req += std::unique(ls.begin(), ls.end()) - ls.begin();
+        }
+    }
+    if (T.req_rows != rows_req) fail("row-order request estimate", T.req_rows, rows_req);
+    // column order: at most one extra line per window boundary beyond the distinct lines
+    if (T.req_lines > T.req_rows + windows) fail("line-order estimate", T.req_lines, T.req_rows);
+    HostLineTiles E;
+    line_group_tiles(S, line, false, E);
+    if (E.req_rows != T.req_rows || E.req_lines != T.req_lines || !E.col.empty()) fail("estimate-only pass", 0, 0);
+}
+
+// a hot head of columns: the column-ordered copy needs fewer requests than the rows
+static void check_line_tiles_gain(std::mt19937& rng) {
+    HostSide S = make_side(rng, 3000, 4000, 0, 100, true);
+    S.tiles.clear();
+    build_tiles(S.rowptr.data(), 0, S.rows, S.tiles);
+    HostLineTiles T;
+    line_group_tiles(S, 16, false, T);
+    if (!(T.req_lines * 10 < T.req_rows * 9)) fail("no gain on a hot head", T.req_lines, T.req_rows);
+}
+
 int main() {
     std::mt19937 rng(12345);
     check_partition_and_tiles(rng);
@@ -212,7 +261,10 @@ int main() {
         const HostSide S = make_side(rng, rows, cols, n_long, maxlong, trial % 2 == 1);
         check_slices(S, S.n_long);   // xslice: long rows
         check_slices(S, S.rows);     // xslice_all: every row
+        check_line_tiles(S, 16);
+        check_line_tiles(S, 4);
     }
+    check_line_tiles_gain(rng);
     std::printf("ok\n");
     return 0;
 }

@@ -489,3 +489,42 @@ def test_row_orders_reach_the_same_optimum(cp, order):
             cp.solve(milp, cp.PDLP("float64", **kw))
     finally:
         os.environ.pop("CPDLP_ORDER", None)
+
+
+@pytest.mark.parametrize("precision,fmt,which", [("float64", "csr", "column"), ("mixed", "csr", "column"),
+                                                 ("float64", "xslice", "power_law"),
+                                                 ("float64", "auto", "wide")])
+def test_line_grouped_tiles_keep_the_bits(cp, precision, fmt, which):
+    """The short tiles' entries stored in column order (CPDLP_LINE_TILES=1: lanes of one
+    load instruction share 128-byte lines of the gathered vector) give the row-ordered
+    tiles' bits (=0): each product lands in its own LDS slot and every row is summed in the
+    same order.  The whole error history, restarts, pass count and solution are identical,
+    in FP64 and mixed, through CSR, the sliced format's short tiles and the tuner's choice
+    (unset: built where the request estimate gains >= 10%, kept where it times faster);
+    the stats report the sides that used them."""
+    import os
+    g = (G.column_lp(400, 1000, 6, seed=23, box_frac=0.3) if which == "column"
+         else G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4) if which == "power_law"
+         else G.power_law_lp(20_000, 1_100_000, 3_000_000, cap=40_000, seed=6))
+    milp = cp.MILP.from_generated(g)
+    kw = dict(termination_reltol=1e-7, powm_tol=1e-6, max_kkt_passes=20_000 if which != "wide" else 4_000)
+    runs = {}
+    for mode in ("0", "1", None):
+        if mode is not None:
+            os.environ["CPDLP_LINE_TILES"] = mode
+        try:
+            runs[mode] = cp.solve(milp, cp.PDLP(precision, None, fmt, **kw))
+        finally:
+            os.environ.pop("CPDLP_LINE_TILES", None)
+    s0, t0 = runs["0"]
+    assert t0.line_tiles == 0
+    assert runs["1"][1].line_tiles == 3, runs["1"][1].line_tiles
+    assert t0.restarts >= 1
+    for mode in ("1", None):
+        s1, t1 = runs[mode]
+        assert t1.kkt_passes == t0.kkt_passes and t1.restarts == t0.restarts, mode
+        assert t1.termination_status == t0.termination_status
+        assert len(t1.error_history) == len(t0.error_history)
+        for (p1, e1), (p0, e0) in zip(t1.error_history, t0.error_history):
+            assert p1 == p0 and e1 == e0, (mode, p1, e1, e0)
+        assert np.array_equal(s1.x, s0.x) and np.array_equal(s1.y, s0.y), mode
