@@ -318,6 +318,8 @@ MatView<V> DevSide::view(bool allow_slices) const {
     v.tiles = tiles.p;
     v.n_tiles = n_tiles;
     if (lg_built && lg_on) {
+        v.lg_tiles = lg_tiles.p;
+        v.lg_n_tiles = lg_n_tiles;
         v.lg_col = lg_col.p;
         v.lg_val = lg_vals<V>();
     }
@@ -360,6 +362,8 @@ template <>
 const float* DevSide::lg_vals<float>() const { return lg_val32.p; }
 
 void DevSide::build_line_tiles(const HostLineTiles& T, hipStream_t s) {
+    lg_n_tiles = (int32_t)T.tiles.size();
+    lg_tiles.upload(T.tiles, s);
     lg_col.upload(T.col, s);
     lg_vmap.upload(T.vmap, s);
     lg_built = true;
@@ -596,17 +600,18 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         else if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && H.n_long > 0))
             side.first->build_slices(H, H.n_long, ns, s);
     }
-    // Line-grouped short tiles: built for a side whose short tiles (>= 1M entries) would
-    // need >= 10% fewer gather requests in column order, counted over 128-byte lines of
-    // the loop's element size; the format tuner then keeps them only if they time
-    // faster.  CPDLP_LINE_TILES=0 never builds them, =1 always does (and keeps them).
+    // Line-grouped tiles of the short rows (the steps of a CSR side): built for a side
+    // whose short rows (>= 1M entries) would need >= 10% fewer gather requests in them,
+    // counted over 128-byte lines of the loop's element size; the format tuner then keeps
+    // them only if they time faster.  CPDLP_LINE_TILES=0 never builds them, =1 always
+    // does (and keeps them).
     lg_mode = -1;
     if (const char* e = std::getenv("CPDLP_LINE_TILES")) lg_mode = std::strtol(e, nullptr, 10) != 0 ? 1 : 0;
-    if (lg_mode != 0 && P.format != CPDLP_FMT_ELL && P.format != CPDLP_FMT_XSLICE_ALL) {
+    if (lg_mode != 0 && (P.format == CPDLP_FMT_AUTO || P.format == CPDLP_FMT_CSR)) {
         for (auto side : {std::make_pair(&A, &L.A), std::make_pair(&At, &L.At)}) {
             const HostSide& H = *side.second;
             const int64_t short_nnz = H.nnz - H.rowptr[H.n_long];
-            if (H.tiles.empty() || !line_tiles_packable(H)) continue;
+            if (H.n_long == H.rows || !line_tiles_packable(H)) continue;
             if (lg_mode < 0 && short_nnz < ((int64_t)1 << 20)) continue;
             HostLineTiles T;
             line_group_tiles(H, (int32_t)(128 / elem), true, T);
@@ -874,7 +879,7 @@ void Handle::choose_formats() {
     // with the chosen format
     for (DevSide* S : {&A, &At}) {
         if (!S->lg_built || lg_mode == 1 || P.format != CPDLP_FMT_AUTO || std::getenv("CPDLP_FORMATS")) continue;
-        if (S->format != CPDLP_FMT_CSR && S->format != CPDLP_FMT_XSLICE) {
+        if (S->format != CPDLP_FMT_CSR) {
             S->lg_on = false;
             continue;
         }
@@ -1511,7 +1516,8 @@ void Handle::finish_stats() {
     stats.step_kernel_launches = step_launches;
     stats.format_A = A.format;
     stats.format_At = At.format;
-    stats.line_tiles = (A.lg_built && A.lg_on ? 1 : 0) | (At.lg_built && At.lg_on ? 2 : 0);
+    auto lg_used = [](const DevSide& S) { return S.lg_built && S.lg_on && S.format == CPDLP_FMT_CSR; };
+    stats.line_tiles = (lg_used(A) ? 1 : 0) | (lg_used(At) ? 2 : 0);
     solved = true;
 }
 

@@ -33,6 +33,8 @@ struct DevSide {
     // line-grouped copy of the short tiles (built when it saves gather requests; used
     // while lg_on)
     bool lg_built = false, lg_on = false;
+    int32_t lg_n_tiles = 0;
+    DevBuf<TileMeta> lg_tiles;
     DevBuf<uint32_t> lg_col;
     DevBuf<int32_t> lg_vmap;
     DevBuf<double> lg_val64;

@@ -31,9 +31,14 @@ namespace cpdlp {
 
 constexpr int TILE_NNZ = 2048;
 constexpr int PER_T = TILE_NNZ / BLOCK;   // nonzeros held per thread in a short tile
-// line-grouped short tiles: an entry packs its slot in the tile (11 bits) above its
-// column (21 bits)
-constexpr int LG_COL_BITS = 21;
+// line-grouped tiles: LG_NT threads, LG_PT entries each, at most LG_NT rows and
+// LG_TILE_NNZ entries; an entry packs its slot in the tile (12 bits) above its column
+// (20 bits).  Measured on the large power-law LP's primal step (threads x entries per
+// thread): 256 x 8 110.1 us, 512 x 8 108.5 us, 1024 x 4 105.8 us (46 VGPRs).
+constexpr int LG_NT = 1024;
+constexpr int LG_PT = 4;
+constexpr int LG_TILE_NNZ = LG_NT * LG_PT;
+constexpr int LG_COL_BITS = 20;
 constexpr uint32_t LG_COL_MASK = (1u << LG_COL_BITS) - 1u;
 constexpr int LONG_ROWS_PER_BLOCK = BLOCK / WAVE;
 
@@ -53,8 +58,11 @@ struct MatView {
     int32_t n_long_blocks = 0;    // ceil(n_long / 4)
     const TileMeta* tiles = nullptr;     // short tiles
     int32_t n_tiles = 0;
-    // line-grouped copy of the short tiles' entries (column-sorted within each tile,
-    // packed slot | column, and their values), or null: the tiles read col / val
+    // line-grouped tiles of the short rows (LG_NT rows and LG_TILE_NNZ entries at most,
+    // entries column-sorted within each tile, packed slot | column, and their values), or
+    // null; the single-vector products without block partials (the steps) use them
+    const TileMeta* lg_tiles = nullptr;
+    int32_t lg_n_tiles = 0;
     const uint32_t* lg_col = nullptr;
     const V* lg_val = nullptr;
     // column-major ELL (format 2): rows x width, col -1 = padding
@@ -179,14 +187,15 @@ __device__ __forceinline__ void long_row_sums(const int32_t* rowptr, const int32
 }
 
 // Short tile rows [r0, r1): thread tid owns row r0 + tid.  Every load unconditional
-// (index clamped to the tile), so all PER_T (col, val) pairs and then all PER_T
+// (index clamped to the tile), so all PT (col, val) pairs and then all PT
 // gathers are in flight together; products go to LDS and each thread sums its row
 // sequentially from zero in ascending k.  Slots past the tile's nnz are written but
-// never summed.  LG: the tile's line-grouped copy (lgc, lgv): entries in column order,
-// each product written to its own slot, so the sums are the same.  (A separate
-// instantiation: the row-ordered kernel keeps its registers and paired LDS stores.)
+// never summed.  NT threads, NT * PT entries at most.  LG: the tile's line-grouped
+// copy (lgc, lgv): entries in column order, each product written to its own slot, so the
+// sums are the same.  (A separate instantiation: the row-ordered kernel keeps its
+// registers and paired LDS stores.)
 // All threads of the block must call this (it synchronises).
-template <class V, class Acc, int K, class XP, bool LG>
+template <class V, class Acc, int K, class XP, bool LG, int NT, int PT = PER_T>
 __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* col,
                                           const V* val, const uint32_t* lgc, const V* lgv,
                                           const TileMeta& tm, const XP& X,
@@ -202,12 +211,12 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
 #pragma unroll
     for (int kv = 0; kv < K; ++kv) acc[kv] = Acc(0);
     if (cnt <= 0) return;   // uniform per block
-    int cc[PER_T];
-    int slot[PER_T];   // LDS slot of product j
-    V vv[PER_T];
+    int cc[PT];
+    int slot[PT];   // LDS slot of product j
+    V vv[PT];
 #pragma unroll
-    for (int j = 0; j < PER_T; ++j) {
-        const int i = tid + j * BLOCK;
+    for (int j = 0; j < PT; ++j) {
+        const int i = tid + j * NT;
         const int e = min(i, cnt - 1);
         if constexpr (LG) {
             const uint32_t p = __builtin_nontemporal_load(lgc + base + e);
@@ -221,13 +230,13 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
         }
     }
     if constexpr (XP::aos) {   // an entry's K values in one load, then the K products in turn
-        VecK<V, K> xk[PER_T];
+        VecK<V, K> xk[PT];
 #pragma unroll
-        for (int j = 0; j < PER_T; ++j) xk[j] = load_k(X, cc[j]);
+        for (int j = 0; j < PT; ++j) xk[j] = load_k(X, cc[j]);
 #pragma unroll
         for (int kv = 0; kv < K; ++kv) {
 #pragma unroll
-            for (int j = 0; j < PER_T; ++j) prod[slot[j]] = (Acc)vv[j] * (Acc)xk[j].v[kv];
+            for (int j = 0; j < PT; ++j) prod[slot[j]] = (Acc)vv[j] * (Acc)xk[j].v[kv];
             __syncthreads();
             acc[kv] = seq_sum(prod, lo, hi);
             __syncthreads();
@@ -235,11 +244,11 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
     } else {
 #pragma unroll
         for (int kv = 0; kv < K; ++kv) {
-            Acc xv[PER_T];
+            Acc xv[PT];
 #pragma unroll
-            for (int j = 0; j < PER_T; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
+            for (int j = 0; j < PT; ++j) xv[j] = (Acc)X.p[kv][cc[j]];
 #pragma unroll
-            for (int j = 0; j < PER_T; ++j) prod[slot[j]] = (Acc)vv[j] * xv[j];
+            for (int j = 0; j < PT; ++j) prod[slot[j]] = (Acc)vv[j] * xv[j];
             __syncthreads();
             acc[kv] = seq_sum(prod, lo, hi);
             __syncthreads();
@@ -249,16 +258,19 @@ __device__ __forceinline__ void tile_sums(const int32_t* rowptr, const int32_t* 
 
 // (65 VGPRs, 7 waves per SIMD for the FP64 primal step.  Forcing 8 waves cut it to 49
 // VGPRs but fewer loads in flight: 145 us instead of 130 us on the large LP.)
-template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>, bool LG = false>
-__global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XP X, Epi epi) {
-    __shared__ Acc prod[TILE_NNZ];
+// NT threads per block: NT / 64 long rows or one tile of NT rows and NT * PT entries;
+// LG: the view's tiles are the line-grouped ones.
+template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>, bool LG = false, int NT = BLOCK,
+          int PT = PER_T>
+__global__ __launch_bounds__(NT) void k_spmv_csr(MatView<V> A, XP X, Epi epi) {
+    __shared__ Acc prod[NT * PT];
     double q[Epi::NQ > 0 ? Epi::NQ : 1];
 #pragma unroll
     for (int i = 0; i < (Epi::NQ > 0 ? Epi::NQ : 1); ++i) q[i] = 0.0;
     if (epi.skip()) return;
     Acc acc[K];
     if ((int)blockIdx.x < A.n_long_blocks) {
-        const int r = blockIdx.x * LONG_ROWS_PER_BLOCK + threadIdx.x / WAVE;
+        const int r = blockIdx.x * (NT / WAVE) + threadIdx.x / WAVE;
         if (r < A.n_long) {
             typename Epi::Pre pre{};
             if ((threadIdx.x & (WAVE - 1)) == 0) pre = epi.load(r);
@@ -274,10 +286,10 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_csr(MatView<V> A, XP X, Epi epi)
         const int r = tm.r0 + (int)threadIdx.x;
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);   // issued before the gathers
-        tile_sums<V, Acc, K, XP, LG>(A.rowptr, A.col, A.val, A.lg_col, A.lg_val, tm, X, prod, acc);
+        tile_sums<V, Acc, K, XP, LG, NT, PT>(A.rowptr, A.col, A.val, A.lg_col, A.lg_val, tm, X, prod, acc);
         if (r < tm.r1) epi.row(r, acc, q, pre);
     }
-    if constexpr (Epi::NQ > 0) block_sum<Epi::NQ>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
+    if constexpr (Epi::NQ > 0) block_sum<Epi::NQ, NT>(q, epi.partials + (size_t)blockIdx.x * Epi::NQ);
 }
 
 // Sum of a value over the SEG lanes of an aligned lane group (butterfly; fixed order).
@@ -299,7 +311,7 @@ __device__ __forceinline__ Acc group_sum(Acc v) {
 // finished by the epilogue exactly as in k_spmv_csr, block partials at index
 // b - ns * sl_max_tiles.  Deterministic; the sliced rows' order of summation differs
 // from a sequential sum (as the wavefront path's does).
-template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>, bool LG = false>
+template <class V, class Acc, int K, class Epi, class XP = XPack<V, K>>
 __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XP X, Epi epi) {
     __shared__ Acc prod[TILE_NNZ];
     __shared__ Acc red[K][BLOCK / WAVE];
@@ -315,7 +327,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XP X, Epi e
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);
         Acc acc[K];
-        tile_sums<V, Acc, K, XP, LG>(A.rowptr, A.col, A.val, A.lg_col, A.lg_val, tm, X, prod, acc);
+        tile_sums<V, Acc, K, XP, false, BLOCK>(A.rowptr, A.col, A.val, nullptr, nullptr, tm, X, prod, acc);
         if (r < tm.r1) epi.row(r, acc, q, pre);
         if constexpr (Epi::NQ > 0)
             block_sum<Epi::NQ>(q, epi.partials + (size_t)(blockIdx.x - nsb) * Epi::NQ);
@@ -531,13 +543,23 @@ inline void launch_spmv(const MatView<V>& A, const XP& X, const Epi& epi, hipStr
     if (g <= 0) return;
     if constexpr (K <= SL_KMAX) {
         if (A.format == CPDLP_FMT_XSLICE) {
-            if (A.lg_col)
-                hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi, XP, true>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
-            else
-                hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi, XP>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
+            hipLaunchKernelGGL((k_spmv_xslice<V, Acc, K, Epi, XP>), dim3(g), dim3(BLOCK), 0, s, A, X, epi);
             if (A.sl_rows > 0)
                 hipLaunchKernelGGL((k_xslice_combine<V, Acc, K, Epi>), dim3(ceil_div(A.sl_rows, BLOCK)),
                                    dim3(BLOCK), 0, s, A, epi);
+            return;
+        }
+    }
+    // a CSR side with line-grouped tiles: the single-vector products without block
+    // partials run on those tiles (the same products and row sums, so the same bits)
+    if constexpr (K == 1 && Epi::NQ == 0 && !XP::aos) {
+        if (A.format == CPDLP_FMT_CSR && A.lg_col) {
+            MatView<V> v = A;
+            v.tiles = A.lg_tiles;
+            v.n_tiles = A.lg_n_tiles;
+            v.n_long_blocks = ceil_div(A.n_long, LG_NT / WAVE);
+            hipLaunchKernelGGL((k_spmv_csr<V, Acc, 1, Epi, XP, true, LG_NT, LG_PT>), dim3(v.grid()), dim3(LG_NT),
+                               0, s, v, X, epi);
             return;
         }
     }
@@ -549,10 +571,7 @@ inline void launch_spmv(const MatView<V>& A, const XP& X, const Epi& epi, hipStr
         // and the block-partial reductions
         MatView<V> v = A;
         v.format = CPDLP_FMT_CSR;
-        if (v.lg_col)
-            hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi, XP, true>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
-        else
-            hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi, XP>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
+        hipLaunchKernelGGL((k_spmv_csr<V, Acc, K, Epi, XP>), dim3(v.grid()), dim3(BLOCK), 0, s, v, X, epi);
     }
 }
 

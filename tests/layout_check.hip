@@ -201,18 +201,37 @@ static void check_build_layout(std::mt19937& rng) {
     }
 }
 
-// line-grouped short tiles: each tile's copy holds every slot of the tile once, sorted by
-// column (stable: equal columns in slot order), with the column and value map of the entry
-// at that slot; the request estimate counts distinct lines per 64-entry window
+// line-grouped tiles: the short rows cut into tiles of <= LG_NT rows and <= LG_TILE_NNZ
+// entries, contiguous; each tile's copy holds every slot of the tile once, sorted by column
+// (stable: equal columns in slot order), with the column and value map of the entry at that
+// slot; the request estimates count distinct lines per 64-entry window (row order: the
+// short tiles; column order: the line-grouped tiles)
+static long windows_lines(const int32_t* c, int cnt, int line) {
+    long req = 0;
+    for (int w0 = 0; w0 < cnt; w0 += 64) {
+        std::vector<int> ls;
+        for (int e = w0; e < std::min(cnt, w0 + 64); ++e) ls.push_back(c[e] / line);
+        std::sort(ls.begin(), ls.end());
+        req += std::unique(ls.begin(), ls.end()) - ls.begin();
+    }
+    return req;
+}
+
 static void check_line_tiles(HostSide S, int line) {
     S.tiles.clear();
     build_tiles(S.rowptr.data(), S.n_long, S.rows, S.tiles);
     HostLineTiles T;
     line_group_tiles(S, line, true, T);
     if ((long)T.col.size() != std::max<long>(S.nnz, 1)) fail("line tiles size", (long)T.col.size(), S.nnz);
-    long windows = 0, rows_req = 0;
-    for (const TileMeta& tm : S.tiles) {
-        std::vector<int> seen(tm.cnt, 0);
+    int expect = S.n_long;
+    long rows_req = 0, lines_req = 0;
+    for (const TileMeta& tm : S.tiles) rows_req += windows_lines(S.col.data() + tm.base, tm.cnt, line);
+    for (const TileMeta& tm : T.tiles) {
+        if (tm.r0 != expect || tm.r1 <= tm.r0 || tm.r1 - tm.r0 > LG_NT) fail("line tile rows", tm.r0, tm.r1);
+        if (tm.base != S.rowptr[tm.r0] || tm.cnt != S.rowptr[tm.r1] - S.rowptr[tm.r0]) fail("line tile base", tm.r0, 0);
+        if (tm.cnt > LG_TILE_NNZ && tm.r1 - tm.r0 > 1) fail("line tile entries", tm.r0, tm.cnt);
+        expect = tm.r1;
+        std::vector<int> seen(tm.cnt, 0), sorted_cols(tm.cnt);
         for (int e = 0; e < tm.cnt; ++e) {
             const uint32_t p = T.col[tm.base + e];
             const int slot = (int)(p >> LG_COL_BITS), c = (int)(p & LG_COL_MASK);
@@ -223,24 +242,19 @@ static void check_line_tiles(HostSide S, int line) {
                 const int cq = (int)(q & LG_COL_MASK), sq = (int)(q >> LG_COL_BITS);
                 if (cq > c || (cq == c && sq > slot)) fail("line tile not sorted stably", tm.base, e);
             }
+            sorted_cols[e] = c;
         }
-        // row-order estimate recounted here: distinct lines of each 64-entry window
-        for (int w0 = 0; w0 < tm.cnt; w0 += 64, ++windows) {
-            std::vector<int> ls;
-            for (int e = w0; e < std::min(tm.cnt, w0 + 64); ++e) ls.push_back(S.col[tm.base + e] / line);
-            std::sort(ls.begin(), ls.end());
-            rows_req += std::unique(ls.begin(), ls.end()) - ls.begin();
-        }
+        lines_req += windows_lines(sorted_cols.data(), tm.cnt, line);
     }
+    if (expect != S.rows) fail("line tiles end early", expect, S.rows);
     if (T.req_rows != rows_req) fail("row-order request estimate", T.req_rows, rows_req);
-    // column order: at most one extra line per window boundary beyond the distinct lines
-    if (T.req_lines > T.req_rows + windows) fail("line-order estimate", T.req_lines, T.req_rows);
+    if (T.req_lines != lines_req) fail("line-order request estimate", T.req_lines, lines_req);
     HostLineTiles E;
     line_group_tiles(S, line, false, E);
     if (E.req_rows != T.req_rows || E.req_lines != T.req_lines || !E.col.empty()) fail("estimate-only pass", 0, 0);
 }
 
-// a hot head of columns: the column-ordered copy needs fewer requests than the rows
+// a hot head of columns: the line-grouped tiles need fewer requests than the rows
 static void check_line_tiles_gain(std::mt19937& rng) {
     HostSide S = make_side(rng, 3000, 4000, 0, 100, true);
     S.tiles.clear();
