@@ -624,7 +624,8 @@ def bench_single(args):
     # reports the reference rule's eta beside it
     algo = cp.PDLP("float64", None, args.format, termination_reltol=reltol, time_limit=args.time_limit,
                    max_kkt_passes=args.max_passes, device=0, powm_tol=args.powm_tol,
-                   show_progress=0, detect_infeasibility=int(args.detect_infeasibility))
+                   show_progress=0, detect_infeasibility=int(args.detect_infeasibility),
+                   long_row=args.long_row)
     t = time.perf_counter()
     eng = cp.Engine(milp, algo)
     torch.cuda.synchronize()
@@ -769,6 +770,7 @@ def bench_single(args):
         "objective": obj, "objective_closed_form": g.objective,
         "objective_rel_err": abs(obj - g.objective) / (1 + abs(g.objective)),
         "formats": {"A": FORMATS[st.format_A], "At": FORMATS[st.format_At]},
+        "long_row": args.long_row,
         "line_grouped_tiles": {"A": bool(st.line_tiles & 1), "At": bool(st.line_tiles & 2)},
         "row_order": os.environ.get("CPDLP_ORDER", "first_touch"),
         "setup_seconds": st.setup_seconds, "eta": st.eta, "powm_iters": st.powm_iters,
@@ -803,6 +805,8 @@ def main():
     # profiling runs only: cap the solve at this many passes (same kernels, fewer of
     # them); the default solves to the tolerance
     ap.add_argument("--max-passes", type=int, default=10**7)
+    ap.add_argument("--long-row", type=int, default=64,
+                    help="rows longer than this are long rows (wavefront rows, XCD slices)")
     # PMC calibration: also run a plain 512 MiB device-to-device copy kernel 5 times
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--format", default="auto", choices=FORMATS)

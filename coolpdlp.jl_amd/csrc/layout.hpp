@@ -27,9 +27,11 @@ namespace cpdlp {
 struct HostSide {
     int32_t rows = 0, cols = 0;
     int64_t nnz = 0;
-    int32_t n_long = 0;
+    int32_t n_long = 0;    // rows [0, n_long): long (a wavefront per row in CSR)
+    int32_t n_slice = 0;   // rows [0, n_slice) (n_slice >= n_long): cut by the XCD slices
     std::vector<int32_t> rowptr, col, vmap;
-    std::vector<TileMeta> tiles;
+    std::vector<TileMeta> tiles;      // short tiles of rows [n_long, rows)
+    std::vector<TileMeta> xs_tiles;   // short tiles of rows [n_slice, rows) (sliced format)
 };
 
 struct HostLayout {
@@ -146,10 +148,11 @@ inline void build_tiles(const int32_t* rowptr, int32_t first, int32_t rows,
 inline void build_side(HostSide& S, int32_t rows, int32_t cols, const std::vector<int32_t>& src_ptr,
                        const std::vector<int32_t>& src_col, const std::vector<int32_t>* src_k,
                        const std::vector<int32_t>& row_of_int, const std::vector<int32_t>* col_map,
-                       int32_t n_long) {
+                       int32_t n_long, int32_t n_slice) {
     S.rows = rows;
     S.cols = cols;
     S.n_long = n_long;
+    S.n_slice = std::max(n_slice, n_long);
     S.nnz = src_ptr[rows];
     S.rowptr.assign(rows + 1, 0);
     for (int32_t r = 0; r < rows; ++r) {
@@ -168,17 +171,21 @@ inline void build_side(HostSide& S, int32_t rows, int32_t cols, const std::vecto
     }
     S.tiles.clear();
     build_tiles(S.rowptr.data(), S.n_long, S.rows, S.tiles);
+    S.xs_tiles.clear();
+    build_tiles(S.rowptr.data(), S.n_slice, S.rows, S.xs_tiles);
 }
 
 // Order of the short rows and columns: first touch (the default, below), density
 // (increasing stored-entry count, stable: the order sort_rows_columns gives the whole MILP,
-// src/components/permutation.jl:16-46) or the caller's own order.  Long rows and columns
-// come first, longest first, in every order.
+// src/components/permutation.jl:16-46) or the caller's own order.  The rows longer than
+// min(long_row, slice_row) come first, longest first, in every order: those longer than
+// long_row are the long rows (a prefix), the rest of that group are short rows that the
+// sliced format also cuts by column slice.
 enum class RowOrder { first_touch = 0, density = 1, given = 2 };
 
 inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
                                std::vector<int32_t>&& rv, int32_t long_row,
-                               RowOrder order = RowOrder::first_touch) {
+                               RowOrder order = RowOrder::first_touch, int32_t slice_row = INT32_MAX) {
     HostLayout L;
     L.m = m;
     L.n = n;
@@ -186,7 +193,8 @@ inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
     L.colptr = std::move(cp);
     L.rowval = std::move(rv);
     csc_to_csr(m, n, L.colptr, L.rowval, L.csr_rowptr, L.csr_col, L.perm);
-    int32_t nlA = 0, nlAt = 0;
+    const int32_t split = std::min(long_row, slice_row);
+    int32_t nlA = 0, nlAt = 0;   // rows longer than split (the group that comes first)
     // First-touch order of the short rows, in two passes.  Pass 1: A's short rows by the
     // first internal row of A' (column of A) that holds them; pass 2: A's short columns
     // by the first (new) row of A that holds them, then A's rows again.  Each step kernel
@@ -202,13 +210,13 @@ inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
         return first;
     };
     if (order == RowOrder::first_touch) {
-        partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt);
+        partition_rows(L.colptr, split, L.col_of_int, L.int_of_col, nlAt);
         std::vector<int32_t> key = first_touch(n, L.colptr, L.rowval, L.int_of_col, m);
-        partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, &key);
+        partition_rows(L.csr_rowptr, split, L.row_of_int, L.int_of_row, nlA, &key);
         key = first_touch(m, L.csr_rowptr, L.csr_col, L.int_of_row, n);
-        partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt, &key);
+        partition_rows(L.colptr, split, L.col_of_int, L.int_of_col, nlAt, &key);
         key = first_touch(n, L.colptr, L.rowval, L.int_of_col, m);
-        partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, &key);
+        partition_rows(L.csr_rowptr, split, L.row_of_int, L.int_of_row, nlA, &key);
     } else {
         const bool density = order == RowOrder::density;
         auto lengths = [](const std::vector<int32_t>& ptr) {
@@ -217,13 +225,22 @@ inline HostLayout build_layout(int64_t m, int64_t n, std::vector<int32_t>&& cp,
             return len;
         };
         const std::vector<int32_t> row_len = lengths(L.csr_rowptr), col_len = lengths(L.colptr);
-        partition_rows(L.csr_rowptr, long_row, L.row_of_int, L.int_of_row, nlA, density ? &row_len : nullptr);
-        partition_rows(L.colptr, long_row, L.col_of_int, L.int_of_col, nlAt, density ? &col_len : nullptr);
+        partition_rows(L.csr_rowptr, split, L.row_of_int, L.int_of_row, nlA, density ? &row_len : nullptr);
+        partition_rows(L.colptr, split, L.col_of_int, L.int_of_col, nlAt, density ? &col_len : nullptr);
     }
+    // the long rows: the prefix of that group longer than long_row (it is longest first)
+    auto prefix_longer = [](const std::vector<int32_t>& ptr, const std::vector<int32_t>& of_int, int32_t n_first,
+                            int32_t len) {
+        int32_t k = 0;
+        while (k < n_first && ptr[of_int[k] + 1] - ptr[of_int[k]] > len) ++k;
+        return k;
+    };
+    const int32_t longA = prefix_longer(L.csr_rowptr, L.row_of_int, nlA, long_row);
+    const int32_t longAt = prefix_longer(L.colptr, L.col_of_int, nlAt, long_row);
     build_side(L.A, (int32_t)m, (int32_t)n, L.csr_rowptr, L.csr_col, &L.perm, L.row_of_int,
-               &L.int_of_col, nlA);
+               &L.int_of_col, longA, nlA);
     build_side(L.At, (int32_t)n, (int32_t)m, L.colptr, L.rowval, nullptr, L.col_of_int,
-               &L.int_of_row, nlAt);
+               &L.int_of_row, longAt, nlAt);
     return L;
 }
 
@@ -308,19 +325,19 @@ inline int32_t build_ell(const HostSide& S, double max_fill, std::vector<int32_t
     return w;
 }
 
-// XCD column slices of the first `nrows` rows of an internal side (its long rows, or all
-// of its rows).  Column slice s holds a contiguous range of internal columns (ranges
+// XCD column slices of the first `nrows` rows of an internal side (its rows [0, n_slice),
+// or all of its rows).  Column slice s holds a contiguous range of internal columns (ranges
 // balanced by entry count, slice_of_columns); slice s's part of row r (r < nrows) is the
 // run of that row's entries whose column lies in the slice, in the row's entry order.  rowptr:
 // ns * (nrows + 1) absolute offsets, slices stored one after the other.  Each slice's
 // rows are cut into tiles of at most TILE_NNZ entries and BLOCK rows (long rows) or
 // SL_RPT * BLOCK rows (short rows, whose parts hold a few entries) that never span
-// the long/short boundary (so the long rows' tiles are a prefix of every slice's tiles,
-// tile_n_long[s] of them); a row part longer than TILE_NNZ is a tile of its own
-// (r1 = r0 + 1, cnt > TILE_NNZ).  Rows past nrows are not sliced.
+// the long/short boundary n_long nor n_slice (so the tiles of rows [0, n_slice) are a
+// prefix of every slice's tiles, tile_n_pre[s] of them); a row part longer than TILE_NNZ
+// is a tile of its own (r1 = r0 + 1, cnt > TILE_NNZ).  Rows past nrows are not sliced.
 struct HostSlices {
-    int32_t ns = 0, rows = 0, max_tiles = 0, max_tiles_long = 0;
-    std::vector<int32_t> rowptr, col, vmap, tile_off, tile_n_long;
+    int32_t ns = 0, rows = 0, max_tiles = 0, max_tiles_pre = 0;
+    std::vector<int32_t> rowptr, col, vmap, tile_off, tile_n_pre;
     std::vector<TileMeta> tiles;
 };
 
@@ -344,7 +361,7 @@ inline std::vector<int8_t> slice_of_columns(const HostSide& S, int32_t ns, int32
 }
 
 inline void build_slices(const HostSide& S, int32_t ns, int32_t nrows, HostSlices& H) {
-    const int32_t nr = nrows, nl = std::min(S.n_long, nrows);
+    const int32_t nr = nrows, nl = std::min(S.n_long, nrows), np = std::min(std::max(S.n_slice, S.n_long), nrows);
     const std::vector<int8_t> slice_of = slice_of_columns(S, ns, nr);
     H.ns = ns;
     H.rows = nr;
@@ -375,15 +392,15 @@ inline void build_slices(const HostSide& S, int32_t ns, int32_t nrows, HostSlice
         }
     H.tiles.clear();
     H.tile_off.assign(ns + 1, 0);
-    H.tile_n_long.assign(ns, 0);
-    H.max_tiles = H.max_tiles_long = 0;
+    H.tile_n_pre.assign(ns, 0);
+    H.max_tiles = H.max_tiles_pre = 0;
     for (int32_t sl = 0; sl < ns; ++sl) {
         H.tile_off[sl] = (int32_t)H.tiles.size();
         const int32_t* rp = H.rowptr.data() + (size_t)sl * (nr + 1);
         int32_t r = 0;
         while (r < nr) {
-            if (r == nl) H.tile_n_long[sl] = (int32_t)H.tiles.size() - H.tile_off[sl];
-            const int32_t r0 = r, lim = r0 < nl ? nl : nr;
+            if (r == np) H.tile_n_pre[sl] = (int32_t)H.tiles.size() - H.tile_off[sl];
+            const int32_t r0 = r, lim = r0 < nl ? nl : r0 < np ? np : nr;
             // short rows' parts are summed a thread per row, SL_RPT rows per thread
             const int32_t max_rows = r0 < nl ? BLOCK : SL_RPT * BLOCK;
             if (rp[r + 1] - rp[r] > TILE_NNZ) {
@@ -395,9 +412,9 @@ inline void build_slices(const HostSide& S, int32_t ns, int32_t nrows, HostSlice
             }
             H.tiles.push_back(TileMeta{r0, r, rp[r0], rp[r] - rp[r0]});
         }
-        if (nl == nr) H.tile_n_long[sl] = (int32_t)H.tiles.size() - H.tile_off[sl];
+        if (np == nr) H.tile_n_pre[sl] = (int32_t)H.tiles.size() - H.tile_off[sl];
         H.max_tiles = std::max(H.max_tiles, (int32_t)H.tiles.size() - H.tile_off[sl]);
-        H.max_tiles_long = std::max(H.max_tiles_long, H.tile_n_long[sl]);
+        H.max_tiles_pre = std::max(H.max_tiles_pre, H.tile_n_pre[sl]);
     }
     H.tile_off[ns] = (int32_t)H.tiles.size();
 }

@@ -92,7 +92,7 @@ int cpdlp_matrix_create(int64_t m, int64_t n, int32_t index_base, int32_t index_
         int32_t n_long = 0;
         partition_rows(rp, 64, M->row_of_int, M->int_of_row, n_long);
         HostSide H;
-        build_side(H, rows, cols, rp, col, transpose ? nullptr : &perm, M->row_of_int, nullptr, n_long);
+        build_side(H, rows, cols, rp, col, transpose ? nullptr : &perm, M->row_of_int, nullptr, n_long, n_long);
         M->S.upload_structure(H, M->stream);
         // an explicit ELL is built whatever its padding (as GPUSparseMatrixELL does);
         // AUTO only considers it within 1.5x nnz

@@ -329,16 +329,18 @@ MatView<V> DevSide::view(bool allow_slices) const {
     v.ell_val = ell_vals<V>();
     v.ns = sl_ns;
     v.sl_all = format == CPDLP_FMT_XSLICE_ALL;
-    v.sl_rows = v.sl_all ? rows : n_long;
+    v.sl_rows = v.sl_all ? rows : n_slice;
     v.sl_stride = sl_built + 1;
     v.sl_rowptr = sl_rowptr.p;
     v.sl_col = sl_col.p;
     v.sl_val = sl_vals<V>();
     v.sl_tiles = sl_tiles.p;
     v.sl_tile_off = sl_tile_off.p;
-    v.sl_tile_n = v.sl_all ? sl_tile_n.p : sl_tile_n_long.p;
-    v.sl_max_tiles = v.sl_all ? sl_max_tiles : sl_max_tiles_long;
+    v.sl_tile_n = v.sl_all ? sl_tile_n.p : sl_tile_n_pre.p;
+    v.sl_max_tiles = v.sl_all ? sl_max_tiles : sl_max_tiles_pre;
     v.sl_part = sl_part.p;
+    v.xs_tiles = xs_tiles.p;
+    v.n_xs_tiles = n_xs_tiles;
     // both slice variants run the sliced kernels
     if (format == CPDLP_FMT_XSLICE_ALL) v.format = CPDLP_FMT_XSLICE;
     if (v.format == CPDLP_FMT_XSLICE && !allow_slices) v.format = CPDLP_FMT_CSR;
@@ -380,7 +382,7 @@ void DevSide::build_slices(const HostSide& S, int32_t nrows, int32_t NS, hipStre
     sl_ns = NS;
     sl_built = nrows;
     sl_max_tiles = H.max_tiles;
-    sl_max_tiles_long = H.max_tiles_long;
+    sl_max_tiles_pre = H.max_tiles_pre;
     sl_nnz = (int64_t)H.col.size();
     sl_rowptr.upload(H.rowptr, s);
     sl_col.alloc(std::max<int64_t>(sl_nnz, 1));
@@ -392,7 +394,7 @@ void DevSide::build_slices(const HostSide& S, int32_t nrows, int32_t NS, hipStre
     std::vector<int32_t> tn(NS);
     for (int32_t k = 0; k < NS; ++k) tn[k] = H.tile_off[k + 1] - H.tile_off[k];
     sl_tile_n.upload(tn, s);
-    sl_tile_n_long.upload(H.tile_n_long, s);
+    sl_tile_n_pre.upload(H.tile_n_pre, s);
     sl_part.alloc(std::max<int64_t>((int64_t)SL_KMAX * NS * nrows, 1));
     HIP_CHECK(hipStreamSynchronize(s));
 }
@@ -402,13 +404,16 @@ void DevSide::upload_structure(const HostSide& S, hipStream_t s) {
     cols = S.cols;
     nnz = S.nnz;
     n_long = S.n_long;
+    n_slice = std::max(S.n_slice, S.n_long);
     n_tiles = (int32_t)S.tiles.size();
+    n_xs_tiles = (int32_t)S.xs_tiles.size();
     rowptr.upload(S.rowptr, s);
     col.alloc(std::max<int64_t>(nnz, 1));
     col.upload(S.col.data(), S.col.size(), s);
     vmap.alloc(std::max<int64_t>(nnz, 1));
     vmap.upload(S.vmap.data(), S.vmap.size(), s);
     tiles.upload(S.tiles, s);
+    xs_tiles.upload(S.xs_tiles, s);
 }
 
 void DevSide::fill_values(const double* src_csc, bool want32, hipStream_t s) {
@@ -542,7 +547,14 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         else if (v == "given") order = RowOrder::given;
         else if (v != "first_touch") throw Error(CPDLP_ERR_ARGUMENT, "CPDLP_ORDER must be first_touch, density or given");
     }
-    L = build_layout(m, n, std::move(cp), std::move(rv), P.long_row, order);
+    // rows longer than slice_row (32; CPDLP_SLICE_ROW for measurements) are cut by the XCD
+    // slices in the sliced format; in CSR only those longer than long_row leave the
+    // row-ordered tiles.  Measured on the large LP's dual step: 143.9 us slicing rows
+    // longer than 64, 134.2 us longer than 32, 134.1 us longer than 16.
+    int32_t slice_row = 32;
+    if (const char* e = std::getenv("CPDLP_SLICE_ROW")) slice_row = (int32_t)std::strtol(e, nullptr, 10);
+    if (slice_row <= 0) throw Error(CPDLP_ERR_ARGUMENT, "CPDLP_SLICE_ROW must be positive");
+    L = build_layout(m, n, std::move(cp), std::move(rv), P.long_row, order, slice_row);
     const hipStream_t s = stream;
     auto up64 = [&](DevBuf<double>& d, const double* h, int64_t len) {
         d.alloc(std::max<int64_t>(len, 1));
@@ -577,9 +589,10 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         A.build_ell(L.A, s, fill);
         At.build_ell(L.At, s, fill);
     }
-    // XCD slices: XSLICE cuts the long rows, XSLICE_ALL every row.  AUTO builds slices
-    // of every row for a large side whose gathered vector outgrows one XCD's 4 MiB L2
-    // (the tuner then times both variants), else of the long rows when it has any.
+    // XCD slices: XSLICE cuts the rows longer than slice_row, XSLICE_ALL every row.  AUTO
+    // builds slices of every row for a large side whose gathered vector outgrows one XCD's
+    // 4 MiB L2 (the tuner then times both variants), else of the rows longer than
+    // slice_row when it has any.
     const size_t elem = P.precision == CPDLP_PREC_FP64 ? 8 : 4;
     // slices per side: 8 (one per XCD) unless CPDLP_SLICES="<A>,<A'>" (2, 4 or 8) says
     // otherwise, for measurements
@@ -597,8 +610,8 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         const bool big = (size_t)H.cols * elem > ((size_t)4 << 20) && H.nnz >= ((int64_t)4 << 20);
         if (P.format == CPDLP_FMT_XSLICE_ALL || (P.format == CPDLP_FMT_AUTO && big))
             side.first->build_slices(H, H.rows, ns, s);
-        else if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && H.n_long > 0))
-            side.first->build_slices(H, H.n_long, ns, s);
+        else if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && H.n_slice > 0))
+            side.first->build_slices(H, H.n_slice, ns, s);
     }
     // Line-grouped tiles of the short rows (the steps of a CSR side): built for a side
     // whose short rows (>= 1M entries) would need >= 10% fewer gather requests in them,
@@ -833,7 +846,7 @@ void Handle::choose_formats() {
             const std::string want = S == &A ? spec.substr(0, comma)
                                              : (comma == std::string::npos ? "" : spec.substr(comma + 1));
             if (want == "ell" && S->ell_w > 0) S->format = CPDLP_FMT_ELL;
-            if (want == "xslice" && S->sl_ns > 0 && S->n_long > 0) S->format = CPDLP_FMT_XSLICE;
+            if (want == "xslice" && S->sl_ns > 0 && S->n_slice > 0) S->format = CPDLP_FMT_XSLICE;
             if (want == "xslice_all" && S->slices_all()) S->format = CPDLP_FMT_XSLICE_ALL;
             continue;
         }
@@ -842,7 +855,7 @@ void Handle::choose_formats() {
         // (25% for the slices of every row)
         std::vector<int32_t> cand{CPDLP_FMT_CSR};
         if (S->ell_w > 0) cand.push_back(CPDLP_FMT_ELL);
-        if (S->sl_ns > 0 && S->n_long > 0) cand.push_back(CPDLP_FMT_XSLICE);
+        if (S->sl_ns > 0 && S->n_slice > 0) cand.push_back(CPDLP_FMT_XSLICE);
         if (S->slices_all()) cand.push_back(CPDLP_FMT_XSLICE_ALL);
         if (cand.size() == 1) continue;
         DevBuf<double> xin(std::max<int64_t>(S->cols, 1)), out(std::max<int64_t>(S->rows, 1));

@@ -12,21 +12,21 @@ namespace cpdlp {
 // one matrix (A or A') on the device, internal layout; FP64 values always, FP32 copy
 // for the FP32 / mixed loop; optional column-major ELL of the same rows
 struct DevSide {
-    int32_t rows = 0, cols = 0, n_long = 0, n_tiles = 0;
+    int32_t rows = 0, cols = 0, n_long = 0, n_slice = 0, n_tiles = 0, n_xs_tiles = 0;
     int64_t nnz = 0;
     int32_t format = CPDLP_FMT_CSR;
     int32_t ell_w = 0;
     DevBuf<int32_t> rowptr, col, vmap;
-    DevBuf<TileMeta> tiles;
+    DevBuf<TileMeta> tiles, xs_tiles;
     DevBuf<double> val64;
     DevBuf<float> val32;
     DevBuf<int32_t> ell_col, ell_vmap;
     DevBuf<double> ell_val64;
     DevBuf<float> ell_val32;
-    // XCD column slices of rows [0, sl_built) (the long rows, or all rows)
-    int32_t sl_ns = 0, sl_built = 0, sl_max_tiles = 0, sl_max_tiles_long = 0;
+    // XCD column slices of rows [0, sl_built) (rows [0, n_slice), or all rows)
+    int32_t sl_ns = 0, sl_built = 0, sl_max_tiles = 0, sl_max_tiles_pre = 0;
     int64_t sl_nnz = 0;
-    DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_off, sl_tile_n, sl_tile_n_long;
+    DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_off, sl_tile_n, sl_tile_n_pre;
     DevBuf<TileMeta> sl_tiles;
     DevBuf<double> sl_val64, sl_part;
     DevBuf<float> sl_val32;

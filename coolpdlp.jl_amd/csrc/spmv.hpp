@@ -70,11 +70,12 @@ struct MatView {
     int32_t ell_width = 0;
     const int32_t* ell_col = nullptr;
     const V* ell_val = nullptr;
-    // XCD column slices (format 3): the first sl_rows rows (the long rows, or every
-    // row when sl_all) are also stored cut by column slice, NS slices (one per XCD); the
-    // slice-s part of every such row is summed by a block with blockIdx % NS == s
-    // (workgroups are dispatched to the XCDs round-robin), so each XCD gathers from one
-    // slice of the vector, which stays in that XCD's L2.
+    // XCD column slices (format 3): the first sl_rows rows (the rows longer than the
+    // slice threshold -- the long rows and the longest short ones --, or every row when
+    // sl_all) are also stored cut by column slice, NS slices (one per XCD); the slice-s
+    // part of every such row is summed by a block with blockIdx % NS == s (workgroups are
+    // dispatched to the XCDs round-robin), so each XCD gathers from one slice of the
+    // vector, which stays in that XCD's L2.  The other rows run as the short tiles xs_tiles.
     int32_t ns = 0;
     int32_t sl_rows = 0;                  // rows [0, sl_rows) are sliced
     bool sl_all = false;                  // sl_rows == rows: no short-tile blocks
@@ -87,6 +88,8 @@ struct MatView {
     const int32_t* sl_tile_n = nullptr;   // ns: tiles of each slice covering [0, sl_rows)
     int32_t sl_max_tiles = 0;             // most of any slice
     void* sl_part = nullptr;              // ns * sl_rows slice sums (Acc) per vector
+    const TileMeta* xs_tiles = nullptr;   // short tiles of rows [sl_rows, rows)
+    int32_t n_xs_tiles = 0;
 
     // CSR short tiles: they start at a block index that is a multiple of 8 (blocks
     // [n_long_blocks, csr_short0()) are idle), and XCD x takes the contiguous tiles
@@ -100,7 +103,7 @@ struct MatView {
         return t < n_tiles ? t : -1;
     }
     // the sliced format's blocks that run the side's own short tiles
-    __host__ __device__ int sl_short_tiles() const { return sl_all ? 0 : n_tiles; }
+    __host__ __device__ int sl_short_tiles() const { return sl_all ? 0 : n_xs_tiles; }
     int grid() const {
         if (format == CPDLP_FMT_ELL) return ceil_div(rows, BLOCK);
         if (format == CPDLP_FMT_XSLICE) return ns * sl_max_tiles + sl_short_tiles();
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XP X, Epi e
         double q[Epi::NQ > 0 ? Epi::NQ : 1];
 #pragma unroll
         for (int i = 0; i < (Epi::NQ > 0 ? Epi::NQ : 1); ++i) q[i] = 0.0;
-        const TileMeta tm = A.tiles[blockIdx.x - nsb];
+        const TileMeta tm = A.xs_tiles[blockIdx.x - nsb];
         const int r = tm.r0 + tid;
         typename Epi::Pre pre{};
         if (r < tm.r1) pre = epi.load(r);

@@ -52,7 +52,7 @@ static HostSide make_side(std::mt19937& rng, int rows, int cols, int n_long, int
 }
 
 static void check_slices(const HostSide& S, int nr) {
-    const int ns = 8, nl = std::min(S.n_long, nr);
+    const int ns = 8, nl = std::min(S.n_long, nr), np = std::min(std::max(S.n_slice, S.n_long), nr);
     HostSlices H;
     build_slices(S, ns, nr, H);
     const std::vector<int8_t> so = slice_of_columns(S, ns, nr);
@@ -83,8 +83,8 @@ static void check_slices(const HostSide& S, int nr) {
         }
         if (at != S.rowptr[r + 1]) fail("row part missing entries", r, at);
     }
-    // tiles: cover [0, nr) per slice, limits, never span the long/short boundary, and the
-    // long rows' tiles are the first tile_n_long of the slice
+    // tiles: cover [0, nr) per slice, limits, never span the long/short boundary nor
+    // n_slice, and the tiles of rows [0, n_slice) are the first tile_n_pre of the slice
     int maxt = 0, maxl = 0;
     for (int s = 0; s < ns; ++s) {
         const int32_t* rp = H.rowptr.data() + (size_t)s * (nr + 1);
@@ -97,15 +97,16 @@ static void check_slices(const HostSide& S, int nr) {
             if (tm.base != rp[tm.r0] || tm.cnt != rp[tm.r1] - rp[tm.r0]) fail("tile base", s, t);
             if (tm.cnt > TILE_NNZ && tm.r1 - tm.r0 != 1) fail("oversized tile of several rows", s, t);
             if (tm.r0 < nl && tm.r1 > nl) fail("tile spans the long/short boundary", s, t);
-            const bool in_long_prefix = t - H.tile_off[s] < H.tile_n_long[s];
-            if (in_long_prefix != (tm.r0 < nl)) fail("long tile count", s, t);
+            if (tm.r0 < np && tm.r1 > np) fail("tile spans n_slice", s, t);
+            const bool in_prefix = t - H.tile_off[s] < H.tile_n_pre[s];
+            if (in_prefix != (tm.r0 < np)) fail("prefix tile count", s, t);
             expect = tm.r1;
         }
         if (expect != nr) fail("tiles end early", s, expect);
         maxt = std::max(maxt, H.tile_off[s + 1] - H.tile_off[s]);
-        maxl = std::max(maxl, H.tile_n_long[s]);
+        maxl = std::max(maxl, H.tile_n_pre[s]);
     }
-    if (maxt != H.max_tiles || maxl != H.max_tiles_long) fail("max tiles", maxt, maxl);
+    if (maxt != H.max_tiles || maxl != H.max_tiles_pre) fail("max tiles", maxt, maxl);
 }
 
 static void check_partition_and_tiles(std::mt19937& rng) {
@@ -264,8 +265,49 @@ static void check_line_tiles_gain(std::mt19937& rng) {
     if (!(T.req_lines * 10 < T.req_rows * 9)) fail("no gain on a hot head", T.req_lines, T.req_rows);
 }
 
+// slice_row below long_row: the rows longer than slice_row come first, longest first; the
+// long rows (longer than long_row) are their prefix; the short tiles cover [n_long, rows)
+// and the sliced format's short tiles [n_slice, rows)
+static void check_slice_split(std::mt19937& rng) {
+    const int m = 2500, n = 6000;
+    std::vector<int32_t> cp(n + 1, 0), rv;
+    for (int j = 0; j < n; ++j) {
+        std::vector<int32_t> rs;
+        const int len = (int)(rng() % 14);
+        for (int k = 0; k < len; ++k) rs.push_back((int32_t)(rng() % (k % 3 == 0 ? 200 : m)));
+        std::sort(rs.begin(), rs.end());
+        rs.erase(std::unique(rs.begin(), rs.end()), rs.end());
+        rv.insert(rv.end(), rs.begin(), rs.end());
+        cp[j + 1] = (int32_t)rv.size();
+    }
+    const HostLayout L = build_layout(m, n, std::move(cp), std::move(rv), 64, RowOrder::first_touch, 8);
+    for (const HostSide* S : {&L.A, &L.At}) {
+        auto len = [&](int r) { return S->rowptr[r + 1] - S->rowptr[r]; };
+        if (S->n_slice < S->n_long) fail("n_slice below n_long", S->n_slice, S->n_long);
+        for (int r = 0; r < S->rows; ++r) {
+            if ((r < S->n_long) != (len(r) > 64)) fail("long rows not the prefix", r, len(r));
+            if ((r < S->n_slice) != (len(r) > 8)) fail("sliced rows not the prefix", r, len(r));
+            if (r > 0 && r < S->n_slice && len(r) > len(r - 1)) fail("sliced rows not longest first", r, 0);
+        }
+        int expect = S->n_slice;
+        for (const TileMeta& tm : S->xs_tiles) {
+            if (tm.r0 != expect || tm.r1 - tm.r0 > BLOCK || tm.cnt > TILE_NNZ) fail("xs tile", tm.r0, tm.r1);
+            expect = tm.r1;
+        }
+        if (expect != S->rows) fail("xs tiles end early", expect, S->rows);
+        expect = S->n_long;
+        for (const TileMeta& tm : S->tiles) {
+            if (tm.r0 != expect) fail("short tile", tm.r0, expect);
+            expect = tm.r1;
+        }
+        if (expect != S->rows) fail("short tiles end early", expect, S->rows);
+    }
+    if (L.A.n_slice == L.A.n_long) fail("no sliced short rows in the test matrix", L.A.n_slice, 0);
+}
+
 int main() {
     std::mt19937 rng(12345);
+    check_slice_split(rng);
     check_partition_and_tiles(rng);
     check_build_layout(rng);
     for (int trial = 0; trial < 6; ++trial) {
@@ -275,6 +317,10 @@ int main() {
         const HostSide S = make_side(rng, rows, cols, n_long, maxlong, trial % 2 == 1);
         check_slices(S, S.n_long);   // xslice: long rows
         check_slices(S, S.rows);     // xslice_all: every row
+        HostSide S2 = S;             // sliced prefix beyond the long rows
+        S2.n_slice = std::min(S.rows, S.n_long + 300);
+        check_slices(S2, S2.n_slice);
+        check_slices(S2, S2.rows);
         check_line_tiles(S, 16);
         check_line_tiles(S, 4);
     }
