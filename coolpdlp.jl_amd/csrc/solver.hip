@@ -547,11 +547,13 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         else if (v == "given") order = RowOrder::given;
         else if (v != "first_touch") throw Error(CPDLP_ERR_ARGUMENT, "CPDLP_ORDER must be first_touch, density or given");
     }
-    // rows longer than slice_row (32; CPDLP_SLICE_ROW for measurements) are cut by the XCD
+    // rows longer than slice_row (16; CPDLP_SLICE_ROW for measurements) are cut by the XCD
     // slices in the sliced format; in CSR only those longer than long_row leave the
-    // row-ordered tiles.  Measured on the large LP's dual step: 143.9 us slicing rows
-    // longer than 64, 134.2 us longer than 32, 134.1 us longer than 16.
-    int32_t slice_row = 32;
+    // row-ordered tiles.  Measured on the large LP (primal / dual step, us): rows longer
+    // than 64 sliced 105.8 / 142.8, than 32 104.8 / 132.7, 20 104.1 / 128.8, 16 102.2 /
+    // 126.7, 12 102.6 / 127.2, 8 109.1 / 128.5 (the split also orders the other side's
+    // rows longer than it by length, not by first touch: at 8 most of A''s).
+    int32_t slice_row = 16;
     if (const char* e = std::getenv("CPDLP_SLICE_ROW")) slice_row = (int32_t)std::strtol(e, nullptr, 10);
     if (slice_row <= 0) throw Error(CPDLP_ERR_ARGUMENT, "CPDLP_SLICE_ROW must be positive");
     L = build_layout(m, n, std::move(cp), std::move(rv), P.long_row, order, slice_row);
