@@ -594,7 +594,8 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
     // XCD slices: XSLICE cuts the rows longer than slice_row, XSLICE_ALL every row.  AUTO
     // builds slices of every row for a large side whose gathered vector outgrows one XCD's
     // 4 MiB L2 (the tuner then times both variants), else of the rows longer than
-    // slice_row when it has any.
+    // slice_row when it has any and its gathered vector exceeds 2 MiB (a smaller one stays
+    // in L2 without them: small LPs skip building and timing them).
     const size_t elem = P.precision == CPDLP_PREC_FP64 ? 8 : 4;
     // slices per side: 8 (one per XCD) unless CPDLP_SLICES="<A>,<A'>" (2, 4 or 8) says
     // otherwise, for measurements
@@ -612,7 +613,8 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         const bool big = (size_t)H.cols * elem > ((size_t)4 << 20) && H.nnz >= ((int64_t)4 << 20);
         if (P.format == CPDLP_FMT_XSLICE_ALL || (P.format == CPDLP_FMT_AUTO && big))
             side.first->build_slices(H, H.rows, ns, s);
-        else if (P.format == CPDLP_FMT_XSLICE || (P.format == CPDLP_FMT_AUTO && H.n_slice > 0))
+        else if (P.format == CPDLP_FMT_XSLICE ||
+                 (P.format == CPDLP_FMT_AUTO && H.n_slice > 0 && (size_t)H.cols * elem > ((size_t)2 << 20)))
             side.first->build_slices(H, H.n_slice, ns, s);
     }
     // Line-grouped tiles of the short rows (the steps of a CSR side): built for a side
