@@ -53,7 +53,7 @@ extern "C" {
 #define CPDLP_FMT_AUTO 0        /* autotuned per matrix */
 #define CPDLP_FMT_CSR 1         /* GPUSparseMatrixCSR, src/utils/mat_csr.jl */
 #define CPDLP_FMT_ELL 2         /* GPUSparseMatrixELL, src/utils/mat_ell.jl */
-#define CPDLP_FMT_XSLICE 3      /* long rows cut into 8 column slices, one per XCD's L2 */
+#define CPDLP_FMT_XSLICE 3      /* rows longer than 16 cut into 8 column slices, one per XCD's L2 */
 #define CPDLP_FMT_XSLICE_ALL 4  /* every row cut into 8 column slices */
 
 /* ---- problem input --------------------------------------------------------------
