@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <limits>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -267,8 +268,11 @@ inline bool line_tiles_packable(const HostSide& S) {
     return S.cols <= (int64_t(1) << LG_COL_BITS) && LG_TILE_NNZ <= (int64_t(1) << (32 - LG_COL_BITS));
 }
 
-// estimate only (T.col / T.vmap stay empty) when build == false
-inline void line_group_tiles(const HostSide& S, int32_t line_entries, bool build, HostLineTiles& T) {
+// estimate only (T.col / T.vmap stay empty) when build == false; sample > 1 estimates from
+// every sample-th tile of each order.  Tiles are independent: the work is split over up to
+// 16 host threads.
+inline void line_group_tiles(const HostSide& S, int32_t line_entries, bool build, HostLineTiles& T,
+                             int32_t sample = 1) {
     T.req_rows = T.req_lines = 0;
     T.tiles.clear();
     build_tiles(S.rowptr.data(), S.n_long, S.rows, T.tiles, LG_NT, LG_TILE_NNZ);
@@ -276,8 +280,9 @@ inline void line_group_tiles(const HostSide& S, int32_t line_entries, bool build
         T.col.assign((size_t)std::max<int64_t>(S.nnz, 1), 0u);
         T.vmap.assign((size_t)std::max<int64_t>(S.nnz, 1), 0);
     }
-    std::vector<int32_t> idx, win;
-    auto lines_in_windows = [&](const int32_t* colv, int32_t cnt) {
+    sample = std::max(sample, 1);
+    // distinct lines per 64-entry window of colv[0, cnt)
+    auto lines_in_windows = [line_entries](const int32_t* colv, int32_t cnt, std::vector<int32_t>& win) {
         int64_t req = 0;
         for (int32_t w0 = 0; w0 < cnt; w0 += WAVE) {
             win.assign(colv + w0, colv + std::min(cnt, w0 + WAVE));
@@ -287,23 +292,44 @@ inline void line_group_tiles(const HostSide& S, int32_t line_entries, bool build
         }
         return req;
     };
-    for (const TileMeta& t : S.tiles)
-        if (t.cnt > 0) T.req_rows += lines_in_windows(S.col.data() + t.base, t.cnt);
-    std::vector<int32_t> sorted_cols;
-    for (const TileMeta& t : T.tiles) {
-        if (t.cnt <= 0) continue;
-        const int32_t* c = S.col.data() + t.base;
-        idx.resize(t.cnt);
-        std::iota(idx.begin(), idx.end(), 0);
-        std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return c[a] < c[b]; });
-        sorted_cols.resize(t.cnt);
-        for (int32_t e = 0; e < t.cnt; ++e) sorted_cols[e] = c[idx[e]];
-        T.req_lines += lines_in_windows(sorted_cols.data(), t.cnt);
-        if (build)
-            for (int32_t e = 0; e < t.cnt; ++e) {
-                T.col[t.base + e] = ((uint32_t)idx[e] << LG_COL_BITS) | (uint32_t)c[idx[e]];
-                T.vmap[t.base + e] = S.vmap[t.base + idx[e]];
+    // worker w takes tiles [w * n / nw, (w + 1) * n / nw) of both lists
+    auto work = [&](size_t w, size_t nw, int64_t& req_rows, int64_t& req_lines) {
+        std::vector<int32_t> idx, win, sorted_cols;
+        const size_t nr = S.tiles.size(), nt = T.tiles.size();
+        for (size_t i = w * nr / nw; i < (w + 1) * nr / nw; ++i) {
+            const TileMeta& t = S.tiles[i];
+            if (t.cnt > 0 && i % sample == 0) req_rows += lines_in_windows(S.col.data() + t.base, t.cnt, win);
+        }
+        for (size_t i = w * nt / nw; i < (w + 1) * nt / nw; ++i) {
+            const TileMeta& t = T.tiles[i];
+            const bool est = i % sample == 0;
+            if (t.cnt <= 0 || (!build && !est)) continue;
+            const int32_t* c = S.col.data() + t.base;
+            idx.resize(t.cnt);
+            std::iota(idx.begin(), idx.end(), 0);
+            std::stable_sort(idx.begin(), idx.end(), [&](int32_t a, int32_t b) { return c[a] < c[b]; });
+            if (est) {
+                sorted_cols.resize(t.cnt);
+                for (int32_t e = 0; e < t.cnt; ++e) sorted_cols[e] = c[idx[e]];
+                req_lines += lines_in_windows(sorted_cols.data(), t.cnt, win);
             }
+            if (build)
+                for (int32_t e = 0; e < t.cnt; ++e) {
+                    T.col[t.base + e] = ((uint32_t)idx[e] << LG_COL_BITS) | (uint32_t)c[idx[e]];
+                    T.vmap[t.base + e] = S.vmap[t.base + idx[e]];
+                }
+        }
+    };
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nw = std::max<size_t>(1, std::min<size_t>({(size_t)16, hw, T.tiles.size() / 64 + 1}));
+    std::vector<int64_t> rr(nw, 0), rl(nw, 0);
+    std::vector<std::thread> pool;
+    for (size_t w = 1; w < nw; ++w) pool.emplace_back(work, w, nw, std::ref(rr[w]), std::ref(rl[w]));
+    work(0, nw, rr[0], rl[0]);
+    for (std::thread& th : pool) th.join();
+    for (size_t w = 0; w < nw; ++w) {
+        T.req_rows += rr[w];
+        T.req_lines += rl[w];
     }
 }
 

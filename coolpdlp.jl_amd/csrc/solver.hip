@@ -631,8 +631,11 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
             if (H.n_long == H.rows || !line_tiles_packable(H)) continue;
             if (lg_mode < 0 && short_nnz < ((int64_t)1 << 20)) continue;
             HostLineTiles T;
+            if (lg_mode < 0) {   // estimate from every 8th tile before building them all
+                line_group_tiles(H, (int32_t)(128 / elem), false, T, 8);
+                if ((double)T.req_lines > 0.9 * (double)T.req_rows) continue;
+            }
             line_group_tiles(H, (int32_t)(128 / elem), true, T);
-            if (lg_mode < 0 && (double)T.req_lines > 0.9 * (double)T.req_rows) continue;
             side.first->build_line_tiles(T, s);
         }
     }
