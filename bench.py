@@ -827,7 +827,7 @@ def main():
                     help="the one-at-a-time baseline: run it on the same arrivals, model it, or skip it")
     ap.add_argument("--serve-checks", type=int, default=20)
     ap.add_argument("--serve-batch-gib", type=int, default=16)
-    ap.add_argument("--serve-medium-workers", type=int, default=2)
+    ap.add_argument("--serve-medium-workers", type=int, default=3)
     ap.add_argument("--serve-batch2-limit", type=int, default=0,
                     help="m + n limit of the library server's second batch route (0: off)")
     ap.add_argument("--serve-batch2-gib", type=float, default=2.0)

@@ -80,7 +80,7 @@ class Server:
     # default: medium LPs in the batch lengthen the rounds of the small ones.
     overflow_queue: int = 0
     # plain lanes (max nonzeros, workers): medium requests, then everything larger
-    lanes: tuple = ((1_000_000, 2), (float("inf"), 1))
+    lanes: tuple = ((1_000_000, 3), (float("inf"), 1))
     time_limit: float = 600.0
     keep_solutions: bool = False
     # opt-in: infeasible / unbounded / NaN requests stop at the check that proves it
@@ -235,7 +235,7 @@ class NativeServer:
 
     def __init__(self, algo=None, *, batch_bytes: int = 16 << 30, plain_bytes: int = 64 << 30,
                  batch_limit: int = 8192, checks_per_round: int = 20,
-                 lanes=((1_000_000, 2), (float("inf"), 1)),
+                 lanes=((1_000_000, 3), (float("inf"), 1)),
                  batch2_limit: int = 0, batch2_bytes: int = 2 << 30, batch2_checks: int = 0):
         import ctypes as C
         from . import _lib

@@ -516,7 +516,7 @@ int cpdlp_server_default_config(cpdlp_server_config* c) {
         c->checks_per_round = 20;
         c->n_lanes = 2;
         c->lane_max_nnz[0] = 1e6;
-        c->lane_workers[0] = 2;
+        c->lane_workers[0] = 3;   // 2 / 3 / 4 measured at 300 QPS: 3 served the most small and medium
         c->lane_max_nnz[1] = INFINITY;
         c->lane_workers[1] = 1;
         c->batch2_bytes = (int64_t)2 << 30;

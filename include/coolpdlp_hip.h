@@ -369,7 +369,7 @@ typedef struct cpdlp_server_config {
     int32_t batch_limit;            /* m + n limit of the batch route, <= 65536; 0: none (8192) */
     int32_t checks_per_round;       /* checks per pump for the smallest instances (20) */
     int32_t n_lanes;                /* plain lanes, 1..CPDLP_SERVER_MAX_LANES (2) */
-    int32_t lane_workers[CPDLP_SERVER_MAX_LANES];   /* worker threads per lane (2, 1) */
+    int32_t lane_workers[CPDLP_SERVER_MAX_LANES];   /* worker threads per lane (3, 1) */
     double lane_max_nnz[CPDLP_SERVER_MAX_LANES];    /* nonzeros a lane admits (1e6, inf) */
     int64_t batch2_bytes;           /* second batch route's HBM arena (2 GiB) */
     int32_t batch2_limit;           /* its m + n limit, <= 65536; none when <= batch_limit (0) */

@@ -119,7 +119,7 @@ def test_server_config_defaults_and_validation():
     _lib.check(_lib.lib.cpdlp_server_default_config(C.byref(cfg)))
     assert (cfg.batch_bytes, cfg.plain_bytes) == (16 << 30, 64 << 30)
     assert (cfg.batch_limit, cfg.checks_per_round, cfg.n_lanes) == (8192, 20, 2)
-    assert list(cfg.lane_workers)[:2] == [2, 1] and cfg.lane_max_nnz[0] == 1e6
+    assert list(cfg.lane_workers)[:2] == [3, 1] and cfg.lane_max_nnz[0] == 1e6
     assert cfg.lane_max_nnz[1] == float("inf")
     assert (cfg.batch2_bytes, cfg.batch2_limit, cfg.batch2_checks) == (2 << 30, 0, 0)
     import coolpdlp_hip as cp
