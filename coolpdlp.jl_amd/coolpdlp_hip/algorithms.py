@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ArgumentError, check, lib
-from .problems import MILP, PrimalDualSolution
+from .problems import MILP, PrimalDualSolution, require_unpreconditioned
 
 
 class TerminationStatus(enum.IntEnum):
@@ -216,6 +216,7 @@ class Engine:
     """One LP resident on the GPU (a cpdlp_handle): build once, solve, query."""
 
     def __init__(self, milp: MILP, algo: Algorithm):
+        require_unpreconditioned(milp)
         self.milp = milp
         self.algo = algo
         A = milp.A
@@ -368,6 +369,57 @@ class Engine:
         eta, omega, it = C.c_double(), C.c_double(), C.c_int64()
         check(lib.cpdlp_get_step_size(self.h, C.byref(eta), C.byref(omega), C.byref(it)))
         return eta.value, omega.value, it.value
+
+
+def preprocess(milp: MILP, sol: PrimalDualSolution | None, algo: Algorithm):
+    """preprocess(milp_init, sol_init, algo), src/algorithms/common.jl:147-168.
+
+    Returns (MILP, PrimalDualSolution): the problem and start point after the
+    preconditioning (Ruiz + Chambolle-Pock, src/components/preconditioning.jl:95-155),
+    whose D1 and D2 the engine computes on the device, applied as precondition
+    (preconditioning.jl:37-91): A~ = (D1 a) D2, c~ = D2 c, lv~ = lv / D2, uv~ = uv / D2,
+    lc~ = D1 lc, uc~ = D1 uc, x~ = x / D2, y~ = y / D1, in the caller's order; the MILP
+    carries D1 and D2 (milp.jl:73).  No start point: zeros."""
+    with Engine(milp, algo) as eng:
+        eng.initialize(sol)
+        st = eng.get_state()
+    D1, D2 = st.D1.copy(), st.D2.copy()
+    A = milp.A.tocsc(copy=True)
+    rows = A.indices
+    cols = np.repeat(np.arange(A.shape[1]), np.diff(A.indptr))
+    A.data = (D1[rows] * A.data) * D2[cols]
+    scaled = MILP(c=D2 * milp.c, lv=milp.lv / D2, uv=milp.uv / D2, A=A, lc=D1 * milp.lc,
+                  uc=D1 * milp.uc, D1=D1 * milp.D1, D2=milp.D2 * D2, int_var=milp.int_var,
+                  var_names=milp.var_names, dataset=milp.dataset, name=milp.name, path=milp.path)
+    if sol is None:
+        x0, y0 = np.zeros(milp.A.shape[1]), np.zeros(milp.A.shape[0])
+    else:
+        x0, y0 = np.asarray(sol.x, np.float64), np.asarray(sol.y, np.float64)
+    return scaled, PrimalDualSolution(x0 / D2, y0 / D1)
+
+
+def initialize(milp: MILP, sol: PrimalDualSolution | None, algo: Algorithm) -> Engine:
+    """initialize(milp, sol, algo), src/algorithms/common.jl:170-176: the state of a solve
+    of `milp` from `sol` (an Engine, initialised; preprocessing included), which
+    solve_inplace runs to termination and Engine.run_checks runs chunk by chunk.  Close it
+    (or use it in a `with` block) to release its device memory."""
+    eng = Engine(milp, algo)
+    try:
+        eng.initialize(sol)
+    except BaseException:
+        eng.close()
+        raise
+    return eng
+
+
+def solve_inplace(state: Engine, milp: MILP | None = None, algo: Algorithm | None = None):
+    """solve!(state, milp, algo), src/algorithms/common.jl:211-216 (pdlp.jl:69-94): runs an
+    initialised state's loop to termination; returns (PrimalDualSolution, ConvergenceStats)
+    as solve does.  `milp` and `algo` are the state's own (accepted for the reference's
+    signature)."""
+    while state.run_checks(100) == STILL_RUNNING:
+        pass
+    return state.solution(), state.stats()
 
 
 def solve(milp: MILP, *args):

@@ -18,7 +18,7 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib
 from .algorithms import Algorithm, ConvergenceStats, KKTErrors, LoopState, PDLP, TerminationStatus
-from .problems import PrimalDualSolution
+from .problems import PrimalDualSolution, require_unpreconditioned
 
 
 def footprint(m: int, n: int, nnz: int) -> int:
@@ -73,6 +73,7 @@ class BatchSolver:
         `sol` is an optional unscaled start, as in solve(milp, sol, algo)
         (src/algorithms/common.jl:186-209).  Raises MemoryError when the arena is full
         (retry after release)."""
+        require_unpreconditioned(lp)
         if hasattr(lp, "A"):   # MILP
             A = lp.A
             m, n = A.shape

@@ -84,6 +84,19 @@ class MILP:
                 f"({nbcons_eq(self)} equalities), {self.A.nnz} nonzeros")
 
 
+def require_unpreconditioned(lp) -> None:
+    """The engine scales on the device and reports errors and solutions through its own
+    D1, D2 only.  The reference composes those with the D1, D2 a MILP already carries
+    (precondition, src/components/preconditioning.jl:73; kkt_errors!, errors.jl:77-84;
+    get_solution, common.jl:233-235), so a preprocessed MILP would be measured and
+    unscaled differently here: it is refused (ArgumentError).  Generated LPs carry none."""
+    for name in ("D1", "D2"):
+        d = getattr(lp, name, None)
+        if d is not None and not np.all(np.asarray(d) == 1.0):
+            raise ArgumentError("the engine takes an unpreconditioned MILP (D1 = D2 = I); "
+                                "pass the problem given to preprocess, not its result")
+
+
 def nbvar(milp: MILP) -> int:
     """milp.jl:164"""
     return len(milp.c)

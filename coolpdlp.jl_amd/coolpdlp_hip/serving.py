@@ -29,7 +29,7 @@ import numpy as np
 from ._lib import ArgumentError, DimensionMismatch
 from .algorithms import Engine, PDLP
 from .batch import BatchSolver, footprint
-from .problems import MILP
+from .problems import MILP, require_unpreconditioned
 
 
 def plain_footprint(m: int, n: int, nnz: int) -> int:
@@ -117,6 +117,7 @@ class Server:
         """Queue an LP; `sol` is an optional start point, as in solve(milp, sol, algo)
         (e.g. the solution of an earlier request for a nearby LP, kept with
         keep_solutions)."""
+        require_unpreconditioned(lp)
         t = time.perf_counter() if t_arrival is None else t_arrival
         r = Request(self.next_id, lp, t, sol=sol)
         self.next_id += 1
@@ -280,6 +281,7 @@ class NativeServer:
         (cpdlp_server_submit_borrowed): they are held here until `get` and must not be
         modified meanwhile.  Otherwise the library copies them at submit."""
         C, _lib = self._C, self._lib
+        require_unpreconditioned(lp)
         if hasattr(lp, "A"):
             A = lp.A
             m, n = A.shape

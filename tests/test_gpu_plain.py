@@ -377,6 +377,35 @@ def test_chunked_loop_equals_solve(cp, algo, prec):
     assert np.array_equal(xd.cpu().numpy(), sol2.x) and np.array_equal(yd.cpu().numpy(), sol2.y)
 
 
+def test_preprocess_initialize_solve_inplace(cp):
+    """The reference's free functions (common.jl:147-216): preprocess gives the scaled
+    problem and start point with D1, D2 equal to the oracle's scaling bit for bit (and
+    the scaled matrix equal to the oracle's (D1 a) D2); initialize + solve_inplace equals
+    solve bit for bit."""
+    lp = G.column_lp(300, 700, 6, seed=9)
+    rng = np.random.default_rng(3)
+    x0, y0 = rng.standard_normal(lp.n), rng.standard_normal(lp.m)
+    milp = cp.MILP.from_generated(lp)
+    algo = cp.PDLP(termination_reltol=1e-6, max_kkt_passes=40_000)
+    start = cp.PrimalDualSolution(x0, y0)
+    scaled, ssol = cp.preprocess(milp, start, algo)
+    oD1, oD2, oa, _ = O.scaling(lp.m, lp.n, lp.colptr, lp.rowval, lp.nzval, 10, 1.0)
+    assert np.array_equal(scaled.D1, oD1) and np.array_equal(scaled.D2, oD2)
+    assert np.array_equal(scaled.A.indptr, lp.colptr) and np.array_equal(scaled.A.data, oa)
+    assert np.array_equal(scaled.c, oD2 * lp.c) and np.array_equal(scaled.uc, oD1 * lp.uc)
+    assert np.array_equal(scaled.lv, lp.lv / oD2) and np.array_equal(ssol.y, y0 / oD1)
+    assert np.array_equal(ssol.x, x0 / oD2)
+    sol, st = cp.solve(milp, start, algo)
+    with cp.initialize(milp, start, algo) as state:
+        first = state.get_state()
+        assert np.array_equal(first.x[0], ssol.x) and np.array_equal(first.y[0], ssol.y)
+        sol2, st2 = cp.solve_inplace(state, milp, algo)
+    assert st.termination_status == st2.termination_status == cp.OPTIMAL
+    assert st2.kkt_passes == st.kkt_passes
+    assert np.array_equal(sol.x, sol2.x) and np.array_equal(sol.y, sol2.y)
+    assert [p for p, _ in st.error_history] == [p for p, _ in st2.error_history]
+
+
 def test_chunked_loop_rejects_mixed(cp):
     g = G.column_lp(50, 100, 4, seed=1)
     with cp.Engine(cp.MILP.from_generated(g), cp.PDLP("mixed")) as eng:

@@ -55,3 +55,26 @@ def test_serving_routes_by_size():
     assert route(3000, 6000, 48_000, 8192, lanes) == 0
     assert route(3000, 6000, 48_000, 8192, lanes, overflow_queue=2) == -1
     assert route(30_000, 60_000, 480_000, 8192, lanes, overflow_queue=2) == 0
+
+
+def test_preprocessed_milp_is_refused_before_any_device_call():
+    """A MILP that already carries D1, D2 (the result of preprocess) is refused with
+    ArgumentError by every route: the reference would compose its scaling with the new
+    one (src/components/preconditioning.jl:73) and measure errors through both
+    (errors.jl:77-84), which the engine does not do.  The check runs on the host."""
+    import pytest
+    import scipy.sparse as sp
+    import coolpdlp_hip as cp
+    from coolpdlp_hip.problems import require_unpreconditioned
+    A = sp.csc_matrix(np.array([[1.0, 2.0], [0.0, 1.0]]))
+    kw = dict(c=[1.0, 1.0], lv=[0.0, 0.0], uv=[1.0, 1.0], A=A, lc=[0.0, 0.0], uc=[1.0, 1.0])
+    require_unpreconditioned(cp.MILP(**kw))
+    require_unpreconditioned(G.column_lp(20, 40, 3, seed=1))
+    for d in (dict(D1=[1.0, 2.0]), dict(D2=[0.5, 1.0])):
+        milp = cp.MILP(**kw, **d)
+        with pytest.raises(cp.ArgumentError):
+            cp.Engine(milp, cp.PDLP())
+        with pytest.raises(cp.ArgumentError):
+            cp.solve(milp, cp.PDLP())
+        with pytest.raises(cp.ArgumentError):
+            cp.preprocess(milp, None, cp.PDLP())
