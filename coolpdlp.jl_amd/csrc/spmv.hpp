@@ -118,6 +118,7 @@ struct MatView {
 };
 
 constexpr int SL_KMAX = 5;   // most vectors per product in the sliced format
+constexpr int SL_NS_MAX = 8; // most column slices (one per XCD)
 constexpr int SL_RPT = 8;    // short rows per thread in a sliced tile
 
 template <class V, int K>
@@ -488,8 +489,15 @@ __global__ __launch_bounds__(BLOCK) void k_xslice_combine(MatView<V> A, Epi epi)
         Acc acc[K];
 #pragma unroll
         for (int kv = 0; kv < K; ++kv) {
-            Acc sum = part[((size_t)kv * A.ns) * A.sl_rows + r];
-            for (int s = 1; s < A.ns; ++s) sum += part[((size_t)kv * A.ns + s) * A.sl_rows + r];
+            // all the slice sums in flight together, then added in slice order
+            Acc pv[SL_NS_MAX];
+#pragma unroll
+            for (int s = 0; s < SL_NS_MAX; ++s)
+                pv[s] = s < A.ns ? part[((size_t)kv * A.ns + s) * A.sl_rows + r] : Acc(0);
+            Acc sum = pv[0];
+#pragma unroll
+            for (int s = 1; s < SL_NS_MAX; ++s)
+                if (s < A.ns) sum += pv[s];
             acc[kv] = sum;
         }
         epi.row(r, acc, q, pre);
