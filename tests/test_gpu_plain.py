@@ -261,6 +261,28 @@ def test_warm_start_and_empty_constraints(cp):
     assert np.array_equal(sol.x, [1.0, -1.0, 0.0])
 
 
+@pytest.mark.parametrize("algo,fmt", [("PDLP", "csr"), ("PDLP", "auto"), ("PDLP", "ell"), ("PDHG", "csr")])
+def test_degenerate_shapes_match_oracle(cp, algo, fmt):
+    """Ragged and degenerate inputs through the whole loop, against the oracle on the same
+    LP: no constraints with a nonzero cost (A has no rows: the step size has no operator
+    norm to follow, every variable goes to the bound its cost points at), an empty row and
+    an empty column, and a single variable with a single free-sided constraint.  Same
+    status and pass count; the solution within 1e-6."""
+    cases = [
+        _lp(np.zeros((0, 3)), [1.0, -2.0, 0.5], [0, -1, -3], [2, 4, 5], [], []),
+        _lp([[1.0, 0, 1.0], [0, 0, 0], [2.0, 0, -1.0]], [1.0, 1.0, 1.0], [0, 0, 0], [10, 10, 10],
+            [1, -1, 0], [1, 1, 5]),
+        _lp([[2.0]], [3.0], [-np.inf], [np.inf], [4.0], [np.inf]),
+    ]
+    A_ = cp.PDLP if algo == "PDLP" else cp.PDHG
+    for k, g in enumerate(cases):
+        ref = O.solve(g, O.default_params(algo, termination_reltol=1e-6))
+        sol, st = cp.solve(cp.MILP.from_generated(g), A_("float64", None, fmt, termination_reltol=1e-6))
+        assert st.termination_status.name == ref.status, (k, st.termination_status, ref.status)
+        assert abs(st.kkt_passes - ref.stats.kkt_passes) <= 100, (k, st.kkt_passes, ref.stats.kkt_passes)
+        np.testing.assert_allclose(sol.x, ref.x, rtol=1e-6, atol=1e-6, err_msg=str(k))
+
+
 def test_float32_and_mixed(cp):
     g = G.column_lp(300, 700, 6, seed=13)
     milp = cp.MILP.from_generated(g)
@@ -330,17 +352,25 @@ def test_min_cost_flow_mixed_vs_oracle_fp64_at_1e8(cp):
     assert np.all(sol.x >= g.lv - 1e-9) and np.all(sol.x <= g.uv + 1e-9)
 
 
-@pytest.mark.parametrize("fmt,code", [("xslice", 3), ("xslice_all", 4)])
-def test_xslice_long_rows_track_csr(cp, fmt, code):
+@pytest.mark.parametrize("fmt,code,slices", [("xslice", 3, None), ("xslice_all", 4, None),
+                                             ("xslice", 3, "2,4"), ("xslice_all", 4, "4,2")])
+def test_xslice_long_rows_track_csr(cp, fmt, code, slices):
     """XCD column slices of the long rows (xslice) or of every row (xslice_all),
     including row parts longer than one tile (rows of 15k-30k nonzeros): the KKT history
     tracks the CSR run's to rounding over the first checks, and the format is reported
-    per side."""
+    per side.  Also with 2 and 4 slices per side (CPDLP_SLICES="<A>,<A'>"), where the
+    combine adds fewer slice sums than its 8 registers hold."""
+    import os
     g = G.power_law_lp(2000, 60000, 300_000, cap=40_000, seed=4)
     milp = cp.MILP.from_generated(g)
     kw = dict(termination_reltol=1e-12, max_kkt_passes=1500, powm_tol=1e-6)
     _, s_csr = cp.solve(milp, cp.PDLP("float64", None, "csr", **kw))
-    _, s_xs = cp.solve(milp, cp.PDLP("float64", None, fmt, **kw))
+    if slices:
+        os.environ["CPDLP_SLICES"] = slices
+    try:
+        _, s_xs = cp.solve(milp, cp.PDLP("float64", None, fmt, **kw))
+    finally:
+        os.environ.pop("CPDLP_SLICES", None)
     assert (s_xs.format_A, s_xs.format_At) == (code, code)
     assert len(s_csr.error_history) == len(s_xs.error_history) >= 10
     for (p1, e1), (p2, e2) in list(zip(s_csr.error_history, s_xs.error_history))[:8]:
