@@ -530,7 +530,7 @@ __global__ __launch_bounds__(NSUB * BB) void k_batch_setup(const int32_t* list, 
     __syncthreads();
     // constant KKT scales and the primal_weight_init norms
     {
-        double q[4] = {0, 0, 0, 0};
+        double q[5] = {0, 0, 0, 0, 0};
         for (int i = tid; i < m && tid < BB; i += BB) {
             const double cb = combine(I.lcs[i], I.ucs[i]);
             const double v = I.iD1[i] * cb;
@@ -541,11 +541,15 @@ __global__ __launch_bounds__(NSUB * BB) void k_batch_setup(const int32_t* list, 
             const double v = I.iD2[j] * I.cs[j];
             q[1] += v * v;
             q[3] += I.cs[j] * I.cs[j];
+            q[4] += I.c[j] != 0.0 ? 1.0 : 0.0;
         }
-        bsum<4, NT>(q, sh);
+        bsum<5, NT>(q, sh);
     }
     const double primal_scale = 1.0 + sqrt(sh[0]), dual_scale = 1.0 + sqrt(sh[1]);
     const double comb_norm = sqrt(sh[2]), c_norm = sqrt(sh[3]);
+    // no constraints and c == 0: the early exit of common.jl:194-198, sol.x =
+    // proj_box(0, lv, uv) of the preprocessed problem, OPTIMAL without a step
+    const bool early = m == 0 && sh[4] == 0.0;
     __syncthreads();
     // power method on A'A (FP64): x = I.x64[0], Ax = I.x64[1], t = I.y64[0]
     double* x = I.x64[0];
@@ -617,7 +621,7 @@ __global__ __launch_bounds__(NSUB * BB) void k_batch_setup(const int32_t* list, 
     // restart = sol (pdlp.jl:51-54).  The start was uploaded into the restart buffers,
     // which nothing above touches; each thread reads its own entries before writing them.
     for (int j = tid; j < n; j += NT) {
-        const double xs = I.warm ? I.xr64[j] / I.D2[j] : 0.0;
+        const double xs = early ? proj_box(0.0, I.lvs[j], I.uvs[j]) : I.warm ? I.xr64[j] / I.D2[j] : 0.0;
         const float xf = (float)xs;
         I.x32[0][j] = I.xa32[j] = I.xr32[j] = xf;
         I.x32[1][j] = I.xal32[j] = I.xb32[j] = 0.f;
@@ -647,8 +651,8 @@ __global__ __launch_bounds__(NSUB * BB) void k_batch_setup(const int32_t* list, 
         for (int f = 0; f < 6; ++f) S->err[f] = NAN;
         S->kkt_passes = S->inner = S->total = S->restarts = S->fp32_passes = 0;
         S->powm_iters = it;
-        S->status = CPDLP_STILL_RUNNING;
-        S->phase = P.precision == CPDLP_PREC_FP64 ? 1 : 0;
+        S->status = early ? CPDLP_OPTIMAL : CPDLP_STILL_RUNNING;
+        S->phase = early || P.precision == CPDLP_PREC_FP64 ? 1 : 0;
         S->cur = 0;
         S->stall = 0;
         __threadfence();

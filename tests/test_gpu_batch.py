@@ -43,6 +43,44 @@ def test_batch_matches_closed_form_and_oracle(cp, precision, reltol):
         assert abs(float(sol.x @ g.c) - float(ref.x @ g.c)) <= 20 * reltol * (1 + abs(g.objective))
 
 
+def _lp(A, c, lv, uv, lc, uc):
+    import scipy.sparse as sp
+    A = sp.csc_matrix(np.asarray(A, float))
+    return G.GeneratedLP(A.shape[0], A.shape[1], A.indptr.astype(np.int64), A.indices.astype(np.int32),
+                         A.data.astype(float), *[np.asarray(v, float) for v in (c, lv, uv, lc, uc)])
+
+
+@pytest.mark.parametrize("precision", ["float64", "mixed"])
+def test_batch_degenerate_shapes(cp, precision):
+    """Ragged and degenerate instances in one batch with an ordinary LP, against the
+    oracle's FP64 solve: no constraints with a nonzero cost, no constraints and c = 0
+    (the reference's early exit: x = proj_box(0), common.jl:194-198), an empty row and an
+    empty column, and a single variable with one constraint.  Same status; the solution
+    within 1e-6 (1e-5 mixed); the ordinary LP keeps its bits from a batch of its own."""
+    cases = [
+        _lp(np.zeros((0, 3)), [1.0, -2.0, 0.5], [0, -1, -3], [2, 4, 5], [], []),
+        _lp(np.zeros((0, 3)), [0.0, 0.0, 0.0], [1, -2, -np.inf], [2, -1, np.inf], [], []),
+        _lp([[1.0, 0, 1.0], [0, 0, 0], [2.0, 0, -1.0]], [1.0, 1.0, 1.0], [0, 0, 0], [10, 10, 10],
+            [1, -1, 0], [1, 1, 5]),
+        _lp([[2.0]], [3.0], [-np.inf], [np.inf], [4.0], [np.inf]),
+    ]
+    ordinary = G.column_lp(120, 300, 6, seed=5)
+    algo = cp.PDLP(precision, termination_reltol=1e-6)
+    with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
+        res = B.solve_all(cases + [ordinary])
+    with cp.BatchSolver(algo, hbm_bytes=64 << 20) as B:
+        (sol_alone, st_alone), = B.solve_all([ordinary])
+    tol = 1e-6 if precision == "float64" else 1e-5
+    for k, (g, (sol, st)) in enumerate(zip(cases, res)):
+        ref = O.solve(g, O.default_params(termination_reltol=1e-6))
+        assert st.termination_status.name == ref.status == "OPTIMAL", (k, st.termination_status)
+        np.testing.assert_allclose(sol.x, ref.x, rtol=tol, atol=tol, err_msg=str(k))
+    assert np.array_equal(res[1][0].x, [1.0, -1.0, 0.0])
+    sol_o, st_o = res[-1]
+    assert st_o.termination_status == cp.OPTIMAL and st_o.kkt_passes == st_alone.kkt_passes
+    assert np.array_equal(sol_o.x, sol_alone.x) and np.array_equal(sol_o.y, sol_alone.y)
+
+
 def test_batch_fp64_agrees_with_plain_engine(cp):
     g = G.column_lp(200, 450, 6, seed=31)
     milp = cp.MILP.from_generated(g)
