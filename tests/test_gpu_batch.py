@@ -55,7 +55,8 @@ def test_batch_degenerate_shapes(cp, precision):
     """Ragged and degenerate instances in one batch with an ordinary LP, against the
     oracle's FP64 solve: no constraints with a nonzero cost, no constraints and c = 0
     (the reference's early exit: x = proj_box(0), common.jl:194-198), an empty row and an
-    empty column, and a single variable with one constraint.  Same status; the solution
+    empty column, a single variable with one constraint, and a free row with a fixed
+    variable.  Same status; the solution
     within 1e-6 (1e-5 mixed); the ordinary LP keeps its bits from a batch of its own."""
     cases = [
         _lp(np.zeros((0, 3)), [1.0, -2.0, 0.5], [0, -1, -3], [2, 4, 5], [], []),
@@ -63,6 +64,9 @@ def test_batch_degenerate_shapes(cp, precision):
         _lp([[1.0, 0, 1.0], [0, 0, 0], [2.0, 0, -1.0]], [1.0, 1.0, 1.0], [0, 0, 0], [10, 10, 10],
             [1, -1, 0], [1, 1, 5]),
         _lp([[2.0]], [3.0], [-np.inf], [np.inf], [4.0], [np.inf]),
+        # a free row (both bounds infinite) and a fixed variable (lv = uv)
+        _lp([[1.0, 1.0, 0.0], [1.0, -1.0, 2.0], [0.0, 1.0, 1.0]], [1.0, 2.0, -1.0], [0, 1.5, -5], [4, 1.5, 5],
+            [-np.inf, 0.5, 1.0], [np.inf, 3.0, 1.0]),
     ]
     ordinary = G.column_lp(120, 300, 6, seed=5)
     algo = cp.PDLP(precision, termination_reltol=1e-6)

@@ -266,13 +266,16 @@ def test_degenerate_shapes_match_oracle(cp, algo, fmt):
     """Ragged and degenerate inputs through the whole loop, against the oracle on the same
     LP: no constraints with a nonzero cost (A has no rows: the step size has no operator
     norm to follow, every variable goes to the bound its cost points at), an empty row and
-    an empty column, and a single variable with a single free-sided constraint.  Same
-    status and pass count; the solution within 1e-6."""
+    an empty column, a single variable with a single free-sided constraint, and a free
+    row with a fixed variable.  Same status and pass count; the solution within 1e-6."""
     cases = [
         _lp(np.zeros((0, 3)), [1.0, -2.0, 0.5], [0, -1, -3], [2, 4, 5], [], []),
         _lp([[1.0, 0, 1.0], [0, 0, 0], [2.0, 0, -1.0]], [1.0, 1.0, 1.0], [0, 0, 0], [10, 10, 10],
             [1, -1, 0], [1, 1, 5]),
         _lp([[2.0]], [3.0], [-np.inf], [np.inf], [4.0], [np.inf]),
+        # a free row (both bounds infinite) and a fixed variable (lv = uv)
+        _lp([[1.0, 1.0, 0.0], [1.0, -1.0, 2.0], [0.0, 1.0, 1.0]], [1.0, 2.0, -1.0], [0, 1.5, -5], [4, 1.5, 5],
+            [-np.inf, 0.5, 1.0], [np.inf, 3.0, 1.0]),
     ]
     A_ = cp.PDLP if algo == "PDLP" else cp.PDHG
     for k, g in enumerate(cases):
