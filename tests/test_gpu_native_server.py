@@ -165,3 +165,37 @@ def test_request_larger_than_the_batch_arena_fails_alone(cp):
         sol, st, _, route = done[i]
         assert route == -1 and st.termination_status == cp.OPTIMAL
         assert abs(float(sol.x @ g.c) - g.objective) <= 2e-5 * (1 + abs(g.objective))
+
+
+@pytest.mark.parametrize("batch_limit", [0, 8000])
+def test_native_server_degenerate_requests(cp, batch_limit):
+    """Degenerate requests on each route (batch_limit 0: every request to a plain lane;
+    8000: every request to the batch route): no constraints with a nonzero cost and with
+    c = 0 (the early exit), an empty row and column, a free row with a fixed variable.
+    Each ends OPTIMAL with the oracle's solution to 1e-6."""
+    import scipy.sparse as sp
+
+    import oracle as O
+    from coolpdlp_hip.serving import NativeServer
+
+    def lp(A, c, lv, uv, lc, uc):
+        A = sp.csc_matrix(np.asarray(A, float))
+        return G.GeneratedLP(A.shape[0], A.shape[1], A.indptr.astype(np.int64), A.indices.astype(np.int32),
+                             A.data.astype(float), *[np.asarray(v, float) for v in (c, lv, uv, lc, uc)])
+
+    cases = [lp(np.zeros((0, 3)), [1.0, -2.0, 0.5], [0, -1, -3], [2, 4, 5], [], []),
+             lp(np.zeros((0, 3)), [0.0, 0.0, 0.0], [1, -2, -np.inf], [2, -1, np.inf], [], []),
+             lp([[1.0, 0, 1.0], [0, 0, 0], [2.0, 0, -1.0]], [1.0, 1.0, 1.0], [0, 0, 0], [10, 10, 10],
+                [1, -1, 0], [1, 1, 5]),
+             lp([[1.0, 1.0, 0.0], [1.0, -1.0, 2.0], [0.0, 1.0, 1.0]], [1.0, 2.0, -1.0], [0, 1.5, -5],
+                [4, 1.5, 5], [-np.inf, 0.5, 1.0], [np.inf, 3.0, 1.0])]
+    algo = cp.PDLP("float64", termination_reltol=1e-6)
+    with NativeServer(algo, batch_bytes=64 << 20, plain_bytes=1 << 30, batch_limit=batch_limit) as srv:
+        ids = {srv.submit(g): g for g in cases}
+        res = _drain(srv, len(ids))
+    for i, g in ids.items():
+        sol, st, _, route = res[i]
+        assert route == (-1 if batch_limit else 0)
+        ref = O.solve(g, O.default_params(termination_reltol=1e-6))
+        assert st.termination_status == cp.OPTIMAL and ref.status == "OPTIMAL"
+        np.testing.assert_allclose(sol.x, ref.x, rtol=1e-6, atol=1e-6)
