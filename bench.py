@@ -704,6 +704,9 @@ def bench_single(args):
                         "frac": bp / (ms_p * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "dual_step": {"ms": ms_d, "bytes": bd, "GBs": bd / (ms_d * 1e-3) / 1e9,
                       "frac": bd / (ms_d * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "timed": "each half-step alone after the solve, 20 launches between HIP events on the "
+                 "solver's stream (launch gaps included, so the two sum to a little more than "
+                 "ms_per_iteration)",
     }
     traffic, traffic_src = measured_traffic(desc["workload"], FORMATS[st.format_A], FORMATS[st.format_At])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
