@@ -92,6 +92,19 @@ def _bounds(s) -> tuple[float, float]:
     raise ArgumentError(f"unsupported set {s!r}")
 
 
+def _set_of(lo: float, up: float):
+    """The set whose finite sides are [lo, up] (None when both are infinite)."""
+    if lo == up:
+        return EqualTo(lo)
+    if np.isfinite(lo) and np.isfinite(up):
+        return Interval(lo, up)
+    if np.isfinite(lo):
+        return GreaterThan(lo)
+    if np.isfinite(up):
+        return LessThan(up)
+    return None
+
+
 @dataclass(frozen=True)
 class VariableIndex:
     value: int   # 0-based
@@ -256,6 +269,36 @@ class Optimizer:
             self._rows.append((terms, lo - k, up - k, type(s)))
             return ConstraintIndex("affine", type(s), len(self._rows) - 1)
         raise ArgumentError(f"unsupported function {func!r}")
+
+    def read_from_file(self, source):
+        """Load an MPS model (a path or an open text file), as the tutorial's
+        JuMP.read_from_file(path; format = MOI.FileFormats.FORMAT_MPS) followed by
+        set_optimizer does (test/tutorial.jl:87-93): the model is emptied, then one variable
+        per column with its bounds (a bound constraint of the set its finite sides make),
+        one affine constraint per row, and the objective with its constant and sense.
+        Integer markers are read and dropped: the optimizer solves the continuous
+        relaxation.  Returns the variables in column order."""
+        from .qps import read_mps
+        q = read_mps(source)
+        self.empty()
+        xs = self.add_variables(len(q.c))
+        for j, v in enumerate(xs):
+            s = _set_of(float(q.lvar[j]), float(q.uvar[j]))
+            if s is not None:
+                self.add_constraint(v, s)
+        rows = [[] for _ in range(len(q.lcon))]
+        for i, j, a in zip(q.arows, q.acols, q.avals):
+            rows[int(i)].append((float(a), xs[int(j)]))
+        for i, terms in enumerate(rows):
+            s = _set_of(float(q.lcon[i]), float(q.ucon[i]))
+            if s is None:   # a free row constrains nothing; kept so rows keep their indices
+                s = Interval(-np.inf, np.inf)
+            self.add_constraint(ScalarAffineFunction(terms, 0.0), s)
+        obj = ScalarAffineFunction([(float(q.c[j]), xs[j]) for j in range(len(q.c)) if q.c[j] != 0.0],
+                                   float(q.c0))
+        self.set_objective(obj, ObjectiveSense.MAX_SENSE if q.objsense.upper().startswith("MAX")
+                           else ObjectiveSense.MIN_SENSE)
+        return xs
 
     def set_objective(self, func, sense=ObjectiveSense.MIN_SENSE):
         if isinstance(func, VariableIndex):

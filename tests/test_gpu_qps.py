@@ -64,3 +64,26 @@ def test_suite_cpu_gpu_coherence():
     for f in ("primal", "primal_scale", "dual", "dual_scale", "gap", "gap_scale"):
         a, b = getattr(st.err, f), getattr(ref.stats.err, f)
         assert abs(a - b) <= 1e-8 * (1 + abs(b)), (name, f, a, b)
+
+
+def test_tutorial_workflow():
+    """test/tutorial.jl end to end on the fourth-smallest fixture (the tutorial takes
+    Netlib's fourth instance): MILP(qps) solved by PDLP at 1e-6 with a 10 s time limit, the
+    FP32 solve, and the Optimizer reading the same MPS file (matrix_type CSR); each x
+    feasible to 1e-3 and each objective within 1e-3 of HiGHS's; the relative error falls
+    from the first check to the last (tutorial.jl:122-131)."""
+    from coolpdlp_hip import GPUSparseMatrixCSR, Optimizer, relative
+    name, milp, obj = suite()[3]
+    sol, stats = solve(milp, PDLP(termination_reltol=1e-6, time_limit=10.0))
+    sol32, stats32 = solve(milp, PDLP("float32", None, "csr", termination_reltol=1e-6, time_limit=10.0))
+    opt = Optimizer()
+    xs = opt.read_from_file(milp.path)
+    opt.set_silent(True)
+    opt.set_raw("termination_reltol", 1e-6)
+    opt.set_raw("matrix_type", GPUSparseMatrixCSR)
+    opt.optimize()
+    x_jump = np.array([opt.variable_primal(v) for v in xs])
+    assert relative(stats.error_history[-1][1]) < relative(stats.error_history[0][1])
+    for x in (sol.x, sol32.x, x_jump):
+        assert is_feasible(x, milp, cons_tol=1e-3, int_tol=INF, verbose=False), name
+        assert abs(objective_value(x, milp) - obj) <= 1e-3 * abs(obj), (name, objective_value(x, milp), obj)

@@ -144,3 +144,56 @@ def test_options_and_errors():
         opt.add_constraint(x, moi.LessThan(1.0))  # a second bound on the same variable
     with pytest.raises(ArgumentError):
         opt.add_constraint(x, "not a set")
+
+
+def test_read_from_file_builds_the_mps_problem():
+    """Optimizer.read_from_file (the tutorial's JuMP.read_from_file of an MPS file): the
+    model it builds is the problem MILP(qps) holds, for every Netlib-style fixture (ranged,
+    free, fixed, upper-only variables, equality and ranged rows), with the objective
+    constant and the sense kept; a second read replaces the first model."""
+    import os
+
+    import scipy.sparse as sp
+
+    from coolpdlp_hip import MILP, read_mps
+    d = os.path.join(os.path.dirname(__file__), "golden", "mps")
+    opt = moi.Optimizer()
+    for f in sorted(os.listdir(d)):
+        if not f.endswith(".mps"):
+            continue
+        path = os.path.join(d, f)
+        xs = opt.read_from_file(path)
+        q = read_mps(path)
+        ref = MILP.from_qps(q)
+        milp, max_sense, constant = opt._problem()
+        assert len(xs) == ref.A.shape[1] and not max_sense and constant == q.c0, f
+        for name in ("c", "lv", "uv", "lc", "uc"):
+            assert np.array_equal(getattr(milp, name), getattr(ref, name)), (f, name)
+        assert (sp.csc_matrix(milp.A) != sp.csc_matrix(ref.A)).nnz == 0, f
+    # a maximisation with a constant and a free row
+    import io
+    text = """NAME T
+OBJSENSE
+    MAX
+ROWS
+ N obj
+ L r1
+ N free
+COLUMNS
+    x1 obj 1.0 r1 1.0
+    x1 free 2.0
+    x2 obj 2.0 r1 1.0
+RHS
+    rhs obj -3.5 r1 4.0
+BOUNDS
+ UP bnd x1 3.0
+ FR bnd x2
+ENDATA
+"""
+    xs = opt.read_from_file(io.StringIO(text))
+    milp, max_sense, constant = opt._problem()
+    assert max_sense and len(xs) == 2
+    assert np.array_equal(milp.c, [-1.0, -2.0])
+    assert np.array_equal(milp.lv, [0.0, -np.inf]) and np.array_equal(milp.uv, [3.0, np.inf])
+    assert milp.A.shape == (1, 2) and np.array_equal(milp.uc, [4.0]) and milp.lc[0] == -np.inf
+    assert constant == read_mps(io.StringIO(text)).c0
