@@ -122,6 +122,28 @@ def test_step_size_matches_oracle(cp):
     assert abs(eta - 0.9 / s) <= 1e-9 * eta
 
 
+def test_step_size_vs_exact_spectral_norm(cp):
+    """test/utils/linalg.jl "Spectral norm": dense N(0,1) 10 x 20 matrices; the engine's
+    power method gives the operator 2-norm of the scaled matrix (the step size is
+    invnorm_scaling / norm, 0.9 by default): within the reference's 1e-1 with its
+    tolerance 1e-7 and the default cap of n iterations, within 1e-6 with the cap lifted."""
+    rng = np.random.default_rng(42)
+    for k in range(10):
+        A = sp.csc_matrix(rng.standard_normal((10, 20)))
+        g = _lp(A.toarray(), np.ones(20), np.zeros(20), np.full(20, np.inf), np.zeros(10), np.full(10, np.inf))
+        milp = cp.MILP.from_generated(g)
+        with cp.Engine(milp, cp.PDLP(powm_tol=1e-7)) as eng:
+            eta, _, it = eng.step_size()
+            D1, D2, a, _ = eng.scaling()
+        with cp.Engine(milp, cp.PDLP(powm_tol=1e-7, powm_maxiter=10_000)) as eng:
+            eta_c, _, it_c = eng.step_size()
+        As = sp.csc_matrix((a, A.indices, A.indptr), shape=A.shape).toarray()
+        assert np.array_equal(As, D1[:, None] * A.toarray() * D2[None, :])
+        exact = np.linalg.norm(As, 2)
+        assert it <= 21 and abs(0.9 / eta - exact) <= 1e-1 * exact, (k, 0.9 / eta, exact, it)
+        assert abs(0.9 / eta_c - exact) <= 1e-6 * exact, (k, 0.9 / eta_c, exact, it_c)
+
+
 # ------------------------------------------------------------------ solves
 def test_moi_two_variable_lp(cp):
     # test/moi.jl:44-52: min 12x + 20y, 6x + 8y >= 100, 7x + 12y >= 120, x >= 0, 0 <= y <= 3
