@@ -313,6 +313,38 @@ def test_degenerate_shapes_match_oracle(cp, algo, fmt, prec):
         np.testing.assert_allclose(sol.x, ref.x, rtol=reltol, atol=reltol, err_msg=str(k))
 
 
+@pytest.mark.parametrize("fmt", ["csr", "auto"])
+def test_random_shapes_match_oracle(cp, fmt):
+    """Twelve feasible LPs of random shape (m 10-300, n/m 1.2-3, 2-8 nonzeros per column,
+    0-60% boxed columns) to 1e-6, and six of the reference's own random problems
+    (src/utils/test.jl: random bounds, a quarter infinite and a quarter equal, which do not
+    converge) over their first checks: the GPU run follows the oracle's FP64 run of the
+    same LP -- the same status and pass count within one check, the objective within 1e-6,
+    and every relative KKT error of the first five checks within 1e-9."""
+    rng = np.random.default_rng(7)
+    lps = []
+    for k in range(12):
+        m = int(rng.integers(10, 300))
+        n, kk = int(m * rng.uniform(1.2, 3.0)), int(rng.integers(2, 9))
+        lps.append((G.column_lp(m, n, kk, seed=500 + k, box_frac=float(rng.uniform(0, 0.6))), 60_000))
+    for k in range(6):
+        m, n = int(rng.integers(5, 200)), int(rng.integers(5, 400))
+        lps.append((G.random_milp_and_sol(m, n, float(rng.uniform(0.03, 0.3)), seed=100 + k)[0], 600))
+    for k, (g, cap) in enumerate(lps):
+        ref = O.solve(g, O.default_params("PDLP", termination_reltol=1e-6, max_kkt_passes=cap))
+        sol, st = cp.solve(cp.MILP.from_generated(g),
+                           cp.PDLP("float64", None, fmt, termination_reltol=1e-6, max_kkt_passes=cap))
+        assert st.termination_status.name == ref.status, (k, st.termination_status, ref.status)
+        assert abs(st.kkt_passes - ref.stats.kkt_passes) <= 100, (k, st.kkt_passes, ref.stats.kkt_passes)
+        if ref.status == "OPTIMAL":
+            obj, obj_ref = float(sol.x @ g.c), float(ref.x @ g.c)
+            assert abs(obj - obj_ref) <= 1e-6 * (1 + abs(obj_ref)), (k, obj, obj_ref)
+        for (p1, d1), (p2, e2) in list(zip(ref.history, st.error_history))[:5]:
+            e1 = max(d1["primal"] / d1["primal_scale"], d1["dual"] / d1["dual_scale"],
+                     d1["gap"] / d1["gap_scale"])
+            assert p1 == p2 and abs(e1 - e2.relative()) <= 1e-9 * e1, (k, p1, e1, e2.relative())
+
+
 def test_float32_and_mixed(cp):
     g = G.column_lp(300, 700, 6, seed=13)
     milp = cp.MILP.from_generated(g)
