@@ -6,6 +6,7 @@ method itself.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 import enum
 import math
@@ -379,8 +380,14 @@ def preprocess(milp: MILP, sol: PrimalDualSolution | None, algo: Algorithm):
     whose D1 and D2 the engine computes on the device, applied as precondition
     (preconditioning.jl:37-91): A~ = (D1 a) D2, c~ = D2 c, lv~ = lv / D2, uv~ = uv / D2,
     lc~ = D1 lc, uc~ = D1 uc, x~ = x / D2, y~ = y / D1, in the caller's order; the MILP
-    carries D1 and D2 (milp.jl:73).  No start point: zeros."""
-    with Engine(milp, algo) as eng:
+    carries D1 and D2 (milp.jl:73).  No start point: zeros.  The scaling is computed in
+    FP64 in every precision, so a mixed-precision algorithm is scaled by its FP64 form."""
+    run = algo
+    if algo.params.precision == _lib.PREC_MIXED:
+        run = copy.copy(algo)
+        run.params = type(algo.params).from_buffer_copy(algo.params)
+        run.params.precision = _lib.PREC_FP64
+    with Engine(milp, run) as eng:
         eng.initialize(sol)
         st = eng.get_state()
     D1, D2 = st.D1.copy(), st.D2.copy()
@@ -402,7 +409,8 @@ def initialize(milp: MILP, sol: PrimalDualSolution | None, algo: Algorithm) -> E
     """initialize(milp, sol, algo), src/algorithms/common.jl:170-176: the state of a solve
     of `milp` from `sol` (an Engine, initialised; preprocessing included), which
     solve_inplace runs to termination and Engine.run_checks runs chunk by chunk.  Close it
-    (or use it in a `with` block) to release its device memory."""
+    (or use it in a `with` block) to release its device memory.  FP64 and FP32 only: the
+    mixed precision's hand-over to FP64 runs inside solve (ValueError here)."""
     eng = Engine(milp, algo)
     try:
         eng.initialize(sol)

@@ -496,6 +496,9 @@ def test_preprocess_initialize_solve_inplace(cp):
     assert st2.kkt_passes == st.kkt_passes
     assert np.array_equal(sol.x, sol2.x) and np.array_equal(sol.y, sol2.y)
     assert [p for p, _ in st.error_history] == [p for p, _ in st2.error_history]
+    # mixed precision: the same (FP64) scaling
+    scaled_m, _ = cp.preprocess(milp, start, cp.PDLP("mixed", termination_reltol=1e-6))
+    assert np.array_equal(scaled_m.D1, scaled.D1) and np.array_equal(scaled_m.D2, scaled.D2)
 
 
 def test_chunked_loop_rejects_mixed(cp):
