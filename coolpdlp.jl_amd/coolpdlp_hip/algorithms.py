@@ -409,8 +409,8 @@ def initialize(milp: MILP, sol: PrimalDualSolution | None, algo: Algorithm) -> E
     """initialize(milp, sol, algo), src/algorithms/common.jl:170-176: the state of a solve
     of `milp` from `sol` (an Engine, initialised; preprocessing included), which
     solve_inplace runs to termination and Engine.run_checks runs chunk by chunk.  Close it
-    (or use it in a `with` block) to release its device memory.  FP64 and FP32 only: the
-    mixed precision's hand-over to FP64 runs inside solve (ValueError here)."""
+    (or use it in a `with` block) to release its device memory.  In mixed precision the
+    loop runs the FP32 phase, then the FP64 one, as solve does."""
     eng = Engine(milp, algo)
     try:
         eng.initialize(sol)

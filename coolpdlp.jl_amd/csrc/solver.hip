@@ -1583,19 +1583,7 @@ void Handle::solve(const double* x0, const double* y0) {
         fp32_passes = stats.kkt_passes;
         const bool reached = stats.termination_status == CPDLP_OPTIMAL && r32 > P.termination_reltol;
         if (polish && (reached || phase_switch)) {
-            st64.alloc(n, m);
-            rst_err_valid = false;   // the restart point in FP64 has FP64 errors
-            if (blowup) {   // restart from the best FP32 iterate with its primal weight
-                omega = best_omega;
-                upcast(st32, st64, n, m, stream);
-                eta_sum = 0.0;
-                inner = 0;
-                total = 0;
-            } else {        // continue: the whole PDLP state in FP64, same scalars
-                continue_fp64(st32, st64, n, m, stream);
-            }
-            active64 = true;
-            stats.termination_status = CPDLP_STILL_RUNNING;
+            hand_over_fp64();
             loop<double>(st64, P.termination_reltol, true);
         }
     }
@@ -1605,9 +1593,6 @@ void Handle::solve(const double* x0, const double* y0) {
 // initialize(milp, sol, algo) (common.jl:176; pdlp.jl:45-67, pdhg.jl:40-54): everything
 // of solve before the loop; the loop then runs in pieces through run_checks.
 void Handle::initialize(const double* x0, const double* y0) {
-    if (P.precision == CPDLP_PREC_MIXED)
-        throw Error(CPDLP_ERR_ARGUMENT,
-                    "mixed precision runs its two phases through cpdlp_solve; use fp64 or fp32");
     if (begin(x0, y0)) {
         finish_stats();
         return;
@@ -1618,9 +1603,32 @@ void Handle::initialize(const double* x0, const double* y0) {
     } else {
         init_state<float>(st32, x0d.p, y0d.p);
         active64 = false;
+        best_rel = INFINITY;
+        best_omega = omega;
+        stall = 0;
+        phase_switch = false;
+        blowup = false;
     }
     stepping = true;
     finish_stats();
+}
+
+// mixed precision: the FP32 phase has ended (its tolerance reached, or switched early);
+// the rest of the solve runs in FP64 from its state, as solve does
+void Handle::hand_over_fp64() {
+    st64.alloc(n, m);
+    rst_err_valid = false;   // the restart point in FP64 has FP64 errors
+    if (blowup) {   // restart from the best FP32 iterate with its primal weight
+        omega = best_omega;
+        upcast(st32, st64, n, m, stream);
+        eta_sum = 0.0;
+        inner = 0;
+        total = 0;
+    } else {        // continue: the whole PDLP state in FP64, same scalars
+        continue_fp64(st32, st64, n, m, stream);
+    }
+    active64 = true;
+    stats.termination_status = CPDLP_STILL_RUNNING;
 }
 
 // the five iterates of a state (sol, sol_last, avg, avg_last, restart: PDLPState,
@@ -1678,7 +1686,7 @@ void Handle::get_state(cpdlp_state* st, double* x, double* y, double* D2, double
 void Handle::resume(const cpdlp_state& st, const double* x, const double* y, const double* D2,
                     const double* D1) {
     if (P.precision == CPDLP_PREC_MIXED)
-        throw Error(CPDLP_ERR_ARGUMENT, "mixed precision runs its two phases through cpdlp_solve; use fp64 or fp32");
+        throw Error(CPDLP_ERR_ARGUMENT, "mixed precision: a loop state carries no FP32/FP64 phase; resume in fp64 or fp32");
     if ((n && (!x || !D2)) || (m && (!y || !D1))) throw Error(CPDLP_ERR_ARGUMENT, "NULL state buffer");
     if (!(std::isfinite(st.eta) && st.eta > 0) || !(std::isfinite(st.omega) && st.omega > 0) ||
         !(std::isfinite(st.eta_sum) && st.eta_sum >= 0))
@@ -1754,10 +1762,22 @@ int32_t Handle::run_checks(int64_t max_checks) {
         throw Error(CPDLP_ERR_STATE, "call cpdlp_initialize first");
     }
     if (max_checks <= 0) return stats.termination_status;
-    if (active64)
+    if (active64) {
         loop<double>(st64, P.termination_reltol, true, max_checks);
-    else
+    } else if (P.precision != CPDLP_PREC_MIXED) {
         loop<float>(st32, P.termination_reltol, true, max_checks);
+    } else {   // the FP32 phase, then (within the same max_checks) the FP64 one, as solve
+        const double r32 = std::max(P.polish_reltol, P.termination_reltol);
+        const int64_t p0 = stats.kkt_passes;
+        loop<float>(st32, r32, false, max_checks);
+        fp32_passes = stats.kkt_passes;
+        const bool reached = stats.termination_status == CPDLP_OPTIMAL && r32 > P.termination_reltol;
+        if (reached || phase_switch) {
+            hand_over_fp64();
+            const int64_t left = max_checks - (stats.kkt_passes - p0) / P.check_every;
+            if (left > 0) loop<double>(st64, P.termination_reltol, true, left);
+        }
+    }
     if (stats.termination_status != CPDLP_STILL_RUNNING) stepping = false;
     finish_stats();
     return stats.termination_status;

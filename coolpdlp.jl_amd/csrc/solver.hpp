@@ -163,6 +163,7 @@ struct Handle {
     void finish_stats();
     void solve(const double* x0, const double* y0);
     void initialize(const double* x0, const double* y0);
+    void hand_over_fp64();
     int32_t run_checks(int64_t max_checks);
     template <class V>
     void state_to(IterState<V>& S, double* xs, double* ys, const double* ones);

@@ -318,8 +318,8 @@ end
 
 Handle of a solve run chunk by chunk: `initialize` (common.jl:176) -> `cpdlp_initialize`,
 `solve!` (pdlp.jl:69-94) -> `cpdlp_run_checks` one chunk of `check_every` steps at a time,
-yielding in between as the reference does (pdlp.jl:77).  T = Float64 or Float32 (the
-mixed precision's two phases run through `solve`).
+yielding in between as the reference does (pdlp.jl:77).  Any precision: with
+`HIPEngine(mixed = true)` the chunks run the FP32 phase, then the FP64 one, as `solve`.
 """
 mutable struct HIPState{T}
     handle::Ptr{Cvoid}

@@ -173,8 +173,8 @@ int cpdlp_solve(cpdlp_handle* h, const double* x0, const double* y0);
 /* initialize(milp, sol, algo), src/algorithms/common.jl:176 (pdlp.jl:45-67,
  * pdhg.jl:40-54): what cpdlp_solve does before its loop (preprocess, step size,
  * primal weight, the start point on the device), the loop then run in pieces by
- * cpdlp_run_checks.  FP64 and FP32 precision; the mixed precision's two phases run
- * through cpdlp_solve (CPDLP_ERR_ARGUMENT here). */
+ * cpdlp_run_checks.  Every precision: in mixed precision cpdlp_run_checks runs the FP32
+ * phase, then the FP64 one from its state, as cpdlp_solve does. */
 int cpdlp_initialize(cpdlp_handle* h, const double* x0, const double* y0);
 
 /* solve!(state, milp, algo)'s loop, pdlp.jl:69-94 / pdhg.jl:58-74, for at most
@@ -196,7 +196,7 @@ int cpdlp_get_state(cpdlp_handle* h, cpdlp_state* st, double* x, double* y, doub
  * resume); others are carried over through the unscaled point (x .* D2 ./ D2_own), so a
  * batch instance continues on this engine (cpdlp_batch_take_state).  The step size is the
  * state's.  The time limit counts from the resume, and the error history restarts.
- * FP64 and FP32 handles (CPDLP_ERR_ARGUMENT for mixed, as cpdlp_initialize).  No
+ * FP64 and FP32 handles (CPDLP_ERR_ARGUMENT for mixed: the state carries no phase).  No
  * reference counterpart. */
 int cpdlp_resume(cpdlp_handle* h, const cpdlp_state* st, const double* x, const double* y,
                  const double* D2, const double* D1);

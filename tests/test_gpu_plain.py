@@ -501,11 +501,27 @@ def test_preprocess_initialize_solve_inplace(cp):
     assert np.array_equal(scaled_m.D1, scaled.D1) and np.array_equal(scaled_m.D2, scaled.D2)
 
 
-def test_chunked_loop_rejects_mixed(cp):
-    g = G.column_lp(50, 100, 4, seed=1)
-    with cp.Engine(cp.MILP.from_generated(g), cp.PDLP("mixed")) as eng:
-        with pytest.raises(ValueError):
-            eng.initialize()
+@pytest.mark.parametrize("reltol,polish_reltol", [(1e-7, 1e-4), (1e-9, 1e-12)])
+def test_chunked_loop_mixed_equals_solve(cp, reltol, polish_reltol):
+    """Mixed precision through initialize + run_checks one chunk at a time: the FP32 phase,
+    the hand-over and the FP64 phase give solve's bits (the same history, passes, FP32
+    passes and solution), whether the FP32 phase ends at its own tolerance (1e-4) or by
+    the early switch (1e-9, which FP32 iterates cannot reach)."""
+    g = G.column_lp(300, 700, 6, seed=9)
+    milp = cp.MILP.from_generated(g)
+    algo = cp.PDLP("mixed", termination_reltol=reltol, polish_reltol=polish_reltol, max_kkt_passes=60_000)
+    with cp.Engine(milp, algo) as eng:
+        sol, st = eng.solve()
+        eng.initialize()
+        status = cp.STILL_RUNNING
+        while status == cp.STILL_RUNNING:
+            status = eng.run_checks(1)
+        sol2, st2 = eng.solution(), eng.stats()
+    assert st.termination_status == st2.termination_status == cp.OPTIMAL
+    assert 0 < st.fp32_passes < st.kkt_passes
+    assert (st2.kkt_passes, st2.fp32_passes, st2.restarts) == (st.kkt_passes, st.fp32_passes, st.restarts)
+    assert np.array_equal(sol.x, sol2.x) and np.array_equal(sol.y, sol2.y)
+    assert [(p, e) for p, e in st.error_history] == [(p, e) for p, e in st2.error_history]
 
 
 # ------------------------------------------------------------------ full size
