@@ -356,13 +356,15 @@ inline int32_t build_ell(const HostSide& S, double max_fill, std::vector<int32_t
 // balanced by entry count, slice_of_columns); slice s's part of row r (r < nrows) is the
 // run of that row's entries whose column lies in the slice, in the row's entry order.  rowptr:
 // ns * (nrows + 1) absolute offsets, slices stored one after the other.  Each slice's
-// rows are cut into tiles of at most TILE_NNZ entries and BLOCK rows (long rows) or
-// SL_RPT * BLOCK rows (short rows, whose parts hold a few entries) that never span
-// the long/short boundary n_long nor n_slice (so the tiles of rows [0, n_slice) are a
-// prefix of every slice's tiles, tile_n_pre[s] of them); a row part longer than TILE_NNZ
-// is a tile of its own (r1 = r0 + 1, cnt > TILE_NNZ).  Rows past nrows are not sliced.
+// rows are cut into tiles of at most TILE_NNZ entries and BLOCK rows (rows whose parts
+// are summed by lane groups: the first n_group rows, by default the long rows n_long) or
+// SL_RPT * BLOCK rows (the others, a thread per row part) that never span that boundary
+// nor n_slice (so the tiles of rows [0, n_slice) are a prefix of every slice's tiles,
+// tile_n_pre[s] of them); a row part longer than TILE_NNZ is a tile of its own (r1 = r0 +
+// 1, cnt > TILE_NNZ).  Rows past nrows are not sliced.
 struct HostSlices {
     int32_t ns = 0, rows = 0, max_tiles = 0, max_tiles_pre = 0;
+    int32_t n_group = 0;   // rows [0, n_group): parts summed by lane groups, the rest a thread each
     std::vector<int32_t> rowptr, col, vmap, tile_off, tile_n_pre;
     std::vector<TileMeta> tiles;
 };
@@ -386,8 +388,10 @@ inline std::vector<int8_t> slice_of_columns(const HostSide& S, int32_t ns, int32
     return sl;
 }
 
-inline void build_slices(const HostSide& S, int32_t ns, int32_t nrows, HostSlices& H) {
-    const int32_t nr = nrows, nl = std::min(S.n_long, nrows), np = std::min(std::max(S.n_slice, S.n_long), nrows);
+inline void build_slices(const HostSide& S, int32_t ns, int32_t nrows, HostSlices& H, int32_t n_group = -1) {
+    const int32_t nr = nrows, np = std::min(std::max(S.n_slice, S.n_long), nrows);
+    const int32_t nl = std::min(n_group < 0 ? S.n_long : std::min(n_group, S.n_long), nrows);
+    H.n_group = nl;
     const std::vector<int8_t> slice_of = slice_of_columns(S, ns, nr);
     H.ns = ns;
     H.rows = nr;

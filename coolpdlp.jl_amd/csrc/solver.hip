@@ -330,6 +330,7 @@ MatView<V> DevSide::view(bool allow_slices) const {
     v.ns = sl_ns;
     v.sl_all = format == CPDLP_FMT_XSLICE_ALL;
     v.sl_rows = v.sl_all ? rows : n_slice;
+    v.sl_n_long = sl_n_group;
     v.sl_stride = sl_built + 1;
     v.sl_rowptr = sl_rowptr.p;
     v.sl_col = sl_col.p;
@@ -375,11 +376,12 @@ void DevSide::build_line_tiles(const HostLineTiles& T, hipStream_t s) {
 
 // NS column slices of rows [0, nrows); NS divides the 8 XCDs of the MI355X, and slice s
 // is summed on the XCDs x with x % NS == s (workgroups go to the XCDs round-robin)
-void DevSide::build_slices(const HostSide& S, int32_t nrows, int32_t NS, hipStream_t s) {
+void DevSide::build_slices(const HostSide& S, int32_t nrows, int32_t NS, hipStream_t s, int32_t n_group) {
     if (NS != 2 && NS != 4 && NS != 8) throw Error(CPDLP_ERR_ARGUMENT, "slice count must be 2, 4 or 8");
     HostSlices H;
-    cpdlp::build_slices(S, NS, nrows, H);
+    cpdlp::build_slices(S, NS, nrows, H, n_group);
     sl_ns = NS;
+    sl_n_group = H.n_group;
     sl_built = nrows;
     sl_max_tiles = H.max_tiles;
     sl_max_tiles_pre = H.max_tiles_pre;
@@ -606,16 +608,27 @@ Handle::Handle(const cpdlp_lp& lp, const cpdlp_params& prm) : P(prm) {
         ns_of[0] = (int32_t)std::strtol(spec.substr(0, comma).c_str(), nullptr, 10);
         ns_of[1] = comma == std::string::npos ? ns_of[0] : (int32_t)std::strtol(spec.substr(comma + 1).c_str(), nullptr, 10);
     }
+    // In the slices, the parts of rows longer than CPDLP_SLICE_GROUP_ROW entries (default
+    // 1,024, at least long_row) are summed by 16-lane groups, the shorter ones a thread per
+    // part (a part holds about 1/8 of its row).  Measured on the large power-law LP
+    // (ms per iteration, bench.py --long-row, which set both thresholds then): 0.2308 at
+    // 32, 0.2295 at 64, 0.2282 at 128, 0.2268 at 256, 0.2261 at 512, 0.2259 at 1,024,
+    // 0.2301 at 2,048.
+    int32_t group_row = 1024;
+    if (const char* e = std::getenv("CPDLP_SLICE_GROUP_ROW")) group_row = (int32_t)std::strtol(e, nullptr, 10);
+    group_row = std::max(group_row, P.long_row);
     int side_k = 0;
     for (auto side : {std::make_pair(&A, &L.A), std::make_pair(&At, &L.At)}) {
         const HostSide& H = *side.second;
         const int32_t ns = ns_of[side_k++];
+        int32_t n_group = 0;   // the long rows are longest first: a prefix
+        while (n_group < H.n_long && H.rowptr[n_group + 1] - H.rowptr[n_group] > group_row) ++n_group;
         const bool big = (size_t)H.cols * elem > ((size_t)4 << 20) && H.nnz >= ((int64_t)4 << 20);
         if (P.format == CPDLP_FMT_XSLICE_ALL || (P.format == CPDLP_FMT_AUTO && big))
-            side.first->build_slices(H, H.rows, ns, s);
+            side.first->build_slices(H, H.rows, ns, s, n_group);
         else if (P.format == CPDLP_FMT_XSLICE ||
                  (P.format == CPDLP_FMT_AUTO && H.n_slice > 0 && (size_t)H.cols * elem > ((size_t)2 << 20)))
-            side.first->build_slices(H, H.n_slice, ns, s);
+            side.first->build_slices(H, H.n_slice, ns, s, n_group);
     }
     // Line-grouped tiles of the short rows (the steps of a CSR side): built for a side
     // whose short rows (>= 1M entries) would need >= 10% fewer gather requests in them,

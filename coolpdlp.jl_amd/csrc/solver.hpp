@@ -25,6 +25,7 @@ struct DevSide {
     DevBuf<float> ell_val32;
     // XCD column slices of rows [0, sl_built) (rows [0, n_slice), or all rows)
     int32_t sl_ns = 0, sl_built = 0, sl_max_tiles = 0, sl_max_tiles_pre = 0;
+    int32_t sl_n_group = 0;   // sliced rows whose parts are summed by lane groups (a prefix)
     int64_t sl_nnz = 0;
     DevBuf<int32_t> sl_rowptr, sl_col, sl_vmap, sl_tile_off, sl_tile_n, sl_tile_n_pre;
     DevBuf<TileMeta> sl_tiles;
@@ -52,7 +53,7 @@ struct DevSide {
     template <class V>
     MatView<V> view(bool allow_slices = true) const;
     void upload_structure(const HostSide& S, hipStream_t s);
-    void build_slices(const HostSide& S, int32_t nrows, int32_t ns, hipStream_t s);
+    void build_slices(const HostSide& S, int32_t nrows, int32_t ns, hipStream_t s, int32_t n_group = -1);
     bool slices_all() const { return sl_ns > 0 && sl_built == rows; }
     void build_ell(const HostSide& S, hipStream_t s, double max_fill);
     void build_line_tiles(const HostLineTiles& T, hipStream_t s);

@@ -78,6 +78,7 @@ struct MatView {
     // vector, which stays in that XCD's L2.  The other rows run as the short tiles xs_tiles.
     int32_t ns = 0;
     int32_t sl_rows = 0;                  // rows [0, sl_rows) are sliced
+    int32_t sl_n_long = 0;                // rows [0, sl_n_long): parts summed by 16-lane groups
     bool sl_all = false;                  // sl_rows == rows: no short-tile blocks
     int32_t sl_stride = 0;                // per-slice rowptr length (built rows + 1)
     const int32_t* sl_rowptr = nullptr;   // ns * sl_stride, absolute offsets
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_xslice(MatView<V> A, XP X, Epi e
     // short rows' tiles (uniform per block: tiles never span the long/short boundary):
     // thread tid sums the parts of rows r0 + tid + j * BLOCK, j < SL_RPT.  Their bounds
     // are read when summed, so that the long rows' launches do not hold registers for them.
-    const bool per_thread = tm.r0 >= A.n_long;
+    const bool per_thread = tm.r0 >= A.sl_n_long;
     // every lane of a group runs the same trips, so the butterfly's partners are active
     auto row_parts = [&](int kv) {
         if (per_thread) {

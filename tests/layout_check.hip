@@ -51,10 +51,12 @@ static HostSide make_side(std::mt19937& rng, int rows, int cols, int n_long, int
     return S;
 }
 
-static void check_slices(const HostSide& S, int nr) {
-    const int ns = 8, nl = std::min(S.n_long, nr), np = std::min(std::max(S.n_slice, S.n_long), nr);
+static void check_slices(const HostSide& S, int nr, int n_group = -1) {
+    const int ns = 8, np = std::min(std::max(S.n_slice, S.n_long), nr);
+    const int nl = std::min(n_group < 0 ? S.n_long : std::min(n_group, S.n_long), nr);
     HostSlices H;
-    build_slices(S, ns, nr, H);
+    build_slices(S, ns, nr, H, n_group);
+    if (H.n_group != nl) fail("lane-group rows", H.n_group, nl);
     const std::vector<int8_t> so = slice_of_columns(S, ns, nr);
     // contiguous column ranges, in slice order
     for (int j = 1; j < S.cols; ++j)
@@ -321,6 +323,10 @@ int main() {
         S2.n_slice = std::min(S.rows, S.n_long + 300);
         check_slices(S2, S2.n_slice);
         check_slices(S2, S2.rows);
+        for (int ng : {0, S.n_long / 3}) {   // fewer rows summed by lane groups
+            check_slices(S2, S2.n_slice, ng);
+            check_slices(S2, S2.rows, ng);
+        }
         check_line_tiles(S, 16);
         check_line_tiles(S, 4);
     }
